@@ -1,0 +1,123 @@
+/*
+ * sdm.h -- C ABI of the MI355X superellipsoid DEM step library (libsdm_hip.so).
+ *
+ * One context owns one scene (bodies, interactions, collider state) resident in
+ * HBM and advances it with the engine list
+ *
+ *   ForceResetter()
+ *   InsertionSortCollider([Bo1_Superquadrics_Aabb(), Bo1_Wall_Aabb()], verletDist=...)
+ *   InteractionLoop([Ig2_Wall_Superquadrics_SuperquadricsGeom(2)(),
+ *                    Ig2_Superquadrics_Superquadrics_SuperquadricsGeom(2)()],
+ *                   [Ip2_SuperquadricsMat(2)_...()], [SuperquadricsLaw(2)()])
+ *   NewtonIntegrator(damping=..., gravity=..., isSuperquadrics=1)
+ *
+ * which is the canonical SudoDEM3D engine list (examples/SudoDEM3D/example1.py:105-116).
+ * A reference engine would call this ABI once per step, never per pair:
+ *   - GlobalEngine::action() of each engine (core/Engine.hpp:34, core/GlobalEngine.hpp:13-17)
+ *     maps onto sdm_step(), which runs all four engines of one Scene::moveToNextTimeStep
+ *     (core/Scene.cpp:70-138);
+ *   - the Python attribute system (SUDODEM_CLASS_BASE_DOC_ATTRS_*,
+ *     lib/serialization/Serializable.hpp:207-223) maps onto sdm_set()/sdm_get() with the
+ *     same attribute names ("state.pos", "geom.PenetrationDepth", "phys.shearForce", ...).
+ *
+ * Conventions: every int return is 0 on success and negative on failure; the message of
+ * the last failure is returned by sdm_last_error(). No C++ exception crosses the ABI.
+ * Host buffers belong to the caller; device buffers belong to the context. A context is
+ * used by one host thread at a time; calls are stream ordered and sdm_get() synchronises.
+ */
+#ifndef SDM_H
+#define SDM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sdm_ctx sdm_ctx;
+
+enum {
+  SDM_LAW_LINEAR = 0, /* Ig2_*_SuperquadricsGeom (Levenberg-Marquardt) + Ip2_SuperquadricsMat_* + SuperquadricsLaw */
+  SDM_LAW_HERTZ = 1   /* Ig2_*_SuperquadricsGeom2 (Nelder-Mead) + Ip2_SuperquadricsMat2_* + SuperquadricsLaw2 */
+};
+
+enum {
+  SDM_FP64 = 0,  /* all arithmetic in double, reference semantics */
+  SDM_MIXED = 1  /* narrowphase iterations in float on relative frames, final contact refinement in double */
+};
+
+enum { SDM_BODY_SUPERQUADRIC = 0, SDM_BODY_WALL = 1 };
+
+/* Engine attributes.  Each field names the reference attribute it carries. */
+typedef struct sdm_params {
+  double dt;                    /* Scene::dt */
+  double gravity[3];            /* NewtonIntegrator.gravity (pkg/dem/NewtonIntegrator.hpp:73) */
+  double damping;               /* NewtonIntegrator.damping (NewtonIntegrator.hpp:70) */
+  int32_t exact_aspherical_rot; /* NewtonIntegrator.exactAsphericalRot (NewtonIntegrator.hpp:75) */
+  double verlet_dist;           /* InsertionSortCollider.verletDist, >= 0 (InsertionSortCollider.hpp:217) */
+  int32_t target_interv;        /* InsertionSortCollider.targetInterv (InsertionSortCollider.hpp:214) */
+  double min_sweep_dist_factor; /* InsertionSortCollider.minSweepDistFactor (InsertionSortCollider.hpp:218) */
+  int32_t sort_axis;            /* InsertionSortCollider.sortAxis (InsertionSortCollider.hpp:211) */
+  int32_t law;                  /* SDM_LAW_* : which Ig2/Ip2/Law2 functor family the InteractionLoop holds */
+  int32_t precision;            /* SDM_FP64 or SDM_MIXED */
+} sdm_params;
+
+/* SuperquadricsMat / SuperquadricsMat2 (pkg/dem/Superquadrics.hpp:247-277) + Material.density */
+typedef struct sdm_material {
+  double density, Kn, Ks, frictionAngle, betan, betas, G, nu;
+} sdm_material;
+
+/* Per-run counters (device side, summed over the steps since the last reset). */
+typedef struct sdm_counters {
+  int64_t steps;          /* engine-loop iterations run */
+  int64_t collider_runs;  /* InsertionSortCollider::action executions */
+  int64_t contact_evals;  /* Ig2 invocations on superquadric pairs (one per pair-step) */
+  int64_t dist_evals;     /* Disfunction / Disfun evaluations inside the narrowphase solvers */
+  int64_t pairs_inserted; /* interactions appended by the collider */
+  int64_t pairs_erased;   /* interactions erased by the collider */
+} sdm_counters;
+
+/* Version string of the library. */
+const char* sdm_version(void);
+
+/* Create a context for n_bodies bodies and n_materials materials on the current device.
+ * Replaces the construction of Scene + O.engines (py/wrapper/sudodemWrapper.cpp:929-991).
+ * Returns NULL on failure (message through sdm_last_error(NULL)). */
+sdm_ctx* sdm_create(const sdm_params* params, int32_t n_bodies, int32_t n_materials);
+void sdm_destroy(sdm_ctx* ctx);
+const char* sdm_last_error(const sdm_ctx* ctx);
+
+/* Material table (O.materials). */
+int sdm_set_materials(sdm_ctx* ctx, const sdm_material* mats, int32_t n);
+
+/* Geometric properties of a superellipsoid, as Superquadrics::Initial computes them
+ * (pkg/dem/Superquadrics.cpp:47-93): r_max, volume and principal inertia per unit density. */
+int sdm_superquadric_props(double rx, double ry, double rz, double eps1, double eps2,
+                           double* r_max, double* volume, double inertia[3]);
+
+/* Attribute access.  `name` is one of the attribute names listed by sdm_field_list(); the
+ * element type (SDM_F64 / SDM_I32 / SDM_I64) and the number of elements (count) are fixed by
+ * the attribute and the current number of bodies or interactions.  Vector attributes are
+ * stored row-major, one row per body or interaction ("state.pos" holds x,y,z of body 0,
+ * then of body 1, ...).  Setting "intr.id1" resizes the interaction container to count.
+ * Attribute reads synchronise with the device. */
+enum { SDM_F64 = 0, SDM_I32 = 1, SDM_I64 = 2 };
+int sdm_field_info(sdm_ctx* ctx, const char* name, int32_t* dtype, int64_t* count);
+int sdm_get(sdm_ctx* ctx, const char* name, void* dst, int64_t count);
+int sdm_set(sdm_ctx* ctx, const char* name, const void* src, int64_t count);
+/* Newline separated list of the attribute names. */
+const char* sdm_field_list(void);
+
+/* Run nsteps iterations of the engine list (Scene::moveToNextTimeStep, core/Scene.cpp:70-138). */
+int sdm_step(sdm_ctx* ctx, int32_t nsteps);
+/* Block until all queued device work of the context is done. */
+int sdm_sync(sdm_ctx* ctx);
+
+int sdm_get_counters(sdm_ctx* ctx, sdm_counters* out);
+int sdm_reset_counters(sdm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDM_H */

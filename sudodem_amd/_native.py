@@ -1,0 +1,168 @@
+"""ctypes binding of the C ABI in include/sdm.h.
+
+`SdmLibrary(path, prefix)` binds one shared library that exports the ABI under the given
+symbol prefix.  The product library is libsdm_hip.so (prefix ``sdm_``), built in-tree next to
+this file; `load_hip()` loads it and raises if it is missing, so nothing silently falls back to
+another implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(PKG_DIR, "libsdm_hip.so")
+
+SDM_LAW_LINEAR, SDM_LAW_HERTZ = 0, 1
+SDM_FP64, SDM_MIXED = 0, 1
+SDM_BODY_SUPERQUADRIC, SDM_BODY_WALL = 0, 1
+SDM_F64, SDM_I32, SDM_I64 = 0, 1, 2
+_NP_DTYPE = {SDM_F64: np.float64, SDM_I32: np.int32, SDM_I64: np.int64}
+
+
+class sdm_params(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
+        ("damping", ctypes.c_double),
+        ("exact_aspherical_rot", ctypes.c_int32),
+        ("verlet_dist", ctypes.c_double),
+        ("target_interv", ctypes.c_int32),
+        ("min_sweep_dist_factor", ctypes.c_double),
+        ("sort_axis", ctypes.c_int32),
+        ("law", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+    ]
+
+
+class sdm_material(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("density", "Kn", "Ks", "frictionAngle", "betan", "betas", "G", "nu")]
+
+
+class sdm_counters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in
+                ("steps", "collider_runs", "contact_evals", "dist_evals", "pairs_inserted", "pairs_erased")]
+
+
+class SdmError(RuntimeError):
+    pass
+
+
+class SdmLibrary:
+    """One implementation of the ABI (symbols `<prefix>create`, `<prefix>step`, ...)."""
+
+    def __init__(self, path: str, prefix: str):
+        if not os.path.exists(path):
+            raise SdmError(f"shared library {path} is missing; build it first (__graft_entry__.build())")
+        self.path = path
+        self.prefix = prefix
+        self.lib = ctypes.CDLL(path)
+        L, p = self.lib, prefix
+        c_ctx = ctypes.c_void_p
+
+        def fn(name, res, args):
+            f = getattr(L, p + name)
+            f.restype = res
+            f.argtypes = args
+            return f
+
+        self.version = fn("version", ctypes.c_char_p, [])
+        self.create = fn("create", c_ctx, [ctypes.POINTER(sdm_params), ctypes.c_int32, ctypes.c_int32])
+        self.destroy = fn("destroy", None, [c_ctx])
+        self.last_error = fn("last_error", ctypes.c_char_p, [c_ctx])
+        self.set_materials = fn("set_materials", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_material), ctypes.c_int32])
+        self.superquadric_props = fn("superquadric_props", ctypes.c_int,
+                                     [ctypes.c_double] * 5 + [ctypes.POINTER(ctypes.c_double)] * 3)
+        self.field_info = fn("field_info", ctypes.c_int,
+                             [c_ctx, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)])
+        self.get = fn("get", ctypes.c_int, [c_ctx, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64])
+        self.set = fn("set", ctypes.c_int, [c_ctx, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64])
+        self.field_list = fn("field_list", ctypes.c_char_p, [])
+        self.step = fn("step", ctypes.c_int, [c_ctx, ctypes.c_int32])
+        self.sync = fn("sync", ctypes.c_int, [c_ctx])
+        self.get_counters = fn("get_counters", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_counters)])
+        self.reset_counters = fn("reset_counters", ctypes.c_int, [c_ctx])
+
+    def fields(self):
+        return self.field_list().decode().split()
+
+    def check(self, rc, ctx=None):
+        if rc != 0:
+            raise SdmError((self.last_error(ctx) or b"").decode())
+
+    def props(self, rx, ry, rz, e1, e2):
+        r = ctypes.c_double()
+        v = ctypes.c_double()
+        inert = (ctypes.c_double * 3)()
+        rc = self.superquadric_props(rx, ry, rz, e1, e2, ctypes.byref(r), ctypes.byref(v), inert)
+        self.check(rc)
+        return r.value, v.value, np.array(list(inert))
+
+
+class Context:
+    """An open scene on one implementation of the ABI."""
+
+    def __init__(self, lib: SdmLibrary, params: sdm_params, n_bodies: int, materials):
+        self.lib = lib
+        self.ptr = lib.create(ctypes.byref(params), int(n_bodies), len(materials))
+        if not self.ptr:
+            raise SdmError((lib.last_error(None) or b"").decode())
+        arr = (sdm_material * max(1, len(materials)))()
+        for i, m in enumerate(materials):
+            arr[i] = m
+        lib.check(lib.set_materials(self.ptr, arr, len(materials)), self.ptr)
+
+    def close(self):
+        if self.ptr:
+            self.lib.destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self, name):
+        dt = ctypes.c_int32()
+        n = ctypes.c_int64()
+        self.lib.check(self.lib.field_info(self.ptr, name.encode(), ctypes.byref(dt), ctypes.byref(n)), self.ptr)
+        return dt.value, n.value
+
+    def get(self, name):
+        dt, n = self.info(name)
+        out = np.empty(n, dtype=_NP_DTYPE[dt])
+        self.lib.check(self.lib.get(self.ptr, name.encode(), out.ctypes.data_as(ctypes.c_void_p), n), self.ptr)
+        return out
+
+    def set(self, name, values):
+        dt, _ = self.info(name)
+        a = np.ascontiguousarray(np.asarray(values, dtype=_NP_DTYPE[dt]).ravel())
+        self.lib.check(self.lib.set(self.ptr, name.encode(), a.ctypes.data_as(ctypes.c_void_p), a.size), self.ptr)
+
+    def step(self, n=1):
+        self.lib.check(self.lib.step(self.ptr, int(n)), self.ptr)
+
+    def sync(self):
+        self.lib.check(self.lib.sync(self.ptr), self.ptr)
+
+    def counters(self):
+        c = sdm_counters()
+        self.lib.check(self.lib.get_counters(self.ptr, ctypes.byref(c)), self.ptr)
+        return {k: getattr(c, k) for k, _ in sdm_counters._fields_}
+
+    def reset_counters(self):
+        self.lib.check(self.lib.reset_counters(self.ptr), self.ptr)
+
+
+_HIP = None
+
+
+def load_hip() -> SdmLibrary:
+    """The MI355X library.  Raises SdmError when it has not been built."""
+    global _HIP
+    if _HIP is None:
+        _HIP = SdmLibrary(HIP_LIB, "sdm_")
+    return _HIP

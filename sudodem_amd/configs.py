@@ -1,0 +1,289 @@
+"""Seeded synthetic superellipsoid packings for the benchmark and the parity tests.
+
+Every generator draws from numpy.random.Generator(PCG64(seed)) with seed 20260000 + the
+configuration index, creates the six walls first (ids 0-5, which the reference's Ip2/Law2
+functors require, pkg/dem/Superquadrics.cpp:1100,1134,1394) and then the particles the way
+NewSuperquadrics_rot2 does (py/_superquadrics_utils.cpp:354-379): mass = density * volume,
+inertia = density * principal inertia, orientation = normalised random quaternion.
+
+Initial overlaps are controlled with the exact support function of each superellipsoid along
+the line of centres (Normal2Phi + getSurface): for convex bodies the overlap along any one
+direction bounds the penetration depth from above, so bounding the centre-line overlap bounds
+the initial contact overlaps.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _native as nat
+from .scene import (BODY_ARRAYS, Bo1_Superquadrics_Aabb, Bo1_Wall_Aabb, ForceResetter,
+                    Ig2_Superquadrics_Superquadrics_SuperquadricsGeom, Ig2_Superquadrics_Superquadrics_SuperquadricsGeom2,
+                    Ig2_Wall_Superquadrics_SuperquadricsGeom, Ig2_Wall_Superquadrics_SuperquadricsGeom2,
+                    InsertionSortCollider, InteractionLoop, Ip2_SuperquadricsMat2_SuperquadricsMat2_HertzMindlinPhys,
+                    Ip2_SuperquadricsMat_SuperquadricsMat_SuperquadricsPhys, NewtonIntegrator, SuperquadricsLaw,
+                    SuperquadricsLaw2, SuperquadricsMat, SuperquadricsMat2, engines_to_params, rot_matrices)
+
+SEED_BASE = 20260000
+DOF_ALL = 63
+
+
+def rng_for(config_index: int, extra: int = 0):
+    return np.random.Generator(np.random.PCG64(SEED_BASE + config_index + 1000 * extra))
+
+
+def random_quaternions(rng, n):
+    q = rng.normal(size=(n, 4))
+    return q / np.linalg.norm(q, axis=1, keepdims=True)
+
+
+def superquadric_props(rxyz, eps):
+    """Vectorised front end of sdm_superquadric_props (Superquadrics::Initial)."""
+    lib = nat.load_hip()
+    n = len(rxyz)
+    rmax = np.empty(n)
+    vol = np.empty(n)
+    inert = np.empty((n, 3))
+    for i in range(n):
+        rmax[i], vol[i], inert[i] = lib.props(*rxyz[i], *eps[i])
+    return rmax, vol, inert
+
+
+def support_h(rxyz, eps, R, d):
+    """Support function h(d) = max_{p in body} p.d for unit directions d (rows), bodies
+    given by semi-axes, exponents and rotation matrices to the global frame (rows of 9)."""
+    rxyz = np.asarray(rxyz, float)
+    eps = np.asarray(eps, float)
+    R = np.asarray(R, float).reshape(-1, 3, 3)
+    d = np.asarray(d, float)
+    n = np.einsum("kji,kj->ki", R, d)  # local direction = R^T d
+    rx, ry, rz = rxyz[:, 0], rxyz[:, 1], rxyz[:, 2]
+    e1, e2 = eps[:, 0], eps[:, 1]
+    a1, a2 = 1.0 / (2.0 - e1), 1.0 / (2.0 - e2)
+    ph0 = np.arctan2(np.sign(n[:, 1]) * np.abs(ry * n[:, 1]) ** a1, np.sign(n[:, 0]) * np.abs(rx * n[:, 0]) ** a1)
+    ph1 = np.arctan2(np.sign(n[:, 2]) * np.abs(rz * n[:, 2] * np.abs(np.cos(ph0)) ** (2.0 - e1)) ** a2,
+                     np.abs(rx * n[:, 0]) ** a2)
+    c0, s0, c1, s1 = np.cos(ph0), np.sin(ph0), np.cos(ph1), np.sin(ph1)
+    p = np.stack([np.sign(c0) * rx * np.abs(c0) ** e1 * np.abs(c1) ** e2,
+                  np.sign(s0) * ry * np.abs(s0) ** e1 * np.abs(c1) ** e2,
+                  np.sign(s1) * rz * np.abs(s1) ** e2], axis=1)
+    return np.einsum("ki,ki->k", p, n)
+
+
+def _empty_arrays(n):
+    A = {
+        "kind": np.zeros(n, np.int32), "pos": np.zeros((n, 3)), "ori": np.tile([1.0, 0, 0, 0], (n, 1)),
+        "vel": np.zeros((n, 3)), "angVel": np.zeros((n, 3)), "angMom": np.zeros((n, 3)), "mass": np.zeros(n),
+        "inertia": np.zeros((n, 3)), "rxyz": np.zeros((n, 3)), "eps": np.ones((n, 2)), "r_max": np.zeros(n),
+        "isSphere": np.zeros(n, np.int32), "blockedDOFs": np.zeros(n, np.int32), "isDamped": np.ones(n, np.int32),
+        "material": np.zeros(n, np.int32), "wallAxis": np.zeros(n, np.int32), "wallSense": np.zeros(n, np.int32),
+        "hasBound": np.zeros(n, np.int32),
+    }
+    assert set(A) == set(BODY_ARRAYS)
+    return A
+
+
+def _box_walls(A, lo, hi, wall_mat):
+    """utils.wall for the six faces, sense pointing inwards, ids 0..5."""
+    k = 0
+    for axis in range(3):
+        for pos, sense in ((lo[axis], 1), (hi[axis], -1)):
+            A["kind"][k] = nat.SDM_BODY_WALL
+            A["pos"][k, axis] = pos
+            A["wallAxis"][k] = axis
+            A["wallSense"][k] = sense
+            A["blockedDOFs"][k] = DOF_ALL
+            A["material"][k] = wall_mat
+            A["hasBound"][k] = 0
+            k += 1
+
+
+def _fill_particles(A, start, pos, rxyz, eps, quat, mat_index, density):
+    n = len(pos)
+    sl = slice(start, start + n)
+    rmax, vol, inert = superquadric_props(rxyz, eps)
+    A["kind"][sl] = nat.SDM_BODY_SUPERQUADRIC
+    A["pos"][sl] = pos
+    A["ori"][sl] = quat
+    A["rxyz"][sl] = rxyz
+    A["eps"][sl] = eps
+    A["r_max"][sl] = rmax
+    A["mass"][sl] = density * vol
+    A["inertia"][sl] = inert * density
+    A["material"][sl] = mat_index
+    A["hasBound"][sl] = 1
+    return vol
+
+
+class Config:
+    """A synthetic scene: body arrays, materials and the engine list."""
+
+    def __init__(self, name, arrays, materials, engines, dt, steps, box):
+        self.name, self.arrays, self.materials, self.engines = name, arrays, materials, engines
+        self.dt, self.steps, self.box = dt, steps, box
+
+    @property
+    def n(self):
+        return len(self.arrays["kind"])
+
+    def params(self, precision=nat.SDM_FP64):
+        return engines_to_params(self.engines, self.dt, precision)
+
+    def open(self, lib=None, precision=nat.SDM_FP64):
+        from .scene import open_arrays
+        return open_arrays(lib or nat.load_hip(), self.params(precision), self.arrays, self.materials)
+
+
+def linear_engines(verlet):
+    return [ForceResetter(), InsertionSortCollider([Bo1_Superquadrics_Aabb(), Bo1_Wall_Aabb()], verletDist=verlet),
+            InteractionLoop([Ig2_Wall_Superquadrics_SuperquadricsGeom(),
+                             Ig2_Superquadrics_Superquadrics_SuperquadricsGeom()],
+                            [Ip2_SuperquadricsMat_SuperquadricsMat_SuperquadricsPhys()], [SuperquadricsLaw()]),
+            NewtonIntegrator(damping=0.3, gravity=(0.0, 0.0, -9.81), isSuperquadrics=1)]
+
+
+def hertz_engines(verlet):
+    return [ForceResetter(), InsertionSortCollider([Bo1_Superquadrics_Aabb(), Bo1_Wall_Aabb()], verletDist=verlet),
+            InteractionLoop([Ig2_Wall_Superquadrics_SuperquadricsGeom2(),
+                             Ig2_Superquadrics_Superquadrics_SuperquadricsGeom2()],
+                            [Ip2_SuperquadricsMat2_SuperquadricsMat2_HertzMindlinPhys()], [SuperquadricsLaw2()]),
+            NewtonIntegrator(damping=0.3, gravity=(0.0, 0.0, -9.81), isSuperquadrics=1)]
+
+
+def _draw_shapes(rng, n):
+    rxyz = rng.uniform(0.8e-3, 1.2e-3, size=(n, 3))
+    eps = rng.uniform(0.6, 1.4, size=(n, 2))
+    return rxyz, eps
+
+
+def _neighbour_pairs(pos, cutoff):
+    """All pairs (i < j) of points closer than cutoff."""
+    from scipy.spatial import cKDTree
+    pr = cKDTree(pos).query_pairs(cutoff, output_type="ndarray")
+    if len(pr) == 0:
+        return np.zeros(0, int), np.zeros(0, int)
+    pr = pr[np.lexsort((pr[:, 1], pr[:, 0]))]
+    return pr[:, 0].astype(np.int64), pr[:, 1].astype(np.int64)
+
+
+def _centre_line_overlap(pos, rxyz, eps, R, i, j):
+    d = pos[j] - pos[i]
+    L = np.linalg.norm(d, axis=1)
+    u = d / L[:, None]
+    return support_h(rxyz[i], eps[i], R[i], u) + support_h(rxyz[j], eps[j], R[j], -u) - L
+
+
+def config0(n_particles=1000, seed_extra=0):
+    """1,000 superellipsoids in a 20x20x60 mm box, linear law, FP64 deposition.
+
+    Centres: the 7x7x21 lattice of spacing 2.86 mm (1,029 sites, 1,000 drawn), jittered by
+    up to 0.1 mm.  A particle whose support along the line of centres overlaps a lattice
+    neighbour (or a wall) gets a new orientation, up to 200 draws, then its semi-axes are
+    scaled down until it is clear."""
+    rng = rng_for(0, seed_extra)
+    box_lo = np.zeros(3)
+    box_hi = np.array([20e-3, 20e-3, 60e-3])
+    sp = 2.86e-3
+    grid = np.array([[i, j, k] for k in range(21) for j in range(7) for i in range(7)], float)
+    sites = (grid + 0.5) * sp
+    sites[:, 2] += 0.0
+    pick = np.sort(rng.choice(len(sites), size=n_particles, replace=False))
+    pos = sites[pick] + rng.uniform(-0.1e-3, 0.1e-3, size=(n_particles, 3))
+    rxyz, eps = _draw_shapes(rng, n_particles)
+    quat = random_quaternions(rng, n_particles)
+    _separate(rng, pos, rxyz, eps, quat, box_lo, box_hi, gap=1e-6)
+    p_mat = SuperquadricsMat(label="mat1", Kn=1e5, Ks=7e4, frictionAngle=math.atan(0.5), density=2650.0)
+    w_mat = SuperquadricsMat(label="wallmat", Kn=1e5, Ks=7e4, frictionAngle=0.0, density=2650.0)
+    A = _empty_arrays(6 + n_particles)
+    _box_walls(A, box_lo, box_hi, wall_mat=1)
+    _fill_particles(A, 6, pos, rxyz, eps, quat, 0, p_mat.density)
+    verlet = 0.1 * float(np.mean(A["r_max"][6:]))
+    return Config("C0: 1000 superellipsoids, linear law, 20x20x60 mm box", A, [p_mat, w_mat], linear_engines(verlet),
+                  dt=1e-6, steps=20000, box=(box_lo, box_hi))
+
+
+def _separate(rng, pos, rxyz, eps, quat, lo, hi, gap):
+    """Re-draw orientations (then shrink) until no centre-line overlap with neighbours or walls."""
+    n = len(pos)
+    cutoff = 2 * 1.2e-3 * math.sqrt(3) + 1e-6
+    ii, jj = _neighbour_pairs(pos, cutoff)
+    for sweep in range(400):
+        R = rot_matrices(quat)
+        bad = np.zeros(n, bool)
+        if len(ii):
+            ov = _centre_line_overlap(pos, rxyz, eps, R, ii, jj)
+            hit = ov > -gap
+            bad[jj[hit]] = True
+        for axis in range(3):
+            e = np.zeros((n, 3))
+            e[:, axis] = 1.0
+            hp = support_h(rxyz, eps, R, e)
+            hm = support_h(rxyz, eps, R, -e)
+            bad |= (pos[:, axis] + hp > hi[axis] - gap) | (pos[:, axis] - hm < lo[axis] + gap)
+        if not bad.any():
+            return
+        k = np.nonzero(bad)[0]
+        if sweep < 200:
+            quat[k] = random_quaternions(rng, len(k))
+        else:
+            rxyz[k] *= 0.97
+    raise RuntimeError("could not separate the initial packing")
+
+
+def config1(n_particles=200000, seed_extra=0, overlap=2e-6):
+    """200,000 superellipsoids in a 120x120x200 mm box, Hertz-Mindlin law.
+
+    Densification: centres on a face-centred cubic lattice of nearest-neighbour distance
+    2.0 mm filling the bottom of the box; each particle's semi-axes are scaled by the largest
+    factor <= 1 that keeps its centre-line overlap with every lattice neighbour and wall
+    below `overlap`.  The solids fraction of the resulting bed is reported by the generator."""
+    rng = rng_for(1, seed_extra)
+    box_lo = np.zeros(3)
+    box_hi = np.array([120e-3, 120e-3, 200e-3])
+    a = 2.0e-3
+    cc = a * math.sqrt(2.0)
+    nx = int((box_hi[0] - 1.6e-3) / cc)
+    ny = int((box_hi[1] - 1.6e-3) / cc)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    per_layer = nx * ny * 4
+    nz = int(math.ceil(n_particles / per_layer))
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), axis=-1).reshape(-1, 3)
+    sites = (g[:, None, :] + basis[None, :, :]).reshape(-1, 3) * cc
+    sites = sites[np.lexsort((sites[:, 0], sites[:, 1], sites[:, 2]))][:n_particles]
+    sites += 1.6e-3
+    pos = sites + rng.uniform(-0.02e-3, 0.02e-3, size=sites.shape)
+    rxyz, eps = _draw_shapes(rng, n_particles)
+    quat = random_quaternions(rng, n_particles)
+    R = rot_matrices(quat)
+    ii, jj = _neighbour_pairs(pos, a * 1.5)
+    scale = np.ones(n_particles)
+    # pairwise bound: lambda_i, lambda_j <= (L + overlap) / (h_i + h_j)
+    d = pos[jj] - pos[ii]
+    L = np.linalg.norm(d, axis=1)
+    u = d / L[:, None]
+    hs = support_h(rxyz[ii], eps[ii], R[ii], u) + support_h(rxyz[jj], eps[jj], R[jj], -u)
+    lim = np.minimum(1.0, (L + overlap) / hs)
+    np.minimum.at(scale, ii, lim)
+    np.minimum.at(scale, jj, lim)
+    for axis in range(3):
+        e = np.zeros((n_particles, 3))
+        e[:, axis] = 1.0
+        hp = support_h(rxyz, eps, R, e)
+        hm = support_h(rxyz, eps, R, -e)
+        scale = np.minimum(scale, np.minimum((box_hi[axis] - pos[:, axis]) / hp, (pos[:, axis] - box_lo[axis]) / hm))
+    rxyz = rxyz * scale[:, None]
+    p_mat = SuperquadricsMat2(label="sq", G=7e7 / (2 * (1 + 0.3)), nu=0.3, frictionAngle=math.atan(0.5),
+                              density=2650.0, Kn=1e5, Ks=7e4)
+    w_mat = SuperquadricsMat2(label="wall", G=7e7 / (2 * (1 + 0.3)), nu=0.3, frictionAngle=math.atan(0.5),
+                              density=2650.0, Kn=1e5, Ks=7e4)
+    A = _empty_arrays(6 + n_particles)
+    _box_walls(A, box_lo, box_hi, wall_mat=1)
+    vol = _fill_particles(A, 6, pos, rxyz, eps, quat, 0, p_mat.density)
+    verlet = 0.1 * float(np.mean(A["r_max"][6:]))
+    bed_h = pos[:, 2].max() + 1.2e-3
+    cfg = Config(f"C1: {n_particles} superellipsoids, Hertz-Mindlin law, 120x120x200 mm box", A, [p_mat, w_mat],
+                 hertz_engines(verlet), dt=1e-6, steps=5000, box=(box_lo, box_hi))
+    cfg.solids_fraction = float(vol.sum() / ((box_hi[0] - box_lo[0]) * (box_hi[1] - box_lo[1]) * bed_h))
+    cfg.mean_scale = float(scale.mean())
+    return cfg

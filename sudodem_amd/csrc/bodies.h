@@ -1,0 +1,234 @@
+// bodies.h -- per-body kernels: bound update (Bo1 functors + adaptive sweep) and the
+// Newton integrator with the per-body force reduction.
+#pragma once
+#include "pairs.h"
+
+// BoundDispatcher::processBody (pkg/common/Dispatching.cpp:30-62) with
+// Bo1_Superquadrics_Aabb::go (pkg/dem/Superquadrics.cpp:639-686: an orientation-free cube of
+// half-size r_max) and Bo1_Wall_Aabb::go (pkg/common/Wall.cpp:54-62).
+__global__ __launch_bounds__(256) void k_bounds(BodyArrays B, int64_t iter, double sweepDist, double minSweepDistFactor,
+                                                int targetInterv) {
+  const int64_t N = B.n;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const V3<double> pos = ld3(B.pos, i, N);
+  double sweep = B.sweep[i];
+  const bool had = B.hasBound[i] != 0;
+  if (had) {
+    if (targetInterv >= 0) {
+      const V3<double> r = ld3(B.refPos, i, N);
+      const V3<double> disp = pos - r;
+      // std::max(a,b) = (a<b)?b:a, which keeps a when b is NaN (first call: refPos is NaN)
+      const double ax = fabs(disp.x), ay = fabs(disp.y), az = fabs(disp.z);
+      const double myz = (ay < az) ? az : ay;
+      const double dist = (ax < myz) ? myz : ax;
+      if (dist != 0.0) {  // NaN counts as true
+        double newLength = dist * (double)targetInterv / (double)(iter - (int64_t)B.lastUpd[i]);
+        const double a = 0.9 * sweep;
+        newLength = (a < newLength) ? newLength : a;
+        const double m = (sweepDist < newLength) ? sweepDist : newLength;
+        const double lo = minSweepDistFactor * sweepDist;
+        sweep = (lo < m) ? m : lo;
+      } else {
+        sweep = 0;
+      }
+    } else {
+      sweep = sweepDist;
+    }
+  } else {
+    sweep = 0;  // the functor allocates the bound: Bound::sweepLength defaults to 0
+  }
+  V3<double> mn, mx;
+  if (B.kind[i] == SDM_KIND_SQ) {
+    const double r = B.rmax[i];
+    mn = pos + mk3(-r, -r, -r);
+    mx = pos + mk3(r, r, r);
+  } else {
+    const double inf = __builtin_huge_val();
+    mn = mk3(-inf, -inf, -inf);
+    mx = mk3(inf, inf, inf);
+    const int ax = B.wallAxis[i];
+    if (ax == 0) { mn.x = pos.x; mx.x = pos.x; }
+    else if (ax == 1) { mn.y = pos.y; mx.y = pos.y; }
+    else { mn.z = pos.z; mx.z = pos.z; }
+  }
+  if (sweep > 0) {
+    mn = mn - mk3(sweep, sweep, sweep);
+    mx = mx + mk3(sweep, sweep, sweep);
+  }
+  st3(B.bmin, i, N, mn);
+  st3(B.bmax, i, N, mx);
+  st3(B.refPos, i, N, pos);
+  B.sweep[i] = sweep;
+  B.lastUpd[i] = (int32_t)iter;
+  B.hasBound[i] = 1;
+}
+
+struct NewtonArgs {
+  BodyArrays B;
+  PairArrays P;
+  const int32_t* csrOff;
+  const int32_t* csrPair;
+  double dt, damping;
+  double g[3];
+  int exactAsph;
+  int* maxDisp;  // 0: no bounded body, 1: all inside their sweep (0.5), 2: some outside (2)
+};
+
+__device__ __forceinline__ Q4 dotq(V3<double> w, Q4 Q) {  // NewtonIntegrator::DotQ (:470-477)
+  Q4 o;
+  o.w = (-Q.x * w.x - Q.y * w.y - Q.z * w.z) / 2;
+  o.x = (Q.w * w.x - Q.z * w.y + Q.y * w.z) / 2;
+  o.y = (Q.z * w.x + Q.w * w.y - Q.x * w.z) / 2;
+  o.z = (-Q.y * w.x + Q.x * w.y + Q.w * w.z) / 2;
+  return o;
+}
+
+__device__ __forceinline__ double cundall(double damping, double a, double b) {
+  return 1 - damping * sgn(a * b);
+}
+
+// NewtonIntegrator::action (pkg/dem/NewtonIntegrator.cpp:99-304) for isSuperquadrics=1, one
+// body per thread.  The force and torque are first summed over the body's interactions in
+// interaction order, which is the order of the reference's serial addForce/addTorque calls.
+__global__ __launch_bounds__(256) void k_newton(NewtonArgs A) {
+  const BodyArrays& B = A.B;
+  const int64_t N = B.n;
+  const int64_t S = A.P.cap;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int code = 0;
+  if (i < N) {
+    V3<double> f = mk3(0.0, 0.0, 0.0), m = f;
+    const int e0 = A.csrOff[i], e1 = A.csrOff[i + 1];
+    for (int e = e0; e < e1; e++) {
+      const int p = A.csrPair[e];
+      if (!A.P.active[p]) continue;
+      const V3<double> F = ld3(A.P.cF, p, S);
+      if (A.P.id1[p] == (int)i) {
+        f = f + F;
+        m = m + ld3(A.P.cT1, p, S);
+      } else {
+        f = f + (-F);
+        m = m + ld3(A.P.cT2, p, S);
+      }
+    }
+    st3(B.force, i, N, f);
+    st3(B.torque, i, N, m);
+    const double dt = A.dt, damping = A.damping;
+    const unsigned blocked = (unsigned)B.blocked[i];
+    const bool isSQ = B.kind[i] == SDM_KIND_SQ;
+    const bool isSphere = B.isSphere[i] != 0;
+    const bool useAsph = A.exactAsph && isSQ && blocked != 63u;
+    const bool damped = B.damped[i] != 0;
+    V3<double> vel = ld3(B.vel, i, N);
+    V3<double> angVel = ld3(B.angVel, i, N);
+    const V3<double> inertia = ld3(B.inertia, i, N);
+    if (blocked != 63u) {
+      const double mass = B.mass[i];
+      V3<double> a;
+      if (blocked == 0) {
+        a = mk3(f.x / mass + A.g[0], f.y / mass + A.g[1], f.z / mass + A.g[2]);
+      } else {
+        a.x = (blocked & 1u) ? 0.0 : 0.0 + (f.x / mass + A.g[0]);
+        a.y = (blocked & 2u) ? 0.0 : 0.0 + (f.y / mass + A.g[1]);
+        a.z = (blocked & 4u) ? 0.0 : 0.0 + (f.z / mass + A.g[2]);
+      }
+      if (damped) {
+        a.x *= cundall(damping, a.x, vel.x + 0.5 * dt * a.x);
+        a.y *= cundall(damping, a.y, vel.y + 0.5 * dt * a.y);
+        a.z *= cundall(damping, a.z, vel.z + 0.5 * dt * a.z);
+      }
+      vel = vel + dt * a;
+      if (!useAsph || (damped && isSphere)) {
+        V3<double> aa;
+        if (blocked == 0) {
+          aa = mk3(m.x / inertia.x, m.y / inertia.y, m.z / inertia.z);
+        } else {
+          aa.x = (blocked & 8u) ? 0.0 : 0.0 + m.x / inertia.x;
+          aa.y = (blocked & 16u) ? 0.0 : 0.0 + m.y / inertia.y;
+          aa.z = (blocked & 32u) ? 0.0 : 0.0 + m.z / inertia.z;
+        }
+        if (damped) {
+          aa.x *= cundall(damping, aa.x, angVel.x + 0.5 * dt * aa.x);
+          aa.y *= cundall(damping, aa.y, angVel.y + 0.5 * dt * aa.y);
+          aa.z *= cundall(damping, aa.z, angVel.z + 0.5 * dt * aa.z);
+        }
+        angVel = angVel + dt * aa;
+      }
+      if (useAsph) {
+        if (blocked & 8u) m.x = 0;
+        if (blocked & 16u) m.y = 0;
+        if (blocked & 32u) m.z = 0;
+        if (damped && !isSphere) {
+          m.x *= cundall(damping, m.x, angVel.x);
+          m.y *= cundall(damping, m.y, angVel.y);
+          m.z *= cundall(damping, m.z, angVel.z);
+        }
+      }
+    }
+    Q4 ori;
+    ori.w = B.ori[i]; ori.x = B.ori[N + i]; ori.y = B.ori[2 * N + i]; ori.z = B.ori[3 * N + i];
+    if (useAsph && !isSphere) {
+      // leapfrogSuperquadricsRotate (:391-429) -> leapfrogAsphericalRotate (:306-335), with the
+      // rotation matrix cached from the previous orientation.
+      const M3<double> Am = ld9(B.rotL, i, N);
+      V3<double> angMom = ld3(B.angMom, i, N);
+      const V3<double> l_n = angMom + (dt / 2.) * m;
+      const V3<double> l_b_n = mul(Am, l_n);
+      const V3<double> w_b_n = mk3(l_b_n.x / inertia.x, l_b_n.y / inertia.y, l_b_n.z / inertia.z);
+      const Q4 dQ_n = dotq(w_b_n, ori);
+      const double h = dt / 2.;
+      Q4 Q_half;
+      Q_half.w = ori.w + dQ_n.w * h; Q_half.x = ori.x + dQ_n.x * h;
+      Q_half.y = ori.y + dQ_n.y * h; Q_half.z = ori.z + dQ_n.z * h;
+      angMom = angMom + dt * m;
+      const V3<double> l_b_half = mul(Am, angMom);
+      const V3<double> w_b_half = mk3(l_b_half.x / inertia.x, l_b_half.y / inertia.y, l_b_half.z / inertia.z);
+      const Q4 dQ_half = dotq(w_b_half, Q_half);
+      ori.w = ori.w + dQ_half.w * dt; ori.x = ori.x + dQ_half.x * dt;
+      ori.y = ori.y + dQ_half.y * dt; ori.z = ori.z + dQ_half.z * dt;
+      angVel = quat_apply(ori, w_b_half);
+      ori = quat_normalized(ori);
+      const M3<double> RL = quat_rot(quat_conj(ori)), RG = quat_rot(ori);
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        B.rotL[k * N + i] = RL.m[k];
+        B.rotG[k * N + i] = RG.m[k];
+      }
+      st3(B.angMom, i, N, angMom);
+    } else {
+      // leapfrogSphericalRotate (:319-334)
+      const double angle2 = sqn(angVel);
+      if (angle2 != 0) {
+        const double angle = sqrt(angle2);
+        const V3<double> ax = mk3(angVel.x / angle, angVel.y / angle, angVel.z / angle);
+        const double ha = 0.5 * (angle * dt);
+        const double sh = sin(ha);
+        Q4 q;
+        q.w = cos(ha); q.x = sh * ax.x; q.y = sh * ax.y; q.z = sh * ax.z;
+        ori = quat_mul(q, ori);
+      }
+      ori = quat_normalized(ori);
+    }
+    B.ori[i] = ori.w; B.ori[N + i] = ori.x; B.ori[2 * N + i] = ori.y; B.ori[3 * N + i] = ori.z;
+    st3(B.vel, i, N, vel);
+    st3(B.angVel, i, N, angVel);
+    V3<double> pos = ld3(B.pos, i, N);
+    pos = pos + vel * dt;  // leapfrogTranslate (:306-317)
+    st3(B.pos, i, N, pos);
+    // saveMaximaDisplacement (:72-85)
+    if (B.hasBound[i]) {
+      const V3<double> d = pos - ld3(B.refPos, i, N);
+      const double ax = fabs(d.x), ay = fabs(d.y), az = fabs(d.z);
+      const double myz = (ay < az) ? az : ay;
+      const double md = (ax < myz) ? myz : ax;
+      code = (!(md != 0.0) || md < B.sweep[i]) ? 1 : 2;
+    }
+  }
+  // wave-level max, then one atomic per wavefront
+  for (int off = 32; off > 0; off >>= 1) {
+    int o = __shfl_down(code, off);
+    code = o > code ? o : code;
+  }
+  if ((threadIdx.x & 63) == 0 && code) atomicMax(A.maxDisp, code);
+}

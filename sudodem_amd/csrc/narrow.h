@@ -1,0 +1,433 @@
+// narrow.h -- superellipsoid support map and the two common-normal solvers.
+//
+// Device restatement of the SudoDEM3D narrowphase:
+//   support map     Superquadrics::Normal2Phi / getSurface / isInside (pkg/dem/Superquadrics.cpp:103-120, 269-307)
+//   curvature       Superquadrics::Curvatures (Superquadrics.cpp:131-255)
+//   LM solver       Disfunction, JacfunctionApp, lmder2 (pkg/dem/Superquadrics_Ig2.cpp:621-687, 1032-1354)
+//   NM solver       CRotationMat, Disfun, amotry, neldermead_simplex (Superquadrics_Ig2.cpp:295-319, 1624-1875)
+//   Hertz geometry  SuperquadricsGeom2::HertzMindlin (Superquadrics.cpp:974-1026)
+// The solvers are templated on the evaluation type T: T=double is the reference
+// arithmetic; T=float evaluates the support map on frames centred at body 1.
+#pragma once
+#include "dev_math.h"
+
+template <typename T>
+struct Shape {
+  T rx, ry, rz, eps1, eps2;
+};
+
+// Normal2Phi, active branches (Superquadrics.cpp:284, 288).
+template <typename T>
+SDM_HD void normal2phi(const Shape<T>& s, V3<T> n, T& ph0, T& ph1) {
+  const T e1 = T(1) / (T(2) - s.eps1);
+  const T e2 = T(1) / (T(2) - s.eps2);
+  ph0 = xatan2(sgn(n.y) * xpow((xfabs(s.ry * n.y)), e1), sgn(n.x) * xpow((xfabs(s.rx * n.x)), e1));
+  ph1 = xatan2(sgn(n.z) * xpow(xfabs(s.rz * n.z * xpow(xfabs(xcos(ph0)), T(2) - s.eps1)), e2),
+               xpow(xfabs(s.rx * n.x), e2));
+}
+
+// getSurface (Superquadrics.cpp:110-120): local coordinates of the point with parameters phi.
+template <typename T>
+SDM_HD V3<T> get_surface(const Shape<T>& s, T ph0, T ph1) {
+  const T c0 = xcos(ph0), s0 = xsin(ph0), c1 = xcos(ph1), s1 = xsin(ph1);
+  const T pc1 = xpow(xfabs(c1), s.eps2);
+  return mk3(sgn(c0) * s.rx * xpow(xfabs(c0), s.eps1) * pc1, sgn(s0) * s.ry * xpow(xfabs(s0), s.eps1) * pc1,
+             sgn(s1) * s.rz * xpow(xfabs(s1), s.eps2));
+}
+
+// Support point of a body along the global direction dir (global frame, relative to its centre).
+template <typename T>
+SDM_HD V3<T> support(const Shape<T>& s, const M3<T>& L, const M3<T>& G, V3<T> dir) {
+  T ph0, ph1;
+  normal2phi(s, mul(L, dir), ph0, ph1);
+  return mul(G, get_surface(s, ph0, ph1));
+}
+
+// isInside (Superquadrics.cpp:103-108); pg relative to the centre, global frame.
+SDM_HD bool is_inside(const Shape<double>& s, const M3<double>& L, V3<double> pg) {
+  V3<double> p = mul(L, pg);
+  return 1 > pow(pow(fabs(p.x / s.rx), 2.0 / s.eps1) + pow(fabs(p.y / s.ry), 2.0 / s.eps1), s.eps1 / s.eps2) +
+                 pow(fabs(p.z / s.rz), 2.0 / s.eps2);
+}
+
+// The pair geometry a solver needs, in evaluation type T.  pos1/pos2 are the centres in the
+// frame the solver works in (global for T=double, body-1-centred for the mixed form).
+template <typename T>
+struct PairGeo {
+  Shape<T> s1, s2;
+  M3<T> L1, G1, L2, G2;
+  V3<T> pos1, pos2;
+};
+
+// Disfunction (Superquadrics_Ig2.cpp:621-687).  stop_flag is sticky: set, never cleared.
+template <typename T>
+SDM_HD void lm_eval(const PairGeo<T>& g, T a0, T a1, V3<T>& p1, V3<T>& p2, V3<T>& center, bool& stop_flag,
+                    int& nev) {
+  nev++;
+  const V3<T> contact = mk3(xcos(a0) * xcos(a1), xsin(a0) * xcos(a1), xsin(a1));
+  p1 = support(g.s1, g.L1, g.G1, contact) + g.pos1;
+  {
+    T ph0, ph1;
+    V3<T> c2 = mul(g.L2, contact);
+    normal2phi(g.s2, mk3(T(-1) * c2.x, T(-1) * c2.y, T(-1) * c2.z), ph0, ph1);
+    p2 = mul(g.G2, get_surface(g.s2, ph0, ph1)) + g.pos2;
+  }
+  V3<T> d = p2 - p1;
+  center = mk3(T(0.5) * (p1.x + p2.x), T(0.5) * (p1.y + p2.y), T(0.5) * (p1.z + p2.z));
+  if (dot(contact, d) > T(0)) stop_flag = true;
+}
+
+template <typename T>
+struct LMResult {
+  T a0, a1;           // final parameters
+  V3<T> p1, p2, ctr;  // points of the last evaluation
+  V3<T> contact;      // direction set at the top of the last iteration
+  bool touching;
+  int stop;
+};
+
+// lmder2 (Superquadrics_Ig2.cpp:1056-1354) with opts = {1, 1e-15, 1e-3, 1e-20} and the
+// forward-difference Jacobian of JacfunctionApp (:1032-1054) whose step is the last increment.
+template <typename T>
+SDM_HD void lmder2(const PairGeo<T>& g, T& pa0, T& pa1, int itmax, LMResult<T>& R, int& nev) {
+  const T tau = T(1.0), eps1 = T(1e-15), eps2 = T(1e-3), eps3 = T(1e-20);
+  int k;
+  R.touching = true;
+  V3<T> p_f = mk3(T(0), T(0), T(0)), pDp_f = mk3(T(0), T(0), T(0));
+  T jTf0, jTf1;
+  T J00, J01, J10, J11, J20, J21;
+  T A00, A01, A11, dA0 = 0, dA1 = 0;
+  T Dp0 = T(1e-7), Dp1 = T(1e-7);
+  bool stop_flag = false;
+  T mu = T(0), tmp;
+  T p_L2, Dp_L2, dF, dL;
+  T p_fL2, pDp_fL2;
+  int nu = 2, nu2, stop = 0;
+  V3<T> p1, p2, ctr;
+  lm_eval(g, pa0, pa1, p1, p2, ctr, stop_flag, nev);
+  p_f = p2 - p1;
+  p_fL2 = norm(p_f);
+  if (!isfinite(p_fL2)) stop = 7;
+  for (k = 0; k < itmax && !stop; ++k) {
+    if (p_fL2 <= eps3) { stop = 6; break; }
+    if (stop_flag) { stop = 9; R.touching = false; break; }
+    {
+      R.contact = mk3(xcos(pa0) * xcos(pa1), xsin(pa0) * xcos(pa1), xsin(pa1));
+      T angle = dot(p_f, R.contact) / (norm(p_f) * norm(R.contact));
+      if (xfabs(angle) > T(0.95)) { stop = 8; break; }
+    }
+    {  // JacfunctionApp
+      V3<T> f0, f1, f2;
+      lm_eval(g, pa0, pa1, p1, p2, ctr, stop_flag, nev);
+      f0 = p2 - p1;
+      lm_eval(g, pa0 + Dp0, pa1, p1, p2, ctr, stop_flag, nev);
+      f1 = p2 - p1;
+      lm_eval(g, pa0, pa1 + Dp1, p1, p2, ctr, stop_flag, nev);
+      f2 = p2 - p1;
+      J00 = (f1.x - f0.x) / Dp0; J10 = (f1.y - f0.y) / Dp0; J20 = (f1.z - f0.z) / Dp0;
+      J01 = (f2.x - f0.x) / Dp1; J11 = (f2.y - f0.y) / Dp1; J21 = (f2.z - f0.z) / Dp1;
+    }
+    A00 = J00 * J00 + (J10 * J10 + J20 * J20);
+    A01 = J00 * J01 + (J10 * J11 + J20 * J21);
+    A11 = J01 * J01 + (J11 * J11 + J21 * J21);
+    const V3<T> mf = mk3(T(-1.0) * p_f.x, T(-1.0) * p_f.y, T(-1.0) * p_f.z);
+    jTf0 = J00 * mf.x + (J10 * mf.y + J20 * mf.z);
+    jTf1 = J01 * mf.x + (J11 * mf.y + J21 * mf.z);
+    T jTf_inf = (xfabs(jTf0) > xfabs(jTf1) ? xfabs(jTf0) : xfabs(jTf1));
+    p_L2 = dsqrt(pa0 * pa0 + pa1 * pa1);
+    if (jTf_inf <= eps1) { stop = 1; break; }
+    if (k == 0) {
+      tmp = (A00 > A11 ? A00 : A11);
+      mu = tau * tmp;
+    }
+    dA0 = A00;
+    dA1 = A11;
+    while (1) {
+      A00 += mu;
+      A11 += mu;
+      T n_a = A00 * A11 - A01 * A01;
+      Dp0 = (jTf0 * A11 - jTf1 * A01) / n_a;
+      Dp1 = (jTf1 * A00 - jTf0 * A01) / n_a;
+      Dp_L2 = dsqrt(Dp0 * Dp0 + Dp1 * Dp1);
+      if (Dp_L2 < T(1e-5)) { stop = 2; break; }
+      const T q0 = pa0 + Dp0, q1 = pa1 + Dp1;
+      if (Dp_L2 >= (p_L2 + eps2) / (T(1E-12) * T(1E-12))) { stop = 4; break; }
+      lm_eval(g, q0, q1, p1, p2, ctr, stop_flag, nev);
+      pDp_f = p2 - p1;
+      pDp_fL2 = norm(pDp_f);
+      if (!isfinite(pDp_fL2)) { stop = 7; break; }
+      dL = Dp0 * (mu * Dp0 + jTf0) + Dp1 * (mu * Dp1 + jTf1);
+      dF = p_fL2 - pDp_fL2;
+      tmp = dF / dL;
+      if (tmp > T(0.0)) {
+        tmp = T(1.0) - xpow(T(2) * tmp - T(1.), T(3.));
+        mu = mu * ((tmp >= T(0.3333333333333334)) ? tmp : T(0.3333333333333334));
+        nu = 2;
+        pa0 = q0;
+        pa1 = q1;
+        p_f = pDp_f;
+        p_fL2 = pDp_fL2;
+        break;
+      }
+      mu *= T(nu);
+      nu2 = int(unsigned(nu) << 1);
+      if (nu2 <= nu) { stop = 5; break; }
+      nu = nu2;
+      A00 = dA0;
+      A11 = dA1;
+    }
+  }
+  if (k >= itmax) stop = 3;
+  R.stop = stop;
+  R.a0 = pa0;
+  R.a1 = pa1;
+  R.p1 = p1;
+  R.p2 = p2;
+  R.ctr = ctr;
+}
+
+// CRotationMat (Superquadrics_Ig2.cpp:295-319).
+SDM_HD M3<double> crotation_mat(V3<double> position1, V3<double> position2) {
+  V3<double> e1 = position2 - position1;
+  V3<double> x1 = mk3(1.0, 0.0, 0.0);
+  double cos_phi = dot(x1, e1) / (norm(x1) * norm(e1));
+  double cos_halfphi = sqrt(0.5 * (1 + cos_phi));
+  double sin_halfphi = sqrt(0.5 * (1 - cos_phi));
+  Q4 q;
+  if (fabs(cos_phi) > 1.0 - 1e-10) {
+    q.w = cos_halfphi; q.x = 0; q.y = 0; q.z = sin_halfphi;
+  } else {
+    V3<double> n1 = normalized(cross(x1, e1));
+    n1 = n1 * sin_halfphi;
+    q.w = cos_halfphi; q.x = n1.x; q.y = n1.y; q.z = n1.z;
+  }
+  return quat_rot(q);
+}
+
+template <typename T>
+struct NMGeo {
+  PairGeo<T> g;
+  M3<T> gC;  // CRotationMat frame
+};
+
+// Disfun (Superquadrics_Ig2.cpp:1624-1678): distance |p2-p1| and the contact test.
+template <typename T>
+SDM_HD T nm_eval(const NMGeo<T>& n, T a0, T a1, V3<T>& p1, V3<T>& p2, bool& touching, int& nev) {
+  nev++;
+  const V3<T> contact = mul(n.gC, mk3(xcos(a0) * xcos(a1), xsin(a0) * xcos(a1), xsin(a1)));
+  p1 = support(n.g.s1, n.g.L1, n.g.G1, contact) + n.g.pos1;
+  {
+    T ph0, ph1;
+    V3<T> c2 = mul(n.g.L2, contact);
+    normal2phi(n.g.s2, mk3(T(-1) * c2.x, T(-1) * c2.y, T(-1) * c2.z), ph0, ph1);
+    p2 = mul(n.g.G2, get_surface(n.g.s2, ph0, ph1)) + n.g.pos2;
+  }
+  V3<T> d = p2 - p1;
+  if (dot(contact, d) > T(0)) touching = false;
+  return norm(d);
+}
+
+template <typename T>
+struct NMResult {
+  T a0, a1;   // best vertex
+  T fmin;     // its distance (info[1])
+  V3<T> p1, p2;  // points of the last evaluation
+  int nfunc;
+  bool touching;
+  // the final simplex (for the double-precision refinement of the mixed form)
+  T v[3][2];
+  T y[3];
+};
+
+// amotry (Superquadrics_Ig2.cpp:1681-1699).
+template <typename T>
+SDM_HD T nm_amotry(const NMGeo<T>& n, T (&p)[3][2], T (&y)[3], T (&psum)[2], int ihi, T fac, V3<T>& pt1,
+                   V3<T>& pt2, bool& touching, int& nev) {
+  T fac1 = (T(1.0) - fac) / T(2.0);
+  T fac2 = fac1 - fac;
+  T t0 = psum[0] * fac1 - p[ihi][0] * fac2;
+  T t1 = psum[1] * fac1 - p[ihi][1] * fac2;
+  T ytry = nm_eval(n, t0, t1, pt1, pt2, touching, nev);
+  if (ytry < y[ihi]) {
+    y[ihi] = ytry;
+    psum[0] += t0 - p[ihi][0];
+    psum[1] += t1 - p[ihi][1];
+    p[ihi][0] = t0;
+    p[ihi][1] = t1;
+  }
+  return ytry;
+}
+
+// neldermead_simplex (Superquadrics_Ig2.cpp:1701-1875), ftol = 1e-3, NMAX = 5000.  The
+// reference leaves the evaluation counter uninitialised (:1713); it starts at 0 here.
+template <typename T>
+SDM_HD void neldermead(const NMGeo<T>& n, T a0, T a1, T del, NMResult<T>& R, int& nev) {
+  const T ftol = T(1e-3);
+  const int NMAX = 5000;
+  const T TINY = T(1.0e-10);
+  int nfunc = 0;
+  int ihi, ilo, inhi;
+  T psum[2], y[3], p[3][2];
+  V3<T> pt1 = mk3(T(0), T(0), T(0)), pt2 = pt1;
+  bool touching = true;
+  for (int i = 0; i < 3; i++) { p[i][0] = a0; p[i][1] = a1; }
+  p[1][0] += del;
+  p[2][1] += del;
+  R.fmin = T(0);
+  bool done = false;
+  for (int i = 0; i < 3 && !done; i++) {
+    y[i] = nm_eval(n, p[i][0], p[i][1], pt1, pt2, touching, nev);
+    nfunc++;
+    if (!touching) {
+      R.fmin = y[0];
+      done = true;
+    }
+  }
+  if (!done) {
+    psum[0] = p[0][0] + p[1][0] + p[2][0];
+    psum[1] = p[0][1] + p[1][1] + p[2][1];
+    for (;;) {
+      ilo = 0;
+      ihi = y[0] > y[1] ? (inhi = 1, 0) : (inhi = 0, 1);
+      for (int i = 0; i < 3; i++) {
+        if (y[i] <= y[ilo]) ilo = i;
+        if (y[i] > y[ihi]) {
+          inhi = ihi;
+          ihi = i;
+        } else if (y[i] > y[inhi] && i != ihi)
+          inhi = i;
+      }
+      T rtol = T(2.0) * xfabs(y[ihi] - y[ilo]) / (xfabs(y[ihi]) + xfabs(y[ilo]) + TINY);
+      if (rtol < ftol) {
+        T t = y[0]; y[0] = y[ilo]; y[ilo] = t;
+        T t0 = p[ilo][0], t1 = p[ilo][1];
+        p[ilo][0] = p[0][0]; p[ilo][1] = p[0][1];
+        p[0][0] = t0; p[0][1] = t1;
+        R.fmin = y[0];
+        break;
+      }
+      if (nfunc >= NMAX) { R.fmin = y[0]; break; }
+      T ytry = nm_amotry(n, p, y, psum, ihi, T(-1.0), pt1, pt2, touching, nev);
+      nfunc++;
+      if (!touching) { R.fmin = y[0]; break; }
+      if (ytry <= y[ilo]) {
+        ytry = nm_amotry(n, p, y, psum, ihi, T(2.0), pt1, pt2, touching, nev);
+        nfunc++;
+        if (!touching) { R.fmin = y[0]; break; }
+      } else if (ytry >= y[inhi]) {
+        T ysave = y[ihi];
+        ytry = nm_amotry(n, p, y, psum, ihi, T(0.5), pt1, pt2, touching, nev);
+        nfunc++;
+        if (!touching) { R.fmin = y[0]; break; }
+        if (ytry >= ysave) {
+          bool out = false;
+          for (int i = 0; i < 3; i++) {
+            if (i != ilo) {
+              psum[0] = T(0.5) * (p[i][0] + p[ilo][0]);
+              psum[1] = T(0.5) * (p[i][1] + p[ilo][1]);
+              p[i][0] = psum[0];
+              p[i][1] = psum[1];
+              y[i] = nm_eval(n, psum[0], psum[1], pt1, pt2, touching, nev);
+              nfunc++;
+              if (!touching) {
+                R.fmin = y[0];
+                out = true;
+                break;
+              }
+            }
+          }
+          if (out) break;
+          psum[0] = p[0][0] + p[1][0] + p[2][0];
+          psum[1] = p[0][1] + p[1][1] + p[2][1];
+        }
+      }
+    }
+  }
+  R.a0 = p[0][0];
+  R.a1 = p[0][1];
+  R.p1 = pt1;
+  R.p2 = pt2;
+  R.nfunc = nfunc;
+  R.touching = touching;
+  for (int i = 0; i < 3; i++) {
+    R.v[i][0] = p[i][0];
+    R.v[i][1] = p[i][1];
+    R.y[i] = y[i];
+  }
+}
+
+// Curvatures (Superquadrics.cpp:131-255).  Keeps the reference's expressions: K ignores the
+// M term, and R11/R22 use the coefficient choices of :172-178.
+SDM_HD void sq_curvatures(const Shape<double>& s, const M3<double>& Rg, double ph0, double ph1, double& k1o,
+                          double& k2o, V3<double>& dir_max, V3<double>& dir_min, bool& max_flag) {
+  max_flag = true;
+  if (fabs(ph0) < 1E-20) ph0 = 1E-20;
+  if (fabs(ph1) < 1E-20) ph1 = 1E-20;
+  const double rx = s.rx, ry = s.ry, rz = s.rz, eps1 = s.eps1, eps2 = s.eps2;
+  double s1 = sin(ph0), s2 = sin(ph1), c1 = cos(ph0), c2 = cos(ph1);
+  double s_s1 = sgn(s1), s_s2 = sgn(s2), s_c1 = sgn(c1), s_c2 = sgn(c2);
+  double abs_s1 = fabs(s1), abs_s2 = fabs(s2), abs_c1 = fabs(c1), abs_c2 = fabs(c2);
+  double s10 = s_s1 * pow(abs_s1, eps1);
+  double s20 = s_s2 * pow(abs_s2, eps2);
+  double c10 = s_c1 * pow(abs_c1, eps1);
+  double c20 = s_c2 * pow(abs_c2, eps2);
+  double s12 = s_s1 * pow(abs_s1, eps1 - 2.0);
+  double s22 = s_s2 * pow(abs_s2, eps2 - 2.0);
+  double c12 = s_c1 * pow(abs_c1, eps1 - 2.0);
+  double c22 = s_c2 * pow(abs_c2, eps2 - 2.0);
+  V3<double> R1 = mk3(-rx * eps1 * c12 * c1 * c20 * s1, ry * eps1 * c1 * c20 * s12 * s1, 0.0);
+  V3<double> R2 = mk3(-rx * eps2 * c10 * c22 * c2 * s2, -ry * eps2 * c22 * c2 * s10 * s2, rz * eps2 * c2 * s22 * s2);
+  V3<double> R11 = mk3(rx * eps1 * c12 * s20 * s1 * s1 * (eps1 - 1.0) - rx * eps1 * c10 * c20,
+                       ry * eps1 * c1 * c1 * c20 * s12 * (eps1 - 1.0) - ry * eps1 * c20 * s10, 0.0);
+  V3<double> R22 = mk3(rx * eps2 * c10 * c22 * s2 * s2 * (eps2 - 1.0) - rx * eps2 * c10 * c20,
+                       ry * eps2 * s10 * s1 * s1 * (eps2 - 1.0) - ry * eps2 * c20 * s10,
+                       rz * eps2 * c2 * c2 * s22 * (eps2 - 1.0) - rz * eps2 * s20);
+  V3<double> RR = cross(R1, R2);
+  double nr = norm(RR);
+  V3<double> n = mk3(RR.x / nr, RR.y / nr, RR.z / nr);
+  double L = dot(R11, n);
+  double N = dot(R22, n);
+  double E = dot(R1, R1);
+  double F = dot(R1, R2);
+  double G = dot(R2, R2);
+  double K = L * N / (E * G - F * F);
+  double H = 0.5 * (E * N + G * L) / (E * G - F * F);
+  double tmp0 = sqrt(fabs(H * H - K));
+  double k1 = H + tmp0;
+  double k2 = H - tmp0;
+  const double a1 = N - k1 * G, a2 = N - k2 * G;
+  V3<double> dir1 = mk3(R1.x * a1 + R2.x * k1 * F, R1.y * a1 + R2.y * k1 * F, R1.z * a1 + R2.z * k1 * F);
+  if (norm(dir1) == 0) {
+    max_flag = false;
+  } else {
+    dir1 = normalized(dir1);
+    dir_max = mul(Rg, dir1);
+  }
+  V3<double> dir2 = mk3(R1.x * a2 + R2.x * k2 * F, R1.y * a2 + R2.y * k2 * F, R1.z * a2 + R2.z * k2 * F);
+  dir2 = normalized(dir2);
+  dir_min = mul(Rg, dir2);
+  k1o = k1;
+  k2o = k2;
+}
+
+// SuperquadricsGeom2::HertzMindlin (Superquadrics.cpp:974-1026).
+SDM_HD void hertz_mindlin(double c10, double c11, double c20, double c21, double theta, double& curvSum,
+                          double& alpha, double& Ko, double& Eo) {
+  double rho11 = fabs(c10), rho12 = fabs(c11), rho21 = fabs(c20), rho22 = fabs(c21);
+  if (rho11 * rho12 < 1E-18) rho11 = rho12 = 0.0;
+  if (rho21 * rho22 < 1E-18) rho21 = rho22 = 0.0;
+  double sum_rho = rho11 + rho12 + rho21 + rho22;
+  double a1 = rho11 - rho12, a2 = rho21 - rho22;
+  double F_rho = -sqrt(a1 * a1 + a2 * a2 + 2 * (fabs(rho11 - rho12)) * fabs(rho21 - rho22) * cos(2 * theta));
+  double A = (sum_rho + F_rho) / 4.;
+  double B = (sum_rho - F_rho) / 4.;
+  curvSum = A + B;
+  double lg = log10(B / A);
+  double x_squ = lg * lg;
+  double gamma1 = 2 * (1 + x_squ * (0.40227436 + x_squ * (3.7491752e-2 + x_squ * (7.4855761e-4 + x_squ * 2.1667028e-6))));
+  double gamma2 = 3 * (1 + x_squ * (0.42678878 + x_squ * (4.2605401e-2 + x_squ * (9.0786922e-4 + x_squ * 2.7868927e-6))));
+  alpha = pow((B / A), gamma1 / gamma2);
+  double m1 = 1 / (alpha * alpha);
+  Ko = (1.3862944 + m1 * (0.1119723 + 0.0725296 * m1)) - log(m1) * (0.5 + m1 * (0.1213478 + 0.0288729 * m1));
+  Eo = (1 + m1 * (0.4630151 + 0.1077812 * m1)) - log(m1) * (0.2452727 + 0.0412496 * m1) * m1;
+}

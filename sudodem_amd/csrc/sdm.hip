@@ -1,0 +1,800 @@
+// sdm.hip -- libsdm_hip.so: the superellipsoid DEM step on one MI355X (gfx950).
+//
+// Host side of the library: context management, the engine loop of one
+// Scene::moveToNextTimeStep (core/Scene.cpp:70-138) with the canonical engine list
+// [ForceResetter, InsertionSortCollider, InteractionLoop, NewtonIntegrator], and the C ABI
+// declared in include/sdm.h.  All state lives in HBM; the host keeps only the scalar
+// collider/integrator flags the reference keeps in its engine objects.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/sdm.h"
+#include "bodies.h"
+#include "collider.h"
+#include "radix.h"
+
+#define SDM_VERSION "sdm-hip 1.0 (gfx950)"
+
+namespace {
+
+std::string g_err;
+
+struct HipError {
+  std::string msg;
+};
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) throw HipError{std::string(#x) + ": " + hipGetErrorString(e_)};      \
+  } while (0)
+
+template <typename T>
+T* dalloc(int64_t n) {
+  void* p = nullptr;
+  if (n <= 0) n = 1;
+  HIPCHK(hipMalloc(&p, (size_t)n * sizeof(T)));
+  return (T*)p;
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+struct sdm_ctx {
+  sdm_params prm;
+  int dev = 0;
+  hipStream_t st = nullptr;
+  BodyArrays B;
+  PairArrays P;
+  int64_t np = 0;
+  MaterialDev* dmats = nullptr;
+  std::vector<sdm_material> mats;
+  // collider persistent state (InsertionSortCollider members)
+  int32_t* bb[3] = {nullptr, nullptr, nullptr};
+  int32_t* bbTmp = nullptr;
+  bool bbValid = false;
+  int32_t strideActive = 0;
+  double fastestBodyMaxDist = -1;
+  int32_t numReinit = 0;
+  // NewtonIntegrator::maxVelocitySq, Scene::iter/time
+  double maxVelocitySq = std::numeric_limits<double>::quiet_NaN();
+  int64_t iter = 0;
+  double time = 0;
+  // per-body interaction lists
+  int32_t* csrOff = nullptr;
+  int32_t* csrPair = nullptr;
+  int64_t csrCap = 0;
+  bool csrValid = false;
+  // scratch
+  uint64_t *k0 = nullptr, *k1 = nullptr;
+  uint32_t *v0 = nullptr, *v1 = nullptr, *u0 = nullptr, *u1 = nullptr;
+  int64_t kcap = 0;
+  uint32_t* rsScratch = nullptr;
+  int64_t rsCap = 0;
+  uint32_t* qs[3] = {nullptr, nullptr, nullptr};
+  uint32_t* smin = nullptr;
+  uint64_t* htab = nullptr;
+  uint64_t* dkeys = nullptr;
+  uint64_t* dvals = nullptr;
+  int64_t hcap = 0;
+  Cands C;
+  int32_t *msrc = nullptr, *mdst = nullptr, *holePos = nullptr;
+  int64_t mcap = 0;
+  FieldDesc* dfields = nullptr;
+  int nfields = 0;
+  unsigned long long* dcnt = nullptr;  // [0] contact evals, [1] dist evals, [2] candidate count
+  int* dMaxDisp = nullptr;
+  bool matChecked = false;
+  sdm_counters cnt;
+  std::string err;
+};
+
+namespace {
+
+int fail(sdm_ctx* c, const std::string& m) {
+  if (c) c->err = m;
+  g_err = m;
+  return -1;
+}
+
+void alloc_bodies(sdm_ctx* c, int32_t n) {
+  c->B.n = n;
+#define SDM_ALLOC(T, m, k, name) c->B.m = dalloc<T>((int64_t)n * k);
+  SDM_BODY_FIELDS(SDM_ALLOC)
+#undef SDM_ALLOC
+  for (int a = 0; a < 3; a++) c->bb[a] = dalloc<int32_t>(2 * (int64_t)n);
+  c->bbTmp = dalloc<int32_t>(2 * (int64_t)n);
+  for (int a = 0; a < 3; a++) c->qs[a] = dalloc<uint32_t>(2 * (int64_t)n);
+  c->smin = dalloc<uint32_t>(2 * (int64_t)n + 1);
+  c->csrOff = dalloc<int32_t>((int64_t)n + 1);
+}
+
+void free_pairs(PairArrays& P) {
+#define SDM_FREE(T, m, k, name) dfree(P.m);
+  SDM_PAIR_FIELDS(SDM_FREE)
+#undef SDM_FREE
+}
+
+void build_field_table(sdm_ctx* c) {
+  std::vector<FieldDesc> fd;
+#define SDM_FD(T, m, k, name) fd.push_back(FieldDesc{(void*)c->P.m, (int32_t)sizeof(T), (int32_t)k});
+  SDM_PAIR_FIELDS(SDM_FD)
+#undef SDM_FD
+  dfree(c->dfields);
+  c->dfields = dalloc<FieldDesc>((int64_t)fd.size());
+  HIPCHK(hipMemcpy(c->dfields, fd.data(), fd.size() * sizeof(FieldDesc), hipMemcpyHostToDevice));
+  c->nfields = (int)fd.size();
+}
+
+// Grow the interaction arrays to hold at least `need` entries, keeping [0, np).
+void ensure_pair_cap(sdm_ctx* c, int64_t need) {
+  if (need <= c->P.cap && c->P.id1) return;
+  int64_t cap = c->P.cap > 0 ? c->P.cap : 1024;
+  while (cap < need) cap = cap * 3 / 2 + 1024;
+  PairArrays Q;
+  Q.cap = cap;
+#define SDM_GROW(T, m, k, name)                                                                     \
+  Q.m = dalloc<T>(cap * k);                                                                         \
+  if (c->P.m && c->np > 0)                                                                          \
+    for (int cc = 0; cc < k; cc++)                                                                  \
+      HIPCHK(hipMemcpyAsync(Q.m + cc * cap, c->P.m + cc * c->P.cap, c->np * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+  SDM_PAIR_FIELDS(SDM_GROW)
+#undef SDM_GROW
+  HIPCHK(hipStreamSynchronize(c->st));
+  free_pairs(c->P);
+  c->P = Q;
+  build_field_table(c);
+  c->csrValid = false;
+}
+
+void ensure_scratch(sdm_ctx* c, int64_t n) {
+  if (n <= c->kcap) return;
+  HIPCHK(hipStreamSynchronize(c->st));
+  int64_t cap = n + n / 2 + 4096;
+  dfree(c->k0); dfree(c->k1); dfree(c->v0); dfree(c->v1); dfree(c->u0); dfree(c->u1);
+  c->k0 = dalloc<uint64_t>(cap);
+  c->k1 = dalloc<uint64_t>(cap);
+  c->v0 = dalloc<uint32_t>(cap);
+  c->v1 = dalloc<uint32_t>(cap);
+  c->u0 = dalloc<uint32_t>(cap + 1);
+  c->u1 = dalloc<uint32_t>(cap + 1);
+  c->kcap = cap;
+  int64_t rs = std::max(sdmrs::sort_scratch(cap), sdmrs::scan_scratch(cap + 1)) + 64;
+  if (rs > c->rsCap) {
+    dfree(c->rsScratch);
+    c->rsScratch = dalloc<uint32_t>(rs);
+    c->rsCap = rs;
+  }
+}
+
+void ensure_cands(sdm_ctx* c, int64_t n) {
+  if (n <= c->C.cap) return;
+  HIPCHK(hipStreamSynchronize(c->st));
+  int64_t cap = n + n / 2 + 4096;
+  dfree(c->C.a); dfree(c->C.b); dfree(c->C.key);
+  c->C.a = dalloc<int32_t>(cap);
+  c->C.b = dalloc<int32_t>(cap);
+  c->C.key = dalloc<uint64_t>(cap);
+  c->C.cap = cap;
+}
+
+void ensure_hash(sdm_ctx* c, int64_t n) {
+  int64_t need = 1024;
+  while (need < 2 * n + 16) need <<= 1;
+  if (need <= c->hcap) return;
+  HIPCHK(hipStreamSynchronize(c->st));
+  dfree(c->htab); dfree(c->dkeys); dfree(c->dvals);
+  c->htab = dalloc<uint64_t>(need);
+  c->dkeys = dalloc<uint64_t>(need);
+  c->dvals = dalloc<uint64_t>(need);
+  c->hcap = need;
+}
+
+void ensure_moves(sdm_ctx* c, int64_t n) {
+  if (n <= c->mcap) return;
+  HIPCHK(hipStreamSynchronize(c->st));
+  int64_t cap = n + n / 2 + 1024;
+  dfree(c->msrc); dfree(c->mdst); dfree(c->holePos);
+  c->msrc = dalloc<int32_t>(cap);
+  c->mdst = dalloc<int32_t>(cap);
+  c->holePos = dalloc<int32_t>(cap);
+  c->mcap = cap;
+}
+
+uint32_t read_u32(sdm_ctx* c, const uint32_t* p) {
+  uint32_t v = 0;
+  HIPCHK(hipMemcpyAsync(&v, p, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return v;
+}
+
+// Exclusive scan of u0[0..n) into u1[0..n], with u1[n] = total; returns the total.
+uint32_t exclusive_total(sdm_ctx* c, const uint32_t* in, uint32_t* out, int64_t n) {
+  if (n == 0) return 0;
+  sdmrs::scan<sdmrs::OpAdd, false, false>(in, out, n, c->rsScratch, c->st);
+  uint32_t a = 0, b = 0;
+  HIPCHK(hipMemcpyAsync(&a, out + n - 1, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(&b, in + n - 1, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return a + b;
+}
+
+// Per-body interaction lists, each in interaction order (stable sort by body id).
+void build_csr(sdm_ctx* c) {
+  const int64_t N = c->B.n;
+  const int64_t m = 2 * c->np;
+  if (m > c->csrCap) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(c->csrPair);
+    c->csrCap = m + m / 2 + 1024;
+    c->csrPair = dalloc<int32_t>(c->csrCap);
+  }
+  ensure_scratch(c, std::max<int64_t>(m, N + 1));
+  int32_t* deg = (int32_t*)c->u0;
+  HIPCHK(hipMemsetAsync(deg, 0, (N + 1) * sizeof(int32_t), c->st));
+  if (c->np > 0) {
+    uint32_t* key = c->u1;
+    hipLaunchKernelGGL(k_csr_entries, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.id1, c->P.id2, c->np, key, c->v0,
+                       deg);
+    int bits = 1;
+    while ((1ll << bits) < N) bits++;
+    // keys are 32-bit: reuse the 64-bit sort buffers through a 32-bit view
+    sdmrs::sort_pairs<uint32_t, uint32_t>(key, c->v0, (uint32_t*)c->k1, c->v1, m, bits, c->rsScratch, c->st);
+    HIPCHK(hipMemcpyAsync(c->csrPair, c->v0, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+  }
+  sdmrs::scan<sdmrs::OpAdd, false, false>((uint32_t*)deg, (uint32_t*)c->csrOff, N + 1, c->rsScratch, c->st);
+  c->csrValid = true;
+}
+
+// InsertionSortCollider::action (InsertionSortCollider.cpp:191-400), non-periodic.
+void collider_run(sdm_ctx* c) {
+  c->cnt.collider_runs++;
+  const int64_t N = c->B.n;
+  const int64_t n2 = 2 * N;
+  bool doInitSort = false;
+  if (!c->bbValid) {
+    // empty bound arrays: fill in body order (min, max) and do the initial sort
+    std::vector<int32_t> h(n2);
+    for (int64_t i = 0; i < n2; i++) h[i] = (int32_t)i;
+    for (int a = 0; a < 3; a++) HIPCHK(hipMemcpy(c->bb[a], h.data(), n2 * sizeof(int32_t), hipMemcpyHostToDevice));
+    c->bbValid = true;
+    doInitSort = true;
+  }
+  // BoundDispatcher::action with sweepDist = verletDist
+  hipLaunchKernelGGL(k_bounds, dim3(nblk(N)), dim3(256), 0, c->st, c->B, c->iter, c->prm.verlet_dist,
+                     c->prm.min_sweep_dist_factor, c->prm.target_interv);
+  if (!c->strideActive && c->prm.verlet_dist > 0 && c->maxVelocitySq >= 0) c->strideActive = 1;
+  ensure_scratch(c, std::max<int64_t>({n2 + 1, c->np + 1, 2 * c->np + 1}));
+  ensure_moves(c, c->np + 1);
+
+  // ---- conditionalyEraseNonReal ----
+  if (c->np > 0) {
+    uint32_t* flag = c->u0;
+    uint32_t* E = c->u1;
+    hipLaunchKernelGGL(k_erase_flags, dim3(nblk(c->np)), dim3(256), 0, c->st, c->B, c->P, c->np, flag);
+    const uint32_t Etot = exclusive_total(c, flag, E, c->np);
+    if (Etot > 0) {
+      const int64_t K = c->np - Etot;
+      hipLaunchKernelGGL(k_hole_pos, dim3(nblk(K)), dim3(256), 0, c->st, flag, E, K, c->holePos);
+      const int64_t tail = c->np - K;
+      hipLaunchKernelGGL(k_tail_moves, dim3(nblk(tail)), dim3(256), 0, c->st, flag, E, K, c->np, Etot, c->holePos,
+                         c->msrc, c->mdst);
+      // moves = holes below K = erased entries below K = E[K] (= kept entries at >= K)
+      const int64_t nmove = read_u32(c, E + K);
+      if (nmove > 0)
+        hipLaunchKernelGGL(k_move_pairs, dim3(nblk(nmove)), dim3(256), 0, c->st, c->dfields, c->nfields, c->P.cap,
+                           c->msrc, c->mdst, nmove);
+      c->np = K;
+      c->cnt.pairs_erased += Etot;
+    }
+  }
+
+  // ---- sort the bounds of each axis ----
+  for (int a = 0; a < 3; a++) {
+    hipLaunchKernelGGL(k_bb_keys, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, c->bb[a], n2, a, c->k0, c->v0);
+    sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, n2, 64, c->rsScratch, c->st);
+    hipLaunchKernelGGL(k_bb_apply, dim3(nblk(n2)), dim3(256), 0, c->st, c->bb[a], c->v0, n2, c->bbTmp);
+    HIPCHK(hipMemcpyAsync(c->qs[a], c->v0, n2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->st));
+    std::swap(c->bb[a], c->bbTmp);
+  }
+
+  // ---- candidates ----
+  int64_t nc = 0;
+  if (!doInitSort) {
+    for (int pass = 0; pass < 2; pass++) {
+      HIPCHK(hipMemsetAsync(c->dcnt + 2, 0, sizeof(unsigned long long), c->st));
+      for (int a = 0; a < 3; a++) {
+        sdmrs::scan<sdmrs::OpMin, true, true>(c->qs[a], c->smin, n2, c->rsScratch, c->st);
+        hipLaunchKernelGGL(k_crossings, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, c->bb[a], c->qs[a], c->smin, n2, a,
+                           c->C);
+      }
+      unsigned long long cnt = 0;
+      HIPCHK(hipMemcpyAsync(&cnt, c->dcnt + 2, sizeof(cnt), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+      nc = (int64_t)cnt;
+      if (nc <= c->C.cap) break;
+      ensure_cands(c, nc);  // rerun with room for all
+    }
+  } else {
+    c->numReinit++;
+    const int32_t* V = c->bb[c->prm.sort_axis];
+    uint32_t* counts = c->u0;
+    uint32_t* offs = c->u1;
+    hipLaunchKernelGGL(k_init_sweep<false>, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, V, n2, counts,
+                       (const uint32_t*)nullptr, c->C);
+    nc = exclusive_total(c, counts, offs, n2);
+    ensure_cands(c, nc);
+    hipLaunchKernelGGL(k_init_sweep<true>, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, V, n2, (uint32_t*)nullptr, offs,
+                       c->C);
+  }
+
+  if (nc > 0) {
+    ensure_hash(c, c->np + nc);
+    ensure_scratch(c, nc + 1);
+    const uint64_t mask = (uint64_t)c->hcap - 1;
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->htab, c->hcap, SDM_HEMPTY);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->dkeys, c->hcap, SDM_HEMPTY);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->dvals, c->hcap, SDM_HEMPTY);
+    if (c->np > 0)
+      hipLaunchKernelGGL(k_hash_insert_pairs, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.id1, c->P.id2, c->np,
+                         c->htab, mask);
+    uint32_t* alive = c->u0;
+    uint32_t* pos = c->u1;
+    hipLaunchKernelGGL(k_cand_dedup, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, c->htab, mask, c->dkeys, c->dvals,
+                       mask, alive);
+    hipLaunchKernelGGL(k_cand_keep, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, c->dkeys, c->dvals, mask, alive);
+    const int64_t ns = exclusive_total(c, alive, pos, nc);
+    if (ns > 0) {
+      hipLaunchKernelGGL(k_cand_compact, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, alive, pos, c->k0, c->v0);
+      sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, ns, 64, c->rsScratch, c->st);
+      ensure_pair_cap(c, c->np + ns);
+      hipLaunchKernelGGL(k_new_pairs, dim3(nblk(ns)), dim3(256), 0, c->st, c->P, c->np, ns, c->C, c->v0, c->iter);
+      c->np += ns;
+      c->cnt.pairs_inserted += ns;
+    }
+  }
+  build_csr(c);
+}
+
+template <int LAW, int PREC>
+void launch_interactions(sdm_ctx* c, const StepArgs& A) {
+  if (A.np <= 0) return;
+  hipLaunchKernelGGL((k_interactions<LAW, PREC>), dim3(nblk(A.np)), dim3(256), 0, c->st, A);
+}
+
+bool collider_is_activated(sdm_ctx* c) {
+  if (!c->strideActive) return true;
+  if (c->fastestBodyMaxDist < 0) {
+    c->fastestBodyMaxDist = 0;
+    return true;
+  }
+  c->fastestBodyMaxDist = c->maxVelocitySq;
+  if (c->fastestBodyMaxDist >= 1 || c->fastestBodyMaxDist == 0) return true;
+  if (!c->bbValid) return true;
+  return false;
+}
+
+void one_step(sdm_ctx* c) {
+  if (collider_is_activated(c)) collider_run(c);
+  if (!c->csrValid) build_csr(c);
+  StepArgs A;
+  A.B = c->B;
+  A.P = c->P;
+  A.mats = c->dmats;
+  A.np = c->np;
+  A.iter = c->iter;
+  A.dt = c->prm.dt;
+  A.counters = c->dcnt;
+  const int law = c->prm.law, prec = c->prm.precision;
+  if (law == SDM_LAW_LINEAR && prec == SDM_FP64) launch_interactions<0, 0>(c, A);
+  else if (law == SDM_LAW_LINEAR) launch_interactions<0, 1>(c, A);
+  else if (prec == SDM_FP64) launch_interactions<1, 0>(c, A);
+  else launch_interactions<1, 1>(c, A);
+  NewtonArgs NA;
+  NA.B = c->B;
+  NA.P = c->P;
+  NA.csrOff = c->csrOff;
+  NA.csrPair = c->csrPair;
+  NA.dt = c->prm.dt;
+  NA.damping = c->prm.damping;
+  NA.g[0] = c->prm.gravity[0];
+  NA.g[1] = c->prm.gravity[1];
+  NA.g[2] = c->prm.gravity[2];
+  NA.exactAsph = c->prm.exact_aspherical_rot;
+  NA.maxDisp = c->dMaxDisp;
+  HIPCHK(hipMemsetAsync(c->dMaxDisp, 0, sizeof(int), c->st));
+  hipLaunchKernelGGL(k_newton, dim3(nblk(c->B.n)), dim3(256), 0, c->st, NA);
+  int code = 0;
+  HIPCHK(hipMemcpyAsync(&code, c->dMaxDisp, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->maxVelocitySq = code == 2 ? 2.0 : (code == 1 ? 0.5 : 0.0);
+  c->iter++;
+  c->time += c->prm.dt;
+  c->cnt.steps++;
+}
+
+// ---- attribute table ----
+enum Scope { S_BODY, S_PAIR, S_SCALAR, S_BOUNDS };
+struct FieldInfo {
+  const char* name;
+  Scope scope;
+  int dtype;
+  int comps;
+  int esize;
+  size_t off;  // offset of the pointer member in BodyArrays / PairArrays, or scalar id
+};
+template <typename T>
+constexpr int dtype_of() {
+  return sizeof(T) == 4 ? SDM_I32 : (std::is_same<T, double>::value ? SDM_F64 : SDM_I64);
+}
+const FieldInfo kFields[] = {
+#define SDM_BF(T, m, k, name) {name, S_BODY, dtype_of<T>(), k, (int)sizeof(T), offsetof(BodyArrays, m)},
+    SDM_BODY_FIELDS(SDM_BF)
+#undef SDM_BF
+#define SDM_PF(T, m, k, name) {name, S_PAIR, dtype_of<T>(), k, (int)sizeof(T), offsetof(PairArrays, m)},
+        SDM_PAIR_FIELDS(SDM_PF)
+#undef SDM_PF
+            {"scene.iter", S_SCALAR, SDM_I64, 1, 8, 0},
+    {"scene.time", S_SCALAR, SDM_F64, 1, 8, 1},
+    {"collider.strideActive", S_SCALAR, SDM_I32, 1, 4, 2},
+    {"collider.fastestBodyMaxDist", S_SCALAR, SDM_F64, 1, 8, 3},
+    {"newton.maxVelocitySq", S_SCALAR, SDM_F64, 1, 8, 4},
+    {"collider.numReinit", S_SCALAR, SDM_I32, 1, 4, 5},
+    {"collider.bounds", S_BOUNDS, SDM_I32, 1, 4, 0},
+};
+
+const FieldInfo* find_field(const char* name) {
+  if (!name) return nullptr;
+  for (const FieldInfo& f : kFields)
+    if (std::strcmp(f.name, name) == 0) return &f;
+  return nullptr;
+}
+
+int64_t field_count(sdm_ctx* c, const FieldInfo* f) {
+  switch (f->scope) {
+    case S_BODY: return (int64_t)c->B.n * f->comps;
+    case S_PAIR: return c->np * f->comps;
+    case S_SCALAR: return 1;
+    case S_BOUNDS: return c->bbValid ? 6 * (int64_t)c->B.n : 0;
+  }
+  return 0;
+}
+
+// Fresh interactions for [from, to) after the host resized the container.
+void init_fresh_pairs(sdm_ctx* c, int64_t from, int64_t to) {
+  if (to <= from) return;
+  const int64_t n = to - from;
+  ensure_cands(c, n);
+  // ids are overwritten by the caller's next "intr.id*" writes; use -1 placeholders
+  std::vector<int32_t> ones(n, -1);
+  HIPCHK(hipMemcpy(c->C.a, ones.data(), n * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->C.b, ones.data(), n * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_new_pairs, dim3(nblk(n)), dim3(256), 0, c->st, c->P, from, n, c->C, (const uint32_t*)nullptr,
+                     c->iter);
+  HIPCHK(hipStreamSynchronize(c->st));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+const char* sdm_version(void) { return SDM_VERSION; }
+
+sdm_ctx* sdm_create(const sdm_params* params, int32_t n_bodies, int32_t n_materials) {
+  if (!params || n_bodies < 0 || n_materials < 0) {
+    g_err = "sdm_create: invalid arguments";
+    return nullptr;
+  }
+  if (!(params->verlet_dist >= 0)) {
+    g_err = "verletDist must be set explicitly (>= 0): the automatic value needs spheres";
+    return nullptr;
+  }
+  if (params->law != SDM_LAW_LINEAR && params->law != SDM_LAW_HERTZ) {
+    g_err = "sdm_create: unknown law";
+    return nullptr;
+  }
+  if (params->precision != SDM_FP64 && params->precision != SDM_MIXED) {
+    g_err = "sdm_create: unknown precision";
+    return nullptr;
+  }
+  if (params->sort_axis < 0 || params->sort_axis > 2) {
+    g_err = "sdm_create: sortAxis must be 0, 1 or 2";
+    return nullptr;
+  }
+  sdm_ctx* c = new sdm_ctx();
+  try {
+    c->prm = *params;
+    HIPCHK(hipGetDevice(&c->dev));
+    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    std::memset(&c->P, 0, sizeof(c->P));
+    std::memset(&c->C, 0, sizeof(c->C));
+    alloc_bodies(c, n_bodies);
+    // defaults of a fresh Body/State/Bound: identity orientation, damped, NaN bound data
+    const int64_t N = n_bodies;
+    std::vector<double> nan(N * 3, std::numeric_limits<double>::quiet_NaN());
+    std::vector<int32_t> zi(N, 0), onei(N, 1);
+#define SDM_ZB(T, m, k, name) HIPCHK(hipMemset(c->B.m, 0, N * k * sizeof(T) + (N == 0 ? 1 : 0)));
+    SDM_BODY_FIELDS(SDM_ZB)
+#undef SDM_ZB
+    if (N > 0) {
+      std::vector<double> one(N, 1.0);
+      HIPCHK(hipMemcpy(c->B.ori, one.data(), N * 8, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->B.damped, onei.data(), N * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->B.refPos, nan.data(), N * 24, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->B.bmin, nan.data(), N * 24, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->B.bmax, nan.data(), N * 24, hipMemcpyHostToDevice));
+      for (int k : {0, 4, 8}) {
+        HIPCHK(hipMemcpy(c->B.rotL + k * N, one.data(), N * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->B.rotG + k * N, one.data(), N * 8, hipMemcpyHostToDevice));
+      }
+    }
+    c->mats.resize(n_materials);
+    c->dmats = dalloc<MaterialDev>(std::max(1, n_materials));
+    c->dcnt = dalloc<unsigned long long>(4);
+    HIPCHK(hipMemset(c->dcnt, 0, 4 * sizeof(unsigned long long)));
+    c->dMaxDisp = dalloc<int>(1);
+    ensure_pair_cap(c, 1024);
+    ensure_cands(c, 4096);
+    c->C.count = c->dcnt + 2;
+    std::memset(&c->cnt, 0, sizeof(c->cnt));
+  } catch (HipError& e) {
+    g_err = e.msg;
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+void sdm_destroy(sdm_ctx* c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->st);
+#define SDM_FB(T, m, k, name) dfree(c->B.m);
+  SDM_BODY_FIELDS(SDM_FB)
+#undef SDM_FB
+  free_pairs(c->P);
+  for (int a = 0; a < 3; a++) {
+    dfree(c->bb[a]);
+    dfree(c->qs[a]);
+  }
+  dfree(c->bbTmp); dfree(c->smin); dfree(c->csrOff); dfree(c->csrPair);
+  dfree(c->k0); dfree(c->k1); dfree(c->v0); dfree(c->v1); dfree(c->u0); dfree(c->u1); dfree(c->rsScratch);
+  dfree(c->htab); dfree(c->dkeys); dfree(c->dvals);
+  dfree(c->C.a); dfree(c->C.b); dfree(c->C.key);
+  dfree(c->msrc); dfree(c->mdst); dfree(c->holePos); dfree(c->dfields);
+  dfree(c->dmats); dfree(c->dcnt); dfree(c->dMaxDisp);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char* sdm_last_error(const sdm_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int sdm_set_materials(sdm_ctx* c, const sdm_material* mats, int32_t n) {
+  if (!c || n < 0 || (n > 0 && !mats)) return fail(c, "sdm_set_materials: invalid arguments");
+  try {
+    c->matChecked = false;
+    c->mats.assign(mats, mats + n);
+    dfree(c->dmats);
+    c->dmats = dalloc<MaterialDev>(std::max(1, n));
+    std::vector<MaterialDev> md(std::max(1, n));
+    for (int i = 0; i < n; i++)
+      md[i] = MaterialDev{mats[i].density, mats[i].Kn, mats[i].Ks, mats[i].frictionAngle,
+                          mats[i].betan,   mats[i].betas, mats[i].G, mats[i].nu};
+    HIPCHK(hipMemcpy(c->dmats, md.data(), md.size() * sizeof(MaterialDev), hipMemcpyHostToDevice));
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+// Superquadrics::Initial (pkg/dem/Superquadrics.cpp:47-93) with Mathr::beta/gammln
+// (lib/base/Math.hpp:193-212): host-side set-up of one body.
+static double sdm_gammln(const double xx) {
+  static const double cof[14] = {57.1562356658629235,     -59.5979603554754912,    14.1360979747417471,
+                                 -0.491913816097620199,   .339946499848118887e-4,  .465236289270485756e-4,
+                                 -.983744753048795646e-4, .158088703224912494e-3,  -.210264441724104883e-3,
+                                 .217439618115212643e-3,  -.164318106536763890e-3, .844182239838527433e-4,
+                                 -.261908384015814087e-4, .368991826595316234e-5};
+  double x = xx, y = xx;
+  double tmp = x + 5.24218750000000000;
+  tmp = (x + 0.5) * std::log(tmp) - tmp;
+  double ser = 0.999999999999997092;
+  for (int j = 0; j < 14; j++) ser += cof[j] / ++y;
+  return tmp + std::log(2.5066282746310005 * ser / x);
+}
+static double sdm_beta(double z, double w) { return std::exp(sdm_gammln(z) + sdm_gammln(w) - sdm_gammln(z + w)); }
+
+int sdm_superquadric_props(double rx, double ry, double rz, double eps1, double eps2, double* r_max, double* volume,
+                           double inertia[3]) {
+  if (!r_max || !volume || !inertia) return fail(nullptr, "sdm_superquadric_props: null output");
+  if (!(rx > 0 && ry > 0 && rz > 0 && eps1 > 0 && eps2 > 0 && eps1 < 2 && eps2 < 2))
+    return fail(nullptr, "sdm_superquadric_props: semi-axes must be > 0 and exponents in (0, 2)");
+  double r = (rx > ry) ? rx : ry;
+  r = r > rz ? r : rz;
+  if ((eps1 < 1.0) || (eps2 < 1.0)) r *= 1.4142135623730951;  // pow(2.0, 0.5)
+  *r_max = r;
+  const double a = rx * ry * rz * eps1 * eps2;
+  const double beta1 = sdm_beta(1.5 * eps1, 0.5 * eps1) * sdm_beta(0.5 * eps2, 2. * eps2 + 1);
+  const double beta2 = sdm_beta(0.5 * eps1, 0.5 * eps1 + 1) * sdm_beta(1.5 * eps2, eps2 + 1);
+  *volume = 2. * a * sdm_beta(eps1 * 0.5, eps1 * 0.5) * sdm_beta(eps2 * 0.5 + 1, eps2);
+  inertia[0] = 0.5 * a * ((ry * ry) * beta1 + 4. * (rz * rz) * beta2);
+  inertia[1] = 0.5 * a * ((rx * rx) * beta1 + 4. * (rz * rz) * beta2);
+  inertia[2] = 0.5 * a * ((rx * rx) + (ry * ry)) * beta1;
+  return 0;
+}
+
+const char* sdm_field_list(void) {
+  static std::string s;
+  if (s.empty())
+    for (const FieldInfo& f : kFields) {
+      s += f.name;
+      s += "\n";
+    }
+  return s.c_str();
+}
+
+int sdm_field_info(sdm_ctx* c, const char* name, int32_t* dtype, int64_t* count) {
+  const FieldInfo* f = find_field(name);
+  if (!c || !f) return fail(c, std::string("unknown attribute ") + (name ? name : "(null)"));
+  if (dtype) *dtype = f->dtype;
+  if (count) *count = field_count(c, f);
+  return 0;
+}
+
+int sdm_get(sdm_ctx* c, const char* name, void* dst, int64_t count) {
+  const FieldInfo* f = find_field(name);
+  if (!c || !f || (!dst && count > 0)) return fail(c, std::string("sdm_get: bad attribute ") + (name ? name : "(null)"));
+  const int64_t n = field_count(c, f);
+  if (count != n) return fail(c, std::string("sdm_get: count mismatch for ") + name);
+  try {
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (f->scope == S_BODY || f->scope == S_PAIR) {
+      const int64_t rows = f->scope == S_BODY ? c->B.n : c->np;
+      const int64_t stride = f->scope == S_BODY ? c->B.n : c->P.cap;
+      const char* base = f->scope == S_BODY ? (const char*)&c->B : (const char*)&c->P;
+      const char* dptr = *(char* const*)(base + f->off);
+      std::vector<char> tmp((size_t)rows * f->esize);
+      char* out = (char*)dst;
+      for (int k = 0; k < f->comps; k++) {
+        if (rows > 0)
+          HIPCHK(hipMemcpy(tmp.data(), dptr + (size_t)k * stride * f->esize, rows * f->esize, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < rows; i++)
+          std::memcpy(out + ((size_t)i * f->comps + k) * f->esize, tmp.data() + (size_t)i * f->esize, f->esize);
+      }
+    } else if (f->scope == S_SCALAR) {
+      switch (f->off) {
+        case 0: *(int64_t*)dst = c->iter; break;
+        case 1: *(double*)dst = c->time; break;
+        case 2: *(int32_t*)dst = c->strideActive; break;
+        case 3: *(double*)dst = c->fastestBodyMaxDist; break;
+        case 4: *(double*)dst = c->maxVelocitySq; break;
+        case 5: *(int32_t*)dst = c->numReinit; break;
+      }
+    } else {
+      const int64_t n2 = 2 * (int64_t)c->B.n;
+      for (int a = 0; a < 3 && c->bbValid; a++)
+        HIPCHK(hipMemcpy((int32_t*)dst + a * n2, c->bb[a], n2 * 4, hipMemcpyDeviceToHost));
+    }
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
+  const FieldInfo* f = find_field(name);
+  if (!c || !f || (!src && count > 0)) return fail(c, std::string("sdm_set: bad attribute ") + (name ? name : "(null)"));
+  try {
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (f->scope == S_PAIR && std::strcmp(name, "intr.id1") == 0) {
+      const int64_t old = c->np;
+      ensure_pair_cap(c, count);
+      c->np = count;
+      init_fresh_pairs(c, old, count);
+      c->csrValid = false;
+    }
+    if (f->scope == S_BOUNDS) {
+      const int64_t n2 = 2 * (int64_t)c->B.n;
+      if (count != 3 * n2) return fail(c, "collider.bounds: count must be 6*N");
+      for (int a = 0; a < 3; a++)
+        HIPCHK(hipMemcpy(c->bb[a], (const int32_t*)src + a * n2, n2 * 4, hipMemcpyHostToDevice));
+      c->bbValid = true;
+      return 0;
+    }
+    const int64_t n = field_count(c, f);
+    if (count != n) return fail(c, std::string("sdm_set: count mismatch for ") + name);
+    if (f->scope == S_BODY || f->scope == S_PAIR) {
+      const int64_t rows = f->scope == S_BODY ? c->B.n : c->np;
+      const int64_t stride = f->scope == S_BODY ? c->B.n : c->P.cap;
+      const char* base = f->scope == S_BODY ? (const char*)&c->B : (const char*)&c->P;
+      char* dptr = *(char* const*)(base + f->off);
+      std::vector<char> tmp((size_t)rows * f->esize);
+      const char* in = (const char*)src;
+      for (int k = 0; k < f->comps; k++) {
+        for (int64_t i = 0; i < rows; i++)
+          std::memcpy(tmp.data() + (size_t)i * f->esize, in + ((size_t)i * f->comps + k) * f->esize, f->esize);
+        if (rows > 0)
+          HIPCHK(hipMemcpy(dptr + (size_t)k * stride * f->esize, tmp.data(), rows * f->esize, hipMemcpyHostToDevice));
+      }
+      if (f->scope == S_PAIR) c->csrValid = false;
+      if (f->scope == S_BODY) c->matChecked = false;
+    } else {
+      switch (f->off) {
+        case 0: c->iter = *(const int64_t*)src; break;
+        case 1: c->time = *(const double*)src; break;
+        case 2: c->strideActive = *(const int32_t*)src; break;
+        case 3: c->fastestBodyMaxDist = *(const double*)src; break;
+        case 4: c->maxVelocitySq = *(const double*)src; break;
+        case 5: c->numReinit = *(const int32_t*)src; break;
+      }
+    }
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+int sdm_step(sdm_ctx* c, int32_t nsteps) {
+  if (!c) return fail(c, "null context");
+  if (nsteps < 0) return fail(c, "sdm_step: nsteps < 0");
+  try {
+    if (!c->matChecked && c->B.n > 0) {
+      std::vector<int32_t> mat(c->B.n);
+      HIPCHK(hipMemcpy(mat.data(), c->B.material, c->B.n * 4, hipMemcpyDeviceToHost));
+      for (int32_t m : mat)
+        if (m < 0 || m >= (int)c->mats.size()) return fail(c, "body material index out of range");
+      c->matChecked = true;
+    }
+    for (int s = 0; s < nsteps; s++) one_step(c);
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+int sdm_sync(sdm_ctx* c) {
+  if (!c) return -1;
+  hipError_t e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess) return fail(c, hipGetErrorString(e));
+  return 0;
+}
+
+int sdm_get_counters(sdm_ctx* c, sdm_counters* out) {
+  if (!c || !out) return fail(c, "sdm_get_counters: bad arguments");
+  try {
+    unsigned long long d[2];
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(d, c->dcnt, sizeof(d), hipMemcpyDeviceToHost));
+    *out = c->cnt;
+    out->contact_evals = (int64_t)d[0];
+    out->dist_evals = (int64_t)d[1];
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+int sdm_reset_counters(sdm_ctx* c) {
+  if (!c) return -1;
+  std::memset(&c->cnt, 0, sizeof(c->cnt));
+  if (hipMemset(c->dcnt, 0, 2 * sizeof(unsigned long long)) != hipSuccess) return fail(c, "reset counters");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// state.h -- device layout of the scene.
+//
+// Bodies and interactions are structure-of-arrays in HBM.  A vector attribute with k
+// components occupies k consecutive planes of length N (bodies) or capacity (interactions):
+// component c of element i is at ptr[c * stride + i], so a wavefront reading one component
+// of 64 consecutive elements issues one coalesced 512-byte load.
+#pragma once
+#include <stdint.h>
+
+// Body attributes: (type, member, components, attribute name).  Names follow the reference
+// attributes (core/State.hpp, pkg/dem/Superquadrics.hpp, core/Bound.hpp).
+#define SDM_BODY_FIELDS(X)                               \
+  X(double, pos, 3, "state.pos")                         \
+  X(double, ori, 4, "state.ori")                         \
+  X(double, vel, 3, "state.vel")                         \
+  X(double, angVel, 3, "state.angVel")                   \
+  X(double, angMom, 3, "state.angMom")                   \
+  X(double, mass, 1, "state.mass")                       \
+  X(double, inertia, 3, "state.inertia")                 \
+  X(int32_t, blocked, 1, "state.blockedDOFs")            \
+  X(int32_t, damped, 1, "state.isDamped")                \
+  X(int32_t, kind, 1, "shape.kind")                      \
+  X(double, rxyz, 3, "shape.rxyz")                       \
+  X(double, eps, 2, "shape.eps")                         \
+  X(double, rmax, 1, "shape.r_max")                      \
+  X(int32_t, isSphere, 1, "shape.isSphere")              \
+  X(int32_t, wallAxis, 1, "shape.wallAxis")              \
+  X(int32_t, wallSense, 1, "shape.wallSense")            \
+  X(double, rotL, 9, "shape.rot_mat2local")              \
+  X(double, rotG, 9, "shape.rot_mat2global")             \
+  X(int32_t, material, 1, "body.material")               \
+  X(int32_t, hasBound, 1, "bound.exists")                \
+  X(double, bmin, 3, "bound.min")                        \
+  X(double, bmax, 3, "bound.max")                        \
+  X(double, refPos, 3, "bound.refPos")                   \
+  X(double, sweep, 1, "bound.sweepLength")               \
+  X(int32_t, lastUpd, 1, "bound.lastUpdateIter")         \
+  X(double, force, 3, "forces.force")                    \
+  X(double, torque, 3, "forces.torque")
+
+// Interaction attributes (core/Interaction.hpp, SuperquadricsGeom(2), SuperquadricsPhys,
+// SuperquadricsHertzMindlinPhys), plus the per-contact scatter and solver diagnostics.
+#define SDM_PAIR_FIELDS(X)                                   \
+  X(int32_t, id1, 1, "intr.id1")                             \
+  X(int32_t, id2, 1, "intr.id2")                             \
+  SDM_PAIR_FIELDS_NOIDS(X)
+
+#define SDM_PAIR_FIELDS_NOIDS(X)                             \
+  X(int32_t, hasGeom, 1, "intr.hasGeom")                     \
+  X(int32_t, hasPhys, 1, "intr.hasPhys")                     \
+  X(int32_t, functorSet, 1, "intr.functorSet")               \
+  X(int32_t, geomExists, 1, "intr.geomExists")               \
+  X(int64_t, iterMadeReal, 1, "intr.iterMadeReal")           \
+  X(double, depth, 1, "geom.PenetrationDepth")               \
+  X(double, ang, 2, "geom.contactAngle")                     \
+  X(double, cpt, 3, "geom.contactPoint")                     \
+  X(double, shearInc, 3, "geom.shearInc")                    \
+  X(double, relVn, 3, "geom.relativeVn")                     \
+  X(double, relVs, 3, "geom.relativeVs")                     \
+  X(double, normal, 3, "geom.normal")                        \
+  X(double, twist, 3, "geom.twist_axis")                     \
+  X(double, orth, 3, "geom.orthonormal_axis")                \
+  X(double, point1, 3, "geom.point1")                        \
+  X(double, point2, 3, "geom.point2")                        \
+  X(int32_t, isShearNew, 1, "geom.isShearNew")               \
+  X(double, curv1, 2, "geom.curvatures1")                    \
+  X(double, curv2, 2, "geom.curvatures2")                    \
+  X(double, curvAngle, 1, "geom.curvatureAngle")             \
+  X(double, alpha, 1, "geom.alpha")                          \
+  X(double, hK, 1, "geom.K")                                 \
+  X(double, hE, 1, "geom.E")                                 \
+  X(double, curvSum, 1, "geom.curvatures_sum")               \
+  X(int32_t, gSphere, 1, "geom.isSphere")                    \
+  X(double, kn, 1, "phys.kn")                                \
+  X(double, ks, 1, "phys.ks")                                \
+  X(double, tanPhi, 1, "phys.tangensOfFrictionAngle")        \
+  X(double, betan, 1, "phys.betan")                          \
+  X(double, betas, 1, "phys.betas")                          \
+  X(double, fn, 3, "phys.normalForce")                       \
+  X(double, fs, 3, "phys.shearForce")                        \
+  X(double, nuab, 1, "phys.nuab")                            \
+  X(double, Gab, 1, "phys.Gab")                              \
+  X(double, mu1, 1, "phys.mu1")                              \
+  X(double, mu2, 1, "phys.mu2")                              \
+  X(double, cF, 3, "contact.force")                          \
+  X(double, cT1, 3, "contact.torque1")                       \
+  X(double, cT2, 3, "contact.torque2")                       \
+  X(int32_t, active, 1, "contact.active")                    \
+  X(int32_t, nEval, 1, "diag.nEval")                         \
+  X(int32_t, stop, 1, "diag.stop")
+
+struct BodyArrays {
+  int32_t n;
+#define SDM_DECL(T, m, k, name) T* m;
+  SDM_BODY_FIELDS(SDM_DECL)
+#undef SDM_DECL
+};
+
+struct PairArrays {
+  int64_t cap;  // plane stride
+#define SDM_DECL(T, m, k, name) T* m;
+  SDM_PAIR_FIELDS(SDM_DECL)
+#undef SDM_DECL
+};
+
+struct MaterialDev {
+  double density, Kn, Ks, frictionAngle, betan, betas, G, nu;
+};
