@@ -1,0 +1,164 @@
+"""Benchmark: particle-steps/s of the superellipsoid DEM step at 200k particles on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 200,000 superellipsoids,
+semi-axes U[0.8,1.2] mm, exponents U[0.6,1.4], density 2650 kg/m^3, deposited bed in a
+120x120x200 mm box of six walls, Hertz-Mindlin law (Ig2_*_SuperquadricsGeom2 / Nelder-Mead,
+Ip2_SuperquadricsMat2_*, SuperquadricsLaw2; E=7e7 Pa, nu=0.3, mu=0.5), gravity, local damping
+0.3, dt = 1e-6 s.  One step = ForceResetter + InsertionSortCollider (when its stride test
+fires) + InteractionLoop + NewtonIntegrator.
+
+Procedure: generate the seeded packing, run W warm-up steps in FP64 (these also let the
+deposited bed load its contacts), snapshot the state, then time K steps of the all-FP64 form
+and K steps of the mixed FP32/FP64 form, each from that same snapshot.  The CPU oracle (the
+restatement of the reference's OpenMP loop) runs a bounded sample from the same snapshot on
+the host cores.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_EQ_PER_EVAL = 392  # pow=8, sin/cos=4, atan2=16, div=4, add/mul=1 over one Disfun evaluation
+PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_run(ctx, steps):
+    ctx.reset_counters()
+    ctx.set_timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    return wall, ctx.counters(), ctx.timers()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=200000)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from sudodem_amd import SDM_FP64, SDM_MIXED, configs, load_hip
+    from sudodem_amd._native import Context
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    lib = load_hip()
+    t0 = time.perf_counter()
+    cfg = configs.config1(n_particles=args.n)
+    log("generated %d bodies in %.1fs (solids fraction %.3f)" % (cfg.n, time.perf_counter() - t0,
+                                                                  cfg.solids_fraction))
+    base = cfg.open(lib, precision=SDM_FP64)
+    t0 = time.perf_counter()
+    base.step(args.warmup)
+    base.sync()
+    log("warm-up %d steps in %.1fs" % (args.warmup, time.perf_counter() - t0))
+    snap = base.snapshot()
+    base.close()
+    depth = snap["geom.PenetrationDepth"]
+    n_contacts = int((depth > 0).sum())
+    n_pairs = len(depth)
+
+    results = {}
+    for name, prec in (("fp64", SDM_FP64), ("mixed", SDM_MIXED)):
+        ctx = Context(lib, cfg.params(prec), cfg.n, [m._native() for m in cfg.materials])
+        ctx.restore(snap)
+        ctx.step(2)  # first launches (queue allocation) outside the timed region
+        ctx.restore(snap)
+        wall, cnt, tm = timed_run(ctx, args.steps)
+        results[name] = (wall, cnt, tm)
+        log("%s: %.3f ms/step, solver %.3f ms/step, %d collider runs" % (
+            name, wall / args.steps * 1e3, tm["solver_ms"] / args.steps, cnt["collider_runs"]))
+        if name == "mixed":
+            mixed_state = {k: ctx.get(k) for k in ("geom.PenetrationDepth",)}
+        ctx.close()
+
+    N = cfg.n - 6
+    K = args.steps
+    w64, c64, t64 = results["fp64"]
+    wmx, cmx, tmx = results["mixed"]
+    pss64 = N * K / w64
+    pssmx = N * K / wmx
+    # dominant kernel: the Nelder-Mead solver of the mixed form
+    solver_s = tmx["solver_ms"] * 1e-3
+    achieved = tmx["solver_evals"] * FLOP_EQ_PER_EVAL / solver_s if solver_s > 0 else 0.0
+    roof = {"bound": "valu", "kernel": "k_solve_nm<float>", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+            "flop_eq_per_eval": FLOP_EQ_PER_EVAL, "evals_per_step": tmx["solver_evals"] / K,
+            "kernel_ms_per_step": tmx["solver_ms"] / K}
+    solver64_s = t64["solver_ms"] * 1e-3
+    ach64 = t64["solver_evals"] * FLOP_EQ_PER_EVAL / solver64_s if solver64_s > 0 else 0.0
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import load_oracle
+        orc = load_oracle()
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        cctx = Context(orc, cfg.params(SDM_FP64), cfg.n, [m._native() for m in cfg.materials])
+        cctx.restore(snap)
+        orc.check(orc.set_threads(cctx.ptr, threads), cctx.ptr)
+        t0 = time.perf_counter()
+        cctx.step(args.cpu_steps)
+        cw = time.perf_counter() - t0
+        cctx.close()
+        cpu = {"value": N * args.cpu_steps / cw, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": "%d steps of the same 200k-particle snapshot, FP64 oracle (restatement of the "
+                         "reference loop), OpenMP %d threads, %.1fs wall" % (args.cpu_steps, threads, cw)}
+        log("cpu baseline: %.0f particle-steps/s" % cpu["value"])
+
+    out = {
+        "metric": "particle-steps/s at 200k superellipsoids: FP32/FP64 mix vs all-FP64 GPU",
+        "value": pssmx,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": wmx / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "200000 superellipsoids, Hertz-Mindlin (Nelder-Mead Geom2 + Law2), deposited bed "
+                               "in a 120x120x200 mm box, dt=1e-6 s", "n_particles": N, "precision": "mixed",
+                   "parallelism": "single GPU"},
+        "fp64": {"value": pss64, "ms_per_step": w64 / K * 1e3, "solver_ms_per_step": t64["solver_ms"] / K,
+                 "solver_achieved_tflops_eq": ach64 / 1e12, "solver_frac_of_fp64_peak": ach64 / PEAK_FP64},
+        "mixed_over_fp64": pssmx / pss64,
+        "contact_evals_per_s": cmx["contact_evals"] / wmx,
+        "dist_evals_per_s": cmx["dist_evals"] / wmx,
+        "interactions": n_pairs,
+        "contacts": n_contacts,
+        "phases_ms_per_step": {k: tmx[k] / K for k in ("collider_ms", "interactions_ms", "newton_ms", "solver_ms")},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -116,6 +116,18 @@ int sdm_sync(sdm_ctx* ctx);
 int sdm_get_counters(sdm_ctx* ctx, sdm_counters* out);
 int sdm_reset_counters(sdm_ctx* ctx);
 
+/* Per-phase device timers (HIP events recorded on the context's stream around each phase),
+ * the equivalent of the reference's per-engine TimingInfo (core/Timing.hpp:5-23,
+ * core/Scene.cpp:90-97).  Off by default; sdm_set_timing(ctx, 1) resets and starts them.
+ * sdm_get_timers fills 8 values summed since then:
+ *   out[0] collider [ms]            out[1] interaction loop [ms]   out[2] Newton + force sums [ms]
+ *   out[3] timed steps              out[4] narrowphase solver kernel [ms]
+ *   out[5] distance evaluations of the solver kernel
+ *   out[6] distance evaluations of the restart / refinement kernels
+ *   out[7] interactions handed to the solver kernel (summed over the timed steps) */
+int sdm_set_timing(sdm_ctx* ctx, int32_t on);
+int sdm_get_timers(sdm_ctx* ctx, double out[8]);
+
 #ifdef __cplusplus
 }
 #endif

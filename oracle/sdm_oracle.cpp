@@ -1668,6 +1668,8 @@ const Field kFields[] = {
     {"collider.fastestBodyMaxDist", S_SCALAR, SDM_F64, 1, 3, true},
     {"newton.maxVelocitySq", S_SCALAR, SDM_F64, 1, 4, true}, {"collider.numReinit", S_SCALAR, SDM_I32, 1, 5, true},
     {"collider.bounds", S_BOUNDS, SDM_I32, 1, 0, true},
+    {"scene.dt", S_SCALAR, SDM_F64, 1, 6, true}, {"newton.damping", S_SCALAR, SDM_F64, 1, 7, true},
+    {"newton.gravity", S_SCALAR, SDM_F64, 3, 8, true},
 };
 #undef BF
 #undef PF
@@ -1683,7 +1685,7 @@ int64_t field_count(sdm_ctx* c, const Field* f) {
   switch (f->scope) {
     case S_BODY: return (int64_t)c->bodies.size() * f->comps;
     case S_PAIR: return (int64_t)c->intrs.size() * f->comps;
-    case S_SCALAR: return 1;
+    case S_SCALAR: return f->comps;
     case S_BOUNDS: return (int64_t)(c->BB[0].size() * 3);
   }
   return 0;
@@ -1786,6 +1788,9 @@ int sdmcpu_get(sdm_ctx* c, const char* name, void* dst, int64_t count) {
       case 3: *(double*)dst = c->fastestBodyMaxDist; break;
       case 4: *(double*)dst = c->maxVelocitySq; break;
       case 5: *(int32_t*)dst = c->numReinit; break;
+      case 6: *(double*)dst = c->prm.dt; break;
+      case 7: *(double*)dst = c->prm.damping; break;
+      case 8: std::memcpy(dst, c->prm.gravity, 3 * sizeof(double)); break;
     }
   } else {
     int32_t* o = (int32_t*)dst;
@@ -1837,6 +1842,9 @@ int sdmcpu_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
       case 3: c->fastestBodyMaxDist = *(const double*)src; break;
       case 4: c->maxVelocitySq = *(const double*)src; break;
       case 5: c->numReinit = *(const int32_t*)src; break;
+      case 6: c->prm.dt = *(const double*)src; break;
+      case 7: c->prm.damping = *(const double*)src; break;
+      case 8: std::memcpy(c->prm.gravity, src, 3 * sizeof(double)); break;
     }
   }
   return 0;
@@ -1860,6 +1868,13 @@ int sdmcpu_get_counters(sdm_ctx* c, sdm_counters* out) {
 int sdmcpu_reset_counters(sdm_ctx* c) {
   if (!c) return -1;
   std::memset(&c->cnt, 0, sizeof(c->cnt));
+  return 0;
+}
+
+int sdmcpu_set_timing(sdm_ctx* c, int32_t) { return c ? 0 : -1; }
+int sdmcpu_get_timers(sdm_ctx* c, double out[8]) {
+  if (!c || !out) return -1;
+  for (int k = 0; k < 8; k++) out[k] = 0;
   return 0;
 }
 

@@ -84,6 +84,8 @@ class SdmLibrary:
         self.sync = fn("sync", ctypes.c_int, [c_ctx])
         self.get_counters = fn("get_counters", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_counters)])
         self.reset_counters = fn("reset_counters", ctypes.c_int, [c_ctx])
+        self.set_timing = fn("set_timing", ctypes.c_int, [c_ctx, ctypes.c_int32])
+        self.get_timers = fn("get_timers", ctypes.c_int, [c_ctx, ctypes.POINTER(ctypes.c_double)])
 
     def fields(self):
         return self.field_list().decode().split()
@@ -155,6 +157,32 @@ class Context:
 
     def reset_counters(self):
         self.lib.check(self.lib.reset_counters(self.ptr), self.ptr)
+
+    def set_timing(self, on=True):
+        self.lib.check(self.lib.set_timing(self.ptr, 1 if on else 0), self.ptr)
+
+    def timers(self):
+        """Per-phase device times [ms] and solver evaluation counts since set_timing(True)."""
+        out = (ctypes.c_double * 8)()
+        self.lib.check(self.lib.get_timers(self.ptr, out), self.ptr)
+        return {"collider_ms": out[0], "interactions_ms": out[1], "newton_ms": out[2], "steps": int(out[3]),
+                "solver_ms": out[4], "solver_evals": int(out[5]), "extra_evals": int(out[6]),
+                "queued": int(out[7])}
+
+    def snapshot(self):
+        """Complete state (bodies, interactions, collider and integrator flags) as arrays."""
+        return {name: self.get(name) for name in self.lib.fields()
+                if not (self.info(name)[1] == 0 and name == "collider.bounds")}
+
+    def restore(self, snap):
+        """Load a snapshot taken from any implementation of the ABI."""
+        self.set("intr.id1", snap["intr.id1"])
+        for name, val in snap.items():
+            if name in ("intr.id1",) or name.startswith("contact.") or name.startswith("diag."):
+                continue
+            if name.startswith("forces."):
+                continue
+            self.set(name, val)
 
 
 _HIP = None

@@ -231,17 +231,63 @@ def _separate(rng, pos, rxyz, eps, quat, lo, hi, gap):
     raise RuntimeError("could not separate the initial packing")
 
 
-def config1(n_particles=200000, seed_extra=0, overlap=2e-6):
+def _deposit(pos, rxyz, eps, R, lo, overlap, iters=6):
+    """Lower the particles layer by layer (increasing height) until each rests on the first
+    particle below it that its support reaches along the line of centres (overlap `overlap`),
+    or on the floor.  Positions are changed along z only."""
+    from scipy.spatial import cKDTree
+    pos = pos.copy()
+    n = len(pos)
+    ez = np.zeros((n, 3))
+    ez[:, 2] = 1.0
+    hdown = support_h(rxyz, eps, R, -ez)
+    zkey = np.round(pos[:, 2] / 1e-4).astype(np.int64)
+    layers = np.unique(zkey)
+    placed = np.zeros(n, bool)
+    for li, lz in enumerate(layers):
+        idx = np.nonzero(zkey == lz)[0]
+        floor_z = lo[2] + hdown[idx] - overlap
+        if li == 0:
+            pos[idx, 2] = floor_z
+            placed[idx] = True
+            continue
+        below = np.nonzero(placed)[0]
+        tree = cKDTree(pos[below, :2])
+        nb = tree.query_ball_point(pos[idx, :2], r=3.0e-3)
+        ii = np.concatenate([np.full(len(b), k) for k, b in enumerate(nb)]) if len(idx) else np.zeros(0, int)
+        jj = below[np.concatenate([np.asarray(b, int) for b in nb])] if len(idx) else np.zeros(0, int)
+        for _ in range(iters):
+            pi = idx[ii]
+            d = pos[pi] - pos[jj]
+            L = np.linalg.norm(d, axis=1)
+            u = d / L[:, None]
+            o = support_h(rxyz[pi], eps[pi], R[pi], -u) + support_h(rxyz[jj], eps[jj], R[jj], u) - L
+            # lowering by dz shortens the centre distance by about dz*u_z
+            uz = np.maximum(u[:, 2], 0.2)
+            dz = (overlap - o) / uz
+            dz[u[:, 2] < 0.2] = np.inf
+            best = np.full(len(idx), np.inf)
+            np.minimum.at(best, ii, dz)
+            cur = pos[idx, 2]
+            target = np.maximum(cur - np.where(np.isfinite(best), best, cur - floor_z), floor_z)
+            pos[idx, 2] = target
+        placed[idx] = True
+    return pos
+
+
+def config1(n_particles=200000, seed_extra=0, overlap=2e-6, spacing=2.0e-3, deposit=True):
     """200,000 superellipsoids in a 120x120x200 mm box, Hertz-Mindlin law.
 
     Densification: centres on a face-centred cubic lattice of nearest-neighbour distance
     2.0 mm filling the bottom of the box; each particle's semi-axes are scaled by the largest
     factor <= 1 that keeps its centre-line overlap with every lattice neighbour and wall
-    below `overlap`.  The solids fraction of the resulting bed is reported by the generator."""
+    below `overlap`; then the lattice layers are deposited bottom-up (_deposit), so that
+    every particle rests on the floor or on a particle below.  The solids fraction of the
+    resulting bed is reported by the generator."""
     rng = rng_for(1, seed_extra)
     box_lo = np.zeros(3)
     box_hi = np.array([120e-3, 120e-3, 200e-3])
-    a = 2.0e-3
+    a = spacing
     cc = a * math.sqrt(2.0)
     nx = int((box_hi[0] - 1.6e-3) / cc)
     ny = int((box_hi[1] - 1.6e-3) / cc)
@@ -273,6 +319,21 @@ def config1(n_particles=200000, seed_extra=0, overlap=2e-6):
         hm = support_h(rxyz, eps, R, -e)
         scale = np.minimum(scale, np.minimum((box_hi[axis] - pos[:, axis]) / hp, (pos[:, axis] - box_lo[axis]) / hm))
     rxyz = rxyz * scale[:, None]
+    if deposit:
+        pos = _deposit(pos, rxyz, eps, R, box_lo, overlap)
+        # the deposition moves neighbours of one layer by different amounts: bound the
+        # centre-line overlaps again on the final positions
+        ii, jj = _neighbour_pairs(pos, a * 1.75)
+        d = pos[jj] - pos[ii]
+        L = np.linalg.norm(d, axis=1)
+        u = d / L[:, None]
+        hs = support_h(rxyz[ii], eps[ii], R[ii], u) + support_h(rxyz[jj], eps[jj], R[jj], -u)
+        lim = np.minimum(1.0, (L + 2.5 * overlap) / hs)
+        s2 = np.ones(n_particles)
+        np.minimum.at(s2, ii, lim)
+        np.minimum.at(s2, jj, lim)
+        rxyz = rxyz * s2[:, None]
+        scale = scale * s2
     p_mat = SuperquadricsMat2(label="sq", G=7e7 / (2 * (1 + 0.3)), nu=0.3, frictionAngle=math.atan(0.5),
                               density=2650.0, Kn=1e5, Ks=7e4)
     w_mat = SuperquadricsMat2(label="wall", G=7e7 / (2 * (1 + 0.3)), nu=0.3, frictionAngle=math.atan(0.5),

@@ -58,10 +58,86 @@ __global__ __launch_bounds__(256) void k_bounds(BodyArrays B, int64_t iter, doub
   }
   st3(B.bmin, i, N, mn);
   st3(B.bmax, i, N, mx);
+  double* gr = B.geo + i * GEO_STRIDE;
+  gr[GEO_BMIN] = mn.x; gr[GEO_BMIN + 1] = mn.y; gr[GEO_BMIN + 2] = mn.z;
+  gr[GEO_BMAX] = mx.x; gr[GEO_BMAX + 1] = mx.y; gr[GEO_BMAX + 2] = mx.z;
   st3(B.refPos, i, N, pos);
   B.sweep[i] = sweep;
   B.lastUpd[i] = (int32_t)iter;
   B.hasBound[i] = 1;
+}
+
+// Rebuild the packed records from the attribute arrays (after the host wrote attributes).
+__global__ __launch_bounds__(256) void k_geo_pack(BodyArrays B) {
+  const int64_t N = B.n;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double* g = B.geo + i * GEO_STRIDE;
+  for (int k = 0; k < 3; k++) g[GEO_POS + k] = B.pos[k * N + i];
+  g[GEO_RMAX] = B.rmax[i];
+  for (int k = 0; k < 9; k++) g[GEO_G + k] = B.rotG[k * N + i];
+  for (int k = 0; k < 3; k++) g[GEO_R + k] = B.rxyz[k * N + i];
+  for (int k = 0; k < 2; k++) g[GEO_E + k] = B.eps[k * N + i];
+  for (int k = 0; k < 3; k++) g[GEO_BMIN + k] = B.bmin[k * N + i];
+  for (int k = 0; k < 3; k++) g[GEO_BMAX + k] = B.bmax[k * N + i];
+}
+
+// Per-body force and torque: the contributions of the body's interactions summed in
+// interaction order, i.e. in the order of the reference's serial addForce/addTorque calls
+// (core/ForceContainer.hpp:218-319), which makes the sums bitwise reproducible.  A group of
+// 16 lanes serves one body: the lanes read 16 list entries at a time (coalesced), and the
+// active ones are added in entry order through lane shuffles.
+__global__ __launch_bounds__(256) void k_force_sum(BodyArrays B, PairArrays P, const int32_t* csrOff,
+                                                   const int32_t* csrPair) {
+  const int64_t N = B.n;
+  const int64_t S = P.cap;
+  const int lane = threadIdx.x & 63;
+  const int gl = lane & 15;                 // lane in the group
+  const int gbase = lane & ~15;             // first lane of the group in the wave
+  const int64_t b = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const bool valid = b < N;
+  const int e0 = valid ? csrOff[b] : 0, e1 = valid ? csrOff[b + 1] : 0;
+  V3<double> f = mk3(0.0, 0.0, 0.0), m = f;
+  // every lane of the wave runs the same number of chunks so that shuffles stay convergent
+  int chunks = (e1 - e0 + 15) >> 4;
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(chunks, off);
+    chunks = o > chunks ? o : chunks;
+  }
+  for (int c = 0; c < chunks; c++) {
+    const int e = e0 + c * 16 + gl;
+    bool act = false;
+    V3<double> F = mk3(0.0, 0.0, 0.0), T = F;
+    if (e < e1) {
+      const int p = csrPair[e];
+      if (P.active[p]) {
+        act = true;
+        F = ld3(P.cF, p, S);
+        if (P.id1[p] == (int)b) {
+          T = ld3(P.cT1, p, S);
+        } else {
+          F = -F;
+          T = ld3(P.cT2, p, S);
+        }
+      }
+    }
+    // positions (0..15) active in any group of the wave; an inactive lane contributes +0,
+    // which leaves a sum unchanged, so the groups can share one loop over positions
+    uint64_t m64 = __ballot(act);
+    uint32_t pos = 0;
+    for (int g = 0; g < 64; g += 16) pos |= (uint32_t)(m64 >> g) & 0xFFFFu;
+    while (pos) {
+      const int k = __ffs(pos) - 1;
+      pos &= pos - 1;
+      const int src = gbase + k;
+      f = f + mk3(__shfl(F.x, src), __shfl(F.y, src), __shfl(F.z, src));
+      m = m + mk3(__shfl(T.x, src), __shfl(T.y, src), __shfl(T.z, src));
+    }
+  }
+  if (valid && gl == 0) {
+    st3(B.force, b, N, f);
+    st3(B.torque, b, N, m);
+  }
 }
 
 struct NewtonArgs {
@@ -98,22 +174,8 @@ __global__ __launch_bounds__(256) void k_newton(NewtonArgs A) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int code = 0;
   if (i < N) {
-    V3<double> f = mk3(0.0, 0.0, 0.0), m = f;
-    const int e0 = A.csrOff[i], e1 = A.csrOff[i + 1];
-    for (int e = e0; e < e1; e++) {
-      const int p = A.csrPair[e];
-      if (!A.P.active[p]) continue;
-      const V3<double> F = ld3(A.P.cF, p, S);
-      if (A.P.id1[p] == (int)i) {
-        f = f + F;
-        m = m + ld3(A.P.cT1, p, S);
-      } else {
-        f = f + (-F);
-        m = m + ld3(A.P.cT2, p, S);
-      }
-    }
-    st3(B.force, i, N, f);
-    st3(B.torque, i, N, m);
+    (void)S;
+    V3<double> f = ld3(B.force, i, N), m = ld3(B.torque, i, N);
     const double dt = A.dt, damping = A.damping;
     const unsigned blocked = (unsigned)B.blocked[i];
     const bool isSQ = B.kind[i] == SDM_KIND_SQ;
@@ -190,10 +252,12 @@ __global__ __launch_bounds__(256) void k_newton(NewtonArgs A) {
       angVel = quat_apply(ori, w_b_half);
       ori = quat_normalized(ori);
       const M3<double> RL = quat_rot(quat_conj(ori)), RG = quat_rot(ori);
+      double* gr = B.geo + i * GEO_STRIDE;
 #pragma unroll
       for (int k = 0; k < 9; k++) {
         B.rotL[k * N + i] = RL.m[k];
         B.rotG[k * N + i] = RG.m[k];
+        gr[GEO_G + k] = RG.m[k];
       }
       st3(B.angMom, i, N, angMom);
     } else {
@@ -216,6 +280,8 @@ __global__ __launch_bounds__(256) void k_newton(NewtonArgs A) {
     V3<double> pos = ld3(B.pos, i, N);
     pos = pos + vel * dt;  // leapfrogTranslate (:306-317)
     st3(B.pos, i, N, pos);
+    double* gp = B.geo + i * GEO_STRIDE;
+    gp[GEO_POS] = pos.x; gp[GEO_POS + 1] = pos.y; gp[GEO_POS + 2] = pos.z;
     // saveMaximaDisplacement (:72-85)
     if (B.hasBound[i]) {
       const V3<double> d = pos - ld3(B.refPos, i, N);

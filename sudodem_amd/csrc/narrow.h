@@ -35,17 +35,26 @@ SDM_HD V3<T> get_surface(const Shape<T>& s, T ph0, T ph1) {
              sgn(s1) * s.rz * xpow(xfabs(s1), s.eps2));
 }
 
+// rot_mat2local * v.  The body caches rot_mat2local and rot_mat2global from the same
+// normalised quaternion (Superquadrics.hpp:86-91, NewtonIntegrator.cpp:426-427), and
+// toRotationMatrix of the conjugate is bitwise the transpose, so only rot_mat2global is read.
+template <typename T>
+SDM_HD V3<T> mulT(const M3<T>& G, V3<T> v) {
+  return mk3(G.m[0] * v.x + (G.m[3] * v.y + G.m[6] * v.z), G.m[1] * v.x + (G.m[4] * v.y + G.m[7] * v.z),
+             G.m[2] * v.x + (G.m[5] * v.y + G.m[8] * v.z));
+}
+
 // Support point of a body along the global direction dir (global frame, relative to its centre).
 template <typename T>
-SDM_HD V3<T> support(const Shape<T>& s, const M3<T>& L, const M3<T>& G, V3<T> dir) {
+SDM_HD V3<T> support(const Shape<T>& s, const M3<T>& G, V3<T> dir) {
   T ph0, ph1;
-  normal2phi(s, mul(L, dir), ph0, ph1);
+  normal2phi(s, mulT(G, dir), ph0, ph1);
   return mul(G, get_surface(s, ph0, ph1));
 }
 
 // isInside (Superquadrics.cpp:103-108); pg relative to the centre, global frame.
-SDM_HD bool is_inside(const Shape<double>& s, const M3<double>& L, V3<double> pg) {
-  V3<double> p = mul(L, pg);
+SDM_HD bool is_inside(const Shape<double>& s, const M3<double>& G, V3<double> pg) {
+  V3<double> p = mulT(G, pg);
   return 1 > pow(pow(fabs(p.x / s.rx), 2.0 / s.eps1) + pow(fabs(p.y / s.ry), 2.0 / s.eps1), s.eps1 / s.eps2) +
                  pow(fabs(p.z / s.rz), 2.0 / s.eps2);
 }
@@ -55,7 +64,7 @@ SDM_HD bool is_inside(const Shape<double>& s, const M3<double>& L, V3<double> pg
 template <typename T>
 struct PairGeo {
   Shape<T> s1, s2;
-  M3<T> L1, G1, L2, G2;
+  M3<T> G1, G2;
   V3<T> pos1, pos2;
 };
 
@@ -65,125 +74,16 @@ SDM_HD void lm_eval(const PairGeo<T>& g, T a0, T a1, V3<T>& p1, V3<T>& p2, V3<T>
                     int& nev) {
   nev++;
   const V3<T> contact = mk3(xcos(a0) * xcos(a1), xsin(a0) * xcos(a1), xsin(a1));
-  p1 = support(g.s1, g.L1, g.G1, contact) + g.pos1;
+  p1 = support(g.s1, g.G1, contact) + g.pos1;
   {
     T ph0, ph1;
-    V3<T> c2 = mul(g.L2, contact);
+    V3<T> c2 = mulT(g.G2, contact);
     normal2phi(g.s2, mk3(T(-1) * c2.x, T(-1) * c2.y, T(-1) * c2.z), ph0, ph1);
     p2 = mul(g.G2, get_surface(g.s2, ph0, ph1)) + g.pos2;
   }
   V3<T> d = p2 - p1;
   center = mk3(T(0.5) * (p1.x + p2.x), T(0.5) * (p1.y + p2.y), T(0.5) * (p1.z + p2.z));
   if (dot(contact, d) > T(0)) stop_flag = true;
-}
-
-template <typename T>
-struct LMResult {
-  T a0, a1;           // final parameters
-  V3<T> p1, p2, ctr;  // points of the last evaluation
-  V3<T> contact;      // direction set at the top of the last iteration
-  bool touching;
-  int stop;
-};
-
-// lmder2 (Superquadrics_Ig2.cpp:1056-1354) with opts = {1, 1e-15, 1e-3, 1e-20} and the
-// forward-difference Jacobian of JacfunctionApp (:1032-1054) whose step is the last increment.
-template <typename T>
-SDM_HD void lmder2(const PairGeo<T>& g, T& pa0, T& pa1, int itmax, LMResult<T>& R, int& nev) {
-  const T tau = T(1.0), eps1 = T(1e-15), eps2 = T(1e-3), eps3 = T(1e-20);
-  int k;
-  R.touching = true;
-  V3<T> p_f = mk3(T(0), T(0), T(0)), pDp_f = mk3(T(0), T(0), T(0));
-  T jTf0, jTf1;
-  T J00, J01, J10, J11, J20, J21;
-  T A00, A01, A11, dA0 = 0, dA1 = 0;
-  T Dp0 = T(1e-7), Dp1 = T(1e-7);
-  bool stop_flag = false;
-  T mu = T(0), tmp;
-  T p_L2, Dp_L2, dF, dL;
-  T p_fL2, pDp_fL2;
-  int nu = 2, nu2, stop = 0;
-  V3<T> p1, p2, ctr;
-  lm_eval(g, pa0, pa1, p1, p2, ctr, stop_flag, nev);
-  p_f = p2 - p1;
-  p_fL2 = norm(p_f);
-  if (!isfinite(p_fL2)) stop = 7;
-  for (k = 0; k < itmax && !stop; ++k) {
-    if (p_fL2 <= eps3) { stop = 6; break; }
-    if (stop_flag) { stop = 9; R.touching = false; break; }
-    {
-      R.contact = mk3(xcos(pa0) * xcos(pa1), xsin(pa0) * xcos(pa1), xsin(pa1));
-      T angle = dot(p_f, R.contact) / (norm(p_f) * norm(R.contact));
-      if (xfabs(angle) > T(0.95)) { stop = 8; break; }
-    }
-    {  // JacfunctionApp
-      V3<T> f0, f1, f2;
-      lm_eval(g, pa0, pa1, p1, p2, ctr, stop_flag, nev);
-      f0 = p2 - p1;
-      lm_eval(g, pa0 + Dp0, pa1, p1, p2, ctr, stop_flag, nev);
-      f1 = p2 - p1;
-      lm_eval(g, pa0, pa1 + Dp1, p1, p2, ctr, stop_flag, nev);
-      f2 = p2 - p1;
-      J00 = (f1.x - f0.x) / Dp0; J10 = (f1.y - f0.y) / Dp0; J20 = (f1.z - f0.z) / Dp0;
-      J01 = (f2.x - f0.x) / Dp1; J11 = (f2.y - f0.y) / Dp1; J21 = (f2.z - f0.z) / Dp1;
-    }
-    A00 = J00 * J00 + (J10 * J10 + J20 * J20);
-    A01 = J00 * J01 + (J10 * J11 + J20 * J21);
-    A11 = J01 * J01 + (J11 * J11 + J21 * J21);
-    const V3<T> mf = mk3(T(-1.0) * p_f.x, T(-1.0) * p_f.y, T(-1.0) * p_f.z);
-    jTf0 = J00 * mf.x + (J10 * mf.y + J20 * mf.z);
-    jTf1 = J01 * mf.x + (J11 * mf.y + J21 * mf.z);
-    T jTf_inf = (xfabs(jTf0) > xfabs(jTf1) ? xfabs(jTf0) : xfabs(jTf1));
-    p_L2 = dsqrt(pa0 * pa0 + pa1 * pa1);
-    if (jTf_inf <= eps1) { stop = 1; break; }
-    if (k == 0) {
-      tmp = (A00 > A11 ? A00 : A11);
-      mu = tau * tmp;
-    }
-    dA0 = A00;
-    dA1 = A11;
-    while (1) {
-      A00 += mu;
-      A11 += mu;
-      T n_a = A00 * A11 - A01 * A01;
-      Dp0 = (jTf0 * A11 - jTf1 * A01) / n_a;
-      Dp1 = (jTf1 * A00 - jTf0 * A01) / n_a;
-      Dp_L2 = dsqrt(Dp0 * Dp0 + Dp1 * Dp1);
-      if (Dp_L2 < T(1e-5)) { stop = 2; break; }
-      const T q0 = pa0 + Dp0, q1 = pa1 + Dp1;
-      if (Dp_L2 >= (p_L2 + eps2) / (T(1E-12) * T(1E-12))) { stop = 4; break; }
-      lm_eval(g, q0, q1, p1, p2, ctr, stop_flag, nev);
-      pDp_f = p2 - p1;
-      pDp_fL2 = norm(pDp_f);
-      if (!isfinite(pDp_fL2)) { stop = 7; break; }
-      dL = Dp0 * (mu * Dp0 + jTf0) + Dp1 * (mu * Dp1 + jTf1);
-      dF = p_fL2 - pDp_fL2;
-      tmp = dF / dL;
-      if (tmp > T(0.0)) {
-        tmp = T(1.0) - xpow(T(2) * tmp - T(1.), T(3.));
-        mu = mu * ((tmp >= T(0.3333333333333334)) ? tmp : T(0.3333333333333334));
-        nu = 2;
-        pa0 = q0;
-        pa1 = q1;
-        p_f = pDp_f;
-        p_fL2 = pDp_fL2;
-        break;
-      }
-      mu *= T(nu);
-      nu2 = int(unsigned(nu) << 1);
-      if (nu2 <= nu) { stop = 5; break; }
-      nu = nu2;
-      A00 = dA0;
-      A11 = dA1;
-    }
-  }
-  if (k >= itmax) stop = 3;
-  R.stop = stop;
-  R.a0 = pa0;
-  R.a1 = pa1;
-  R.p1 = p1;
-  R.p2 = p2;
-  R.ctr = ctr;
 }
 
 // CRotationMat (Superquadrics_Ig2.cpp:295-319).
@@ -215,145 +115,16 @@ template <typename T>
 SDM_HD T nm_eval(const NMGeo<T>& n, T a0, T a1, V3<T>& p1, V3<T>& p2, bool& touching, int& nev) {
   nev++;
   const V3<T> contact = mul(n.gC, mk3(xcos(a0) * xcos(a1), xsin(a0) * xcos(a1), xsin(a1)));
-  p1 = support(n.g.s1, n.g.L1, n.g.G1, contact) + n.g.pos1;
+  p1 = support(n.g.s1, n.g.G1, contact) + n.g.pos1;
   {
     T ph0, ph1;
-    V3<T> c2 = mul(n.g.L2, contact);
+    V3<T> c2 = mulT(n.g.G2, contact);
     normal2phi(n.g.s2, mk3(T(-1) * c2.x, T(-1) * c2.y, T(-1) * c2.z), ph0, ph1);
     p2 = mul(n.g.G2, get_surface(n.g.s2, ph0, ph1)) + n.g.pos2;
   }
   V3<T> d = p2 - p1;
   if (dot(contact, d) > T(0)) touching = false;
   return norm(d);
-}
-
-template <typename T>
-struct NMResult {
-  T a0, a1;   // best vertex
-  T fmin;     // its distance (info[1])
-  V3<T> p1, p2;  // points of the last evaluation
-  int nfunc;
-  bool touching;
-  // the final simplex (for the double-precision refinement of the mixed form)
-  T v[3][2];
-  T y[3];
-};
-
-// amotry (Superquadrics_Ig2.cpp:1681-1699).
-template <typename T>
-SDM_HD T nm_amotry(const NMGeo<T>& n, T (&p)[3][2], T (&y)[3], T (&psum)[2], int ihi, T fac, V3<T>& pt1,
-                   V3<T>& pt2, bool& touching, int& nev) {
-  T fac1 = (T(1.0) - fac) / T(2.0);
-  T fac2 = fac1 - fac;
-  T t0 = psum[0] * fac1 - p[ihi][0] * fac2;
-  T t1 = psum[1] * fac1 - p[ihi][1] * fac2;
-  T ytry = nm_eval(n, t0, t1, pt1, pt2, touching, nev);
-  if (ytry < y[ihi]) {
-    y[ihi] = ytry;
-    psum[0] += t0 - p[ihi][0];
-    psum[1] += t1 - p[ihi][1];
-    p[ihi][0] = t0;
-    p[ihi][1] = t1;
-  }
-  return ytry;
-}
-
-// neldermead_simplex (Superquadrics_Ig2.cpp:1701-1875), ftol = 1e-3, NMAX = 5000.  The
-// reference leaves the evaluation counter uninitialised (:1713); it starts at 0 here.
-template <typename T>
-SDM_HD void neldermead(const NMGeo<T>& n, T a0, T a1, T del, NMResult<T>& R, int& nev) {
-  const T ftol = T(1e-3);
-  const int NMAX = 5000;
-  const T TINY = T(1.0e-10);
-  int nfunc = 0;
-  int ihi, ilo, inhi;
-  T psum[2], y[3], p[3][2];
-  V3<T> pt1 = mk3(T(0), T(0), T(0)), pt2 = pt1;
-  bool touching = true;
-  for (int i = 0; i < 3; i++) { p[i][0] = a0; p[i][1] = a1; }
-  p[1][0] += del;
-  p[2][1] += del;
-  R.fmin = T(0);
-  bool done = false;
-  for (int i = 0; i < 3 && !done; i++) {
-    y[i] = nm_eval(n, p[i][0], p[i][1], pt1, pt2, touching, nev);
-    nfunc++;
-    if (!touching) {
-      R.fmin = y[0];
-      done = true;
-    }
-  }
-  if (!done) {
-    psum[0] = p[0][0] + p[1][0] + p[2][0];
-    psum[1] = p[0][1] + p[1][1] + p[2][1];
-    for (;;) {
-      ilo = 0;
-      ihi = y[0] > y[1] ? (inhi = 1, 0) : (inhi = 0, 1);
-      for (int i = 0; i < 3; i++) {
-        if (y[i] <= y[ilo]) ilo = i;
-        if (y[i] > y[ihi]) {
-          inhi = ihi;
-          ihi = i;
-        } else if (y[i] > y[inhi] && i != ihi)
-          inhi = i;
-      }
-      T rtol = T(2.0) * xfabs(y[ihi] - y[ilo]) / (xfabs(y[ihi]) + xfabs(y[ilo]) + TINY);
-      if (rtol < ftol) {
-        T t = y[0]; y[0] = y[ilo]; y[ilo] = t;
-        T t0 = p[ilo][0], t1 = p[ilo][1];
-        p[ilo][0] = p[0][0]; p[ilo][1] = p[0][1];
-        p[0][0] = t0; p[0][1] = t1;
-        R.fmin = y[0];
-        break;
-      }
-      if (nfunc >= NMAX) { R.fmin = y[0]; break; }
-      T ytry = nm_amotry(n, p, y, psum, ihi, T(-1.0), pt1, pt2, touching, nev);
-      nfunc++;
-      if (!touching) { R.fmin = y[0]; break; }
-      if (ytry <= y[ilo]) {
-        ytry = nm_amotry(n, p, y, psum, ihi, T(2.0), pt1, pt2, touching, nev);
-        nfunc++;
-        if (!touching) { R.fmin = y[0]; break; }
-      } else if (ytry >= y[inhi]) {
-        T ysave = y[ihi];
-        ytry = nm_amotry(n, p, y, psum, ihi, T(0.5), pt1, pt2, touching, nev);
-        nfunc++;
-        if (!touching) { R.fmin = y[0]; break; }
-        if (ytry >= ysave) {
-          bool out = false;
-          for (int i = 0; i < 3; i++) {
-            if (i != ilo) {
-              psum[0] = T(0.5) * (p[i][0] + p[ilo][0]);
-              psum[1] = T(0.5) * (p[i][1] + p[ilo][1]);
-              p[i][0] = psum[0];
-              p[i][1] = psum[1];
-              y[i] = nm_eval(n, psum[0], psum[1], pt1, pt2, touching, nev);
-              nfunc++;
-              if (!touching) {
-                R.fmin = y[0];
-                out = true;
-                break;
-              }
-            }
-          }
-          if (out) break;
-          psum[0] = p[0][0] + p[1][0] + p[2][0];
-          psum[1] = p[0][1] + p[1][1] + p[2][1];
-        }
-      }
-    }
-  }
-  R.a0 = p[0][0];
-  R.a1 = p[0][1];
-  R.p1 = pt1;
-  R.p2 = pt2;
-  R.nfunc = nfunc;
-  R.touching = touching;
-  for (int i = 0; i < 3; i++) {
-    R.v[i][0] = p[i][0];
-    R.v[i][1] = p[i][1];
-    R.y[i] = y[i];
-  }
 }
 
 // Curvatures (Superquadrics.cpp:131-255).  Keeps the reference's expressions: K ignores the

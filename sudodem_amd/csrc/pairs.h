@@ -1,18 +1,23 @@
-// pairs.h -- one interaction per thread: Ig2 functor, Ip2 functor and contact law.
+// pairs.h -- the interaction loop: Ig2 functor, Ip2 functor and contact law per interaction.
 //
-// Device restatement of InteractionLoop::action (pkg/common/InteractionLoop.cpp:23-149)
-// for one interaction, with
+// Device restatement of InteractionLoop::action (pkg/common/InteractionLoop.cpp:23-149) with
 //   Ig2_Superquadrics_Superquadrics_SuperquadricsGeom::go   (Superquadrics_Ig2.cpp:1880-2317)
 //   Ig2_Superquadrics_Superquadrics_SuperquadricsGeom2::go  (Superquadrics_Ig2.cpp:2322-2517)
 //   Ig2_Wall_Superquadrics_SuperquadricsGeom(2)::go         (Superquadrics_Ig2.cpp:2522-2703)
 //   SuperquadricsGeom::precompute / rotate                 (Superquadrics.cpp:1030-1080)
 //   Ip2_SuperquadricsMat(2)_* ::go                         (Superquadrics.cpp:1086-1160)
 //   SuperquadricsLaw::go / SuperquadricsLaw2::go           (Superquadrics.cpp:1181-1557)
-// The force/torque of each contact is written to the interaction; the per-body sums are
-// formed later in interaction order by the integrator kernel, which reproduces the serial
-// ForceContainer summation order (core/ForceContainer.hpp:218-319) without atomics.
+// split in three kernels so that none carries the register footprint of all of them:
+//   k_pairs_prep   every interaction: functor dispatch, wall / sphere / bounding-sphere
+//                  cases completely; pairs that need an iterative solve are queued;
+//   (solve.h)      persistent solvers over the queue;
+//   k_pairs_finish queued pairs: solver outputs -> geometry, curvature and Hertz terms,
+//                  precompute, Ip2, law.
+// Each contact's force and torques are stored on the interaction; the per-body sums are
+// formed by the integrator kernel in interaction order, which is the order of the
+// reference's serial addForce/addTorque calls (core/ForceContainer.hpp:218-319).
 #pragma once
-#include "narrow.h"
+#include "solve.h"
 #include "state.h"
 
 enum { SDM_KIND_SQ = 0, SDM_KIND_WALL = 1 };
@@ -59,14 +64,24 @@ struct BodyView {
   double rmax;
 };
 
+__device__ __forceinline__ void load_bounds(const double* geo, int id, V3<double>& mn, V3<double>& mx) {
+  const double* g = geo + (int64_t)id * GEO_STRIDE + GEO_BMIN;
+  const double2* r = reinterpret_cast<const double2*>(g);
+  const double2 a = r[0], b = r[1], c = r[2];
+  mn = mk3(a.x, a.y, b.x);
+  mx = mk3(b.y, c.x, c.y);
+}
+
 __device__ __forceinline__ BodyView load_body(const BodyArrays& B, int id) {
   const int64_t N = B.n;
   BodyView b;
-  b.pos = ld3(B.pos, id, N);
+  const GeoRec r = load_rec(B.geo, id);
+  b.pos = r.pos;
+  b.s = r.s;
+  b.rmax = r.rmax;
+  load_bounds(B.geo, id, b.bmin, b.bmax);
   b.vel = ld3(B.vel, id, N);
   b.angVel = ld3(B.angVel, id, N);
-  b.bmin = ld3(B.bmin, id, N);
-  b.bmax = ld3(B.bmax, id, N);
   b.mass = B.mass[id];
   b.blocked = B.blocked[id];
   b.material = B.material[id];
@@ -74,13 +89,28 @@ __device__ __forceinline__ BodyView load_body(const BodyArrays& B, int id) {
   b.isSphere = B.isSphere[id];
   b.wallAxis = B.wallAxis[id];
   b.wallSense = B.wallSense[id];
-  b.s.rx = B.rxyz[id];
-  b.s.ry = B.rxyz[N + id];
-  b.s.rz = B.rxyz[2 * N + id];
-  b.s.eps1 = B.eps[id];
-  b.s.eps2 = B.eps[N + id];
-  b.rmax = B.rmax[id];
   return b;
+}
+
+// Previous geometry of an interaction (a fresh SuperquadricsGeom(2) is all zero, isShearNew).
+__device__ __forceinline__ void load_geom_prev(const PairArrays& P, int64_t p, bool hadGeom, GeomState& g) {
+  const int64_t S = P.cap;
+  g.normal = mk3(0.0, 0.0, 0.0);
+  g.cpt = g.normal; g.shearInc = g.normal; g.relVn = g.normal; g.relVs = g.normal;
+  g.twist = g.normal; g.orth = g.normal; g.point1 = g.normal; g.point2 = g.normal;
+  g.c10 = g.c11 = g.c20 = g.c21 = 0; g.curvAngle = g.alpha = g.K = g.E = g.curvSum = 0;
+  if (!hadGeom) {
+    g.depth = 0; g.a0 = 0; g.a1 = 0;
+    g.isShearNew = true;
+    g.gSphere = false;
+  } else {
+    g.depth = P.depth[p];
+    g.a0 = P.ang[p];
+    g.a1 = P.ang[S + p];
+    g.normal = ld3(P.normal, p, S);
+    g.gSphere = P.gSphere[p] != 0;
+    g.isShearNew = g.depth <= 0;  // Superquadrics_Ig2.cpp:1931, 2344, 2537
+  }
 }
 
 // SuperquadricsGeom::precompute (Superquadrics.cpp:1030-1050); g.normal is the previous normal.
@@ -104,157 +134,8 @@ __device__ __forceinline__ void precompute(GeomState& g, const BodyView& b1, con
   g.shearInc = g.relVs * dt;
 }
 
-// Outcome of a functor: EARLY = only depth/isShearNew (and maybe the angles) changed.
+// Outcome of a functor: EARLY = only depth/isShearNew changed; EARLY_ANG also the angles.
 enum { GEO_EARLY = 0, GEO_EARLY_ANG = 1, GEO_FULL = 2 };
-
-// Frame of the mixed form: everything relative to body 1's centre, in float.
-template <typename T>
-__device__ __forceinline__ PairGeo<T> make_geo(const BodyView& b1, const BodyView& b2, const M3<double>& L1,
-                                               const M3<double>& G1, const M3<double>& L2, const M3<double>& G2,
-                                               bool relative) {
-  PairGeo<T> g;
-  g.s1.rx = T(b1.s.rx); g.s1.ry = T(b1.s.ry); g.s1.rz = T(b1.s.rz); g.s1.eps1 = T(b1.s.eps1); g.s1.eps2 = T(b1.s.eps2);
-  g.s2.rx = T(b2.s.rx); g.s2.ry = T(b2.s.ry); g.s2.rz = T(b2.s.rz); g.s2.eps1 = T(b2.s.eps1); g.s2.eps2 = T(b2.s.eps2);
-  g.L1 = cast9<T>(L1); g.G1 = cast9<T>(G1); g.L2 = cast9<T>(L2); g.G2 = cast9<T>(G2);
-  if (relative) {
-    g.pos1 = mk3(T(0), T(0), T(0));
-    g.pos2 = cast3<T>(b2.pos - b1.pos);
-  } else {
-    g.pos1 = cast3<T>(b1.pos);
-    g.pos2 = cast3<T>(b2.pos);
-  }
-  return g;
-}
-
-// Ig2_Superquadrics_Superquadrics_SuperquadricsGeom::go, non-sphere branch after the
-// bounding-sphere test: LM solve with the reference's two restarts.
-template <int PREC>
-__device__ int lm_geometry(GeomState& g, const BodyView& b1, const BodyView& b2, const M3<double>& L1,
-                           const M3<double>& G1, const M3<double>& L2, const M3<double>& G2, double& para0,
-                           double& para1, V3<double>& contactNormal, int& nev, int& stopOut) {
-  const V3<double> position1 = b1.pos, position2 = b2.pos;
-  const PairGeo<double> gd = make_geo<double>(b1, b2, L1, G1, L2, G2, false);
-  LMResult<double> R;
-  R.contact = mk3(0.0, 0.0, 0.0);
-  if (PREC == 1) {
-    // Mixed form: iterate in float on the body-1 frame, then finish in double from the
-    // float solution (the double solver stops after its first angle test when converged).
-    const PairGeo<float> gf = make_geo<float>(b1, b2, L1, G1, L2, G2, true);
-    LMResult<float> Rf;
-    Rf.contact = mk3(0.f, 0.f, 0.f);
-    float f0 = float(para0), f1 = float(para1);
-    lmder2<float>(gf, f0, f1, 50, Rf, nev);
-    if (Rf.touching && isfinite(f0) && isfinite(f1) && Rf.stop != 7) {
-      para0 = f0;
-      para1 = f1;
-    }
-  }
-  lmder2<double>(gd, para0, para1, 50, R, nev);
-  int stop = R.stop;
-  if (isnan(para0) || isnan(para1) || 7 == stop) {
-    if (R.touching) {
-      V3<double> d0 = position2 - position1;
-      para0 = atan2(d0.y, d0.x);
-      para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
-      para0 += 1e-5;
-      para1 += 1e-5;
-      lmder2<double>(gd, para0, para1, 50, R, nev);
-      stop = R.stop + 100;
-    }
-  }
-  V3<double> p1 = R.p1, p2 = R.p2;
-  double pDepth = dot(p1 - p2, R.contact);
-  if (R.touching) {
-    if (10.0 * norm(p2 - p1) > b1.rmax) {
-      if (!(is_inside(b1.s, L1, p2 - position1)) || !(is_inside(b2.s, L2, p1 - position2))) {
-        V3<double> d0 = position2 - position1;
-        para0 = atan2(d0.y, d0.x);
-        para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
-        para0 += 1e-5;
-        para1 += 1e-5;
-        lmder2<double>(gd, para0, para1, 50, R, nev);
-        stop = R.stop + 200;
-        p1 = R.p1;
-        p2 = R.p2;
-        pDepth = dot(p1 - p2, R.contact);
-      }
-    }
-  }
-  stopOut = stop;
-  if (!R.touching) return 0;
-  g.point1 = p1;
-  g.point2 = p2;
-  g.cpt = R.ctr;
-  g.depth = pDepth;
-  contactNormal = normalized(R.contact);
-  return 1;
-}
-
-// Ig2_Superquadrics_Superquadrics_SuperquadricsGeom2::go, non-sphere branch after the
-// bounding-sphere test: NM solve, curvatures and Hertz geometry.
-template <int PREC>
-__device__ int nm_geometry(GeomState& g, const BodyView& b1, const BodyView& b2, const M3<double>& L1,
-                           const M3<double>& G1, const M3<double>& L2, const M3<double>& G2, double dels,
-                           double& para0, double& para1, V3<double>& contactNormal, int& nev, int& nfuncOut) {
-  NMGeo<double> nd;
-  nd.g = make_geo<double>(b1, b2, L1, G1, L2, G2, false);
-  nd.gC = crotation_mat(b1.pos, b2.pos);
-  NMResult<double> R;
-  if (PREC == 1) {
-    NMGeo<float> nf;
-    nf.g = make_geo<float>(b1, b2, L1, G1, L2, G2, true);
-    nf.gC = cast9<float>(nd.gC);
-    NMResult<float> Rf;
-    neldermead<float>(nf, float(para0), float(para1), float(dels), Rf, nev);
-    if (!Rf.touching) {
-      para0 = Rf.a0;
-      para1 = Rf.a1;
-      nfuncOut = Rf.nfunc;
-      return 0;
-    }
-    // Finish in double: restart the simplex at the float optimum with the warm-start size.
-    neldermead<double>(nd, double(Rf.a0), double(Rf.a1), 0.001, R, nev);
-    R.nfunc += Rf.nfunc;
-  } else {
-    neldermead<double>(nd, para0, para1, dels, R, nev);
-  }
-  para0 = R.a0;
-  para1 = R.a1;
-  nfuncOut = R.nfunc;
-  if (!R.touching) return 0;
-  const V3<double> p1 = R.p1, p2 = R.p2;
-  contactNormal = normalized(p1 - p2);
-  g.point1 = p1;
-  g.point2 = p2;
-  g.cpt = mk3(0.5 * (p1.x + p2.x), 0.5 * (p1.y + p2.y), 0.5 * (p1.z + p2.z));
-  g.depth = R.fmin;
-  // curvature direction convention of :2463-2465
-  V3<double> contact = mul(nd.gC, mk3(cos(para0) * sin(para1), sin(para0) * sin(para1), cos(para1)));
-  V3<double> c1 = mul(L1, contact), c2 = mul(L2, contact);
-  V3<double> dmax1 = mk3(0.0, 0.0, 0.0), dmax2 = dmax1, dmin1, dmin2;
-  bool f1, f2;
-  double ph0, ph1, k10, k11, k20, k21;
-  normal2phi(b1.s, c1, ph0, ph1);
-  sq_curvatures(b1.s, G1, ph0, ph1, k10, k11, dmax1, dmin1, f1);
-  normal2phi(b2.s, mk3(-1.0 * c2.x, -1.0 * c2.y, -1.0 * c2.z), ph0, ph1);
-  sq_curvatures(b2.s, G2, ph0, ph1, k20, k21, dmax2, dmin2, f2);
-  V3<double> cd1, cd2;
-  if (f1 && f2) {
-    cd1 = dmax1;
-    cd2 = dmax2;
-  } else {
-    cd1 = dmin1;
-    cd2 = dmin2;
-  }
-  V3<double> dir1 = mk3(cd1.x * contactNormal.x, cd1.y * contactNormal.y, cd1.z * contactNormal.z) - cd1;
-  V3<double> dir2 = mk3(cd2.x * contactNormal.x, cd2.y * contactNormal.y, cd2.z * contactNormal.z) - cd2;
-  double angle = acos(dot(dir1, dir2) / (norm(dir1) * norm(dir2)));
-  if (isnan(angle)) angle = 0.0;
-  g.c10 = k10; g.c11 = k11; g.c20 = k20; g.c21 = k21;
-  g.curvAngle = angle;
-  hertz_mindlin(k10, k11, k20, k21, angle, g.curvSum, g.alpha, g.K, g.E);
-  return 1;
-}
 
 struct PhysState {
   double kn, ks, tanPhi, betan, betas, nuab, Gab, mu1, mu2;
@@ -269,167 +150,19 @@ __device__ __forceinline__ V3<double> rotate_shear(const GeomState& g, V3<double
   return fs;
 }
 
-template <int LAW, int PREC>
-__device__ void process_pair(const StepArgs& A, int64_t p, unsigned long long& ce, unsigned long long& de) {
+// Ip2 (first time only), the law, and the write-back of one interaction.
+template <int LAW>
+__device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1, const BodyView& b2, int id1,
+                               GeomState& g, int outcome, bool hadGeom, bool hadPhys, int nev, int stopCode) {
   const PairArrays& P = A.P;
-  const BodyArrays& B = A.B;
   const int64_t S = P.cap;
-  const int64_t N = B.n;
-  P.active[p] = 0;
-  if (!P.geomExists[p]) return;
-  int id1 = P.id1[p], id2 = P.id2[p];
-  if (!P.functorSet[p]) {
-    // dispatch order (Wall,Superquadrics) / (Superquadrics,Superquadrics); swapOrder otherwise
-    const int k1 = B.kind[id1], k2 = B.kind[id2];
-    if (k1 == SDM_KIND_WALL && k2 == SDM_KIND_WALL) {
-      P.geomExists[p] = 0;
-      return;
-    }
-    P.functorSet[p] = 1;
-    if (k1 == SDM_KIND_SQ && k2 == SDM_KIND_WALL) {
-      int t = id1; id1 = id2; id2 = t;
-      P.id1[p] = id1;
-      P.id2[p] = id2;
-    }
-  }
-  const bool hadGeom = P.hasGeom[p] != 0;
-  const bool hadPhys = P.hasPhys[p] != 0;
-  const BodyView b1 = load_body(B, id1);
-  const BodyView b2 = load_body(B, id2);
-
-  // ---------------- geometry functor ----------------
-  GeomState g;
-  int outcome;
-  int nev = 0, stopCode = 0;
-  const bool isNew = !hadGeom;
-  if (isNew) {
-    // fresh SuperquadricsGeom(2): all zero, isShearNew = true
-    g.depth = 0; g.a0 = 0; g.a1 = 0;
-    g.normal = mk3(0.0, 0.0, 0.0);
-    g.cpt = g.normal; g.shearInc = g.normal; g.relVn = g.normal; g.relVs = g.normal;
-    g.twist = g.normal; g.orth = g.normal; g.point1 = g.normal; g.point2 = g.normal;
-    g.isShearNew = true;
-    g.c10 = g.c11 = g.c20 = g.c21 = 0; g.curvAngle = g.alpha = g.K = g.E = g.curvSum = 0;
-    g.gSphere = false;
-  } else {
-    g.depth = P.depth[p];
-    g.a0 = P.ang[p];
-    g.a1 = P.ang[S + p];
-    g.normal = ld3(P.normal, p, S);
-    g.gSphere = P.gSphere[p] != 0;
-    g.isShearNew = g.depth <= 0;
-  }
-  V3<double> contactNormal = mk3(0.0, 0.0, 0.0);
-  if (b1.kind == SDM_KIND_WALL) {
-    // Ig2_Wall_Superquadrics_SuperquadricsGeom(2)::go
-    const int PA = b1.wallAxis;
-    V3<double> wn = mk3(0.0, 0.0, 0.0);
-    double sense = (double)b1.wallSense;
-    if (PA == 0) wn.x = sense; else if (PA == 1) wn.y = sense; else wn.z = sense;
-    const double wallc = PA == 0 ? b1.pos.x : (PA == 1 ? b1.pos.y : b1.pos.z);
-    outcome = GEO_FULL;
-    if (b2.isSphere) {
-      V3<double> q1 = b2.pos;
-      if (PA == 0) q1.x = wallc; else if (PA == 1) q1.y = wallc; else q1.z = wallc;
-      V3<double> point2 = b2.pos - wn * b2.rmax;
-      double depth = b2.rmax - norm(b2.pos - q1);
-      if (depth < 0.0) {
-        g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
-      } else {
-        g.point1 = q1; g.point2 = point2;
-        g.cpt = mk3(0.5 * (point2.x + q1.x), 0.5 * (point2.y + q1.y), 0.5 * (point2.z + q1.z));
-        g.depth = depth;
-      }
-    } else {
-      const M3<double> L2 = ld9(B.rotL, id2, N), G2 = ld9(B.rotG, id2, N);
-      V3<double> point2 = support(b2.s, L2, G2, -wn) + b2.pos;
-      nev = 1;
-      V3<double> q1 = point2;
-      if (PA == 0) q1.x = wallc; else if (PA == 1) q1.y = wallc; else q1.z = wallc;
-      V3<double> d = point2 - q1;
-      const double nPA = PA == 0 ? wn.x : (PA == 1 ? wn.y : wn.z);
-      const double dPA = PA == 0 ? d.x : (PA == 1 ? d.y : d.z);
-      if (nPA * dPA >= 0.0) {
-        g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
-      } else {
-        g.point1 = q1; g.point2 = point2;
-        g.cpt = mk3(0.5 * (point2.x + q1.x), 0.5 * (point2.y + q1.y), 0.5 * (point2.z + q1.z));
-        g.depth = norm(d);
-      }
-    }
-    contactNormal = wn;
-  } else {
-    ce++;
-    double para0, para1;
-    if (LAW == 0) {
-      if (isNew) {
-        V3<double> d0 = b2.pos - b1.pos;
-        para0 = atan2(d0.y, d0.x);
-        para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
-        para0 += 1e-5;
-        para1 += 1e-5;
-      } else {
-        para0 = g.a0;
-        para1 = g.a1;
-      }
-    } else {
-      para0 = g.a0;
-      para1 = g.a1;
-    }
-    if (b1.isSphere && b2.isSphere) {
-      V3<double> dist = b2.pos - b1.pos;
-      double r_sum = b1.rmax + b2.rmax;
-      double depth = r_sum - norm(dist);
-      if (depth < 1E-18) {
-        g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
-      } else {
-        dist = normalized(dist);
-        V3<double> q1 = b1.pos + b1.rmax * dist;
-        V3<double> q2 = b2.pos - b2.rmax * dist;
-        contactNormal = normalized(q1 - q2);
-        g.point1 = q1; g.point2 = q2;
-        g.cpt = mk3(0.5 * (q1.x + q2.x), 0.5 * (q1.y + q2.y), 0.5 * (q1.z + q2.z));
-        g.depth = depth;
-        if (LAW == 1) {
-          g.curvSum = r_sum / (b1.rmax * b2.rmax);
-          g.gSphere = true;
-          g.a0 = 0;  // Geom2 keeps its local para(0,0) on this branch (:2335)
-          g.a1 = 0;
-        } else {
-          g.a0 = para0;
-          g.a1 = para1;
-        }
-        outcome = GEO_FULL;
-      }
-    } else {
-      double r_sum = b1.rmax + b2.rmax;
-      if (norm(b1.pos - b2.pos) > r_sum) {
-        g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
-      } else {
-        const M3<double> L1 = ld9(B.rotL, id1, N), G1 = ld9(B.rotG, id1, N);
-        const M3<double> L2 = ld9(B.rotL, id2, N), G2 = ld9(B.rotG, id2, N);
-        int ok;
-        if (LAW == 0)
-          ok = lm_geometry<PREC>(g, b1, b2, L1, G1, L2, G2, para0, para1, contactNormal, nev, stopCode);
-        else
-          ok = nm_geometry<PREC>(g, b1, b2, L1, G1, L2, G2, isNew ? 0.1 : 0.001, para0, para1, contactNormal, nev,
-                                 stopCode);
-        g.a0 = para0;
-        g.a1 = para1;
-        if (!ok) {
-          g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY_ANG;
-        } else {
-          outcome = GEO_FULL;
-        }
-      }
-    }
-    de += nev;
-  }
-  if (outcome == GEO_FULL) precompute(g, b1, b2, A.dt, contactNormal, g.isShearNew);
-
   // ---------------- Ip2 functor ----------------
   PhysState ph;
-  if (hadPhys) {
+  if (hadPhys && outcome != GEO_FULL) {
+    // no contact this step: the law reads nothing of the phys but the erase test
+    ph.fs = mk3(0.0, 0.0, 0.0);
+    ph.kn = ph.ks = 0;
+  } else if (hadPhys) {
     ph.kn = P.kn[p]; ph.ks = P.ks[p]; ph.tanPhi = P.tanPhi[p];
     ph.betan = P.betan[p]; ph.betas = P.betas[p];
     ph.nuab = P.nuab[p]; ph.Gab = P.Gab[p]; ph.mu1 = P.mu1[p]; ph.mu2 = P.mu2[p];
@@ -484,8 +217,8 @@ __device__ void process_pair(const StepArgs& A, int64_t p, unsigned long long& c
         ph.kn = 8. / 3. * ph.Gab * tmp;
         ph.ks = 8. / (2.0 * ph.mu1 - ph.mu2) * tmp;
       } else {
-        // The reference forms these two in long double through M_PIl (:1413, :1417); the
-        // device has no extended type, so they are evaluated in double here.
+        // The reference forms these two in long double through M_PIl (Superquadrics.cpp:1413,
+        // 1417); the device has no extended type, so they are evaluated in double.
         const double PI = 3.141592653589793238462643383279502884;
         double a = sqrt(depth * g.E / cs / g.K);
         ph.kn = 4. / 3. * PI * a * ph.Gab * g.alpha / g.K;
@@ -540,9 +273,10 @@ __device__ void process_pair(const StepArgs& A, int64_t p, unsigned long long& c
   // ---------------- write back ----------------
   P.nEval[p] = nev;
   P.stop[p] = stopCode;
+  P.active[p] = contributed ? 1 : 0;
   if (!keep) {
-    // Interaction::reset (core/Interaction.cpp:25-29): geom and phys are dropped; store the
-    // defaults a fresh geom/phys will start from.
+    // Interaction::reset (core/Interaction.cpp:25-29): drop geom and phys; store the defaults
+    // a fresh geom/phys starts from.
     P.hasGeom[p] = 0;
     P.hasPhys[p] = 0;
     P.geomExists[p] = 1;
@@ -586,30 +320,358 @@ __device__ void process_pair(const StepArgs& A, int64_t p, unsigned long long& c
     P.tanPhi[p] = ph.tanPhi; P.betan[p] = ph.betan; P.betas[p] = ph.betas;
     P.nuab[p] = ph.nuab; P.Gab[p] = ph.Gab; P.mu1[p] = ph.mu1; P.mu2[p] = ph.mu2;
   }
-  P.kn[p] = ph.kn;
-  P.ks[p] = ph.ks;
+  if (!hadPhys || contributed) {  // kn/ks change only in Ip2 and in Law2's contact branch
+    P.kn[p] = ph.kn;
+    P.ks[p] = ph.ks;
+  }
   st3(P.fn, p, S, ph.fn);
   st3(P.fs, p, S, ph.fs);
   if (contributed) {
-    P.active[p] = 1;
     st3(P.cF, p, S, F);
     st3(P.cT1, p, S, -cross(b1.pos - g.cpt, F));
     st3(P.cT2, p, S, cross(b2.pos - g.cpt, F));
   }
 }
 
-template <int LAW, int PREC>
-__global__ __launch_bounds__(256) void k_interactions(StepArgs A) {
-  unsigned long long ce = 0, de = 0;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.np; p += (int64_t)gridDim.x * blockDim.x)
-    process_pair<LAW, PREC>(A, p, ce, de);
-  // one atomic per wavefront
+// Reduce a wave's counters into the global ones with one atomic per wave.
+__device__ __forceinline__ void add_counters(unsigned long long* c, unsigned long long ce, unsigned long long de) {
   for (int off = 32; off > 0; off >>= 1) {
     ce += __shfl_down(ce, off);
     de += __shfl_down(de, off);
   }
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&A.counters[0], ce);
-    atomicAdd(&A.counters[1], de);
+    if (ce) atomicAdd(&c[0], ce);
+    if (de) atomicAdd(&c[1], de);
   }
+}
+
+// Every interaction: dispatch, the cheap functor branches completely, queue the rest.
+template <int LAW>
+__global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, SolveIO S, unsigned long long* qcount) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long ce = 0, de = 0;
+  bool enqueue = false;
+  int id1 = 0, id2 = 0;
+  double sa0 = 0, sa1 = 0, sdel = 0;
+  const PairArrays& P = A.P;
+  const BodyArrays& B = A.B;
+  if (p < A.np) {
+    const int64_t N = B.n;
+    P.active[p] = 0;
+    if (P.geomExists[p]) {
+      id1 = P.id1[p];
+      id2 = P.id2[p];
+      bool skip = false;
+      if (!P.functorSet[p]) {
+        // dispatch order (Wall,Superquadrics) / (Superquadrics,Superquadrics); swapOrder otherwise
+        const int k1 = B.kind[id1], k2 = B.kind[id2];
+        if (k1 == SDM_KIND_WALL && k2 == SDM_KIND_WALL) {
+          P.geomExists[p] = 0;
+          skip = true;
+        } else {
+          P.functorSet[p] = 1;
+          if (k1 == SDM_KIND_SQ && k2 == SDM_KIND_WALL) {
+            int t = id1; id1 = id2; id2 = t;
+            P.id1[p] = id1;
+            P.id2[p] = id2;
+          }
+        }
+      }
+      if (!skip) {
+        const bool hadGeom = P.hasGeom[p] != 0;
+        const bool hadPhys = P.hasPhys[p] != 0;
+        const bool isNew = !hadGeom;
+        const bool wallPair = B.kind[id1] == SDM_KIND_WALL;
+        const bool spherePair = !wallPair && B.isSphere[id1] && B.isSphere[id2];
+        bool lean = false;
+        if (!wallPair && !spherePair) {
+          // Superquadric pair: bounding spheres, then the separation bound of the solver's
+          // first evaluation.  Only the packed records are read on this path.
+          ce++;
+          const GeoRec r1 = load_rec(B.geo, id1), r2 = load_rec(B.geo, id2);
+          double para0, para1;
+          if (LAW == 0 && isNew) {
+            const V3<double> d0 = r2.pos - r1.pos;
+            para0 = atan2(d0.y, d0.x);
+            para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
+            para0 += 1e-5;
+            para1 += 1e-5;
+          } else if (isNew) {
+            para0 = 0;
+            para1 = 0;
+          } else {
+            para0 = P.ang[p];
+            para1 = P.ang[P.cap + p];
+          }
+          int outcome = -1, stopCode = 0;
+          if (norm(r1.pos - r2.pos) > r1.rmax + r2.rmax) {
+            outcome = GEO_EARLY;  // Superquadrics_Ig2.cpp:1975-1979, 2384-2388
+          } else {
+            // The first distance evaluation of either solver is at `para` along the contact
+            // direction c; it reports "not touching" when c.(p2 - p1) > 0, and
+            //   c.(p2 - p1) = c.(x2 - x1) - h1(c) - h2(-c)
+            // with h the support function of each body.  Every point of a superellipsoid lies
+            // in its box [-rx,rx]x[-ry,ry]x[-rz,rz], so h(n) <= rx|nx| + ry|ny| + rz|nz| in body
+            // axes.  When the box bound already proves c.(p2 - p1) > 0 (with a margin far above
+            // the rounding of the evaluation), the solver would stop after that evaluation with
+            // the parameters unchanged: LM at stop 9 (lmder2 :1145-1149), NM after one function
+            // call (neldermead_simplex :1731-1737).  The outcome is then written directly.
+            V3<double> c;
+            if (LAW == 0) {
+              c = mk3(cos(para0) * cos(para1), sin(para0) * cos(para1), sin(para1));
+            } else {
+              const M3<double> gC = crotation_mat(r1.pos, r2.pos);
+              c = mul(gC, mk3(cos(para0) * cos(para1), sin(para0) * cos(para1), sin(para1)));
+            }
+            const V3<double> n1 = mulT(r1.G, c), n2 = mulT(r2.G, c);
+            const double hb1 = r1.s.rx * fabs(n1.x) + r1.s.ry * fabs(n1.y) + r1.s.rz * fabs(n1.z);
+            const double hb2 = r2.s.rx * fabs(n2.x) + r2.s.ry * fabs(n2.y) + r2.s.rz * fabs(n2.z);
+            const V3<double> dx = r2.pos - r1.pos;
+            const double gap = dot(c, dx) - hb1 - hb2;
+            if (gap > 1e-12 * (1.0 + norm(r1.pos) + norm(r2.pos) + hb1 + hb2)) {
+              outcome = GEO_EARLY_ANG;
+              stopCode = (LAW == 0) ? 9 : 1;
+            }
+          }
+          if (outcome >= 0) {
+            lean = true;
+            BodyView b1, b2;
+            load_bounds(B.geo, id1, b1.bmin, b1.bmax);
+            load_bounds(B.geo, id2, b2.bmin, b2.bmax);
+            b1.kind = b2.kind = SDM_KIND_SQ;
+            b1.mass = b2.mass = 0;
+            b1.blocked = b2.blocked = 0;
+            b1.material = hadPhys ? 0 : B.material[id1];
+            b2.material = hadPhys ? 0 : B.material[id2];
+            GeomState g;
+            g.depth = 0;
+            g.isShearNew = true;
+            g.a0 = para0;
+            g.a1 = para1;
+            g.gSphere = false;
+            phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, 0, stopCode);
+          } else {
+            lean = true;
+            enqueue = true;
+            sa0 = para0;
+            sa1 = para1;
+            sdel = isNew ? 0.1 : 0.001;
+          }
+        }
+        if (!lean) {
+        const BodyView b1 = load_body(B, id1);
+        const BodyView b2 = load_body(B, id2);
+        GeomState g;
+        load_geom_prev(P, p, hadGeom, g);
+        int outcome = GEO_FULL, nev = 0;
+        V3<double> cn = mk3(0.0, 0.0, 0.0);
+        bool handled = true;
+        if (b1.kind == SDM_KIND_WALL) {
+          // Ig2_Wall_Superquadrics_SuperquadricsGeom(2)::go
+          const int PA = b1.wallAxis;
+          V3<double> wn = mk3(0.0, 0.0, 0.0);
+          const double sense = (double)b1.wallSense;
+          if (PA == 0) wn.x = sense; else if (PA == 1) wn.y = sense; else wn.z = sense;
+          const double wallc = PA == 0 ? b1.pos.x : (PA == 1 ? b1.pos.y : b1.pos.z);
+          if (b2.isSphere) {
+            V3<double> q1 = b2.pos;
+            if (PA == 0) q1.x = wallc; else if (PA == 1) q1.y = wallc; else q1.z = wallc;
+            const V3<double> point2 = b2.pos - wn * b2.rmax;
+            const double depth = b2.rmax - norm(b2.pos - q1);
+            if (depth < 0.0) {
+              g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
+            } else {
+              g.point1 = q1; g.point2 = point2;
+              g.cpt = mk3(0.5 * (point2.x + q1.x), 0.5 * (point2.y + q1.y), 0.5 * (point2.z + q1.z));
+              g.depth = depth;
+            }
+          } else {
+            const M3<double> G2 = ld9(B.rotG, id2, N);
+            const V3<double> point2 = support(b2.s, G2, -wn) + b2.pos;
+            nev = 1;
+            V3<double> q1 = point2;
+            if (PA == 0) q1.x = wallc; else if (PA == 1) q1.y = wallc; else q1.z = wallc;
+            const V3<double> d = point2 - q1;
+            const double nPA = PA == 0 ? wn.x : (PA == 1 ? wn.y : wn.z);
+            const double dPA = PA == 0 ? d.x : (PA == 1 ? d.y : d.z);
+            if (nPA * dPA >= 0.0) {
+              g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
+            } else {
+              g.point1 = q1; g.point2 = point2;
+              g.cpt = mk3(0.5 * (point2.x + q1.x), 0.5 * (point2.y + q1.y), 0.5 * (point2.z + q1.z));
+              g.depth = norm(d);
+            }
+          }
+          cn = wn;
+        } else {
+          ce++;
+          double para0, para1;
+          if (LAW == 0 && isNew) {
+            const V3<double> d0 = b2.pos - b1.pos;
+            para0 = atan2(d0.y, d0.x);
+            para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
+            para0 += 1e-5;
+            para1 += 1e-5;
+          } else {
+            para0 = g.a0;
+            para1 = g.a1;
+          }
+          if (b1.isSphere && b2.isSphere) {
+            V3<double> dist = b2.pos - b1.pos;
+            const double r_sum = b1.rmax + b2.rmax;
+            const double depth = r_sum - norm(dist);
+            if (depth < 1E-18) {
+              g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
+            } else {
+              dist = normalized(dist);
+              const V3<double> q1 = b1.pos + b1.rmax * dist;
+              const V3<double> q2 = b2.pos - b2.rmax * dist;
+              cn = normalized(q1 - q2);
+              g.point1 = q1; g.point2 = q2;
+              g.cpt = mk3(0.5 * (q1.x + q2.x), 0.5 * (q1.y + q2.y), 0.5 * (q1.z + q2.z));
+              g.depth = depth;
+              if (LAW == 1) {
+                g.curvSum = r_sum / (b1.rmax * b2.rmax);
+                g.gSphere = true;
+                g.a0 = 0;  // Geom2 keeps its local para(0,0) on this branch (:2335)
+                g.a1 = 0;
+              } else {
+                g.a0 = para0;
+                g.a1 = para1;
+              }
+            }
+          }
+        }
+        if (handled) {
+          if (outcome == GEO_FULL) precompute(g, b1, b2, A.dt, cn, g.isShearNew);
+          de += (b1.kind == SDM_KIND_WALL) ? 0 : nev;
+          phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, nev, 0);
+        }
+        }
+      }
+    }
+  }
+  add_counters(A.counters, ce, de);
+  const int64_t slot = fetch_work(enqueue, qcount);
+  if (enqueue) {
+    S.pair[slot] = (int32_t)p;
+    S.id1[slot] = id1;
+    S.id2[slot] = id2;
+    S.a0[slot] = sa0;
+    S.a1[slot] = sa1;
+    S.del[slot] = sdel;
+    S.nev[slot] = 0;
+    S.tag[slot] = 0;
+    S.touch[slot] = 1;
+    st3s(S.ct, slot, S.cap, mk3(0.0, 0.0, 0.0));
+  }
+}
+
+// Restart checks of Ig2_..._SuperquadricsGeom::go after an LM pass (Superquadrics_Ig2.cpp:2013-2052).
+// kind 1: NaN parameters or stop 7 while touching; kind 2: a suspiciously deep contact whose
+// points are not inside the other body.  Restarted slots are listed for another pass.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int32_t* list, unsigned long long* cnt) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool restart = false;
+  const SolveIO& S = SA.S;
+  if (w < SA.nwork && S.touch[w]) {
+    const int64_t N = A.B.n;
+    const int id1 = S.id1[w], id2 = S.id2[w];
+    if (KIND == 1) {
+      restart = isnan(S.a0[w]) || isnan(S.a1[w]) || S.stop[w] == 7;
+    } else {
+      const V3<double> p1 = ld3s(S.p1, w, S.cap), p2 = ld3s(S.p2, w, S.cap);
+      const V3<double> x1 = ld3(A.B.pos, id1, N), x2 = ld3(A.B.pos, id2, N);
+      if (10.0 * norm(p2 - p1) > A.B.rmax[id1]) {
+        Shape<double> s1, s2;
+        s1.rx = A.B.rxyz[id1]; s1.ry = A.B.rxyz[N + id1]; s1.rz = A.B.rxyz[2 * N + id1];
+        s1.eps1 = A.B.eps[id1]; s1.eps2 = A.B.eps[N + id1];
+        s2.rx = A.B.rxyz[id2]; s2.ry = A.B.rxyz[N + id2]; s2.rz = A.B.rxyz[2 * N + id2];
+        s2.eps1 = A.B.eps[id2]; s2.eps2 = A.B.eps[N + id2];
+        const M3<double> G1 = ld9(A.B.rotG, id1, N), G2 = ld9(A.B.rotG, id2, N);
+        restart = !(is_inside(s1, G1, p2 - x1)) || !(is_inside(s2, G2, p1 - x2));
+      }
+    }
+    if (restart) {
+      const V3<double> d0 = ld3(A.B.pos, id2, N) - ld3(A.B.pos, id1, N);
+      double a0 = atan2(d0.y, d0.x);
+      double a1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
+      S.a0[w] = a0 + 1e-5;
+      S.a1[w] = a1 + 1e-5;
+      S.tag[w] = KIND == 1 ? 100 : 200;
+    }
+  }
+  const int64_t slot = fetch_work(restart, cnt);
+  if (restart) list[slot] = (int32_t)w;
+}
+
+// Queued pairs after the solvers: geometry outputs, curvature / Hertz terms, precompute, law.
+template <int LAW>
+__global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, int64_t nq) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long de = 0;
+  if (w < nq) {
+    const PairArrays& P = A.P;
+    const BodyArrays& B = A.B;
+    const int64_t N = B.n;
+    const int64_t p = S.pair[w];
+    const int id1 = S.id1[w], id2 = S.id2[w];
+    const bool hadGeom = P.hasGeom[p] != 0;
+    const bool hadPhys = P.hasPhys[p] != 0;
+    const BodyView b1 = load_body(B, id1);
+    const BodyView b2 = load_body(B, id2);
+    GeomState g;
+    load_geom_prev(P, p, hadGeom, g);
+    const int nev = S.nev[w];
+    de = nev;
+    g.a0 = S.a0[w];
+    g.a1 = S.a1[w];
+    int outcome = GEO_FULL;
+    V3<double> cn = mk3(0.0, 0.0, 0.0);
+    const int stopCode = S.stop[w] + S.tag[w];
+    if (!S.touch[w]) {
+      g.depth = 0;
+      g.isShearNew = true;
+      outcome = GEO_EARLY_ANG;
+    } else if (LAW == 0) {
+      const V3<double> p1 = ld3s(S.p1, w, S.cap), p2 = ld3s(S.p2, w, S.cap), ct = ld3s(S.ct, w, S.cap);
+      g.point1 = p1;
+      g.point2 = p2;
+      g.cpt = ld3s(S.ctr, w, S.cap);
+      g.depth = dot(p1 - p2, ct);
+      cn = normalized(ct);
+    } else {
+      const V3<double> p1 = ld3s(S.p1, w, S.cap), p2 = ld3s(S.p2, w, S.cap);
+      cn = normalized(p1 - p2);
+      g.point1 = p1;
+      g.point2 = p2;
+      g.cpt = mk3(0.5 * (p1.x + p2.x), 0.5 * (p1.y + p2.y), 0.5 * (p1.z + p2.z));
+      g.depth = S.f[w];
+      // curvature direction convention of :2463-2465
+      const M3<double> gC = crotation_mat(b1.pos, b2.pos);
+      const M3<double> G1 = ld9(B.rotG, id1, N), G2 = ld9(B.rotG, id2, N);
+      const V3<double> contact = mul(gC, mk3(cos(g.a0) * sin(g.a1), sin(g.a0) * sin(g.a1), cos(g.a1)));
+      const V3<double> c1 = mulT(G1, contact), c2 = mulT(G2, contact);
+      V3<double> dmax1 = mk3(0.0, 0.0, 0.0), dmax2 = dmax1, dmin1, dmin2;
+      bool f1, f2;
+      double ph0, ph1, k10, k11, k20, k21;
+      normal2phi(b1.s, c1, ph0, ph1);
+      sq_curvatures(b1.s, G1, ph0, ph1, k10, k11, dmax1, dmin1, f1);
+      normal2phi(b2.s, mk3(-1.0 * c2.x, -1.0 * c2.y, -1.0 * c2.z), ph0, ph1);
+      sq_curvatures(b2.s, G2, ph0, ph1, k20, k21, dmax2, dmin2, f2);
+      const V3<double> cd1 = (f1 && f2) ? dmax1 : dmin1;
+      const V3<double> cd2 = (f1 && f2) ? dmax2 : dmin2;
+      const V3<double> dir1 = mk3(cd1.x * cn.x, cd1.y * cn.y, cd1.z * cn.z) - cd1;
+      const V3<double> dir2 = mk3(cd2.x * cn.x, cd2.y * cn.y, cd2.z * cn.z) - cd2;
+      double angle = acos(dot(dir1, dir2) / (norm(dir1) * norm(dir2)));
+      if (isnan(angle)) angle = 0.0;
+      g.c10 = k10; g.c11 = k11; g.c20 = k20; g.c21 = k21;
+      g.curvAngle = angle;
+      hertz_mindlin(k10, k11, k20, k21, angle, g.curvSum, g.alpha, g.K, g.E);
+    }
+    if (outcome == GEO_FULL) precompute(g, b1, b2, A.dt, cn, g.isShearNew);
+    phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, nev, stopCode);
+  }
+  add_counters(A.counters, 0, de);
 }

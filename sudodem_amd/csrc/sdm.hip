@@ -96,9 +96,18 @@ struct sdm_ctx {
   int64_t mcap = 0;
   FieldDesc* dfields = nullptr;
   int nfields = 0;
-  unsigned long long* dcnt = nullptr;  // [0] contact evals, [1] dist evals, [2] candidate count
+  unsigned long long* dcnt = nullptr;  // [0] contact evals, [1] dist evals, [2] candidates, [3] queues
   int* dMaxDisp = nullptr;
+  // narrowphase work queue (indexed by queue slot)
+  SolveIO S;
+  int32_t* retryList = nullptr;
   bool matChecked = false;
+  bool solverRan = false;
+  bool geoDirty = true;
+  // per-phase timing
+  bool timing = false;
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double tms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   sdm_counters cnt;
   std::string err;
 };
@@ -121,6 +130,7 @@ void alloc_bodies(sdm_ctx* c, int32_t n) {
   for (int a = 0; a < 3; a++) c->qs[a] = dalloc<uint32_t>(2 * (int64_t)n);
   c->smin = dalloc<uint32_t>(2 * (int64_t)n + 1);
   c->csrOff = dalloc<int32_t>((int64_t)n + 1);
+  c->B.geo = dalloc<double>((int64_t)n * GEO_STRIDE);
 }
 
 void free_pairs(PairArrays& P) {
@@ -213,6 +223,29 @@ void ensure_moves(sdm_ctx* c, int64_t n) {
   c->mdst = dalloc<int32_t>(cap);
   c->holePos = dalloc<int32_t>(cap);
   c->mcap = cap;
+}
+
+void free_queue(SolveIO& S) {
+  dfree(S.pair); dfree(S.tag); dfree(S.id1); dfree(S.id2); dfree(S.a0); dfree(S.a1); dfree(S.del);
+  dfree(S.f); dfree(S.la0); dfree(S.la1); dfree(S.p1); dfree(S.p2); dfree(S.ctr); dfree(S.ct);
+  dfree(S.nev); dfree(S.stop); dfree(S.touch);
+}
+
+void ensure_queue(sdm_ctx* c, int64_t n) {
+  if (n <= c->S.cap && c->S.pair) return;
+  HIPCHK(hipStreamSynchronize(c->st));
+  free_queue(c->S);
+  dfree(c->retryList);
+  const int64_t cap = n + n / 2 + 1024;
+  SolveIO& S = c->S;
+  S.cap = cap;
+  S.pair = dalloc<int32_t>(cap); S.tag = dalloc<int32_t>(cap); S.id1 = dalloc<int32_t>(cap);
+  S.id2 = dalloc<int32_t>(cap); S.a0 = dalloc<double>(cap); S.a1 = dalloc<double>(cap);
+  S.del = dalloc<double>(cap); S.f = dalloc<double>(cap); S.la0 = dalloc<double>(cap);
+  S.la1 = dalloc<double>(cap); S.p1 = dalloc<double>(3 * cap); S.p2 = dalloc<double>(3 * cap);
+  S.ctr = dalloc<double>(3 * cap); S.ct = dalloc<double>(3 * cap); S.nev = dalloc<int32_t>(cap);
+  S.stop = dalloc<int32_t>(cap); S.touch = dalloc<int32_t>(cap);
+  c->retryList = dalloc<int32_t>(cap);
 }
 
 uint32_t read_u32(sdm_ctx* c, const uint32_t* p) {
@@ -370,10 +403,80 @@ void collider_run(sdm_ctx* c) {
   build_csr(c);
 }
 
-template <int LAW, int PREC>
-void launch_interactions(sdm_ctx* c, const StepArgs& A) {
+unsigned long long read_counter(sdm_ctx* c, int k) {
+  unsigned long long v = 0;
+  HIPCHK(hipMemcpyAsync(&v, c->dcnt + k, sizeof(v), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return v;
+}
+
+void zero_counter(sdm_ctx* c, int k) { HIPCHK(hipMemsetAsync(c->dcnt + k, 0, sizeof(unsigned long long), c->st)); }
+
+// Grid of a persistent solver: enough waves to cover the queue, capped at a few per SIMD.
+unsigned solver_grid(int64_t nwork) {
+  const int64_t b = (nwork + 255) / 256;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 2048));
+}
+
+// InteractionLoop::action: prep (every interaction), solvers (queued pairs), finish.
+template <int LAW>
+void interaction_loop(sdm_ctx* c, const StepArgs& A) {
   if (A.np <= 0) return;
-  hipLaunchKernelGGL((k_interactions<LAW, PREC>), dim3(nblk(A.np)), dim3(256), 0, c->st, A);
+  ensure_queue(c, A.np);
+  zero_counter(c, 3);
+  hipLaunchKernelGGL((k_pairs_prep<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, c->dcnt + 3);
+  const int64_t nq = (int64_t)read_counter(c, 3);
+  if (c->timing) c->tms[7] += (double)nq;
+  if (nq == 0) return;
+  SolveArgs SA;
+  SA.geo = c->B.geo;
+  SA.N = c->B.n;
+  SA.S = c->S;
+  SA.list = nullptr;
+  SA.nwork = nq;
+  SA.next = c->dcnt + 3;
+  SA.evals = c->dcnt + 4;
+  if (LAW == 0) {
+    zero_counter(c, 3);
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
+    hipLaunchKernelGGL(k_solve_lm, dim3(solver_grid(nq)), dim3(256), 0, c->st, SA);
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
+    // the two restarts of Ig2_..._SuperquadricsGeom::go, each a further pass over its list
+    for (int kind = 1; kind <= 2; kind++) {
+      SolveArgs CA = SA;
+      CA.list = nullptr;
+      CA.nwork = nq;
+      zero_counter(c, 3);
+      if (kind == 1)
+        hipLaunchKernelGGL(k_lm_check<1>, dim3(nblk(nq)), dim3(256), 0, c->st, A, CA, c->retryList, c->dcnt + 3);
+      else
+        hipLaunchKernelGGL(k_lm_check<2>, dim3(nblk(nq)), dim3(256), 0, c->st, A, CA, c->retryList, c->dcnt + 3);
+      const int64_t nr = (int64_t)read_counter(c, 3);
+      if (nr > 0) {
+        SolveArgs RA = SA;
+        RA.list = c->retryList;
+        RA.nwork = nr;
+        RA.evals = c->dcnt + 5;
+        zero_counter(c, 3);
+        hipLaunchKernelGGL(k_solve_lm, dim3(solver_grid(nr)), dim3(256), 0, c->st, RA);
+      }
+    }
+  } else {
+    zero_counter(c, 3);
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
+    if (c->prm.precision == SDM_MIXED)
+      hipLaunchKernelGGL(k_solve_nm<float>, dim3(solver_grid(nq)), dim3(256), 0, c->st, SA);
+    else
+      hipLaunchKernelGGL(k_solve_nm<double>, dim3(solver_grid(nq)), dim3(256), 0, c->st, SA);
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
+    if (c->prm.precision == SDM_MIXED) {
+      SolveArgs RF = SA;
+      RF.evals = c->dcnt + 5;
+      hipLaunchKernelGGL(k_nm_refine, dim3(nblk(nq)), dim3(256), 0, c->st, RF);
+    }
+  }
+  hipLaunchKernelGGL((k_pairs_finish<LAW>), dim3(nblk(nq)), dim3(256), 0, c->st, A, c->S, nq);
+  c->solverRan = true;
 }
 
 bool collider_is_activated(sdm_ctx* c) {
@@ -389,8 +492,15 @@ bool collider_is_activated(sdm_ctx* c) {
 }
 
 void one_step(sdm_ctx* c) {
+  c->solverRan = false;
+  if (c->geoDirty) {
+    hipLaunchKernelGGL(k_geo_pack, dim3(nblk(c->B.n)), dim3(256), 0, c->st, c->B);
+    c->geoDirty = false;
+  }
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[0], c->st));
   if (collider_is_activated(c)) collider_run(c);
   if (!c->csrValid) build_csr(c);
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[1], c->st));
   StepArgs A;
   A.B = c->B;
   A.P = c->P;
@@ -399,11 +509,11 @@ void one_step(sdm_ctx* c) {
   A.iter = c->iter;
   A.dt = c->prm.dt;
   A.counters = c->dcnt;
-  const int law = c->prm.law, prec = c->prm.precision;
-  if (law == SDM_LAW_LINEAR && prec == SDM_FP64) launch_interactions<0, 0>(c, A);
-  else if (law == SDM_LAW_LINEAR) launch_interactions<0, 1>(c, A);
-  else if (prec == SDM_FP64) launch_interactions<1, 0>(c, A);
-  else launch_interactions<1, 1>(c, A);
+  if (c->prm.law == SDM_LAW_LINEAR)
+    interaction_loop<0>(c, A);
+  else
+    interaction_loop<1>(c, A);
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[2], c->st));
   NewtonArgs NA;
   NA.B = c->B;
   NA.P = c->P;
@@ -417,10 +527,28 @@ void one_step(sdm_ctx* c) {
   NA.exactAsph = c->prm.exact_aspherical_rot;
   NA.maxDisp = c->dMaxDisp;
   HIPCHK(hipMemsetAsync(c->dMaxDisp, 0, sizeof(int), c->st));
+  hipLaunchKernelGGL(k_force_sum, dim3((unsigned)((c->B.n + 15) / 16)), dim3(256), 0, c->st, c->B, c->P, c->csrOff,
+                     c->csrPair);
   hipLaunchKernelGGL(k_newton, dim3(nblk(c->B.n)), dim3(256), 0, c->st, NA);
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[3], c->st));
   int code = 0;
   HIPCHK(hipMemcpyAsync(&code, c->dMaxDisp, sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  if (c->timing) {
+    float a = 0, b = 0, d = 0;
+    HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIPCHK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    HIPCHK(hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
+    c->tms[0] += a;
+    c->tms[1] += b;
+    c->tms[2] += d;
+    c->tms[3] += 1;
+    if (c->solverRan) {
+      float e = 0;
+      HIPCHK(hipEventElapsedTime(&e, c->ev[4], c->ev[5]));
+      c->tms[4] += e;
+    }
+  }
   c->maxVelocitySq = code == 2 ? 2.0 : (code == 1 ? 0.5 : 0.0);
   c->iter++;
   c->time += c->prm.dt;
@@ -455,6 +583,9 @@ const FieldInfo kFields[] = {
     {"newton.maxVelocitySq", S_SCALAR, SDM_F64, 1, 8, 4},
     {"collider.numReinit", S_SCALAR, SDM_I32, 1, 4, 5},
     {"collider.bounds", S_BOUNDS, SDM_I32, 1, 4, 0},
+    {"scene.dt", S_SCALAR, SDM_F64, 1, 8, 6},
+    {"newton.damping", S_SCALAR, SDM_F64, 1, 8, 7},
+    {"newton.gravity", S_SCALAR, SDM_F64, 3, 8, 8},
 };
 
 const FieldInfo* find_field(const char* name) {
@@ -468,7 +599,7 @@ int64_t field_count(sdm_ctx* c, const FieldInfo* f) {
   switch (f->scope) {
     case S_BODY: return (int64_t)c->B.n * f->comps;
     case S_PAIR: return c->np * f->comps;
-    case S_SCALAR: return 1;
+    case S_SCALAR: return f->comps;
     case S_BOUNDS: return c->bbValid ? 6 * (int64_t)c->B.n : 0;
   }
   return 0;
@@ -525,6 +656,7 @@ sdm_ctx* sdm_create(const sdm_params* params, int32_t n_bodies, int32_t n_materi
     HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     std::memset(&c->P, 0, sizeof(c->P));
     std::memset(&c->C, 0, sizeof(c->C));
+    std::memset(&c->S, 0, sizeof(c->S));
     alloc_bodies(c, n_bodies);
     // defaults of a fresh Body/State/Bound: identity orientation, damped, NaN bound data
     const int64_t N = n_bodies;
@@ -547,8 +679,8 @@ sdm_ctx* sdm_create(const sdm_params* params, int32_t n_bodies, int32_t n_materi
     }
     c->mats.resize(n_materials);
     c->dmats = dalloc<MaterialDev>(std::max(1, n_materials));
-    c->dcnt = dalloc<unsigned long long>(4);
-    HIPCHK(hipMemset(c->dcnt, 0, 4 * sizeof(unsigned long long)));
+    c->dcnt = dalloc<unsigned long long>(8);
+    HIPCHK(hipMemset(c->dcnt, 0, 8 * sizeof(unsigned long long)));
     c->dMaxDisp = dalloc<int>(1);
     ensure_pair_cap(c, 1024);
     ensure_cands(c, 4096);
@@ -568,7 +700,10 @@ void sdm_destroy(sdm_ctx* c) {
 #define SDM_FB(T, m, k, name) dfree(c->B.m);
   SDM_BODY_FIELDS(SDM_FB)
 #undef SDM_FB
+  dfree(c->B.geo);
   free_pairs(c->P);
+  free_queue(c->S);
+  dfree(c->retryList);
   for (int a = 0; a < 3; a++) {
     dfree(c->bb[a]);
     dfree(c->qs[a]);
@@ -579,6 +714,8 @@ void sdm_destroy(sdm_ctx* c) {
   dfree(c->C.a); dfree(c->C.b); dfree(c->C.key);
   dfree(c->msrc); dfree(c->mdst); dfree(c->holePos); dfree(c->dfields);
   dfree(c->dmats); dfree(c->dcnt); dfree(c->dMaxDisp);
+  for (int k = 0; k < 6; k++)
+    if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -685,6 +822,9 @@ int sdm_get(sdm_ctx* c, const char* name, void* dst, int64_t count) {
         case 3: *(double*)dst = c->fastestBodyMaxDist; break;
         case 4: *(double*)dst = c->maxVelocitySq; break;
         case 5: *(int32_t*)dst = c->numReinit; break;
+        case 6: *(double*)dst = c->prm.dt; break;
+        case 7: *(double*)dst = c->prm.damping; break;
+        case 8: std::memcpy(dst, c->prm.gravity, 3 * sizeof(double)); break;
       }
     } else {
       const int64_t n2 = 2 * (int64_t)c->B.n;
@@ -733,7 +873,10 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
           HIPCHK(hipMemcpy(dptr + (size_t)k * stride * f->esize, tmp.data(), rows * f->esize, hipMemcpyHostToDevice));
       }
       if (f->scope == S_PAIR) c->csrValid = false;
-      if (f->scope == S_BODY) c->matChecked = false;
+      if (f->scope == S_BODY) {
+        c->matChecked = false;
+        c->geoDirty = true;
+      }
     } else {
       switch (f->off) {
         case 0: c->iter = *(const int64_t*)src; break;
@@ -742,6 +885,9 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
         case 3: c->fastestBodyMaxDist = *(const double*)src; break;
         case 4: c->maxVelocitySq = *(const double*)src; break;
         case 5: c->numReinit = *(const int32_t*)src; break;
+        case 6: c->prm.dt = *(const double*)src; break;
+        case 7: c->prm.damping = *(const double*)src; break;
+        case 8: std::memcpy(c->prm.gravity, src, 3 * sizeof(double)); break;
       }
     }
   } catch (HipError& e) {
@@ -787,6 +933,32 @@ int sdm_get_counters(sdm_ctx* c, sdm_counters* out) {
   } catch (HipError& e) {
     return fail(c, e.msg);
   }
+  return 0;
+}
+
+int sdm_set_timing(sdm_ctx* c, int32_t on) {
+  if (!c) return -1;
+  try {
+    for (int k = 0; k < 6; k++)
+      if (!c->ev[k]) HIPCHK(hipEventCreate(&c->ev[k]));
+    HIPCHK(hipMemsetAsync(c->dcnt + 4, 0, 2 * sizeof(unsigned long long), c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  c->timing = on != 0;
+  for (int k = 0; k < 8; k++) c->tms[k] = 0;
+  return 0;
+}
+
+int sdm_get_timers(sdm_ctx* c, double out[8]) {
+  if (!c || !out) return -1;
+  unsigned long long ev[2] = {0, 0};
+  if (hipMemcpy(ev, c->dcnt + 4, sizeof(ev), hipMemcpyDeviceToHost) != hipSuccess) return fail(c, "timers");
+  for (int k = 0; k < 5; k++) out[k] = c->tms[k];
+  out[5] = (double)ev[0];
+  out[6] = (double)ev[1];
+  out[7] = c->tms[7];
   return 0;
 }
 

@@ -1,0 +1,519 @@
+// solve.h -- persistent, one-evaluation-per-iteration narrowphase solvers.
+//
+// The two common-normal solvers of the reference take a data-dependent number of distance
+// evaluations per pair (Nelder-Mead: 1 to ~60, Levenberg-Marquardt: 1 to ~200).  Run one
+// pair per thread to completion and a wavefront waits for its slowest lane.  Here each
+// solver is a per-lane state machine whose loop body is exactly one distance evaluation;
+// a lane that finishes its pair pulls the next one from the work queue, so all 64 lanes
+// of a wave execute the same evaluation code in lockstep whatever stage their pair is in.
+// The sequence of evaluations and decisions of every pair is that of the reference:
+//   neldermead_simplex + amotry (pkg/dem/Superquadrics_Ig2.cpp:1681-1875)
+//   lmder2 + JacfunctionApp     (Superquadrics_Ig2.cpp:1032-1354)
+//
+// Mixed precision (Nelder-Mead only): the distance is evaluated in float on a frame centred
+// at body 1, while the simplex vertices, sums and moves stay in double.  When every
+// comparison of float distances agrees with the double one, the vertices are bitwise those
+// of the double solver, and the final double evaluations (k_nm_refine) reproduce its depth
+// and points exactly.
+#pragma once
+#include "narrow.h"
+#include "state.h"
+
+struct SolveIO {  // indexed by work slot
+  int32_t* pair;   // interaction index
+  int32_t* tag;    // LM restart marker added to the reported stop code (100, 200)
+  int32_t* id1;
+  int32_t* id2;
+  double* a0;      // in: start parameters; out: best vertex (NM) / final parameters (LM)
+  double* a1;
+  double* del;     // NM: initial simplex size
+  double* f;       // NM: fmin
+  double* la0;     // NM: last evaluated point
+  double* la1;
+  double* p1;      // [3][cap] points of the last evaluation
+  double* p2;
+  double* ctr;     // LM: midpoint of the last evaluation
+  double* ct;      // LM: contact direction set at the top of the last iteration (in/out)
+  int32_t* nev;    // distance evaluations (accumulated over solver passes)
+  int32_t* stop;   // LM stop code / NM nfunc
+  int32_t* touch;  // 1 = touching
+  int64_t cap;
+};
+
+struct SolveArgs {
+  const double* geo;  // packed body records (state.h): pos, rot_mat2global, semi-axes, exponents, r_max
+  int64_t N;
+  SolveIO S;
+  const int32_t* list;  // work slots to process (nullptr: slots 0..nwork-1)
+  int64_t nwork;
+  unsigned long long* next;   // work counter, zero at launch
+  unsigned long long* evals;  // distance evaluations done by this launch (added)
+};
+
+__device__ __forceinline__ void add_evals(unsigned long long* c, unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(c, v);
+}
+
+__device__ __forceinline__ void st3s(double* a, int64_t i, int64_t s, V3<double> v) {
+  a[i] = v.x;
+  a[s + i] = v.y;
+  a[2 * s + i] = v.z;
+}
+__device__ __forceinline__ V3<double> ld3s(const double* a, int64_t i, int64_t s) {
+  return mk3(a[i], a[s + i], a[2 * s + i]);
+}
+
+// One packed body record (bounds excluded).
+struct GeoRec {
+  V3<double> pos;
+  M3<double> G;
+  Shape<double> s;
+  double rmax;
+};
+__device__ __forceinline__ GeoRec load_rec(const double* geo, int id) {
+  const double2* r = reinterpret_cast<const double2*>(geo + (int64_t)id * GEO_STRIDE);
+  double v[GEO_BMIN];
+#pragma unroll
+  for (int k = 0; k < GEO_BMIN / 2; k++) {
+    const double2 t = r[k];
+    v[2 * k] = t.x;
+    v[2 * k + 1] = t.y;
+  }
+  GeoRec g;
+  g.pos = mk3(v[GEO_POS], v[GEO_POS + 1], v[GEO_POS + 2]);
+#pragma unroll
+  for (int k = 0; k < 9; k++) g.G.m[k] = v[GEO_G + k];
+  g.s.rx = v[GEO_R]; g.s.ry = v[GEO_R + 1]; g.s.rz = v[GEO_R + 2];
+  g.s.eps1 = v[GEO_E]; g.s.eps2 = v[GEO_E + 1];
+  g.rmax = v[GEO_RMAX];
+  return g;
+}
+
+template <typename T>
+__device__ __forceinline__ Shape<T> cast_shape(const Shape<double>& s) {
+  Shape<T> r;
+  r.rx = T(s.rx); r.ry = T(s.ry); r.rz = T(s.rz); r.eps1 = T(s.eps1); r.eps2 = T(s.eps2);
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void pair_geo(const GeoRec& r1, const GeoRec& r2, bool relative, PairGeo<T>& g) {
+  g.s1 = cast_shape<T>(r1.s);
+  g.s2 = cast_shape<T>(r2.s);
+  g.G1 = cast9<T>(r1.G);
+  g.G2 = cast9<T>(r2.G);
+  const V3<double> x1 = r1.pos;
+  const V3<double> x2 = r2.pos;
+  if (relative) {
+    g.pos1 = mk3(T(0), T(0), T(0));
+    g.pos2 = cast3<T>(x2 - x1);
+  } else {
+    g.pos1 = cast3<T>(x1);
+    g.pos2 = cast3<T>(x2);
+  }
+}
+
+// Wave-level work fetch: the idle lanes take consecutive queue entries with one atomic per wave.
+__device__ __forceinline__ int64_t fetch_work(bool want, unsigned long long* counter) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return -1;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  const uint64_t below = lane ? (m & (~0ull >> (64 - lane))) : 0ull;
+  return want ? (int64_t)(base + __popcll(below)) : -1;
+}
+
+// ---------------------------------------------------------------------------------------
+// Nelder-Mead (neldermead_simplex, ftol = 1e-3, NMAX = 5000; nfunc starts at 0)
+// ---------------------------------------------------------------------------------------
+enum { NM_V = 0, NM_REFL = 1, NM_EXP = 2, NM_CON = 3, NM_SHR = 4 };
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
+  constexpr bool kRel = sizeof(T) == 4;
+  const T ftol = T(1e-3);
+  const T TINY = T(1.0e-10);
+  const int NMAX = 5000;
+  NMGeo<T> n;
+  double v[3][2], psum[2], pt[2];
+  T y[3], ysave = T(0);
+  int ihi = 0, ilo = 0, inhi = 0, state = NM_V, vi = 0, nfunc = 0, nev = 0;
+  bool touching = true;
+  int64_t w = -1;
+  V3<T> q1 = mk3(T(0), T(0), T(0)), q2 = q1;
+  bool done = false;
+  unsigned long long tot = 0;
+  for (;;) {
+    const bool idle = (w < 0) && !done;
+    const int64_t got = fetch_work(idle, A.next);
+    if (idle) {
+      if (got >= A.nwork) {
+        done = true;
+      } else {
+        w = A.list ? A.list[got] : got;
+        const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
+        pair_geo<T>(r1, r2, kRel, n.g);
+        n.gC = cast9<T>(crotation_mat(r1.pos, r2.pos));
+        const double a0 = A.S.a0[w], a1 = A.S.a1[w], del = A.S.del[w];
+        v[0][0] = a0; v[0][1] = a1;
+        v[1][0] = a0 + del; v[1][1] = a1;
+        v[2][0] = a0; v[2][1] = a1 + del;
+        pt[0] = v[0][0]; pt[1] = v[0][1];
+        state = NM_V;
+        vi = 0;
+        nfunc = 0;
+        nev = 0;
+        touching = true;
+      }
+    }
+    if (__ballot(!done) == 0) break;
+    if (done) continue;
+    // ---- the one distance evaluation of this iteration ----
+    const T f = nm_eval(n, T(pt[0]), T(pt[1]), q1, q2, touching, nev);
+    nfunc++;
+    bool finished = false;
+    T fmin = T(0);
+    bool decide = false;
+    if (state == NM_V) {
+      y[vi] = f;
+      if (!touching) {
+        fmin = y[0];
+        finished = true;
+      } else if (vi < 2) {
+        vi++;
+        pt[0] = v[vi][0];
+        pt[1] = v[vi][1];
+      } else {
+        psum[0] = v[0][0] + v[1][0] + v[2][0];
+        psum[1] = v[0][1] + v[1][1] + v[2][1];
+        decide = true;
+      }
+    } else if (state == NM_SHR) {
+      y[vi] = f;
+      if (!touching) {
+        fmin = y[0];
+        finished = true;
+      } else {
+        int nx = vi + 1;
+        if (nx == ilo) nx++;
+        if (nx < 3) {
+          vi = nx;
+          psum[0] = 0.5 * (v[vi][0] + v[ilo][0]);
+          psum[1] = 0.5 * (v[vi][1] + v[ilo][1]);
+          v[vi][0] = psum[0];
+          v[vi][1] = psum[1];
+          pt[0] = psum[0];
+          pt[1] = psum[1];
+        } else {
+          psum[0] = v[0][0] + v[1][0] + v[2][0];
+          psum[1] = v[0][1] + v[1][1] + v[2][1];
+          decide = true;
+        }
+      }
+    } else {
+      // amotry: accept the trial point if it beats the highest vertex
+      const T ytry = f;
+      if (ytry < y[ihi]) {
+        y[ihi] = ytry;
+        psum[0] += pt[0] - v[ihi][0];
+        psum[1] += pt[1] - v[ihi][1];
+        v[ihi][0] = pt[0];
+        v[ihi][1] = pt[1];
+      }
+      if (!touching) {
+        fmin = y[0];
+        finished = true;
+      } else if (state == NM_REFL) {
+        if (ytry <= y[ilo]) {
+          const double fac = 2.0, fac1 = (1.0 - fac) / 2.0, fac2 = fac1 - fac;
+          pt[0] = psum[0] * fac1 - v[ihi][0] * fac2;
+          pt[1] = psum[1] * fac1 - v[ihi][1] * fac2;
+          state = NM_EXP;
+        } else if (ytry >= y[inhi]) {
+          ysave = y[ihi];
+          const double fac = 0.5, fac1 = (1.0 - fac) / 2.0, fac2 = fac1 - fac;
+          pt[0] = psum[0] * fac1 - v[ihi][0] * fac2;
+          pt[1] = psum[1] * fac1 - v[ihi][1] * fac2;
+          state = NM_CON;
+        } else {
+          decide = true;
+        }
+      } else if (state == NM_EXP) {
+        decide = true;
+      } else {  // NM_CON
+        if (ytry >= ysave) {
+          vi = (ilo == 0) ? 1 : 0;
+          psum[0] = 0.5 * (v[vi][0] + v[ilo][0]);
+          psum[1] = 0.5 * (v[vi][1] + v[ilo][1]);
+          v[vi][0] = psum[0];
+          v[vi][1] = psum[1];
+          pt[0] = psum[0];
+          pt[1] = psum[1];
+          state = NM_SHR;
+        } else {
+          decide = true;
+        }
+      }
+    }
+    if (decide) {
+      // test of doneness (:1779-1806) and the reflection of the next iteration (:1814)
+      ilo = 0;
+      if (y[0] > y[1]) { ihi = 0; inhi = 1; } else { ihi = 1; inhi = 0; }
+      for (int i = 0; i < 3; i++) {
+        if (y[i] <= y[ilo]) ilo = i;
+        if (y[i] > y[ihi]) {
+          inhi = ihi;
+          ihi = i;
+        } else if (y[i] > y[inhi] && i != ihi)
+          inhi = i;
+      }
+      const T rtol = T(2.0) * xfabs(y[ihi] - y[ilo]) / (xfabs(y[ihi]) + xfabs(y[ilo]) + TINY);
+      if (rtol < ftol) {
+        const T t = y[0]; y[0] = y[ilo]; y[ilo] = t;
+        const double t0 = v[ilo][0], t1 = v[ilo][1];
+        v[ilo][0] = v[0][0]; v[ilo][1] = v[0][1];
+        v[0][0] = t0; v[0][1] = t1;
+        fmin = y[0];
+        finished = true;
+      } else if (nfunc >= NMAX) {
+        fmin = y[0];
+        finished = true;
+      } else {
+        const double fac = -1.0, fac1 = (1.0 - fac) / 2.0, fac2 = fac1 - fac;
+        pt[0] = psum[0] * fac1 - v[ihi][0] * fac2;
+        pt[1] = psum[1] * fac1 - v[ihi][1] * fac2;
+        state = NM_REFL;
+      }
+    }
+    if (finished) {
+      const int64_t S = A.S.cap;
+      A.S.a0[w] = v[0][0];
+      A.S.a1[w] = v[0][1];
+      A.S.f[w] = double(fmin);
+      A.S.la0[w] = pt[0];
+      A.S.la1[w] = pt[1];
+      st3s(A.S.p1, w, S, cast3<double>(q1));
+      st3s(A.S.p2, w, S, cast3<double>(q2));
+      A.S.nev[w] += nev;
+      A.S.stop[w] = nfunc;
+      A.S.touch[w] = touching ? 1 : 0;
+      tot += nev;
+      w = -1;
+    }
+  }
+  add_evals(A.evals, tot);
+}
+
+// Mixed form, final refinement in double: the depth is the double distance at the best
+// vertex and the points (hence the normal) those of the last evaluated point, as the
+// reference takes them (Superquadrics_Ig2.cpp:2439-2454).  One work item per touching pair.
+__global__ __launch_bounds__(256) void k_nm_refine(SolveArgs A) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.nwork) return;
+  const int64_t w = A.list ? A.list[k] : k;
+  if (!A.S.touch[w]) return;
+  NMGeo<double> n;
+  const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
+  pair_geo<double>(r1, r2, false, n.g);
+  n.gC = crotation_mat(r1.pos, r2.pos);
+  V3<double> q1, q2;
+  bool touching = true;
+  int nev = 0;
+  const double f = nm_eval(n, A.S.a0[w], A.S.a1[w], q1, q2, touching, nev);
+  bool t2 = true;
+  nm_eval(n, A.S.la0[w], A.S.la1[w], q1, q2, t2, nev);
+  const int64_t S = A.S.cap;
+  A.S.f[w] = f;
+  st3s(A.S.p1, w, S, q1);
+  st3s(A.S.p2, w, S, q2);
+  A.S.nev[w] += nev;
+}
+
+// ---------------------------------------------------------------------------------------
+// Levenberg-Marquardt (lmder2 with opts {1, 1e-15, 1e-3, 1e-20}, itmax 50)
+// ---------------------------------------------------------------------------------------
+enum { LM_INIT = 0, LM_J0 = 1, LM_J1 = 2, LM_J2 = 3, LM_TRIAL = 4 };
+
+__global__ __launch_bounds__(256) void k_solve_lm(SolveArgs A) {
+  const double tau = 1.0, eps1 = 1e-15, eps2 = 1e-3, eps3 = 1e-20;
+  const int itmax = 50;
+  PairGeo<double> g;
+  double p0 = 0, p1v = 0, Dp0 = 0, Dp1 = 0, mu = 0, p_fL2 = 0, A00 = 0, A01 = 0, A11 = 0, dA0 = 0, dA1 = 0;
+  double jTf0 = 0, jTf1 = 0, p_L2 = 0, q0 = 0, qq1 = 0;
+  V3<double> p_f = mk3(0.0, 0.0, 0.0), f0 = p_f, f1 = p_f, ct = p_f, lp1 = p_f, lp2 = p_f, lc = p_f;
+  int nu = 2, stop = 0, k = 0, state = LM_INIT, nev = 0;
+  bool stop_flag = false, touching = true;
+  int64_t w = -1;
+  bool done = false;
+  unsigned long long tot = 0;
+  for (;;) {
+    const bool idle = (w < 0) && !done;
+    const int64_t got = fetch_work(idle, A.next);
+    if (idle) {
+      if (got >= A.nwork) {
+        done = true;
+      } else {
+        w = A.list ? A.list[got] : got;
+        pair_geo<double>(load_rec(A.geo, A.S.id1[w]), load_rec(A.geo, A.S.id2[w]), false, g);
+        p0 = A.S.a0[w];
+        p1v = A.S.a1[w];
+        ct = ld3s(A.S.ct, w, A.S.cap);
+        Dp0 = 1e-7; Dp1 = 1e-7;
+        mu = 0; nu = 2; stop = 0; k = 0; nev = 0;
+        stop_flag = false;
+        touching = true;
+        state = LM_INIT;
+      }
+    }
+    if (__ballot(!done) == 0) break;
+    if (done) continue;
+    // ---- the one distance evaluation of this iteration ----
+    double e0 = p0, e1 = p1v;
+    if (state == LM_J1) e0 = p0 + Dp0;
+    if (state == LM_J2) e1 = p1v + Dp1;
+    if (state == LM_TRIAL) { e0 = q0; e1 = qq1; }
+    lm_eval(g, e0, e1, lp1, lp2, lc, stop_flag, nev);
+    const V3<double> fe = lp2 - lp1;
+    bool finished = false;
+    bool outer = false;  // go to the outer-iteration checks
+    bool solve = false;  // go to the damped normal-equation solve
+    if (state == LM_INIT) {
+      p_f = fe;
+      p_fL2 = norm(p_f);
+      if (!isfinite(p_fL2)) {
+        stop = 7;
+        finished = true;
+      } else {
+        outer = true;
+      }
+    } else if (state == LM_J0) {
+      f0 = fe;
+      state = LM_J1;
+    } else if (state == LM_J1) {
+      f1 = fe;
+      state = LM_J2;
+    } else if (state == LM_J2) {
+      const V3<double> f2 = fe;
+      const double J00 = (f1.x - f0.x) / Dp0, J10 = (f1.y - f0.y) / Dp0, J20 = (f1.z - f0.z) / Dp0;
+      const double J01 = (f2.x - f0.x) / Dp1, J11 = (f2.y - f0.y) / Dp1, J21 = (f2.z - f0.z) / Dp1;
+      A00 = J00 * J00 + (J10 * J10 + J20 * J20);
+      A01 = J00 * J01 + (J10 * J11 + J20 * J21);
+      A11 = J01 * J01 + (J11 * J11 + J21 * J21);
+      const V3<double> mf = mk3(-1.0 * p_f.x, -1.0 * p_f.y, -1.0 * p_f.z);
+      jTf0 = J00 * mf.x + (J10 * mf.y + J20 * mf.z);
+      jTf1 = J01 * mf.x + (J11 * mf.y + J21 * mf.z);
+      const double jTf_inf = (fabs(jTf0) > fabs(jTf1) ? fabs(jTf0) : fabs(jTf1));
+      p_L2 = sqrt(p0 * p0 + p1v * p1v);
+      if (jTf_inf <= eps1) {
+        stop = 1;
+        finished = true;
+      } else {
+        if (k == 0) mu = tau * (A00 > A11 ? A00 : A11);
+        dA0 = A00;
+        dA1 = A11;
+        solve = true;
+      }
+    } else {  // LM_TRIAL
+      const V3<double> pDp_f = fe;
+      const double pDp_fL2 = norm(pDp_f);
+      if (!isfinite(pDp_fL2)) {
+        stop = 7;
+        k++;
+        finished = true;
+      } else {
+        const double dL = Dp0 * (mu * Dp0 + jTf0) + Dp1 * (mu * Dp1 + jTf1);
+        const double dF = p_fL2 - pDp_fL2;
+        double tmp = dF / dL;
+        if (tmp > 0.0) {
+          tmp = 1.0 - pow(2 * tmp - 1., 3.);
+          mu = mu * ((tmp >= 0.3333333333333334) ? tmp : 0.3333333333333334);
+          nu = 2;
+          p0 = q0;
+          p1v = qq1;
+          p_f = pDp_f;
+          p_fL2 = pDp_fL2;
+          k++;
+          outer = true;
+        } else {
+          mu *= nu;
+          const int nu2 = int(unsigned(nu) << 1);
+          if (nu2 <= nu) {
+            stop = 5;
+            k++;
+            finished = true;
+          } else {
+            nu = nu2;
+            A00 = dA0;
+            A11 = dA1;
+            solve = true;
+          }
+        }
+      }
+    }
+    // Damped normal equations without further evaluations; loops until it needs an evaluation.
+    if (solve) {
+      A00 += mu;
+      A11 += mu;
+      const double n_a = A00 * A11 - A01 * A01;
+      Dp0 = (jTf0 * A11 - jTf1 * A01) / n_a;
+      Dp1 = (jTf1 * A00 - jTf0 * A01) / n_a;
+      const double Dp_L2 = sqrt(Dp0 * Dp0 + Dp1 * Dp1);
+      if (Dp_L2 < 1e-5) {
+        stop = 2;
+        k++;
+        finished = true;
+      } else {
+        q0 = p0 + Dp0;
+        qq1 = p1v + Dp1;
+        if (Dp_L2 >= (p_L2 + eps2) / (1E-12 * 1E-12)) {
+          stop = 4;
+          k++;
+          finished = true;
+        } else {
+          state = LM_TRIAL;
+        }
+      }
+    }
+    if (outer) {
+      if (k >= itmax) {
+        finished = true;
+      } else if (p_fL2 <= eps3) {
+        stop = 6;
+        finished = true;
+      } else if (stop_flag) {
+        stop = 9;
+        touching = false;
+        finished = true;
+      } else {
+        ct = mk3(cos(p0) * cos(p1v), sin(p0) * cos(p1v), sin(p1v));
+        const double angle = dot(p_f, ct) / (norm(p_f) * norm(ct));
+        if (fabs(angle) > 0.95) {
+          stop = 8;
+          finished = true;
+        } else {
+          state = LM_J0;
+        }
+      }
+    }
+    if (finished) {
+      if (k >= itmax) stop = 3;
+      const int64_t S = A.S.cap;
+      A.S.a0[w] = p0;
+      A.S.a1[w] = p1v;
+      st3s(A.S.p1, w, S, lp1);
+      st3s(A.S.p2, w, S, lp2);
+      st3s(A.S.ctr, w, S, lc);
+      st3s(A.S.ct, w, S, ct);
+      A.S.nev[w] += nev;
+      A.S.stop[w] = stop;
+      A.S.touch[w] = touching ? 1 : 0;
+      tot += nev;
+      w = -1;
+    }
+  }
+  add_evals(A.evals, tot);
+}

@@ -94,7 +94,14 @@ struct BodyArrays {
 #define SDM_DECL(T, m, k, name) T* m;
   SDM_BODY_FIELDS(SDM_DECL)
 #undef SDM_DECL
+  // Packed copy of what the pair kernels gather per body, one 192-byte record per body:
+  // pos[3], r_max, rot_mat2global[9], semi-axes[3], exponents[2], bound min[3], max[3].
+  // Interactions reference bodies in no spatial order, so a record (12 x 16-byte loads,
+  // the first two holding pos and r_max) replaces 24 scattered 8-byte reads.
+  double* geo;
 };
+
+enum { GEO_POS = 0, GEO_RMAX = 3, GEO_G = 4, GEO_R = 13, GEO_E = 16, GEO_BMIN = 18, GEO_BMAX = 21, GEO_STRIDE = 24 };
 
 struct PairArrays {
   int64_t cap;  // plane stride

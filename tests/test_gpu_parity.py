@@ -45,3 +45,48 @@ def test_steps_state_parity(gpu, oracle, c0_small):
         pg, pc = g.get("state.pos"), c.get("state.pos")
         assert np.max(np.abs(pg - pc)) <= 1e-12 * max(1.0, np.max(np.abs(pc))), k
     assert g.counters()["collider_runs"] == c.counters()["collider_runs"]
+
+
+def _rel(a, b, floor):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+@pytest.fixture(scope="module")
+def c1_small():
+    return configs.config1(n_particles=3000)
+
+
+def test_hertz_nm_first_steps(gpu, oracle, c1_small):
+    """Geom2 (Nelder-Mead) + Law2 on a bed with contacts: per-contact quantities to 1e-10."""
+    g = c1_small.open(gpu)
+    c = c1_small.open(oracle)
+    oracle.set_init_sort(c.ptr, 1)
+    g.step(3)
+    c.step(3)
+    assert np.array_equal(_pairs(g), _pairs(c))
+    dg, dc = g.get("geom.PenetrationDepth"), c.get("geom.PenetrationDepth")
+    act = (dc > 0) & (dg > 0)
+    assert act.sum() > 50
+    # contacts classified the same way
+    assert np.array_equal(dg > 0, dc > 0)
+    same_path = g.get("diag.stop") == c.get("diag.stop")
+    ok = act & same_path
+    assert ok.sum() >= 0.97 * act.sum()
+    assert np.max(_rel(dg[ok], dc[ok], 1e-12)) < 1e-10
+    for name in ("geom.normal", "geom.contactPoint", "phys.normalForce", "contact.force"):
+        a = g.get(name).reshape(-1, 3)[ok]
+        b = c.get(name).reshape(-1, 3)[ok]
+        scale = np.max(np.abs(b), axis=1, keepdims=True)
+        assert np.max(np.abs(a - b) / np.maximum(scale, 1e-300)) < 1e-10, name
+    # torque = (x - contactPoint) x F: its natural scale is |x - cp| |F| (for near-central
+    # contacts |T| is far smaller and the cross product cancels)
+    F = c.get("contact.force").reshape(-1, 3)[ok]
+    cp = c.get("geom.contactPoint").reshape(-1, 3)[ok]
+    pos = c.get("state.pos").reshape(-1, 3)
+    for name, ids in (("contact.torque1", "intr.id1"), ("contact.torque2", "intr.id2")):
+        # positions before the last step's integration are not kept; use |cp - x| at the
+        # end of the step, which differs by one step's displacement
+        r = np.linalg.norm(cp - pos[c.get(ids)[ok]], axis=1) * np.linalg.norm(F, axis=1)
+        a = g.get(name).reshape(-1, 3)[ok]
+        b = c.get(name).reshape(-1, 3)[ok]
+        assert np.max(np.max(np.abs(a - b), axis=1) / np.maximum(r, 1e-300)) < 1e-10, name
