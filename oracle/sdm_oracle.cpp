@@ -1715,6 +1715,18 @@ sdm_ctx* sdmcpu_create(const sdm_params* params, int32_t n_bodies, int32_t n_mat
     g_err = "verletDist must be set explicitly (>= 0): the automatic value needs spheres";
     return nullptr;
   }
+  if (params->law != SDM_LAW_LINEAR && params->law != SDM_LAW_HERTZ) {
+    g_err = "sdmcpu_create: unknown law";
+    return nullptr;
+  }
+  if (params->precision != SDM_FP64 && params->precision != SDM_MIXED) {
+    g_err = "sdmcpu_create: unknown precision";
+    return nullptr;
+  }
+  if (params->sort_axis < 0 || params->sort_axis > 2) {
+    g_err = "sdmcpu_create: sortAxis must be 0, 1 or 2";
+    return nullptr;
+  }
   sdm_ctx* c = new sdm_ctx();
   c->prm = *params;
   c->bodies.resize(n_bodies);

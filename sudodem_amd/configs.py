@@ -13,6 +13,7 @@ the initial contact overlaps.
 """
 from __future__ import annotations
 
+import copy
 import math
 
 import numpy as np
@@ -134,6 +135,18 @@ class Config:
         from .scene import open_arrays
         return open_arrays(lib or nat.load_hip(), self.params(precision), self.arrays, self.materials)
 
+    def update_stats(self):
+        """solids_fraction: particle volume over the box footprint times the bed height (top of
+        the highest bounding sphere); mean_scale: mean semi-axis scale against the drawn sizes."""
+        A = self.arrays
+        part = A["kind"] == nat.SDM_BODY_SUPERQUADRIC
+        lo, hi = self.box
+        top = float((A["pos"][part, 2] + A["r_max"][part]).max()) - lo[2]
+        vol = float((A["mass"][part] / self.materials[A["material"][part][0]].density).sum())
+        self.solids_fraction = vol / ((hi[0] - lo[0]) * (hi[1] - lo[1]) * top)
+        drawn = getattr(self, "rxyz_drawn", None)
+        self.mean_scale = float((A["rxyz"][part, 0] / drawn[part, 0]).mean()) if drawn is not None else 1.0
+
 
 def linear_engines(verlet):
     return [ForceResetter(), InsertionSortCollider([Bo1_Superquadrics_Aabb(), Bo1_Wall_Aabb()], verletDist=verlet),
@@ -231,62 +244,19 @@ def _separate(rng, pos, rxyz, eps, quat, lo, hi, gap):
     raise RuntimeError("could not separate the initial packing")
 
 
-def _deposit(pos, rxyz, eps, R, lo, overlap, iters=6):
-    """Lower the particles layer by layer (increasing height) until each rests on the first
-    particle below it that its support reaches along the line of centres (overlap `overlap`),
-    or on the floor.  Positions are changed along z only."""
-    from scipy.spatial import cKDTree
-    pos = pos.copy()
-    n = len(pos)
-    ez = np.zeros((n, 3))
-    ez[:, 2] = 1.0
-    hdown = support_h(rxyz, eps, R, -ez)
-    zkey = np.round(pos[:, 2] / 1e-4).astype(np.int64)
-    layers = np.unique(zkey)
-    placed = np.zeros(n, bool)
-    for li, lz in enumerate(layers):
-        idx = np.nonzero(zkey == lz)[0]
-        floor_z = lo[2] + hdown[idx] - overlap
-        if li == 0:
-            pos[idx, 2] = floor_z
-            placed[idx] = True
-            continue
-        below = np.nonzero(placed)[0]
-        tree = cKDTree(pos[below, :2])
-        nb = tree.query_ball_point(pos[idx, :2], r=3.0e-3)
-        ii = np.concatenate([np.full(len(b), k) for k, b in enumerate(nb)]) if len(idx) else np.zeros(0, int)
-        jj = below[np.concatenate([np.asarray(b, int) for b in nb])] if len(idx) else np.zeros(0, int)
-        for _ in range(iters):
-            pi = idx[ii]
-            d = pos[pi] - pos[jj]
-            L = np.linalg.norm(d, axis=1)
-            u = d / L[:, None]
-            o = support_h(rxyz[pi], eps[pi], R[pi], -u) + support_h(rxyz[jj], eps[jj], R[jj], u) - L
-            # lowering by dz shortens the centre distance by about dz*u_z
-            uz = np.maximum(u[:, 2], 0.2)
-            dz = (overlap - o) / uz
-            dz[u[:, 2] < 0.2] = np.inf
-            best = np.full(len(idx), np.inf)
-            np.minimum.at(best, ii, dz)
-            cur = pos[idx, 2]
-            target = np.maximum(cur - np.where(np.isfinite(best), best, cur - floor_z), floor_z)
-            pos[idx, 2] = target
-        placed[idx] = True
-    return pos
+def config1(n_particles=200000, seed_extra=0, spacing=2.0e-3, box_xy=120e-3):
+    """200,000 superellipsoids in a 120x120x200 mm box, Hertz-Mindlin law: the initial lattice.
 
-
-def config1(n_particles=200000, seed_extra=0, overlap=2e-6, spacing=2.0e-3, deposit=True):
-    """200,000 superellipsoids in a 120x120x200 mm box, Hertz-Mindlin law.
-
-    Densification: centres on a face-centred cubic lattice of nearest-neighbour distance
-    2.0 mm filling the bottom of the box; each particle's semi-axes are scaled by the largest
-    factor <= 1 that keeps its centre-line overlap with every lattice neighbour and wall
-    below `overlap`; then the lattice layers are deposited bottom-up (_deposit), so that
-    every particle rests on the floor or on a particle below.  The solids fraction of the
-    resulting bed is reported by the generator."""
+    Centres on a face-centred cubic lattice of nearest-neighbour distance 2.0 mm filling the
+    bottom of the box (jittered by up to 0.02 mm); semi-axes U[0.8,1.2] mm and exponents
+    U[0.6,1.4] drawn per particle, then scaled by the largest factor <= 1 that keeps the
+    particle's support along every line of centres to a lattice neighbour, and along the
+    wall normals, inside the gap: no two particles overlap, so the lattice has no contacts.
+    `densify` turns it into a contact-carrying bed.  The drawn (unscaled) semi-axes are kept
+    as `cfg.rxyz_drawn`."""
     rng = rng_for(1, seed_extra)
     box_lo = np.zeros(3)
-    box_hi = np.array([120e-3, 120e-3, 200e-3])
+    box_hi = np.array([box_xy, box_xy, 200e-3])
     a = spacing
     cc = a * math.sqrt(2.0)
     nx = int((box_hi[0] - 1.6e-3) / cc)
@@ -300,16 +270,17 @@ def config1(n_particles=200000, seed_extra=0, overlap=2e-6, spacing=2.0e-3, depo
     sites += 1.6e-3
     pos = sites + rng.uniform(-0.02e-3, 0.02e-3, size=sites.shape)
     rxyz, eps = _draw_shapes(rng, n_particles)
+    drawn = rxyz.copy()
     quat = random_quaternions(rng, n_particles)
     R = rot_matrices(quat)
     ii, jj = _neighbour_pairs(pos, a * 1.5)
     scale = np.ones(n_particles)
-    # pairwise bound: lambda_i, lambda_j <= (L + overlap) / (h_i + h_j)
+    # pairwise bound: lambda_i, lambda_j <= L / (h_i + h_j)
     d = pos[jj] - pos[ii]
     L = np.linalg.norm(d, axis=1)
     u = d / L[:, None]
     hs = support_h(rxyz[ii], eps[ii], R[ii], u) + support_h(rxyz[jj], eps[jj], R[jj], -u)
-    lim = np.minimum(1.0, (L + overlap) / hs)
+    lim = np.minimum(1.0, L / hs)
     np.minimum.at(scale, ii, lim)
     np.minimum.at(scale, jj, lim)
     for axis in range(3):
@@ -319,32 +290,82 @@ def config1(n_particles=200000, seed_extra=0, overlap=2e-6, spacing=2.0e-3, depo
         hm = support_h(rxyz, eps, R, -e)
         scale = np.minimum(scale, np.minimum((box_hi[axis] - pos[:, axis]) / hp, (pos[:, axis] - box_lo[axis]) / hm))
     rxyz = rxyz * scale[:, None]
-    if deposit:
-        pos = _deposit(pos, rxyz, eps, R, box_lo, overlap)
-        # the deposition moves neighbours of one layer by different amounts: bound the
-        # centre-line overlaps again on the final positions
-        ii, jj = _neighbour_pairs(pos, a * 1.75)
-        d = pos[jj] - pos[ii]
-        L = np.linalg.norm(d, axis=1)
-        u = d / L[:, None]
-        hs = support_h(rxyz[ii], eps[ii], R[ii], u) + support_h(rxyz[jj], eps[jj], R[jj], -u)
-        lim = np.minimum(1.0, (L + 2.5 * overlap) / hs)
-        s2 = np.ones(n_particles)
-        np.minimum.at(s2, ii, lim)
-        np.minimum.at(s2, jj, lim)
-        rxyz = rxyz * s2[:, None]
-        scale = scale * s2
     p_mat = SuperquadricsMat2(label="sq", G=7e7 / (2 * (1 + 0.3)), nu=0.3, frictionAngle=math.atan(0.5),
                               density=2650.0, Kn=1e5, Ks=7e4)
     w_mat = SuperquadricsMat2(label="wall", G=7e7 / (2 * (1 + 0.3)), nu=0.3, frictionAngle=math.atan(0.5),
                               density=2650.0, Kn=1e5, Ks=7e4)
     A = _empty_arrays(6 + n_particles)
     _box_walls(A, box_lo, box_hi, wall_mat=1)
-    vol = _fill_particles(A, 6, pos, rxyz, eps, quat, 0, p_mat.density)
+    _fill_particles(A, 6, pos, rxyz, eps, quat, 0, p_mat.density)
     verlet = 0.1 * float(np.mean(A["r_max"][6:]))
-    bed_h = pos[:, 2].max() + 1.2e-3
-    cfg = Config(f"C1: {n_particles} superellipsoids, Hertz-Mindlin law, 120x120x200 mm box", A, [p_mat, w_mat],
-                 hertz_engines(verlet), dt=1e-6, steps=5000, box=(box_lo, box_hi))
-    cfg.solids_fraction = float(vol.sum() / ((box_hi[0] - box_lo[0]) * (box_hi[1] - box_lo[1]) * bed_h))
-    cfg.mean_scale = float(scale.mean())
+    cfg = Config(f"C1: {n_particles} superellipsoids, Hertz-Mindlin law, {box_xy * 1e3:.0f}x{box_xy * 1e3:.0f}x200 mm box",
+                 A, [p_mat, w_mat], hertz_engines(verlet), dt=1e-6, steps=5000, box=(box_lo, box_hi))
+    cfg.rxyz_drawn = np.concatenate([np.zeros((6, 3)), drawn])
+    cfg.update_stats()
     return cfg
+
+
+def densify(cfg, lib, iterations=150, target=2e-6, growth=0.003, max_push=5e-5, relax=25, log=None):
+    """Grow-and-relax densification of a lattice configuration (a force-biased packing
+    algorithm) evaluated by the contact engine itself.
+
+    Each iteration runs one engine step with dt = 0 (geometry only: Ig2 gives every
+    interaction's penetration depth and normal), then (1) moves both particles of every
+    contact deeper than `target` apart along the contact normal by 0.35 x the excess (capped at
+    `max_push`; a wall contact moves the particle by 0.7 x the excess), and (2) grows every
+    particle whose deepest contact is shallower than 2 x `target` by the factor 1 + `growth`,
+    up to its drawn semi-axes; the last `relax` iterations only move.  The collider is re-run
+    every iteration.  Masses, inertia and
+    r_max are recomputed from the final semi-axes (Superquadrics::Initial); velocities stay
+    zero.  Deterministic for a given engine library."""
+    A = cfg.arrays
+    n = cfg.n
+    ctx = cfg.open(lib)
+    try:
+        ctx.set("scene.dt", np.array([0.0]))
+        part = A["kind"] == nat.SDM_BODY_SUPERQUADRIC
+        rx0 = A["rxyz"].copy()
+        rm0 = A["r_max"].copy()
+        with np.errstate(divide="ignore", invalid="ignore"):
+            cap = np.where(part, cfg.rxyz_drawn[:, 0] / np.where(part, rx0[:, 0], 1.0), 1.0)
+        lam = np.ones(n)
+        pos = A["pos"].copy()
+        for it in range(iterations):
+            if it:
+                ctx.set("shape.rxyz", rx0 * lam[:, None])
+                ctx.set("shape.r_max", rm0 * lam)
+                ctx.set("state.pos", pos)
+            ctx.set("collider.strideActive", np.array([0], np.int32))
+            ctx.step(1)
+            i1 = ctx.get("intr.id1")
+            i2 = ctx.get("intr.id2")
+            d = ctx.get("geom.PenetrationDepth")
+            nrm = ctx.get("geom.normal").reshape(-1, 3)
+            ex = np.minimum(np.where(d > target, d - target, 0.0), max_push)
+            w = np.where(A["kind"][i1] == nat.SDM_BODY_WALL, 0.7, 0.35) * ex
+            for k in range(3):
+                pos[:, k] += (np.bincount(i2, w * nrm[:, k], minlength=n) - np.bincount(i1, w * nrm[:, k], minlength=n)) * part
+            dm = np.zeros(n)
+            np.maximum.at(dm, i1, d)
+            np.maximum.at(dm, i2, d)
+            gr = part & (dm < 2 * target) & (lam < cap) & (it < iterations - relax)
+            lam[gr] = np.minimum(cap[gr], lam[gr] * (1 + growth))
+            if log and (it % 25 == 0 or it == iterations - 1):
+                log("densify %d: %d contacts, %d growing" % (it, int((d > 0).sum()), int(gr.sum())))
+    finally:
+        ctx.close()
+    B = {k: v.copy() for k, v in A.items()}
+    idx = np.nonzero(part)[0]
+    rxyz = rx0[idx] * lam[idx, None]
+    B["pos"][idx] = pos[idx]
+    B["rxyz"][idx] = rxyz
+    rmax, vol, inert = superquadric_props(rxyz, B["eps"][idx])
+    rho = cfg.materials[0].density
+    B["r_max"][idx] = rmax
+    B["mass"][idx] = rho * vol
+    B["inertia"][idx] = inert * rho
+    out = Config(cfg.name + ", densified", B, cfg.materials, copy.deepcopy(cfg.engines), cfg.dt, cfg.steps, cfg.box)
+    out.rxyz_drawn = cfg.rxyz_drawn
+    out.engines[1].verletDist = 0.1 * float(np.mean(B["r_max"][idx]))
+    out.update_stats()
+    return out

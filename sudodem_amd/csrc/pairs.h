@@ -345,18 +345,44 @@ __device__ __forceinline__ void add_counters(unsigned long long* c, unsigned lon
   }
 }
 
-// Every interaction: dispatch, the cheap functor branches completely, queue the rest.
+// Outcome of a superquadric pair that needs no further evaluation: depth 0 and a new
+// shear state; EARLY leaves the stored angles, EARLY_ANG stores (a0, a1).  Only the bounds
+// (the law's erase test) and, for a first contact, the materials are read.
 template <int LAW>
-__global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, SolveIO S, unsigned long long* qcount) {
+__device__ __forceinline__ void early_store(const StepArgs& A, int64_t p, int id1, int id2, bool hadGeom,
+                                            bool hadPhys, double a0, double a1, int outcome, int nev, int stopCode) {
+  const BodyArrays& B = A.B;
+  BodyView b1, b2;
+  load_bounds(B.geo, id1, b1.bmin, b1.bmax);
+  load_bounds(B.geo, id2, b2.bmin, b2.bmax);
+  b1.kind = b2.kind = SDM_KIND_SQ;
+  b1.mass = b2.mass = 0;
+  b1.blocked = b2.blocked = 0;
+  b1.material = hadPhys ? 0 : B.material[id1];
+  b2.material = hadPhys ? 0 : B.material[id2];
+  GeomState g;
+  g.depth = 0;
+  g.isShearNew = true;
+  g.a0 = a0;
+  g.a1 = a1;
+  g.gSphere = false;
+  phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, nev, stopCode);
+}
+
+// Every interaction: functor dispatch; superquadric pairs that the bounding spheres or the
+// box bound settle are completed here, the others queued for the solver; wall and sphere
+// pairs are listed for k_pairs_misc.
+template <int LAW>
+__global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, SolveIO S, unsigned long long* qcount,
+                                                    int32_t* mlist, unsigned long long* mcount) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long ce = 0, de = 0;
-  bool enqueue = false;
+  unsigned long long ce = 0;
+  bool enqueue = false, misc = false;
   int id1 = 0, id2 = 0;
   double sa0 = 0, sa1 = 0, sdel = 0;
   const PairArrays& P = A.P;
   const BodyArrays& B = A.B;
   if (p < A.np) {
-    const int64_t N = B.n;
     P.active[p] = 0;
     if (P.geomExists[p]) {
       id1 = P.id1[p];
@@ -378,15 +404,16 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, SolveIO S, unsig
         }
       }
       if (!skip) {
-        const bool hadGeom = P.hasGeom[p] != 0;
-        const bool hadPhys = P.hasPhys[p] != 0;
-        const bool isNew = !hadGeom;
         const bool wallPair = B.kind[id1] == SDM_KIND_WALL;
         const bool spherePair = !wallPair && B.isSphere[id1] && B.isSphere[id2];
-        bool lean = false;
-        if (!wallPair && !spherePair) {
+        if (wallPair || spherePair) {
+          misc = true;
+        } else {
           // Superquadric pair: bounding spheres, then the separation bound of the solver's
           // first evaluation.  Only the packed records are read on this path.
+          const bool hadGeom = P.hasGeom[p] != 0;
+          const bool hadPhys = P.hasPhys[p] != 0;
+          const bool isNew = !hadGeom;
           ce++;
           const GeoRec r1 = load_rec(B.geo, id1), r2 = load_rec(B.geo, id2);
           double para0, para1;
@@ -434,124 +461,20 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, SolveIO S, unsig
             }
           }
           if (outcome >= 0) {
-            lean = true;
-            BodyView b1, b2;
-            load_bounds(B.geo, id1, b1.bmin, b1.bmax);
-            load_bounds(B.geo, id2, b2.bmin, b2.bmax);
-            b1.kind = b2.kind = SDM_KIND_SQ;
-            b1.mass = b2.mass = 0;
-            b1.blocked = b2.blocked = 0;
-            b1.material = hadPhys ? 0 : B.material[id1];
-            b2.material = hadPhys ? 0 : B.material[id2];
-            GeomState g;
-            g.depth = 0;
-            g.isShearNew = true;
-            g.a0 = para0;
-            g.a1 = para1;
-            g.gSphere = false;
-            phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, 0, stopCode);
+            early_store<LAW>(A, p, id1, id2, hadGeom, hadPhys, para0, para1, outcome, 0, stopCode);
           } else {
-            lean = true;
             enqueue = true;
             sa0 = para0;
             sa1 = para1;
             sdel = isNew ? 0.1 : 0.001;
           }
         }
-        if (!lean) {
-        const BodyView b1 = load_body(B, id1);
-        const BodyView b2 = load_body(B, id2);
-        GeomState g;
-        load_geom_prev(P, p, hadGeom, g);
-        int outcome = GEO_FULL, nev = 0;
-        V3<double> cn = mk3(0.0, 0.0, 0.0);
-        bool handled = true;
-        if (b1.kind == SDM_KIND_WALL) {
-          // Ig2_Wall_Superquadrics_SuperquadricsGeom(2)::go
-          const int PA = b1.wallAxis;
-          V3<double> wn = mk3(0.0, 0.0, 0.0);
-          const double sense = (double)b1.wallSense;
-          if (PA == 0) wn.x = sense; else if (PA == 1) wn.y = sense; else wn.z = sense;
-          const double wallc = PA == 0 ? b1.pos.x : (PA == 1 ? b1.pos.y : b1.pos.z);
-          if (b2.isSphere) {
-            V3<double> q1 = b2.pos;
-            if (PA == 0) q1.x = wallc; else if (PA == 1) q1.y = wallc; else q1.z = wallc;
-            const V3<double> point2 = b2.pos - wn * b2.rmax;
-            const double depth = b2.rmax - norm(b2.pos - q1);
-            if (depth < 0.0) {
-              g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
-            } else {
-              g.point1 = q1; g.point2 = point2;
-              g.cpt = mk3(0.5 * (point2.x + q1.x), 0.5 * (point2.y + q1.y), 0.5 * (point2.z + q1.z));
-              g.depth = depth;
-            }
-          } else {
-            const M3<double> G2 = ld9(B.rotG, id2, N);
-            const V3<double> point2 = support(b2.s, G2, -wn) + b2.pos;
-            nev = 1;
-            V3<double> q1 = point2;
-            if (PA == 0) q1.x = wallc; else if (PA == 1) q1.y = wallc; else q1.z = wallc;
-            const V3<double> d = point2 - q1;
-            const double nPA = PA == 0 ? wn.x : (PA == 1 ? wn.y : wn.z);
-            const double dPA = PA == 0 ? d.x : (PA == 1 ? d.y : d.z);
-            if (nPA * dPA >= 0.0) {
-              g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
-            } else {
-              g.point1 = q1; g.point2 = point2;
-              g.cpt = mk3(0.5 * (point2.x + q1.x), 0.5 * (point2.y + q1.y), 0.5 * (point2.z + q1.z));
-              g.depth = norm(d);
-            }
-          }
-          cn = wn;
-        } else {
-          ce++;
-          double para0, para1;
-          if (LAW == 0 && isNew) {
-            const V3<double> d0 = b2.pos - b1.pos;
-            para0 = atan2(d0.y, d0.x);
-            para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
-            para0 += 1e-5;
-            para1 += 1e-5;
-          } else {
-            para0 = g.a0;
-            para1 = g.a1;
-          }
-          if (b1.isSphere && b2.isSphere) {
-            V3<double> dist = b2.pos - b1.pos;
-            const double r_sum = b1.rmax + b2.rmax;
-            const double depth = r_sum - norm(dist);
-            if (depth < 1E-18) {
-              g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
-            } else {
-              dist = normalized(dist);
-              const V3<double> q1 = b1.pos + b1.rmax * dist;
-              const V3<double> q2 = b2.pos - b2.rmax * dist;
-              cn = normalized(q1 - q2);
-              g.point1 = q1; g.point2 = q2;
-              g.cpt = mk3(0.5 * (q1.x + q2.x), 0.5 * (q1.y + q2.y), 0.5 * (q1.z + q2.z));
-              g.depth = depth;
-              if (LAW == 1) {
-                g.curvSum = r_sum / (b1.rmax * b2.rmax);
-                g.gSphere = true;
-                g.a0 = 0;  // Geom2 keeps its local para(0,0) on this branch (:2335)
-                g.a1 = 0;
-              } else {
-                g.a0 = para0;
-                g.a1 = para1;
-              }
-            }
-          }
-        }
-        if (handled) {
-          if (outcome == GEO_FULL) precompute(g, b1, b2, A.dt, cn, g.isShearNew);
-          de += (b1.kind == SDM_KIND_WALL) ? 0 : nev;
-          phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, nev, 0);
-        }
-        }
       }
     }
   }
-  add_counters(A.counters, ce, de);
+  add_counters(A.counters, ce, 0);
+  const int64_t ms = fetch_work(misc, mcount);
+  if (misc) mlist[ms] = (int32_t)p;
   const int64_t slot = fetch_work(enqueue, qcount);
   if (enqueue) {
     S.pair[slot] = (int32_t)p;
@@ -564,6 +487,142 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, SolveIO S, unsig
     S.tag[slot] = 0;
     S.touch[slot] = 1;
     st3s(S.ct, slot, S.cap, mk3(0.0, 0.0, 0.0));
+  }
+}
+
+// Wall-superquadric pairs (Ig2_Wall_Superquadrics_SuperquadricsGeom(2)::go,
+// Superquadrics_Ig2.cpp:2522-2703) and sphere-sphere pairs (the sphere branches of both Ig2
+// superquadric functors, :1950-1970, :2331-2350), listed by k_pairs_prep.
+template <int LAW>
+__global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* mlist, int64_t nm) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long ce = 0, de = 0;
+  if (k < nm) {
+    const PairArrays& P = A.P;
+    const BodyArrays& B = A.B;
+    const int64_t N = B.n;
+    const int64_t p = mlist[k];
+    const int id1 = P.id1[p], id2 = P.id2[p];
+    const bool hadGeom = P.hasGeom[p] != 0;
+    const bool hadPhys = P.hasPhys[p] != 0;
+    const bool isNew = !hadGeom;
+        const BodyView b1 = load_body(B, id1);
+    const BodyView b2 = load_body(B, id2);
+    GeomState g;
+    load_geom_prev(P, p, hadGeom, g);
+    int outcome = GEO_FULL, nev = 0;
+    V3<double> cn = mk3(0.0, 0.0, 0.0);
+    if (b1.kind == SDM_KIND_WALL) {
+      // Ig2_Wall_Superquadrics_SuperquadricsGeom(2)::go
+      const int PA = b1.wallAxis;
+      V3<double> wn = mk3(0.0, 0.0, 0.0);
+      const double sense = (double)b1.wallSense;
+      if (PA == 0) wn.x = sense; else if (PA == 1) wn.y = sense; else wn.z = sense;
+      const double wallc = PA == 0 ? b1.pos.x : (PA == 1 ? b1.pos.y : b1.pos.z);
+      if (b2.isSphere) {
+        V3<double> q1 = b2.pos;
+        if (PA == 0) q1.x = wallc; else if (PA == 1) q1.y = wallc; else q1.z = wallc;
+        const V3<double> point2 = b2.pos - wn * b2.rmax;
+        const double depth = b2.rmax - norm(b2.pos - q1);
+        if (depth < 0.0) {
+          g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
+        } else {
+          g.point1 = q1; g.point2 = point2;
+          g.cpt = mk3(0.5 * (point2.x + q1.x), 0.5 * (point2.y + q1.y), 0.5 * (point2.z + q1.z));
+          g.depth = depth;
+        }
+      } else {
+        const M3<double> G2 = ld9(B.rotG, id2, N);
+        const V3<double> point2 = support(b2.s, G2, -wn) + b2.pos;
+        nev = 1;
+        V3<double> q1 = point2;
+        if (PA == 0) q1.x = wallc; else if (PA == 1) q1.y = wallc; else q1.z = wallc;
+        const V3<double> d = point2 - q1;
+        const double nPA = PA == 0 ? wn.x : (PA == 1 ? wn.y : wn.z);
+        const double dPA = PA == 0 ? d.x : (PA == 1 ? d.y : d.z);
+        if (nPA * dPA >= 0.0) {
+          g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
+        } else {
+          g.point1 = q1; g.point2 = point2;
+          g.cpt = mk3(0.5 * (point2.x + q1.x), 0.5 * (point2.y + q1.y), 0.5 * (point2.z + q1.z));
+          g.depth = norm(d);
+        }
+      }
+      cn = wn;
+    } else {
+      ce++;
+      double para0, para1;
+      if (LAW == 0 && isNew) {
+        const V3<double> d0 = b2.pos - b1.pos;
+        para0 = atan2(d0.y, d0.x);
+        para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
+        para0 += 1e-5;
+        para1 += 1e-5;
+      } else {
+        para0 = g.a0;
+        para1 = g.a1;
+      }
+      if (b1.isSphere && b2.isSphere) {
+        V3<double> dist = b2.pos - b1.pos;
+        const double r_sum = b1.rmax + b2.rmax;
+        const double depth = r_sum - norm(dist);
+        if (depth < 1E-18) {
+          g.depth = 0; g.isShearNew = true; outcome = GEO_EARLY;
+        } else {
+          dist = normalized(dist);
+          const V3<double> q1 = b1.pos + b1.rmax * dist;
+          const V3<double> q2 = b2.pos - b2.rmax * dist;
+          cn = normalized(q1 - q2);
+          g.point1 = q1; g.point2 = q2;
+          g.cpt = mk3(0.5 * (q1.x + q2.x), 0.5 * (q1.y + q2.y), 0.5 * (q1.z + q2.z));
+          g.depth = depth;
+          if (LAW == 1) {
+            g.curvSum = r_sum / (b1.rmax * b2.rmax);
+            g.gSphere = true;
+            g.a0 = 0;  // Geom2 keeps its local para(0,0) on this branch (:2335)
+            g.a1 = 0;
+          } else {
+            g.a0 = para0;
+            g.a1 = para1;
+          }
+        }
+      }
+    }
+    if (outcome == GEO_FULL) precompute(g, b1, b2, A.dt, cn, g.isShearNew);
+    de += (b1.kind == SDM_KIND_WALL) ? 0 : nev;
+    phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, nev, 0);
+  }
+  add_counters(A.counters, ce, de);
+}
+
+// The first Nelder-Mead evaluation of every queued pair (T: evaluation type).  A pair not
+// touching there is complete: its outcome is stored here (angles unchanged, one function
+// call).  Touching pairs are listed with a sort key that puts the expected long solves first.
+template <typename T>
+__global__ __launch_bounds__(256) void k_nm_first(StepArgs A, SolveArgs SA, uint64_t* keys, uint32_t* list,
+                                                  unsigned long long* cnt) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool go = false;
+  uint32_t key = 0;
+  if (w < SA.nwork) {
+    const PairArrays& P = A.P;
+    const int64_t p = SA.S.pair[w];
+    const bool hadGeom = P.hasGeom[p] != 0;
+    if (nm_first_eval<T>(SA, w)) {
+      go = true;
+      key = nm_cost_key(P.nEval[p], hadGeom && P.depth[p] > 0, SA.S.del[w]);
+    } else {
+      early_store<1>(A, p, SA.S.id1[w], SA.S.id2[w], hadGeom, P.hasPhys[p] != 0, SA.S.a0[w], SA.S.a1[w],
+                     GEO_EARLY_ANG, 1, 1);
+    }
+  }
+  add_evals(SA.evals, w < SA.nwork ? 1ull : 0ull);
+  // listed pairs are counted when they finish; the others are complete here
+  add_counters(A.counters, 0, (w < SA.nwork && !go) ? 1ull : 0ull);
+  const int64_t slot = fetch_work(go, cnt);
+  if (go) {
+    keys[slot] = key;
+    list[slot] = (uint32_t)w;
   }
 }
 
@@ -608,10 +667,11 @@ __global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int3
 
 // Queued pairs after the solvers: geometry outputs, curvature / Hertz terms, precompute, law.
 template <int LAW>
-__global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, int64_t nq) {
-  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, const uint32_t* list, int64_t nq) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long de = 0;
-  if (w < nq) {
+  if (k < nq) {
+    const int64_t w = list ? (int64_t)list[k] : k;
     const PairArrays& P = A.P;
     const BodyArrays& B = A.B;
     const int64_t N = B.n;

@@ -96,11 +96,17 @@ struct sdm_ctx {
   int64_t mcap = 0;
   FieldDesc* dfields = nullptr;
   int nfields = 0;
-  unsigned long long* dcnt = nullptr;  // [0] contact evals, [1] dist evals, [2] candidates, [3] queues
+  // device counters: [0] contact evals, [1] dist evals, [2] candidates, [3] solver queue,
+  // [4] solver evaluations, [5] other evaluations, [6] misc list, [7] touching list
+  unsigned long long* dcnt = nullptr;
   int* dMaxDisp = nullptr;
   // narrowphase work queue (indexed by queue slot)
   SolveIO S;
   int32_t* retryList = nullptr;
+  int32_t* miscList = nullptr;
+  int numCU = 0;
+  int residentNM[2] = {0, 0};  // resident blocks of k_solve_nm<float>, <double> (0: not queried)
+  int residentLM = 0;
   bool matChecked = false;
   bool solverRan = false;
   bool geoDirty = true;
@@ -236,6 +242,7 @@ void ensure_queue(sdm_ctx* c, int64_t n) {
   HIPCHK(hipStreamSynchronize(c->st));
   free_queue(c->S);
   dfree(c->retryList);
+  dfree(c->miscList);
   const int64_t cap = n + n / 2 + 1024;
   SolveIO& S = c->S;
   S.cap = cap;
@@ -246,6 +253,7 @@ void ensure_queue(sdm_ctx* c, int64_t n) {
   S.ctr = dalloc<double>(3 * cap); S.ct = dalloc<double>(3 * cap); S.nev = dalloc<int32_t>(cap);
   S.stop = dalloc<int32_t>(cap); S.touch = dalloc<int32_t>(cap);
   c->retryList = dalloc<int32_t>(cap);
+  c->miscList = dalloc<int32_t>(cap);
 }
 
 uint32_t read_u32(sdm_ctx* c, const uint32_t* p) {
@@ -412,20 +420,68 @@ unsigned long long read_counter(sdm_ctx* c, int k) {
 
 void zero_counter(sdm_ctx* c, int k) { HIPCHK(hipMemsetAsync(c->dcnt + k, 0, sizeof(unsigned long long), c->st)); }
 
-// Grid of a persistent solver: enough waves to cover the queue, capped at a few per SIMD.
-unsigned solver_grid(int64_t nwork) {
+// Grid of a persistent solver: the blocks that are resident at once (occupancy x CUs), or
+// fewer when the list is short.  Each lane then works through several pairs, so the lanes
+// a short solve frees are refilled from the list instead of idling until the wave ends.
+template <typename K>
+unsigned resident_grid(sdm_ctx* c, K kernel, int& cache, int64_t nwork) {
+  if (cache == 0) {
+    int nb = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0));
+    cache = std::max(1, nb) * std::max(1, c->numCU);
+  }
   const int64_t b = (nwork + 255) / 256;
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 2048));
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, cache));
 }
 
-// InteractionLoop::action: prep (every interaction), solvers (queued pairs), finish.
+// Nelder-Mead pass of the Geom2 functor over the queued pairs: the first evaluation of every
+// pair (dense, one pair per lane), then the persistent solver over the touching ones, longest
+// expected solves first, the mixed form's double refinement, and the finish kernel.
+template <typename T>
+void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA, int64_t nq) {
+  ensure_scratch(c, nq);
+  zero_counter(c, 7);
+  SA.evals = c->dcnt + 5;
+  hipLaunchKernelGGL(k_nm_first<T>, dim3(nblk(nq)), dim3(256), 0, c->st, A, SA, c->k0, c->v0, c->dcnt + 7);
+  const int64_t n2 = (int64_t)read_counter(c, 7);
+  if (n2 == 0) return;
+  sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, n2, 8, c->rsScratch, c->st);
+  const uint32_t* list = c->v0;
+  SolveArgs S2 = SA;
+  S2.list = reinterpret_cast<const int32_t*>(list);
+  S2.nwork = n2;
+  S2.evals = c->dcnt + 4;
+  zero_counter(c, 3);
+  const int ti = sizeof(T) == 4 ? 0 : 1;
+  const unsigned grid = resident_grid(c, k_solve_nm<T>, c->residentNM[ti], n2);
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
+  hipLaunchKernelGGL(k_solve_nm<T>, dim3(grid), dim3(256), 0, c->st, S2);
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
+  c->solverRan = true;
+  if (sizeof(T) == 4) {
+    SolveArgs RF = S2;
+    RF.evals = c->dcnt + 5;
+    hipLaunchKernelGGL(k_nm_refine, dim3(nblk(n2)), dim3(256), 0, c->st, RF);
+  }
+  hipLaunchKernelGGL((k_pairs_finish<1>), dim3(nblk(n2)), dim3(256), 0, c->st, A, c->S, list, n2);
+}
+
+// InteractionLoop::action: prep (every interaction), wall / sphere pairs, solvers (queued
+// pairs), finish.
 template <int LAW>
 void interaction_loop(sdm_ctx* c, const StepArgs& A) {
   if (A.np <= 0) return;
   ensure_queue(c, A.np);
-  zero_counter(c, 3);
-  hipLaunchKernelGGL((k_pairs_prep<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, c->dcnt + 3);
-  const int64_t nq = (int64_t)read_counter(c, 3);
+  HIPCHK(hipMemsetAsync(c->dcnt + 3, 0, sizeof(unsigned long long), c->st));
+  HIPCHK(hipMemsetAsync(c->dcnt + 6, 0, sizeof(unsigned long long), c->st));
+  hipLaunchKernelGGL((k_pairs_prep<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, c->dcnt + 3, c->miscList,
+                     c->dcnt + 6);
+  unsigned long long qc[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(qc, c->dcnt + 3, sizeof(qc), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const int64_t nq = (int64_t)qc[0], nm = (int64_t)qc[3];
+  if (nm > 0)
+    hipLaunchKernelGGL((k_pairs_misc<LAW>), dim3(nblk(nm)), dim3(256), 0, c->st, A, c->miscList, nm);
   if (c->timing) c->tms[7] += (double)nq;
   if (nq == 0) return;
   SolveArgs SA;
@@ -438,9 +494,11 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
   SA.evals = c->dcnt + 4;
   if (LAW == 0) {
     zero_counter(c, 3);
+    const unsigned grid = resident_grid(c, k_solve_lm, c->residentLM, nq);
     if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
-    hipLaunchKernelGGL(k_solve_lm, dim3(solver_grid(nq)), dim3(256), 0, c->st, SA);
+    hipLaunchKernelGGL(k_solve_lm, dim3(grid), dim3(256), 0, c->st, SA);
     if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
+    c->solverRan = true;
     // the two restarts of Ig2_..._SuperquadricsGeom::go, each a further pass over its list
     for (int kind = 1; kind <= 2; kind++) {
       SolveArgs CA = SA;
@@ -458,25 +516,16 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
         RA.nwork = nr;
         RA.evals = c->dcnt + 5;
         zero_counter(c, 3);
-        hipLaunchKernelGGL(k_solve_lm, dim3(solver_grid(nr)), dim3(256), 0, c->st, RA);
+        hipLaunchKernelGGL(k_solve_lm, dim3(resident_grid(c, k_solve_lm, c->residentLM, nr)), dim3(256), 0, c->st,
+                           RA);
       }
     }
+    hipLaunchKernelGGL((k_pairs_finish<0>), dim3(nblk(nq)), dim3(256), 0, c->st, A, c->S, nullptr, nq);
+  } else if (c->prm.precision == SDM_MIXED) {
+    nm_pass<float>(c, A, SA, nq);
   } else {
-    zero_counter(c, 3);
-    if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
-    if (c->prm.precision == SDM_MIXED)
-      hipLaunchKernelGGL(k_solve_nm<float>, dim3(solver_grid(nq)), dim3(256), 0, c->st, SA);
-    else
-      hipLaunchKernelGGL(k_solve_nm<double>, dim3(solver_grid(nq)), dim3(256), 0, c->st, SA);
-    if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
-    if (c->prm.precision == SDM_MIXED) {
-      SolveArgs RF = SA;
-      RF.evals = c->dcnt + 5;
-      hipLaunchKernelGGL(k_nm_refine, dim3(nblk(nq)), dim3(256), 0, c->st, RF);
-    }
+    nm_pass<double>(c, A, SA, nq);
   }
-  hipLaunchKernelGGL((k_pairs_finish<LAW>), dim3(nblk(nq)), dim3(256), 0, c->st, A, c->S, nq);
-  c->solverRan = true;
 }
 
 bool collider_is_activated(sdm_ctx* c) {
@@ -679,8 +728,9 @@ sdm_ctx* sdm_create(const sdm_params* params, int32_t n_bodies, int32_t n_materi
     }
     c->mats.resize(n_materials);
     c->dmats = dalloc<MaterialDev>(std::max(1, n_materials));
-    c->dcnt = dalloc<unsigned long long>(8);
-    HIPCHK(hipMemset(c->dcnt, 0, 8 * sizeof(unsigned long long)));
+    c->dcnt = dalloc<unsigned long long>(16);
+    HIPCHK(hipMemset(c->dcnt, 0, 16 * sizeof(unsigned long long)));
+    HIPCHK(hipDeviceGetAttribute(&c->numCU, hipDeviceAttributeMultiprocessorCount, c->dev));
     c->dMaxDisp = dalloc<int>(1);
     ensure_pair_cap(c, 1024);
     ensure_cands(c, 4096);

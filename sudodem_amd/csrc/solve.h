@@ -162,10 +162,12 @@ __global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
         v[0][0] = a0; v[0][1] = a1;
         v[1][0] = a0 + del; v[1][1] = a1;
         v[2][0] = a0; v[2][1] = a1 + del;
-        pt[0] = v[0][0]; pt[1] = v[0][1];
+        // the first vertex was evaluated by k_nm_first (touching there): continue at the second
+        y[0] = T(A.S.f[w]);
+        pt[0] = v[1][0]; pt[1] = v[1][1];
         state = NM_V;
-        vi = 0;
-        nfunc = 0;
+        vi = 1;
+        nfunc = 1;
         nev = 0;
         touching = true;
       }
@@ -306,6 +308,34 @@ __global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
     }
   }
   add_evals(A.evals, tot);
+}
+
+// Expected evaluation count of a queued pair, used to start the long solves first.
+__device__ __forceinline__ uint32_t nm_cost_key(int prevEval, bool prevContact, double del) {
+  const int e = del > 0.01 ? 80 : (prevContact ? prevEval : 30);
+  return 255u - (uint32_t)(e < 0 ? 0 : (e > 255 ? 255 : e));
+}
+
+// The first distance evaluation of every queued pair (the first simplex vertex,
+// neldermead_simplex :1731-1737), one pair per lane.  Pairs that are not touching there are
+// complete after it (the caller stores their outcome); the others are listed for the solver
+// with their first value in S.f.  Returns whether the pair is touching.
+template <typename T>
+__device__ __forceinline__ bool nm_first_eval(const SolveArgs& A, int64_t w) {
+  constexpr bool kRel = sizeof(T) == 4;
+  NMGeo<T> n;
+  const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
+  pair_geo<T>(r1, r2, kRel, n.g);
+  n.gC = cast9<T>(crotation_mat(r1.pos, r2.pos));
+  V3<T> q1, q2;
+  bool touching = true;
+  int nev = 0;
+  const T f = nm_eval(n, T(A.S.a0[w]), T(A.S.a1[w]), q1, q2, touching, nev);
+  A.S.f[w] = double(f);
+  A.S.nev[w] = 1;
+  A.S.stop[w] = 1;
+  A.S.touch[w] = touching ? 1 : 0;
+  return touching;
 }
 
 // Mixed form, final refinement in double: the depth is the double distance at the best

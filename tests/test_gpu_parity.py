@@ -52,41 +52,57 @@ def _rel(a, b, floor):
 
 
 @pytest.fixture(scope="module")
-def c1_small():
-    return configs.config1(n_particles=3000)
+def c1_small(gpu):
+    # a 3,000-particle column (24x24 mm footprint) densified into a contact-carrying bed
+    return configs.densify(configs.config1(n_particles=3000, box_xy=24e-3), gpu, iterations=60)
 
 
 def test_hertz_nm_first_steps(gpu, oracle, c1_small):
-    """Geom2 (Nelder-Mead) + Law2 on a bed with contacts: per-contact quantities to 1e-10."""
+    """Geom2 (Nelder-Mead) + Law2 on a bed with contacts, one step from identical states:
+    the interaction set and the contact classification agree exactly, and per-contact depth,
+    normal, contact point, normal force, total force and torques agree to 1e-10 relative.
+
+    The two sides evaluate pow/atan2/sin/cos with different math libraries (last-ulp
+    differences), which the NM decisions absorb but the outputs carry.  A normal is
+    normalize(p1 - p2) of two surface points about `depth` apart at coordinates |x|, so its
+    error is at least ulp(|x|) / depth; for the few contacts shallower than 1e-4 particle sizes
+    the test allows 1e-10 or 1e3 ulp(|x|) / depth, whichever is larger."""
     g = c1_small.open(gpu)
     c = c1_small.open(oracle)
     oracle.set_init_sort(c.ptr, 1)
-    g.step(3)
-    c.step(3)
+    g.step(2)
+    c.restore(g.snapshot())
+    g.step(1)
+    c.step(1)
     assert np.array_equal(_pairs(g), _pairs(c))
     dg, dc = g.get("geom.PenetrationDepth"), c.get("geom.PenetrationDepth")
     act = (dc > 0) & (dg > 0)
-    assert act.sum() > 50
+    assert act.sum() > 1000
     # contacts classified the same way
     assert np.array_equal(dg > 0, dc > 0)
     same_path = g.get("diag.stop") == c.get("diag.stop")
     ok = act & same_path
-    assert ok.sum() >= 0.97 * act.sum()
+    assert ok.sum() >= 0.99 * act.sum()
     assert np.max(_rel(dg[ok], dc[ok], 1e-12)) < 1e-10
+    pos = c.get("state.pos").reshape(-1, 3)
+    xmag = np.linalg.norm(pos[c.get("intr.id2")[ok]], axis=1)
+    cond = np.maximum(1e-10, 1e3 * np.finfo(float).eps * xmag / dc[ok])
     for name in ("geom.normal", "geom.contactPoint", "phys.normalForce", "contact.force"):
         a = g.get(name).reshape(-1, 3)[ok]
         b = c.get(name).reshape(-1, 3)[ok]
-        scale = np.max(np.abs(b), axis=1, keepdims=True)
-        assert np.max(np.abs(a - b) / np.maximum(scale, 1e-300)) < 1e-10, name
+        scale = np.max(np.abs(b), axis=1)
+        err = np.max(np.abs(a - b), axis=1) / np.maximum(scale, 1e-300)
+        tol = 1e-10 if name == "geom.contactPoint" else cond
+        assert np.all(err < tol), (name, float(err.max()), int(np.sum(err >= 1e-10)))
     # torque = (x - contactPoint) x F: its natural scale is |x - cp| |F| (for near-central
     # contacts |T| is far smaller and the cross product cancels)
     F = c.get("contact.force").reshape(-1, 3)[ok]
     cp = c.get("geom.contactPoint").reshape(-1, 3)[ok]
-    pos = c.get("state.pos").reshape(-1, 3)
     for name, ids in (("contact.torque1", "intr.id1"), ("contact.torque2", "intr.id2")):
-        # positions before the last step's integration are not kept; use |cp - x| at the
-        # end of the step, which differs by one step's displacement
+        # positions before the step's integration are not kept; use |cp - x| at the end of
+        # the step, which differs by one step's displacement
         r = np.linalg.norm(cp - pos[c.get(ids)[ok]], axis=1) * np.linalg.norm(F, axis=1)
         a = g.get(name).reshape(-1, 3)[ok]
         b = c.get(name).reshape(-1, 3)[ok]
-        assert np.max(np.max(np.abs(a - b), axis=1) / np.maximum(r, 1e-300)) < 1e-10, name
+        err = np.max(np.abs(a - b), axis=1) / np.maximum(r, 1e-300)
+        assert np.all(err < cond), (name, float(err.max()))
