@@ -80,6 +80,11 @@ __global__ __launch_bounds__(256) void k_geo_pack(BodyArrays B) {
   for (int k = 0; k < 2; k++) g[GEO_E + k] = B.eps[k * N + i];
   for (int k = 0; k < 3; k++) g[GEO_BMIN + k] = B.bmin[k * N + i];
   for (int k = 0; k < 3; k++) g[GEO_BMAX + k] = B.bmax[k * N + i];
+  for (int k = 0; k < 3; k++) g[GEO_VEL + k] = B.vel[k * N + i];
+  for (int k = 0; k < 3; k++) g[GEO_ANGVEL + k] = B.angVel[k * N + i];
+  g[GEO_MASS] = B.mass[i];
+  const uint64_t fl = geo_flags(B.material[i], B.blocked[i], B.kind[i], B.isSphere[i], B.wallAxis[i], B.wallSense[i]);
+  g[GEO_FLAGS] = __longlong_as_double((long long)fl);
 }
 
 // Per-body force and torque: the contributions of the body's interactions summed in
@@ -110,14 +115,14 @@ __global__ __launch_bounds__(256) void k_force_sum(BodyArrays B, PairArrays P, c
     V3<double> F = mk3(0.0, 0.0, 0.0), T = F;
     if (e < e1) {
       const int p = csrPair[e];
-      if (P.active[p]) {
+      if (P.r[p].active[0]) {
         act = true;
-        F = ld3(P.cF, p, S);
-        if (P.id1[p] == (int)b) {
-          T = ld3(P.cT1, p, S);
+        F = ldv3(P.r[p].cF);
+        if (P.r[p].id1[0] == (int)b) {
+          T = ldv3(P.r[p].cT1);
         } else {
           F = -F;
-          T = ld3(P.cT2, p, S);
+          T = ldv3(P.r[p].cT2);
         }
       }
     }
@@ -282,6 +287,8 @@ __global__ __launch_bounds__(256) void k_newton(NewtonArgs A) {
     st3(B.pos, i, N, pos);
     double* gp = B.geo + i * GEO_STRIDE;
     gp[GEO_POS] = pos.x; gp[GEO_POS + 1] = pos.y; gp[GEO_POS + 2] = pos.z;
+    gp[GEO_VEL] = vel.x; gp[GEO_VEL + 1] = vel.y; gp[GEO_VEL + 2] = vel.z;
+    gp[GEO_ANGVEL] = angVel.x; gp[GEO_ANGVEL + 1] = angVel.y; gp[GEO_ANGVEL + 2] = angVel.z;
     // saveMaximaDisplacement (:72-85)
     if (B.hasBound[i]) {
       const V3<double> d = pos - ld3(B.refPos, i, N);

@@ -139,10 +139,10 @@ __global__ void k_fill_u64(uint64_t* a, int64_t n, uint64_t v) {
   if (i < n) a[i] = v;
 }
 
-__global__ void k_hash_insert_pairs(const int32_t* id1, const int32_t* id2, int64_t np, uint64_t* tab, uint64_t mask) {
+__global__ void k_hash_insert_pairs(const PairRec* R, int64_t np, uint64_t* tab, uint64_t mask) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const uint64_t key = upair(id1[p], id2[p]);
+  const uint64_t key = upair(R[p].id1[0], R[p].id2[0]);
   uint64_t h = hmix(key) & mask;
   while (true) {
     const unsigned long long prev = atomicCAS((unsigned long long*)&tab[h], SDM_HEMPTY, (unsigned long long)key);
@@ -207,8 +207,8 @@ __global__ void k_cand_compact(const Cands C, int64_t nc, const uint32_t* alive,
 __global__ void k_erase_flags(BodyArrays B, PairArrays P, int64_t np, uint32_t* flag) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const bool real = P.hasGeom[p] && P.hasPhys[p];
-  flag[p] = (!real && !spatial_overlap(B, P.id1[p], P.id2[p])) ? 1u : 0u;
+  const bool real = P.r[p].hasGeom[0] && P.r[p].hasPhys[0];
+  flag[p] = (!real && !spatial_overlap(B, P.r[p].id1[0], P.r[p].id2[0])) ? 1u : 0u;
 }
 
 __global__ void k_hole_pos(const uint32_t* flag, const uint32_t* E, int64_t K, int32_t* holePos) {
@@ -228,29 +228,33 @@ __global__ void k_tail_moves(const uint32_t* flag, const uint32_t* E, int64_t K,
   mdst[t] = holePos[t];
 }
 
-struct FieldDesc {
-  void* ptr;
-  int32_t esize;
-  int32_t comps;
-};
-
-__global__ void k_move_pairs(const FieldDesc* fd, int nf, int64_t stride, const int32_t* msrc, const int32_t* mdst,
-                             int64_t nm) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Move whole interaction records: record msrc[k] to slot mdst[k] (the swap-with-last
+// compaction of InteractionContainer::erase).  One 16-byte word per lane, 40 lanes per record.
+__global__ void k_move_pairs(PairRec* R, const int32_t* msrc, const int32_t* mdst, int64_t nm) {
+  constexpr int W = (int)(sizeof(PairRec) / 16);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = t / W;
+  const int w = (int)(t % W);
   if (k >= nm) return;
   const int64_t s = msrc[k], d = mdst[k];
-  for (int f = 0; f < nf; f++) {
-    const FieldDesc F = fd[f];
-    for (int c = 0; c < F.comps; c++) {
-      if (F.esize == 8) {
-        uint64_t* a = (uint64_t*)F.ptr;
-        a[c * stride + d] = a[c * stride + s];
-      } else {
-        uint32_t* a = (uint32_t*)F.ptr;
-        a[c * stride + d] = a[c * stride + s];
-      }
-    }
-  }
+  reinterpret_cast<uint4*>(R + d)[w] = reinterpret_cast<const uint4*>(R + s)[w];
+}
+
+// One attribute of every interaction between the records and a row-major buffer
+// (`words` 32-bit words per interaction, at byte offset `off` in the record).
+__global__ void k_gather_field(const PairRec* R, int64_t n, int off, int words, uint32_t* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * words) return;
+  const int64_t i = t / words;
+  const int w = (int)(t % words);
+  out[t] = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(R + i) + off)[w];
+}
+__global__ void k_scatter_field(PairRec* R, int64_t n, int off, int words, const uint32_t* in) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * words) return;
+  const int64_t i = t / words;
+  const int w = (int)(t % words);
+  reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(R + i) + off)[w] = in[t];
 }
 
 // Fresh Interaction(id1,id2) (core/Interaction.cpp:15): no geom, no phys, geomExists.
@@ -259,26 +263,25 @@ __global__ void k_new_pairs(PairArrays P, int64_t start, int64_t count, const Ca
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
   const int64_t p = start + k;
-  const int64_t S = P.cap;
   const uint32_t c = idx ? idx[k] : (uint32_t)k;
-  P.id1[p] = C.a[c];
-  P.id2[p] = C.b[c];
+  PairRec& R = P.r[p];
 #define SDM_ZERO(T, m, kk, name)              \
-  for (int cc = 0; cc < kk; cc++) P.m[cc * S + p] = (T)0;
+  for (int cc = 0; cc < kk; cc++) R.m[cc] = (T)0;
   SDM_PAIR_FIELDS_NOIDS(SDM_ZERO)
 #undef SDM_ZERO
-  P.geomExists[p] = 1;
-  P.isShearNew[p] = 1;
-  P.iterMadeReal[p] = -1;
-  P.tanPhi[p] = __builtin_nan("");
+  R.id1[0] = C.a[c];
+  R.id2[0] = C.b[c];
+  P.r[p].geomExists[0] = 1;
+  P.r[p].isShearNew[0] = 1;
+  P.r[p].iterMadeReal[0] = -1;
+  P.r[p].tanPhi[0] = __builtin_nan("");
 }
 
 // Interaction endpoints for the per-body lists: entry 2p -> (id1, p), 2p+1 -> (id2, p).
-__global__ void k_csr_entries(const int32_t* id1, const int32_t* id2, int64_t np, uint32_t* key, uint32_t* val,
-                              int32_t* deg) {
+__global__ void k_csr_entries(const PairRec* R, int64_t np, uint32_t* key, uint32_t* val, int32_t* deg) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const int a = id1[p], b = id2[p];
+  const int a = R[p].id1[0], b = R[p].id2[0];
   key[2 * p] = (uint32_t)a;
   val[2 * p] = (uint32_t)p;
   key[2 * p + 1] = (uint32_t)b;

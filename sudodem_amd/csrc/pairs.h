@@ -30,8 +30,24 @@ struct StepArgs {
   int64_t iter;
   double dt;
   unsigned long long* counters;  // [0] contact evals, [1] distance evaluations
+  int fullWrite;                 // store every value (first step after host writes)
 };
 
+// Store v unless `lazy` and the stored value is bitwise equal.
+__device__ __forceinline__ void put(int32_t& d, int32_t v, bool lazy) {
+  if (!lazy || d != v) d = v;
+}
+__device__ __forceinline__ void put(double& d, double v, bool lazy) {
+  if (!lazy || __double_as_longlong(d) != __double_as_longlong(v)) d = v;
+}
+
+// Three consecutive values of a record field.
+__device__ __forceinline__ V3<double> ldv3(const double* a) { return mk3(a[0], a[1], a[2]); }
+__device__ __forceinline__ void stv3(double* a, V3<double> v) {
+  a[0] = v.x;
+  a[1] = v.y;
+  a[2] = v.z;
+}
 __device__ __forceinline__ V3<double> ld3(const double* a, int64_t i, int64_t s) {
   return mk3(a[i], a[s + i], a[2 * s + i]);
 }
@@ -73,22 +89,31 @@ __device__ __forceinline__ void load_bounds(const double* geo, int id, V3<double
 }
 
 __device__ __forceinline__ BodyView load_body(const BodyArrays& B, int id) {
-  const int64_t N = B.n;
   BodyView b;
-  const GeoRec r = load_rec(B.geo, id);
-  b.pos = r.pos;
-  b.s = r.s;
-  b.rmax = r.rmax;
-  load_bounds(B.geo, id, b.bmin, b.bmax);
-  b.vel = ld3(B.vel, id, N);
-  b.angVel = ld3(B.angVel, id, N);
-  b.mass = B.mass[id];
-  b.blocked = B.blocked[id];
-  b.material = B.material[id];
-  b.kind = B.kind[id];
-  b.isSphere = B.isSphere[id];
-  b.wallAxis = B.wallAxis[id];
-  b.wallSense = B.wallSense[id];
+  const double2* r = reinterpret_cast<const double2*>(B.geo + (int64_t)id * GEO_STRIDE);
+  double v[GEO_STRIDE];
+#pragma unroll
+  for (int k = 0; k < GEO_STRIDE / 2; k++) {
+    const double2 t = r[k];
+    v[2 * k] = t.x;
+    v[2 * k + 1] = t.y;
+  }
+  b.pos = mk3(v[GEO_POS], v[GEO_POS + 1], v[GEO_POS + 2]);
+  b.s.rx = v[GEO_R]; b.s.ry = v[GEO_R + 1]; b.s.rz = v[GEO_R + 2];
+  b.s.eps1 = v[GEO_E]; b.s.eps2 = v[GEO_E + 1];
+  b.rmax = v[GEO_RMAX];
+  b.bmin = mk3(v[GEO_BMIN], v[GEO_BMIN + 1], v[GEO_BMIN + 2]);
+  b.bmax = mk3(v[GEO_BMAX], v[GEO_BMAX + 1], v[GEO_BMAX + 2]);
+  b.vel = mk3(v[GEO_VEL], v[GEO_VEL + 1], v[GEO_VEL + 2]);
+  b.angVel = mk3(v[GEO_ANGVEL], v[GEO_ANGVEL + 1], v[GEO_ANGVEL + 2]);
+  b.mass = v[GEO_MASS];
+  const uint64_t fl = (uint64_t)__double_as_longlong(v[GEO_FLAGS]);
+  b.material = (int)(uint32_t)fl;
+  b.blocked = (int)((fl >> 32) & 0xFF);
+  b.kind = (int)((fl >> 40) & 3);
+  b.isSphere = (int)((fl >> 42) & 1);
+  b.wallAxis = (int)((fl >> 43) & 3);
+  b.wallSense = (int)((fl >> 45) & 3) - 1;
   return b;
 }
 
@@ -104,11 +129,11 @@ __device__ __forceinline__ void load_geom_prev(const PairArrays& P, int64_t p, b
     g.isShearNew = true;
     g.gSphere = false;
   } else {
-    g.depth = P.depth[p];
-    g.a0 = P.ang[p];
-    g.a1 = P.ang[S + p];
-    g.normal = ld3(P.normal, p, S);
-    g.gSphere = P.gSphere[p] != 0;
+    g.depth = P.r[p].depth[0];
+    g.a0 = P.r[p].ang[0];
+    g.a1 = P.r[p].ang[1];
+    g.normal = ldv3(P.r[p].normal);
+    g.gSphere = P.r[p].gSphere[0] != 0;
     g.isShearNew = g.depth <= 0;  // Superquadrics_Ig2.cpp:1931, 2344, 2537
   }
 }
@@ -163,10 +188,10 @@ __device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
     ph.fs = mk3(0.0, 0.0, 0.0);
     ph.kn = ph.ks = 0;
   } else if (hadPhys) {
-    ph.kn = P.kn[p]; ph.ks = P.ks[p]; ph.tanPhi = P.tanPhi[p];
-    ph.betan = P.betan[p]; ph.betas = P.betas[p];
-    ph.nuab = P.nuab[p]; ph.Gab = P.Gab[p]; ph.mu1 = P.mu1[p]; ph.mu2 = P.mu2[p];
-    ph.fs = ld3(P.fs, p, S);
+    ph.kn = P.r[p].kn[0]; ph.ks = P.r[p].ks[0]; ph.tanPhi = P.r[p].tanPhi[0];
+    ph.betan = P.r[p].betan[0]; ph.betas = P.r[p].betas[0];
+    ph.nuab = P.r[p].nuab[0]; ph.Gab = P.r[p].Gab[0]; ph.mu1 = P.r[p].mu1[0]; ph.mu2 = P.r[p].mu2[0];
+    ph.fs = ldv3(P.r[p].fs);
   } else {
     const MaterialDev m1 = A.mats[b1.material];
     const MaterialDev m2 = A.mats[b2.material];
@@ -271,78 +296,81 @@ __device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
   }
 
   // ---------------- write back ----------------
-  P.nEval[p] = nev;
-  P.stop[p] = stopCode;
-  P.active[p] = contributed ? 1 : 0;
+  // Outside the first step after the host wrote interaction data (A.fullWrite), a value is
+  // only stored when it differs from the stored one: an interaction that stays apart leaves
+  // its record untouched.
+  PairRec& R = P.r[p];
+  const bool lazy = !A.fullWrite;
+  const double prevDepth = R.depth[0];
+  put(R.nEval[0], nev, lazy);
+  put(R.stop[0], stopCode, lazy);
+  put(R.active[0], contributed ? 1 : 0, lazy);
   if (!keep) {
     // Interaction::reset (core/Interaction.cpp:25-29): drop geom and phys; store the defaults
     // a fresh geom/phys starts from.
-    P.hasGeom[p] = 0;
-    P.hasPhys[p] = 0;
-    P.geomExists[p] = 1;
-    P.iterMadeReal[p] = -1;
+    P.r[p].hasGeom[0] = 0;
+    P.r[p].hasPhys[0] = 0;
+    P.r[p].geomExists[0] = 1;
+    P.r[p].iterMadeReal[0] = -1;
     const V3<double> z = mk3(0.0, 0.0, 0.0);
-    P.depth[p] = 0; P.ang[p] = 0; P.ang[S + p] = 0;
-    st3(P.cpt, p, S, z); st3(P.shearInc, p, S, z); st3(P.relVn, p, S, z); st3(P.relVs, p, S, z);
-    st3(P.normal, p, S, z); st3(P.twist, p, S, z); st3(P.orth, p, S, z); st3(P.point1, p, S, z);
-    st3(P.point2, p, S, z);
-    P.isShearNew[p] = 1;
-    P.curv1[p] = 0; P.curv1[S + p] = 0; P.curv2[p] = 0; P.curv2[S + p] = 0;
-    P.curvAngle[p] = 0; P.alpha[p] = 0; P.hK[p] = 0; P.hE[p] = 0; P.curvSum[p] = 0; P.gSphere[p] = 0;
-    P.kn[p] = 0; P.ks[p] = 0; P.tanPhi[p] = __builtin_nan(""); P.betan[p] = 0; P.betas[p] = 0;
-    st3(P.fn, p, S, z); st3(P.fs, p, S, z);
-    P.nuab[p] = 0; P.Gab[p] = 0; P.mu1[p] = 0; P.mu2[p] = 0;
+    P.r[p].depth[0] = 0; P.r[p].ang[0] = 0; P.r[p].ang[1] = 0;
+    stv3(P.r[p].cpt, z); stv3(P.r[p].shearInc, z); stv3(P.r[p].relVn, z); stv3(P.r[p].relVs, z);
+    stv3(P.r[p].normal, z); stv3(P.r[p].twist, z); stv3(P.r[p].orth, z); stv3(P.r[p].point1, z);
+    stv3(P.r[p].point2, z);
+    P.r[p].isShearNew[0] = 1;
+    P.r[p].curv1[0] = 0; P.r[p].curv1[1] = 0; P.r[p].curv2[0] = 0; P.r[p].curv2[1] = 0;
+    P.r[p].curvAngle[0] = 0; P.r[p].alpha[0] = 0; P.r[p].hK[0] = 0; P.r[p].hE[0] = 0; P.r[p].curvSum[0] = 0; P.r[p].gSphere[0] = 0;
+    P.r[p].kn[0] = 0; P.r[p].ks[0] = 0; P.r[p].tanPhi[0] = __builtin_nan(""); P.r[p].betan[0] = 0; P.r[p].betas[0] = 0;
+    stv3(P.r[p].fn, z); stv3(P.r[p].fs, z);
+    P.r[p].nuab[0] = 0; P.r[p].Gab[0] = 0; P.r[p].mu1[0] = 0; P.r[p].mu2[0] = 0;
     return;
   }
-  if (!(hadGeom && hadPhys)) P.iterMadeReal[p] = A.iter;
-  P.hasGeom[p] = 1;
-  P.hasPhys[p] = 1;
-  P.depth[p] = g.depth;
-  P.isShearNew[p] = g.isShearNew ? 1 : 0;
+  if (!(hadGeom && hadPhys)) P.r[p].iterMadeReal[0] = A.iter;
+  put(R.hasGeom[0], 1, lazy);
+  put(R.hasPhys[0], 1, lazy);
+  put(R.depth[0], g.depth, lazy);
+  put(R.isShearNew[0], g.isShearNew ? 1 : 0, lazy);
   if (outcome != GEO_EARLY) {
-    P.ang[p] = g.a0;
-    P.ang[S + p] = g.a1;
+    put(R.ang[0], g.a0, lazy);
+    put(R.ang[1], g.a1, lazy);
   }
   if (outcome == GEO_FULL) {
-    st3(P.normal, p, S, g.normal); st3(P.cpt, p, S, g.cpt); st3(P.shearInc, p, S, g.shearInc);
-    st3(P.relVn, p, S, g.relVn); st3(P.relVs, p, S, g.relVs); st3(P.twist, p, S, g.twist);
-    st3(P.orth, p, S, g.orth); st3(P.point1, p, S, g.point1); st3(P.point2, p, S, g.point2);
+    stv3(P.r[p].normal, g.normal); stv3(P.r[p].cpt, g.cpt); stv3(P.r[p].shearInc, g.shearInc);
+    stv3(P.r[p].relVn, g.relVn); stv3(P.r[p].relVs, g.relVs); stv3(P.r[p].twist, g.twist);
+    stv3(P.r[p].orth, g.orth); stv3(P.r[p].point1, g.point1); stv3(P.r[p].point2, g.point2);
     if (LAW == 1 && b1.kind != SDM_KIND_WALL) {
-      P.curvSum[p] = g.curvSum;
-      P.gSphere[p] = g.gSphere ? 1 : 0;
+      P.r[p].curvSum[0] = g.curvSum;
+      P.r[p].gSphere[0] = g.gSphere ? 1 : 0;
       if (!g.gSphere) {
-        P.curv1[p] = g.c10; P.curv1[S + p] = g.c11; P.curv2[p] = g.c20; P.curv2[S + p] = g.c21;
-        P.curvAngle[p] = g.curvAngle; P.alpha[p] = g.alpha; P.hK[p] = g.K; P.hE[p] = g.E;
+        P.r[p].curv1[0] = g.c10; P.r[p].curv1[1] = g.c11; P.r[p].curv2[0] = g.c20; P.r[p].curv2[1] = g.c21;
+        P.r[p].curvAngle[0] = g.curvAngle; P.r[p].alpha[0] = g.alpha; P.r[p].hK[0] = g.K; P.r[p].hE[0] = g.E;
       }
     }
   }
   if (!hadPhys) {
-    P.tanPhi[p] = ph.tanPhi; P.betan[p] = ph.betan; P.betas[p] = ph.betas;
-    P.nuab[p] = ph.nuab; P.Gab[p] = ph.Gab; P.mu1[p] = ph.mu1; P.mu2[p] = ph.mu2;
+    P.r[p].tanPhi[0] = ph.tanPhi; P.r[p].betan[0] = ph.betan; P.r[p].betas[0] = ph.betas;
+    P.r[p].nuab[0] = ph.nuab; P.r[p].Gab[0] = ph.Gab; P.r[p].mu1[0] = ph.mu1; P.r[p].mu2[0] = ph.mu2;
   }
   if (!hadPhys || contributed) {  // kn/ks change only in Ip2 and in Law2's contact branch
-    P.kn[p] = ph.kn;
-    P.ks[p] = ph.ks;
+    P.r[p].kn[0] = ph.kn;
+    P.r[p].ks[0] = ph.ks;
   }
-  st3(P.fn, p, S, ph.fn);
-  st3(P.fs, p, S, ph.fs);
+  // A kept interaction without force stores zero forces (:1217-1220); after such a step
+  // (stored depth <= 1e-18, phys present) they are zero already.
+  if (contributed || !lazy || !hadPhys || prevDepth > 1E-18) {
+    stv3(R.fn, ph.fn);
+    stv3(R.fs, ph.fs);
+  }
   if (contributed) {
-    st3(P.cF, p, S, F);
-    st3(P.cT1, p, S, -cross(b1.pos - g.cpt, F));
-    st3(P.cT2, p, S, cross(b2.pos - g.cpt, F));
+    stv3(P.r[p].cF, F);
+    stv3(P.r[p].cT1, -cross(b1.pos - g.cpt, F));
+    stv3(P.r[p].cT2, cross(b2.pos - g.cpt, F));
   }
 }
 
-// Reduce a wave's counters into the global ones with one atomic per wave.
+// Add the block's counters into the global ones (one atomic per block).
 __device__ __forceinline__ void add_counters(unsigned long long* c, unsigned long long ce, unsigned long long de) {
-  for (int off = 32; off > 0; off >>= 1) {
-    ce += __shfl_down(ce, off);
-    de += __shfl_down(de, off);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    if (ce) atomicAdd(&c[0], ce);
-    if (de) atomicAdd(&c[1], de);
-  }
+  block_add2(c, ce, de);
 }
 
 // Outcome of a superquadric pair that needs no further evaluation: depth 0 and a new
@@ -373,33 +401,32 @@ __device__ __forceinline__ void early_store(const StepArgs& A, int64_t p, int id
 // box bound settle are completed here, the others queued for the solver; wall and sphere
 // pairs are listed for k_pairs_misc.
 template <int LAW>
-__global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, SolveIO S, unsigned long long* qcount,
-                                                    int32_t* mlist, unsigned long long* mcount) {
+__global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag, int32_t* mlist,
+                                                    unsigned long long* mcount) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long ce = 0;
   bool enqueue = false, misc = false;
   int id1 = 0, id2 = 0;
-  double sa0 = 0, sa1 = 0, sdel = 0;
   const PairArrays& P = A.P;
   const BodyArrays& B = A.B;
   if (p < A.np) {
-    P.active[p] = 0;
-    if (P.geomExists[p]) {
-      id1 = P.id1[p];
-      id2 = P.id2[p];
+    if (P.r[p].active[0]) P.r[p].active[0] = 0;
+    if (P.r[p].geomExists[0]) {
+      id1 = P.r[p].id1[0];
+      id2 = P.r[p].id2[0];
       bool skip = false;
-      if (!P.functorSet[p]) {
+      if (!P.r[p].functorSet[0]) {
         // dispatch order (Wall,Superquadrics) / (Superquadrics,Superquadrics); swapOrder otherwise
         const int k1 = B.kind[id1], k2 = B.kind[id2];
         if (k1 == SDM_KIND_WALL && k2 == SDM_KIND_WALL) {
-          P.geomExists[p] = 0;
+          P.r[p].geomExists[0] = 0;
           skip = true;
         } else {
-          P.functorSet[p] = 1;
+          P.r[p].functorSet[0] = 1;
           if (k1 == SDM_KIND_SQ && k2 == SDM_KIND_WALL) {
             int t = id1; id1 = id2; id2 = t;
-            P.id1[p] = id1;
-            P.id2[p] = id2;
+            P.r[p].id1[0] = id1;
+            P.r[p].id2[0] = id2;
           }
         }
       }
@@ -411,8 +438,8 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, SolveIO S, unsig
         } else {
           // Superquadric pair: bounding spheres, then the separation bound of the solver's
           // first evaluation.  Only the packed records are read on this path.
-          const bool hadGeom = P.hasGeom[p] != 0;
-          const bool hadPhys = P.hasPhys[p] != 0;
+          const bool hadGeom = P.r[p].hasGeom[0] != 0;
+          const bool hadPhys = P.r[p].hasPhys[0] != 0;
           const bool isNew = !hadGeom;
           ce++;
           const GeoRec r1 = load_rec(B.geo, id1), r2 = load_rec(B.geo, id2);
@@ -427,8 +454,8 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, SolveIO S, unsig
             para0 = 0;
             para1 = 0;
           } else {
-            para0 = P.ang[p];
-            para1 = P.ang[P.cap + p];
+            para0 = P.r[p].ang[0];
+            para1 = P.r[p].ang[1];
           }
           int outcome = -1, stopCode = 0;
           if (norm(r1.pos - r2.pos) > r1.rmax + r2.rmax) {
@@ -464,30 +491,55 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, SolveIO S, unsig
             early_store<LAW>(A, p, id1, id2, hadGeom, hadPhys, para0, para1, outcome, 0, stopCode);
           } else {
             enqueue = true;
-            sa0 = para0;
-            sa1 = para1;
-            sdel = isNew ? 0.1 : 0.001;
           }
         }
       }
     }
   }
+  if (p < A.np) qflag[p] = enqueue ? 1u : 0u;
   add_counters(A.counters, ce, 0);
-  const int64_t ms = fetch_work(misc, mcount);
+  const int64_t ms = fetch_work(misc, mcount);  // wall / sphere pairs are few: rare atomics
   if (misc) mlist[ms] = (int32_t)p;
-  const int64_t slot = fetch_work(enqueue, qcount);
-  if (enqueue) {
-    S.pair[slot] = (int32_t)p;
-    S.id1[slot] = id1;
-    S.id2[slot] = id2;
-    S.a0[slot] = sa0;
-    S.a1[slot] = sa1;
-    S.del[slot] = sdel;
-    S.nev[slot] = 0;
-    S.tag[slot] = 0;
-    S.touch[slot] = 1;
-    st3s(S.ct, slot, S.cap, mk3(0.0, 0.0, 0.0));
+}
+
+// The solver queue in interaction order: slot qpos[p] for every flagged interaction p, with
+// the start parameters of the Ig2 functor (new LM pair: the line of centres plus 1e-5,
+// Superquadrics_Ig2.cpp:1919-1924; new NM pair: (0, 0) in its CRotationMat frame, :2355;
+// otherwise the stored contactAngle) and the NM simplex size (0.1 new, 0.001 otherwise).
+template <int LAW>
+__global__ __launch_bounds__(256) void k_queue_fill(StepArgs A, SolveIO S, const uint32_t* qflag,
+                                                    const uint32_t* qpos, int64_t np) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np || !qflag[p]) return;
+  const int64_t slot = qpos[p];
+  const PairRec& R = A.P.r[p];
+  const int id1 = R.id1[0], id2 = R.id2[0];
+  const bool isNew = !R.hasGeom[0];
+  double a0, a1;
+  if (LAW == 0 && isNew) {
+    const double* g1 = A.B.geo + (int64_t)id1 * GEO_STRIDE;
+    const double* g2 = A.B.geo + (int64_t)id2 * GEO_STRIDE;
+    const V3<double> d0 = mk3(g2[GEO_POS], g2[GEO_POS + 1], g2[GEO_POS + 2]) -
+                          mk3(g1[GEO_POS], g1[GEO_POS + 1], g1[GEO_POS + 2]);
+    a0 = atan2(d0.y, d0.x) + 1e-5;
+    a1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y)) + 1e-5;
+  } else if (isNew) {
+    a0 = 0;
+    a1 = 0;
+  } else {
+    a0 = R.ang[0];
+    a1 = R.ang[1];
   }
+  S.pair[slot] = (int32_t)p;
+  S.id1[slot] = id1;
+  S.id2[slot] = id2;
+  S.a0[slot] = a0;
+  S.a1[slot] = a1;
+  S.del[slot] = isNew ? 0.1 : 0.001;
+  S.nev[slot] = 0;
+  S.tag[slot] = 0;
+  S.touch[slot] = 1;
+  st3s(S.ct, slot, S.cap, mk3(0.0, 0.0, 0.0));
 }
 
 // Wall-superquadric pairs (Ig2_Wall_Superquadrics_SuperquadricsGeom(2)::go,
@@ -502,9 +554,9 @@ __global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* m
     const BodyArrays& B = A.B;
     const int64_t N = B.n;
     const int64_t p = mlist[k];
-    const int id1 = P.id1[p], id2 = P.id2[p];
-    const bool hadGeom = P.hasGeom[p] != 0;
-    const bool hadPhys = P.hasPhys[p] != 0;
+    const int id1 = P.r[p].id1[0], id2 = P.r[p].id2[0];
+    const bool hadGeom = P.r[p].hasGeom[0] != 0;
+    const bool hadPhys = P.r[p].hasPhys[0] != 0;
     const bool isNew = !hadGeom;
         const BodyView b1 = load_body(B, id1);
     const BodyView b2 = load_body(B, id2);
@@ -599,31 +651,38 @@ __global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* m
 // touching there is complete: its outcome is stored here (angles unchanged, one function
 // call).  Touching pairs are listed with a sort key that puts the expected long solves first.
 template <typename T>
-__global__ __launch_bounds__(256) void k_nm_first(StepArgs A, SolveArgs SA, uint64_t* keys, uint32_t* list,
-                                                  unsigned long long* cnt) {
+__global__ __launch_bounds__(256) void k_nm_first(StepArgs A, SolveArgs SA, uint32_t* tflag, uint64_t* tkey) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool go = false;
   uint32_t key = 0;
   if (w < SA.nwork) {
     const PairArrays& P = A.P;
     const int64_t p = SA.S.pair[w];
-    const bool hadGeom = P.hasGeom[p] != 0;
+    const bool hadGeom = P.r[p].hasGeom[0] != 0;
     if (nm_first_eval<T>(SA, w)) {
       go = true;
-      key = nm_cost_key(P.nEval[p], hadGeom && P.depth[p] > 0, SA.S.del[w]);
+      key = nm_cost_key(P.r[p].nEval[0], hadGeom && P.r[p].depth[0] > 0, SA.S.del[w]);
     } else {
-      early_store<1>(A, p, SA.S.id1[w], SA.S.id2[w], hadGeom, P.hasPhys[p] != 0, SA.S.a0[w], SA.S.a1[w],
+      early_store<1>(A, p, SA.S.id1[w], SA.S.id2[w], hadGeom, P.r[p].hasPhys[0] != 0, SA.S.a0[w], SA.S.a1[w],
                      GEO_EARLY_ANG, 1, 1);
     }
+  }
+  if (w < SA.nwork) {
+    tflag[w] = go ? 1u : 0u;
+    tkey[w] = key;
   }
   add_evals(SA.evals, w < SA.nwork ? 1ull : 0ull);
   // listed pairs are counted when they finish; the others are complete here
   add_counters(A.counters, 0, (w < SA.nwork && !go) ? 1ull : 0ull);
-  const int64_t slot = fetch_work(go, cnt);
-  if (go) {
-    keys[slot] = key;
-    list[slot] = (uint32_t)w;
-  }
+}
+
+// Compaction of the touching slots (in slot order) with their sort keys.
+__global__ __launch_bounds__(256) void k_list_fill(const uint32_t* flag, const uint32_t* pos, const uint64_t* key,
+                                                   int64_t n, uint64_t* keysOut, uint32_t* listOut) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n || !flag[w]) return;
+  keysOut[pos[w]] = key[w];
+  listOut[pos[w]] = (uint32_t)w;
 }
 
 // Restart checks of Ig2_..._SuperquadricsGeom::go after an LM pass (Superquadrics_Ig2.cpp:2013-2052).
@@ -677,8 +736,8 @@ __global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, con
     const int64_t N = B.n;
     const int64_t p = S.pair[w];
     const int id1 = S.id1[w], id2 = S.id2[w];
-    const bool hadGeom = P.hasGeom[p] != 0;
-    const bool hadPhys = P.hasPhys[p] != 0;
+    const bool hadGeom = P.r[p].hasGeom[0] != 0;
+    const bool hadPhys = P.r[p].hasPhys[0] != 0;
     const BodyView b1 = load_body(B, id1);
     const BodyView b2 = load_body(B, id2);
     GeomState g;

@@ -94,8 +94,6 @@ struct sdm_ctx {
   Cands C;
   int32_t *msrc = nullptr, *mdst = nullptr, *holePos = nullptr;
   int64_t mcap = 0;
-  FieldDesc* dfields = nullptr;
-  int nfields = 0;
   // device counters: [0] contact evals, [1] dist evals, [2] candidates, [3] solver queue,
   // [4] solver evaluations, [5] other evaluations, [6] misc list, [7] touching list
   unsigned long long* dcnt = nullptr;
@@ -104,12 +102,17 @@ struct sdm_ctx {
   SolveIO S;
   int32_t* retryList = nullptr;
   int32_t* miscList = nullptr;
+  uint32_t* qflag = nullptr;  // per interaction / per slot flags and their exclusive scan
+  uint32_t* qpos = nullptr;
   int numCU = 0;
   int residentNM[2] = {0, 0};  // resident blocks of k_solve_nm<float>, <double> (0: not queried)
   int residentLM = 0;
   bool matChecked = false;
   bool solverRan = false;
   bool geoDirty = true;
+  bool pairsWritten = true;  // host wrote interaction data since the last step
+  uint32_t* stage = nullptr;  // attribute staging buffer (32-bit words)
+  int64_t stageCap = 0;
   // per-phase timing
   bool timing = false;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -139,41 +142,31 @@ void alloc_bodies(sdm_ctx* c, int32_t n) {
   c->B.geo = dalloc<double>((int64_t)n * GEO_STRIDE);
 }
 
-void free_pairs(PairArrays& P) {
-#define SDM_FREE(T, m, k, name) dfree(P.m);
-  SDM_PAIR_FIELDS(SDM_FREE)
-#undef SDM_FREE
-}
+void free_pairs(PairArrays& P) { dfree(P.r); }
 
-void build_field_table(sdm_ctx* c) {
-  std::vector<FieldDesc> fd;
-#define SDM_FD(T, m, k, name) fd.push_back(FieldDesc{(void*)c->P.m, (int32_t)sizeof(T), (int32_t)k});
-  SDM_PAIR_FIELDS(SDM_FD)
-#undef SDM_FD
-  dfree(c->dfields);
-  c->dfields = dalloc<FieldDesc>((int64_t)fd.size());
-  HIPCHK(hipMemcpy(c->dfields, fd.data(), fd.size() * sizeof(FieldDesc), hipMemcpyHostToDevice));
-  c->nfields = (int)fd.size();
+uint32_t* staging(sdm_ctx* c, int64_t words) {
+  if (words > c->stageCap) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(c->stage);
+    c->stageCap = words + words / 4 + 1024;
+    c->stage = dalloc<uint32_t>(c->stageCap);
+  }
+  return c->stage;
 }
 
 // Grow the interaction arrays to hold at least `need` entries, keeping [0, np).
 void ensure_pair_cap(sdm_ctx* c, int64_t need) {
-  if (need <= c->P.cap && c->P.id1) return;
+  if (need <= c->P.cap && c->P.r) return;
   int64_t cap = c->P.cap > 0 ? c->P.cap : 1024;
   while (cap < need) cap = cap * 3 / 2 + 1024;
   PairArrays Q;
   Q.cap = cap;
-#define SDM_GROW(T, m, k, name)                                                                     \
-  Q.m = dalloc<T>(cap * k);                                                                         \
-  if (c->P.m && c->np > 0)                                                                          \
-    for (int cc = 0; cc < k; cc++)                                                                  \
-      HIPCHK(hipMemcpyAsync(Q.m + cc * cap, c->P.m + cc * c->P.cap, c->np * sizeof(T), hipMemcpyDeviceToDevice, c->st));
-  SDM_PAIR_FIELDS(SDM_GROW)
-#undef SDM_GROW
+  Q.r = dalloc<PairRec>(cap);
+  if (c->P.r && c->np > 0)
+    HIPCHK(hipMemcpyAsync(Q.r, c->P.r, c->np * sizeof(PairRec), hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   free_pairs(c->P);
   c->P = Q;
-  build_field_table(c);
   c->csrValid = false;
 }
 
@@ -243,6 +236,8 @@ void ensure_queue(sdm_ctx* c, int64_t n) {
   free_queue(c->S);
   dfree(c->retryList);
   dfree(c->miscList);
+  dfree(c->qflag);
+  dfree(c->qpos);
   const int64_t cap = n + n / 2 + 1024;
   SolveIO& S = c->S;
   S.cap = cap;
@@ -254,6 +249,8 @@ void ensure_queue(sdm_ctx* c, int64_t n) {
   S.stop = dalloc<int32_t>(cap); S.touch = dalloc<int32_t>(cap);
   c->retryList = dalloc<int32_t>(cap);
   c->miscList = dalloc<int32_t>(cap);
+  c->qflag = dalloc<uint32_t>(cap + 1);
+  c->qpos = dalloc<uint32_t>(cap + 1);
 }
 
 uint32_t read_u32(sdm_ctx* c, const uint32_t* p) {
@@ -289,7 +286,7 @@ void build_csr(sdm_ctx* c) {
   HIPCHK(hipMemsetAsync(deg, 0, (N + 1) * sizeof(int32_t), c->st));
   if (c->np > 0) {
     uint32_t* key = c->u1;
-    hipLaunchKernelGGL(k_csr_entries, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.id1, c->P.id2, c->np, key, c->v0,
+    hipLaunchKernelGGL(k_csr_entries, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.r, c->np, key, c->v0,
                        deg);
     int bits = 1;
     while ((1ll << bits) < N) bits++;
@@ -337,8 +334,8 @@ void collider_run(sdm_ctx* c) {
       // moves = holes below K = erased entries below K = E[K] (= kept entries at >= K)
       const int64_t nmove = read_u32(c, E + K);
       if (nmove > 0)
-        hipLaunchKernelGGL(k_move_pairs, dim3(nblk(nmove)), dim3(256), 0, c->st, c->dfields, c->nfields, c->P.cap,
-                           c->msrc, c->mdst, nmove);
+        hipLaunchKernelGGL(k_move_pairs, dim3(nblk(nmove * (int64_t)(sizeof(PairRec) / 16))), dim3(256), 0, c->st,
+                           c->P.r, c->msrc, c->mdst, nmove);
       c->np = K;
       c->cnt.pairs_erased += Etot;
     }
@@ -391,7 +388,7 @@ void collider_run(sdm_ctx* c) {
     hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->dkeys, c->hcap, SDM_HEMPTY);
     hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->dvals, c->hcap, SDM_HEMPTY);
     if (c->np > 0)
-      hipLaunchKernelGGL(k_hash_insert_pairs, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.id1, c->P.id2, c->np,
+      hipLaunchKernelGGL(k_hash_insert_pairs, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.r, c->np,
                          c->htab, mask);
     uint32_t* alive = c->u0;
     uint32_t* pos = c->u1;
@@ -439,18 +436,24 @@ unsigned resident_grid(sdm_ctx* c, K kernel, int& cache, int64_t nwork) {
 // expected solves first, the mixed form's double refinement, and the finish kernel.
 template <typename T>
 void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA, int64_t nq) {
-  ensure_scratch(c, nq);
-  zero_counter(c, 7);
+  ensure_scratch(c, nq + 1);
   SA.evals = c->dcnt + 5;
-  hipLaunchKernelGGL(k_nm_first<T>, dim3(nblk(nq)), dim3(256), 0, c->st, A, SA, c->k0, c->v0, c->dcnt + 7);
-  const int64_t n2 = (int64_t)read_counter(c, 7);
+  hipLaunchKernelGGL(k_nm_first<T>, dim3(nblk(nq)), dim3(256), 0, c->st, A, SA, c->qflag, c->k1);
+  const int64_t n2 = exclusive_total(c, c->qflag, c->qpos, nq);
   if (n2 == 0) return;
+  hipLaunchKernelGGL(k_list_fill, dim3(nblk(nq)), dim3(256), 0, c->st, c->qflag, c->qpos, c->k1, nq, c->k0, c->v0);
+  // the list in queue order (increasing interaction index) for the finish kernel, whose
+  // record reads and writes then follow memory order; a copy sorted by expected cost for
+  // the solver
+  uint32_t* byPair = c->u0;
+  HIPCHK(hipMemcpyAsync(byPair, c->v0, n2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->st));
   sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, n2, 8, c->rsScratch, c->st);
   const uint32_t* list = c->v0;
   SolveArgs S2 = SA;
   S2.list = reinterpret_cast<const int32_t*>(list);
   S2.nwork = n2;
   S2.evals = c->dcnt + 4;
+  S2.next = c->dcnt + 3;
   zero_counter(c, 3);
   const int ti = sizeof(T) == 4 ? 0 : 1;
   const unsigned grid = resident_grid(c, k_solve_nm<T>, c->residentNM[ti], n2);
@@ -460,10 +463,11 @@ void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA, int64_t nq) {
   c->solverRan = true;
   if (sizeof(T) == 4) {
     SolveArgs RF = S2;
+    RF.list = reinterpret_cast<const int32_t*>(byPair);
     RF.evals = c->dcnt + 5;
     hipLaunchKernelGGL(k_nm_refine, dim3(nblk(n2)), dim3(256), 0, c->st, RF);
   }
-  hipLaunchKernelGGL((k_pairs_finish<1>), dim3(nblk(n2)), dim3(256), 0, c->st, A, c->S, list, n2);
+  hipLaunchKernelGGL((k_pairs_finish<1>), dim3(nblk(n2)), dim3(256), 0, c->st, A, c->S, byPair, n2);
 }
 
 // InteractionLoop::action: prep (every interaction), wall / sphere pairs, solvers (queued
@@ -472,14 +476,15 @@ template <int LAW>
 void interaction_loop(sdm_ctx* c, const StepArgs& A) {
   if (A.np <= 0) return;
   ensure_queue(c, A.np);
-  HIPCHK(hipMemsetAsync(c->dcnt + 3, 0, sizeof(unsigned long long), c->st));
+  ensure_scratch(c, A.np + 1);
   HIPCHK(hipMemsetAsync(c->dcnt + 6, 0, sizeof(unsigned long long), c->st));
-  hipLaunchKernelGGL((k_pairs_prep<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, c->dcnt + 3, c->miscList,
+  hipLaunchKernelGGL((k_pairs_prep<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->qflag, c->miscList,
                      c->dcnt + 6);
-  unsigned long long qc[4] = {0, 0, 0, 0};
-  HIPCHK(hipMemcpyAsync(qc, c->dcnt + 3, sizeof(qc), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  const int64_t nq = (int64_t)qc[0], nm = (int64_t)qc[3];
+  // the solver queue: scan of the flags (no shared counter), then the slots in interaction order
+  const int64_t nq = exclusive_total(c, c->qflag, c->qpos, A.np);
+  const int64_t nm = (int64_t)read_counter(c, 6);
+  if (nq > 0)
+    hipLaunchKernelGGL((k_queue_fill<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, c->qflag, c->qpos, A.np);
   if (nm > 0)
     hipLaunchKernelGGL((k_pairs_misc<LAW>), dim3(nblk(nm)), dim3(256), 0, c->st, A, c->miscList, nm);
   if (c->timing) c->tms[7] += (double)nq;
@@ -558,6 +563,8 @@ void one_step(sdm_ctx* c) {
   A.iter = c->iter;
   A.dt = c->prm.dt;
   A.counters = c->dcnt;
+  A.fullWrite = c->pairsWritten ? 1 : 0;
+  c->pairsWritten = false;
   if (c->prm.law == SDM_LAW_LINEAR)
     interaction_loop<0>(c, A);
   else
@@ -612,7 +619,7 @@ struct FieldInfo {
   int dtype;
   int comps;
   int esize;
-  size_t off;  // offset of the pointer member in BodyArrays / PairArrays, or scalar id
+  size_t off;  // offset of the pointer member in BodyArrays, of the field in PairRec, or scalar id
 };
 template <typename T>
 constexpr int dtype_of() {
@@ -622,7 +629,7 @@ const FieldInfo kFields[] = {
 #define SDM_BF(T, m, k, name) {name, S_BODY, dtype_of<T>(), k, (int)sizeof(T), offsetof(BodyArrays, m)},
     SDM_BODY_FIELDS(SDM_BF)
 #undef SDM_BF
-#define SDM_PF(T, m, k, name) {name, S_PAIR, dtype_of<T>(), k, (int)sizeof(T), offsetof(PairArrays, m)},
+#define SDM_PF(T, m, k, name) {name, S_PAIR, dtype_of<T>(), k, (int)sizeof(T), offsetof(PairRec, m)},
         SDM_PAIR_FIELDS(SDM_PF)
 #undef SDM_PF
             {"scene.iter", S_SCALAR, SDM_I64, 1, 8, 0},
@@ -762,7 +769,7 @@ void sdm_destroy(sdm_ctx* c) {
   dfree(c->k0); dfree(c->k1); dfree(c->v0); dfree(c->v1); dfree(c->u0); dfree(c->u1); dfree(c->rsScratch);
   dfree(c->htab); dfree(c->dkeys); dfree(c->dvals);
   dfree(c->C.a); dfree(c->C.b); dfree(c->C.key);
-  dfree(c->msrc); dfree(c->mdst); dfree(c->holePos); dfree(c->dfields);
+  dfree(c->msrc); dfree(c->mdst); dfree(c->holePos); dfree(c->stage);
   dfree(c->dmats); dfree(c->dcnt); dfree(c->dMaxDisp);
   for (int k = 0; k < 6; k++)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
@@ -851,11 +858,20 @@ int sdm_get(sdm_ctx* c, const char* name, void* dst, int64_t count) {
   if (count != n) return fail(c, std::string("sdm_get: count mismatch for ") + name);
   try {
     HIPCHK(hipStreamSynchronize(c->st));
-    if (f->scope == S_BODY || f->scope == S_PAIR) {
-      const int64_t rows = f->scope == S_BODY ? c->B.n : c->np;
-      const int64_t stride = f->scope == S_BODY ? c->B.n : c->P.cap;
-      const char* base = f->scope == S_BODY ? (const char*)&c->B : (const char*)&c->P;
-      const char* dptr = *(char* const*)(base + f->off);
+    if (f->scope == S_PAIR) {
+      // one row per interaction: gathered out of the records on the device, then one copy
+      const int words = f->comps * f->esize / 4;
+      if (c->np > 0) {
+        uint32_t* buf = staging(c, c->np * words);
+        hipLaunchKernelGGL(k_gather_field, dim3(nblk(c->np * words)), dim3(256), 0, c->st, c->P.r, c->np, (int)f->off,
+                           words, buf);
+        HIPCHK(hipMemcpyAsync(dst, buf, (size_t)c->np * words * 4, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+      }
+    } else if (f->scope == S_BODY) {
+      const int64_t rows = c->B.n;
+      const int64_t stride = c->B.n;
+      const char* dptr = *(char* const*)((const char*)&c->B + f->off);
       std::vector<char> tmp((size_t)rows * f->esize);
       char* out = (char*)dst;
       for (int k = 0; k < f->comps; k++) {
@@ -909,11 +925,21 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
     }
     const int64_t n = field_count(c, f);
     if (count != n) return fail(c, std::string("sdm_set: count mismatch for ") + name);
-    if (f->scope == S_BODY || f->scope == S_PAIR) {
-      const int64_t rows = f->scope == S_BODY ? c->B.n : c->np;
-      const int64_t stride = f->scope == S_BODY ? c->B.n : c->P.cap;
-      const char* base = f->scope == S_BODY ? (const char*)&c->B : (const char*)&c->P;
-      char* dptr = *(char* const*)(base + f->off);
+    if (f->scope == S_PAIR) {
+      const int words = f->comps * f->esize / 4;
+      if (c->np > 0) {
+        uint32_t* buf = staging(c, c->np * words);
+        HIPCHK(hipMemcpyAsync(buf, src, (size_t)c->np * words * 4, hipMemcpyHostToDevice, c->st));
+        hipLaunchKernelGGL(k_scatter_field, dim3(nblk(c->np * words)), dim3(256), 0, c->st, c->P.r, c->np, (int)f->off,
+                           words, buf);
+        HIPCHK(hipStreamSynchronize(c->st));
+      }
+      c->csrValid = false;
+      c->pairsWritten = true;
+    } else if (f->scope == S_BODY) {
+      const int64_t rows = c->B.n;
+      const int64_t stride = c->B.n;
+      char* dptr = *(char* const*)((const char*)&c->B + f->off);
       std::vector<char> tmp((size_t)rows * f->esize);
       const char* in = (const char*)src;
       for (int k = 0; k < f->comps; k++) {
@@ -922,11 +948,8 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
         if (rows > 0)
           HIPCHK(hipMemcpy(dptr + (size_t)k * stride * f->esize, tmp.data(), rows * f->esize, hipMemcpyHostToDevice));
       }
-      if (f->scope == S_PAIR) c->csrValid = false;
-      if (f->scope == S_BODY) {
-        c->matChecked = false;
-        c->geoDirty = true;
-      }
+      c->matChecked = false;
+      c->geoDirty = true;
     } else {
       switch (f->off) {
         case 0: c->iter = *(const int64_t*)src; break;

@@ -4,7 +4,7 @@
 // evaluations per pair (Nelder-Mead: 1 to ~60, Levenberg-Marquardt: 1 to ~200).  Run one
 // pair per thread to completion and a wavefront waits for its slowest lane.  Here each
 // solver is a per-lane state machine whose loop body is exactly one distance evaluation;
-// a lane that finishes its pair pulls the next one from the work queue, so all 64 lanes
+// a lane that finishes its pair takes the next one from its wave's batch, so all 64 lanes
 // of a wave execute the same evaluation code in lockstep whatever stage their pair is in.
 // The sequence of evaluations and decisions of every pair is that of the reference:
 //   neldermead_simplex + amotry (pkg/dem/Superquadrics_Ig2.cpp:1681-1875)
@@ -46,13 +46,47 @@ struct SolveArgs {
   SolveIO S;
   const int32_t* list;  // work slots to process (nullptr: slots 0..nwork-1)
   int64_t nwork;
-  unsigned long long* next;   // work counter, zero at launch
+  unsigned long long* next;   // work counter (64-entry batches), zero at launch
   unsigned long long* evals;  // distance evaluations done by this launch (added)
 };
 
+// Add per-thread values a, b into c[0], c[1] with one atomic per block: a device-scope
+// atomic on one address from every wavefront serialises across the chip.  Every thread of
+// the block must call it.
+__device__ __forceinline__ void block_add2(unsigned long long* c, unsigned long long a, unsigned long long b) {
+  __shared__ unsigned long long part[2][16];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off);
+    b += __shfl_down(b, off);
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    part[0][wv] = a;
+    part[1][wv] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sa = 0, sb = 0;
+    for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) {
+      sa += part[0][k];
+      sb += part[1][k];
+    }
+    if (sa) atomicAdd(&c[0], sa);
+    if (sb) atomicAdd(&c[1], sb);
+  }
+  __syncthreads();
+}
 __device__ __forceinline__ void add_evals(unsigned long long* c, unsigned long long v) {
+  __shared__ unsigned long long part[16];
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(c, v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s = 0;
+    for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) s += part[k];
+    if (s) atomicAdd(c, s);
+  }
+  __syncthreads();
 }
 
 __device__ __forceinline__ void st3s(double* a, int64_t i, int64_t s, V3<double> v) {
@@ -127,6 +161,43 @@ __device__ __forceinline__ int64_t fetch_work(bool want, unsigned long long* cou
   return want ? (int64_t)(base + __popcll(below)) : -1;
 }
 
+// Work distribution of the persistent solvers: a wavefront claims 64 consecutive list
+// entries with one atomic, and its lanes take them one at a time as they go idle.  One
+// shared-counter atomic per 64 pairs (instead of one per wavefront iteration) keeps the
+// counter off the critical path, while idle lanes still refill at once.
+struct WaveBatch {
+  int64_t base = 0, end = 0;
+  bool exhausted = false;
+};
+__device__ __forceinline__ int64_t take_work(bool want, WaveBatch& b, unsigned long long* counter, int64_t n) {
+  const int lane = threadIdx.x & 63;
+  uint64_t m = __ballot(want);
+  int64_t got = -1;
+  while (m) {
+    if (b.base >= b.end) {
+      if (b.exhausted) break;
+      long long base = 0;
+      if (lane == 0) base = (long long)atomicAdd(counter, 64ull);
+      base = __shfl(base, 0);
+      if (base >= n) {
+        b.exhausted = true;
+        break;
+      }
+      b.base = base;
+      b.end = base + 64 < n ? base + 64 : n;
+    }
+    const int64_t avail = b.end - b.base;
+    const uint64_t below = lane ? (m & (~0ull >> (64 - lane))) : 0ull;
+    const int rank = __popcll(below);
+    const bool wanting = (m >> lane) & 1ull;
+    if (wanting && rank < avail) got = b.base + rank;
+    const int64_t total = __popcll(m);
+    b.base += total < avail ? total : avail;
+    m = __ballot(wanting && rank >= avail);
+  }
+  return got;
+}
+
 // ---------------------------------------------------------------------------------------
 // Nelder-Mead (neldermead_simplex, ftol = 1e-3, NMAX = 5000; nfunc starts at 0)
 // ---------------------------------------------------------------------------------------
@@ -147,11 +218,12 @@ __global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
   V3<T> q1 = mk3(T(0), T(0), T(0)), q2 = q1;
   bool done = false;
   unsigned long long tot = 0;
+  WaveBatch batch;
   for (;;) {
     const bool idle = (w < 0) && !done;
-    const int64_t got = fetch_work(idle, A.next);
+    const int64_t got = take_work(idle, batch, A.next, A.nwork);
     if (idle) {
-      if (got >= A.nwork) {
+      if (got < 0) {
         done = true;
       } else {
         w = A.list ? A.list[got] : got;
@@ -380,11 +452,12 @@ __global__ __launch_bounds__(256) void k_solve_lm(SolveArgs A) {
   int64_t w = -1;
   bool done = false;
   unsigned long long tot = 0;
+  WaveBatch batch;
   for (;;) {
     const bool idle = (w < 0) && !done;
-    const int64_t got = fetch_work(idle, A.next);
+    const int64_t got = take_work(idle, batch, A.next, A.nwork);
     if (idle) {
-      if (got >= A.nwork) {
+      if (got < 0) {
         done = true;
       } else {
         w = A.list ? A.list[got] : got;

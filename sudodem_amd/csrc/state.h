@@ -45,39 +45,29 @@
   X(int32_t, id2, 1, "intr.id2")                             \
   SDM_PAIR_FIELDS_NOIDS(X)
 
+// Field order = record layout: the first 64 bytes (ids, flags, iterMadeReal, depth) and the
+// angles are what every interaction reads each step; a contact also touches the next lines.
 #define SDM_PAIR_FIELDS_NOIDS(X)                             \
   X(int32_t, hasGeom, 1, "intr.hasGeom")                     \
   X(int32_t, hasPhys, 1, "intr.hasPhys")                     \
   X(int32_t, functorSet, 1, "intr.functorSet")               \
   X(int32_t, geomExists, 1, "intr.geomExists")               \
+  X(int32_t, isShearNew, 1, "geom.isShearNew")               \
+  X(int32_t, gSphere, 1, "geom.isSphere")                    \
+  X(int32_t, active, 1, "contact.active")                    \
+  X(int32_t, nEval, 1, "diag.nEval")                         \
+  X(int32_t, stop, 1, "diag.stop")                           \
   X(int64_t, iterMadeReal, 1, "intr.iterMadeReal")           \
   X(double, depth, 1, "geom.PenetrationDepth")               \
   X(double, ang, 2, "geom.contactAngle")                     \
-  X(double, cpt, 3, "geom.contactPoint")                     \
-  X(double, shearInc, 3, "geom.shearInc")                    \
-  X(double, relVn, 3, "geom.relativeVn")                     \
-  X(double, relVs, 3, "geom.relativeVs")                     \
   X(double, normal, 3, "geom.normal")                        \
-  X(double, twist, 3, "geom.twist_axis")                     \
-  X(double, orth, 3, "geom.orthonormal_axis")                \
-  X(double, point1, 3, "geom.point1")                        \
-  X(double, point2, 3, "geom.point2")                        \
-  X(int32_t, isShearNew, 1, "geom.isShearNew")               \
-  X(double, curv1, 2, "geom.curvatures1")                    \
-  X(double, curv2, 2, "geom.curvatures2")                    \
-  X(double, curvAngle, 1, "geom.curvatureAngle")             \
-  X(double, alpha, 1, "geom.alpha")                          \
-  X(double, hK, 1, "geom.K")                                 \
-  X(double, hE, 1, "geom.E")                                 \
-  X(double, curvSum, 1, "geom.curvatures_sum")               \
-  X(int32_t, gSphere, 1, "geom.isSphere")                    \
+  X(double, fs, 3, "phys.shearForce")                        \
+  X(double, fn, 3, "phys.normalForce")                       \
   X(double, kn, 1, "phys.kn")                                \
   X(double, ks, 1, "phys.ks")                                \
   X(double, tanPhi, 1, "phys.tangensOfFrictionAngle")        \
   X(double, betan, 1, "phys.betan")                          \
   X(double, betas, 1, "phys.betas")                          \
-  X(double, fn, 3, "phys.normalForce")                       \
-  X(double, fs, 3, "phys.shearForce")                        \
   X(double, nuab, 1, "phys.nuab")                            \
   X(double, Gab, 1, "phys.Gab")                              \
   X(double, mu1, 1, "phys.mu1")                              \
@@ -85,29 +75,62 @@
   X(double, cF, 3, "contact.force")                          \
   X(double, cT1, 3, "contact.torque1")                       \
   X(double, cT2, 3, "contact.torque2")                       \
-  X(int32_t, active, 1, "contact.active")                    \
-  X(int32_t, nEval, 1, "diag.nEval")                         \
-  X(int32_t, stop, 1, "diag.stop")
+  X(double, cpt, 3, "geom.contactPoint")                     \
+  X(double, shearInc, 3, "geom.shearInc")                    \
+  X(double, relVn, 3, "geom.relativeVn")                     \
+  X(double, relVs, 3, "geom.relativeVs")                     \
+  X(double, twist, 3, "geom.twist_axis")                     \
+  X(double, orth, 3, "geom.orthonormal_axis")                \
+  X(double, point1, 3, "geom.point1")                        \
+  X(double, point2, 3, "geom.point2")                        \
+  X(double, curv1, 2, "geom.curvatures1")                    \
+  X(double, curv2, 2, "geom.curvatures2")                    \
+  X(double, curvAngle, 1, "geom.curvatureAngle")             \
+  X(double, alpha, 1, "geom.alpha")                          \
+  X(double, hK, 1, "geom.K")                                 \
+  X(double, hE, 1, "geom.E")                                 \
+  X(double, curvSum, 1, "geom.curvatures_sum")
+
+// One interaction = one record (array of structures).  The kernels reach an interaction
+// through queues and per-body lists in no particular order, and a contact reads and
+// writes ~60 values: as one record these are 5 whole 128-byte lines, as separate arrays
+// they were ~60 partly used lines.  640 bytes keeps every record line-aligned.
+struct PairRec {
+#define SDM_PREC(T, m, k, name) T m[k];
+  SDM_PAIR_FIELDS(SDM_PREC)
+#undef SDM_PREC
+  double pad_[10];
+};
+static_assert(sizeof(PairRec) == 640, "interaction record size");
 
 struct BodyArrays {
   int32_t n;
 #define SDM_DECL(T, m, k, name) T* m;
   SDM_BODY_FIELDS(SDM_DECL)
 #undef SDM_DECL
-  // Packed copy of what the pair kernels gather per body, one 192-byte record per body:
-  // pos[3], r_max, rot_mat2global[9], semi-axes[3], exponents[2], bound min[3], max[3].
-  // Interactions reference bodies in no spatial order, so a record (12 x 16-byte loads,
-  // the first two holding pos and r_max) replaces 24 scattered 8-byte reads.
+  // Packed copy of what the pair kernels gather per body, one 256-byte record per body:
+  // pos[3], r_max, rot_mat2global[9], semi-axes[3], exponents[2], bound min[3], max[3],
+  // vel[3], angVel[3], mass, and the packed flags (geo_flags).  Interactions reference
+  // bodies in no spatial order, so a record (two 128-byte lines) replaces ~30 scattered
+  // reads; the narrowphase reads only the first 144 bytes.
   double* geo;
 };
 
-enum { GEO_POS = 0, GEO_RMAX = 3, GEO_G = 4, GEO_R = 13, GEO_E = 16, GEO_BMIN = 18, GEO_BMAX = 21, GEO_STRIDE = 24 };
+enum {
+  GEO_POS = 0, GEO_RMAX = 3, GEO_G = 4, GEO_R = 13, GEO_E = 16, GEO_BMIN = 18, GEO_BMAX = 21,
+  GEO_VEL = 24, GEO_ANGVEL = 27, GEO_MASS = 30, GEO_FLAGS = 31, GEO_STRIDE = 32
+};
+// Static per-body integers in one 64-bit word: material (low 32 bits), then blockedDOFs (8),
+// kind (2), isSphere (1), wallAxis (2), wallSense + 1 (2).
+__host__ __device__ inline uint64_t geo_flags(int material, int blocked, int kind, int isSphere, int wallAxis,
+                                              int wallSense) {
+  return (uint64_t)(uint32_t)material | ((uint64_t)(blocked & 0xFF) << 32) | ((uint64_t)(kind & 3) << 40) |
+         ((uint64_t)(isSphere & 1) << 42) | ((uint64_t)(wallAxis & 3) << 43) | ((uint64_t)((wallSense + 1) & 3) << 45);
+}
 
 struct PairArrays {
-  int64_t cap;  // plane stride
-#define SDM_DECL(T, m, k, name) T* m;
-  SDM_PAIR_FIELDS(SDM_DECL)
-#undef SDM_DECL
+  PairRec* r;   // [cap] records; the first np are the interaction container in linIntrs order
+  int64_t cap;
 };
 
 struct MaterialDev {
