@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -32,6 +34,32 @@ PEAK_FP64 = 78.6e12
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+ERROR_FIELDS = ("geom.PenetrationDepth", "geom.normal", "geom.contactPoint", "phys.normalForce", "contact.force",
+                "diag.stop")
+
+
+def mixed_error(ref, mix):
+    """Per-contact error of the mixed form against the FP64 path after one step from the same
+    state: relative depth error and relative vector errors (|a - b| / |b|) over the contacts
+    both forms report, with the share of contacts whose solve took the same path."""
+    d64, dmx = ref["geom.PenetrationDepth"], mix["geom.PenetrationDepth"]
+    both = (d64 > 0) & (dmx > 0)
+    out = {"contacts": int(both.sum()), "classified_differently": int(((d64 > 0) != (dmx > 0)).sum()),
+           "same_solver_path": float((ref["diag.stop"][both] == mix["diag.stop"][both]).mean())}
+
+    def stats(err):
+        q = np.percentile(err, [50, 99, 99.9]) if len(err) else [0, 0, 0]
+        return {"median": float(q[0]), "p99": float(q[1]), "p999": float(q[2]), "max": float(err.max()) if len(err) else 0,
+                "frac_le_1e-5": float((err <= 1e-5).mean()) if len(err) else 1.0}
+    out["depth"] = stats(np.abs(dmx[both] - d64[both]) / d64[both])
+    for k, name in (("geom.normal", "normal"), ("geom.contactPoint", "contact_point"), ("phys.normalForce", "normal_force"),
+                    ("contact.force", "force")):
+        a = mix[k].reshape(-1, 3)[both]
+        b = ref[k].reshape(-1, 3)[both]
+        out[name] = stats(np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300))
+    return out
 
 
 def timed_run(ctx, steps):
@@ -60,7 +88,6 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     from sudodem_amd import SDM_FP64, SDM_MIXED, configs, load_hip
@@ -91,6 +118,7 @@ def main():
     n_pairs = len(depth)
 
     results = {}
+    one_step = {}
     for name, prec in (("fp64", SDM_FP64), ("mixed", SDM_MIXED)):
         ctx = Context(lib, cfg.params(prec), cfg.n, [m._native() for m in cfg.materials])
         ctx.restore(snap)
@@ -100,8 +128,10 @@ def main():
         results[name] = (wall, cnt, tm)
         log("%s: %.3f ms/step, solver %.3f ms/step, %d collider runs" % (
             name, wall / args.steps * 1e3, tm["solver_ms"] / args.steps, cnt["collider_runs"]))
-        if name == "mixed":
-            mixed_state = {k: ctx.get(k) for k in ("geom.PenetrationDepth",)}
+        # one step from the snapshot again, kept for the per-contact error of the mixed form
+        ctx.restore(snap)
+        ctx.step(1)
+        one_step[name] = {k: ctx.get(k) for k in ERROR_FIELDS}
         ctx.close()
 
     N = cfg.n - 6
@@ -164,6 +194,7 @@ def main():
         "solids_fraction": cfg.solids_fraction,
         "queued_per_step": {"fp64": t64["queued"] / K, "mixed": tmx["queued"] / K},
         "phases_ms_per_step": {k: tmx[k] / K for k in ("collider_ms", "interactions_ms", "newton_ms", "solver_ms")},
+        "mixed_error_vs_fp64": mixed_error(one_step["fp64"], one_step["mixed"]),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -52,6 +52,56 @@ SDM_HD V3<T> support(const Shape<T>& s, const M3<T>& G, V3<T> dir) {
   return mul(G, get_surface(s, ph0, ph1));
 }
 
+// ---------------------------------------------------------------------------------------
+// Float support map of the mixed form.  Same formula as normal2phi + get_surface, evaluated
+// with two identities and a short pow, since the float path only steers the solver (its
+// result is re-evaluated in double):
+//  * cos(atan2(a, b)) = b / |(a, b)| and sin(atan2(a, b)) = a / |(a, b)| (atan2(0, 0) = 0):
+//    the four sin/cos of getSurface and the cos of Normal2Phi follow from the two atan2
+//    arguments with one reciprocal square root each;
+//  * x^y for x >= 0, y > 0 as 2^(y log2 x): hardware log2, one Newton step on 2^L = x for
+//    the logarithm, y L as an exact product (fma residual), and hardware exp2 with a first
+//    order correction for the low part -- a few ulp, against ~150 instructions for powf.
+// ---------------------------------------------------------------------------------------
+SDM_HD float fpow(float x, float y) {
+  if (!(x > 0.0f)) return 0.0f;
+  const float L0 = __builtin_amdgcn_logf(x);               // log2 x, hardware
+  const float r = fmaf(x, __builtin_amdgcn_exp2f(-L0), -1.0f);  // x 2^-L0 - 1, |r| ~ 1e-7
+  const float Llo = r * 1.44269504088896341f;              // r / ln 2: Newton correction of L0
+  const float p = y * L0;
+  const float plo = fmaf(y, L0, -p) + y * Llo;             // y L = p + plo
+  const float z = __builtin_amdgcn_exp2f(p);
+  return fmaf(z, plo * 0.693147180559945309f, z);          // 2^p (1 + plo ln 2)
+}
+
+// (cos, sin) of atan2(a, b) without the angle.
+SDM_HD void unit_angle(float a, float b, float& c, float& s) {
+  const float m = fmaxf(fabsf(a), fabsf(b));
+  if (!(m > 0.0f)) {
+    c = 1.0f;
+    s = 0.0f;
+    return;
+  }
+  const float im = 1.0f / m;
+  const float an = a * im, bn = b * im;
+  const float ih = rsqrtf(an * an + bn * bn);
+  c = bn * ih;
+  s = an * ih;
+}
+
+SDM_HD V3<float> support_f(const Shape<float>& s, const M3<float>& G, V3<float> dir) {
+  const V3<float> n = mulT(G, dir);
+  const float e1 = 1.0f / (2.0f - s.eps1), e2 = 1.0f / (2.0f - s.eps2);
+  const float ax = fabsf(s.rx * n.x);
+  float c0, s0, c1, s1;
+  unit_angle(sgn(n.y) * fpow(fabsf(s.ry * n.y), e1), sgn(n.x) * fpow(ax, e1), c0, s0);
+  unit_angle(sgn(n.z) * fpow(fabsf(s.rz * n.z * fpow(fabsf(c0), 2.0f - s.eps1)), e2), fpow(ax, e2), c1, s1);
+  const float pc1 = fpow(fabsf(c1), s.eps2);
+  const V3<float> q = mk3(sgn(c0) * s.rx * fpow(fabsf(c0), s.eps1) * pc1, sgn(s0) * s.ry * fpow(fabsf(s0), s.eps1) * pc1,
+                          sgn(s1) * s.rz * fpow(fabsf(s1), s.eps2));
+  return mul(G, q);
+}
+
 // isInside (Superquadrics.cpp:103-108); pg relative to the centre, global frame.
 SDM_HD bool is_inside(const Shape<double>& s, const M3<double>& G, V3<double> pg) {
   V3<double> p = mulT(G, pg);
@@ -115,8 +165,11 @@ template <typename T>
 SDM_HD T nm_eval(const NMGeo<T>& n, T a0, T a1, V3<T>& p1, V3<T>& p2, bool& touching, int& nev) {
   nev++;
   const V3<T> contact = mul(n.gC, mk3(xcos(a0) * xcos(a1), xsin(a0) * xcos(a1), xsin(a1)));
-  p1 = support(n.g.s1, n.g.G1, contact) + n.g.pos1;
-  {
+  if constexpr (sizeof(T) == 4) {
+    p1 = support_f(n.g.s1, n.g.G1, contact) + n.g.pos1;
+    p2 = support_f(n.g.s2, n.g.G2, mk3(-contact.x, -contact.y, -contact.z)) + n.g.pos2;
+  } else {
+    p1 = support(n.g.s1, n.g.G1, contact) + n.g.pos1;
     T ph0, ph1;
     V3<T> c2 = mulT(n.g.G2, contact);
     normal2phi(n.g.s2, mk3(T(-1) * c2.x, T(-1) * c2.y, T(-1) * c2.z), ph0, ph1);
