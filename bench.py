@@ -27,6 +27,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# HBM bytes per launch of k_solve_nm<float> on this workload: rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (median over the timed steps), scaled by the same counters on a 1 GiB copy
+# (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); profiles/c1_200k_hbm_traffic.md.
+SOLVER_TRAFFIC_BYTES = (220800 * 2.0 + 166498 * 1.0) * 1024
 FLOP_EQ_PER_EVAL = 392  # pow=8, sin/cos=4, atan2=16, div=4, add/mul=1 over one Disfun evaluation
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
@@ -73,16 +77,35 @@ def timed_run(ctx, steps):
     return wall, ctx.counters(), ctx.timers()
 
 
+def calibrate_copy(nbytes=1 << 30, reps=4):
+    """A plain device-to-device copy of `nbytes`, `reps` times: per dispatch it reads and
+    writes exactly nbytes, which scales the HBM byte counters of a profiler run."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_dispatch": nbytes, "bytes_written_per_dispatch": nbytes,
+                      "dispatches": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=200000)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--densify", type=int, default=150, help="iterations of the packing algorithm")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a device copy of known size (the reference for rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
                           "n_gpus": args.gpus}))
@@ -144,7 +167,7 @@ def main():
     solver_s = tmx["solver_ms"] * 1e-3
     achieved = tmx["solver_evals"] * FLOP_EQ_PER_EVAL / solver_s if solver_s > 0 else 0.0
     roof = {"bound": "valu", "kernel": "k_solve_nm<float>", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": SOLVER_TRAFFIC_BYTES if args.n == 200000 else None,
             "flop_eq_per_eval": FLOP_EQ_PER_EVAL, "evals_per_step": tmx["solver_evals"] / K,
             "kernel_ms_per_step": tmx["solver_ms"] / K}
     solver64_s = t64["solver_ms"] * 1e-3
