@@ -114,15 +114,18 @@ __global__ __launch_bounds__(256) void k_force_sum(BodyArrays B, PairArrays P, c
     bool act = false;
     V3<double> F = mk3(0.0, 0.0, 0.0), T = F;
     if (e < e1) {
-      const int p = csrPair[e];
-      if (P.r[p].active[0]) {
+      // the entry says whether the interaction contributed this step and which side the
+      // body is on; only contributing interactions are read
+      const int32_t v = csrPair[e];
+      if (v & 1) {
         act = true;
-        F = ldv3(P.r[p].cF);
-        if (P.r[p].id1[0] == (int)b) {
-          T = ldv3(P.r[p].cT1);
+        const PairRec& R = P.r[v >> 2];
+        F = ldv3(R.cF);
+        if (((v >> 1) & 1) == 0) {
+          T = ldv3(R.cT1);
         } else {
           F = -F;
-          T = ldv3(P.r[p].cT2);
+          T = ldv3(R.cT2);
         }
       }
     }

@@ -240,6 +240,19 @@ __global__ void k_move_pairs(PairRec* R, const int32_t* msrc, const int32_t* mds
   reinterpret_cast<uint4*>(R + d)[w] = reinterpret_cast<const uint4*>(R + s)[w];
 }
 
+// List entry e (sorted by body, interaction order within a body) -> its interaction: the
+// entry holds p << 2 | side << 1 | active, and the record remembers where its two entries
+// are, so the law can mark a contributing contact in both lists.
+__global__ void k_csr_finalize(PairRec* R, const uint32_t* sorted, int64_t m, int32_t* csr) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  const uint32_t v = sorted[e];
+  const int64_t p = v >> 1;
+  const int side = (int)(v & 1u);
+  R[p].csrPos[side] = (int32_t)e;
+  csr[e] = csr_entry(p, side, R[p].active[0] != 0);
+}
+
 // One attribute of every interaction between the records and a row-major buffer
 // (`words` 32-bit words per interaction, at byte offset `off` in the record).
 __global__ void k_gather_field(const PairRec* R, int64_t n, int off, int words, uint32_t* out) {
@@ -277,15 +290,15 @@ __global__ void k_new_pairs(PairArrays P, int64_t start, int64_t count, const Ca
   P.r[p].tanPhi[0] = __builtin_nan("");
 }
 
-// Interaction endpoints for the per-body lists: entry 2p -> (id1, p), 2p+1 -> (id2, p).
+// Interaction endpoints for the per-body lists: entry 2p -> (id1, 2p), 2p+1 -> (id2, 2p+1).
 __global__ void k_csr_entries(const PairRec* R, int64_t np, uint32_t* key, uint32_t* val, int32_t* deg) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const int a = R[p].id1[0], b = R[p].id2[0];
   key[2 * p] = (uint32_t)a;
-  val[2 * p] = (uint32_t)p;
+  val[2 * p] = (uint32_t)(2 * p);
   key[2 * p + 1] = (uint32_t)b;
-  val[2 * p + 1] = (uint32_t)p;
+  val[2 * p + 1] = (uint32_t)(2 * p + 1);
   atomicAdd(&deg[a], 1);
   atomicAdd(&deg[b], 1);
 }

@@ -31,7 +31,12 @@ struct StepArgs {
   double dt;
   unsigned long long* counters;  // [0] contact evals, [1] distance evaluations
   int fullWrite;                 // store every value (first step after host writes)
+  int32_t* csr;                  // per-body interaction lists (entries carry the active bit)
 };
+
+__device__ __forceinline__ int32_t csr_entry(int64_t p, int side, bool active) {
+  return (int32_t)((p << 2) | (side << 1) | (active ? 1 : 0));
+}
 
 // Store v unless `lazy` and the stored value is bitwise equal.
 __device__ __forceinline__ void put(int32_t& d, int32_t v, bool lazy) {
@@ -362,6 +367,8 @@ __device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
     stv3(R.fs, ph.fs);
   }
   if (contributed) {
+    A.csr[R.csrPos[0]] = csr_entry(p, 0, true);
+    A.csr[R.csrPos[1]] = csr_entry(p, 1, true);
     stv3(P.r[p].cF, F);
     stv3(P.r[p].cT1, -cross(b1.pos - g.cpt, F));
     stv3(P.r[p].cT2, cross(b2.pos - g.cpt, F));
@@ -410,7 +417,11 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
   const PairArrays& P = A.P;
   const BodyArrays& B = A.B;
   if (p < A.np) {
-    if (P.r[p].active[0]) P.r[p].active[0] = 0;
+    if (P.r[p].active[0]) {  // ForceResetter: last step's contribution is withdrawn
+      P.r[p].active[0] = 0;
+      A.csr[P.r[p].csrPos[0]] = csr_entry(p, 0, false);
+      A.csr[P.r[p].csrPos[1]] = csr_entry(p, 1, false);
+    }
     if (P.r[p].geomExists[0]) {
       id1 = P.r[p].id1[0];
       id2 = P.r[p].id2[0];
@@ -443,9 +454,11 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
           const bool isNew = !hadGeom;
           ce++;
           const GeoRec r1 = load_rec(B.geo, id1), r2 = load_rec(B.geo, id2);
+          const V3<double> x1 = r1.pos, x2 = r2.pos;
+          const double rmax1 = r1.rmax, rmax2 = r2.rmax;
           double para0, para1;
           if (LAW == 0 && isNew) {
-            const V3<double> d0 = r2.pos - r1.pos;
+            const V3<double> d0 = x2 - x1;
             para0 = atan2(d0.y, d0.x);
             para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
             para0 += 1e-5;
@@ -458,7 +471,7 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
             para1 = P.r[p].ang[1];
           }
           int outcome = -1, stopCode = 0;
-          if (norm(r1.pos - r2.pos) > r1.rmax + r2.rmax) {
+          if (norm(x1 - x2) > rmax1 + rmax2) {
             outcome = GEO_EARLY;  // Superquadrics_Ig2.cpp:1975-1979, 2384-2388
           } else {
             // The first distance evaluation of either solver is at `para` along the contact

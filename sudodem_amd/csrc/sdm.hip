@@ -292,7 +292,7 @@ void build_csr(sdm_ctx* c) {
     while ((1ll << bits) < N) bits++;
     // keys are 32-bit: reuse the 64-bit sort buffers through a 32-bit view
     sdmrs::sort_pairs<uint32_t, uint32_t>(key, c->v0, (uint32_t*)c->k1, c->v1, m, bits, c->rsScratch, c->st);
-    HIPCHK(hipMemcpyAsync(c->csrPair, c->v0, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+    hipLaunchKernelGGL(k_csr_finalize, dim3(nblk(m)), dim3(256), 0, c->st, c->P.r, c->v0, m, c->csrPair);
   }
   sdmrs::scan<sdmrs::OpAdd, false, false>((uint32_t*)deg, (uint32_t*)c->csrOff, N + 1, c->rsScratch, c->st);
   c->csrValid = true;
@@ -563,6 +563,7 @@ void one_step(sdm_ctx* c) {
   A.iter = c->iter;
   A.dt = c->prm.dt;
   A.counters = c->dcnt;
+  A.csr = c->csrPair;
   A.fullWrite = c->pairsWritten ? 1 : 0;
   c->pairsWritten = false;
   if (c->prm.law == SDM_LAW_LINEAR)

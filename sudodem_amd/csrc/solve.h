@@ -98,7 +98,7 @@ __device__ __forceinline__ V3<double> ld3s(const double* a, int64_t i, int64_t s
   return mk3(a[i], a[s + i], a[2 * s + i]);
 }
 
-// One packed body record (bounds excluded).
+// One packed body record (the part the narrowphase uses: up to the exponents).
 struct GeoRec {
   V3<double> pos;
   M3<double> G;
@@ -107,9 +107,9 @@ struct GeoRec {
 };
 __device__ __forceinline__ GeoRec load_rec(const double* geo, int id) {
   const double2* r = reinterpret_cast<const double2*>(geo + (int64_t)id * GEO_STRIDE);
-  double v[GEO_BMIN];
+  double v[GEO_VEL];
 #pragma unroll
-  for (int k = 0; k < GEO_BMIN / 2; k++) {
+  for (int k = 0; k < GEO_VEL / 2; k++) {
     const double2 t = r[k];
     v[2 * k] = t.x;
     v[2 * k + 1] = t.y;
@@ -425,9 +425,13 @@ __global__ __launch_bounds__(256) void k_nm_refine(SolveArgs A) {
   V3<double> q1, q2;
   bool touching = true;
   int nev = 0;
-  const double f = nm_eval(n, A.S.a0[w], A.S.a1[w], q1, q2, touching, nev);
-  bool t2 = true;
-  nm_eval(n, A.S.la0[w], A.S.la1[w], q1, q2, t2, nev);
+  const double a0 = A.S.a0[w], a1 = A.S.a1[w], l0 = A.S.la0[w], l1 = A.S.la1[w];
+  const double f = nm_eval(n, a0, a1, q1, q2, touching, nev);
+  // the last point is often the best vertex: its points are those just computed
+  if (__double_as_longlong(l0) != __double_as_longlong(a0) || __double_as_longlong(l1) != __double_as_longlong(a1)) {
+    bool t2 = true;
+    nm_eval(n, l0, l1, q1, q2, t2, nev);
+  }
   const int64_t S = A.S.cap;
   A.S.f[w] = f;
   st3s(A.S.p1, w, S, q1);

@@ -99,7 +99,9 @@ struct PairRec {
 #define SDM_PREC(T, m, k, name) T m[k];
   SDM_PAIR_FIELDS(SDM_PREC)
 #undef SDM_PREC
-  double pad_[10];
+  // positions of the interaction's two entries in the per-body lists (rebuilt with them)
+  int32_t csrPos[2];
+  double pad_[9];
 };
 static_assert(sizeof(PairRec) == 640, "interaction record size");
 
@@ -109,15 +111,16 @@ struct BodyArrays {
   SDM_BODY_FIELDS(SDM_DECL)
 #undef SDM_DECL
   // Packed copy of what the pair kernels gather per body, one 256-byte record per body:
-  // pos[3], r_max, rot_mat2global[9], semi-axes[3], exponents[2], bound min[3], max[3],
+  // pos[3], r_max, bound min[3], max[3], rot_mat2global[9], semi-axes[3], exponents[2],
   // vel[3], angVel[3], mass, and the packed flags (geo_flags).  Interactions reference
   // bodies in no spatial order, so a record (two 128-byte lines) replaces ~30 scattered
-  // reads; the narrowphase reads only the first 144 bytes.
+  // reads.  The first line alone (pos, r_max, bounds) settles the bounding-sphere test and
+  // the law's erase test, which is all most interactions need.
   double* geo;
 };
 
 enum {
-  GEO_POS = 0, GEO_RMAX = 3, GEO_G = 4, GEO_R = 13, GEO_E = 16, GEO_BMIN = 18, GEO_BMAX = 21,
+  GEO_POS = 0, GEO_RMAX = 3, GEO_BMIN = 4, GEO_BMAX = 7, GEO_G = 10, GEO_R = 19, GEO_E = 22,
   GEO_VEL = 24, GEO_ANGVEL = 27, GEO_MASS = 30, GEO_FLAGS = 31, GEO_STRIDE = 32
 };
 // Static per-body integers in one 64-bit word: material (low 32 bits), then blockedDOFs (8),
