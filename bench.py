@@ -91,6 +91,56 @@ def calibrate_copy(nbytes=1 << 30, reps=4):
                       "dispatches": reps}))
 
 
+def batch_bench(args):
+    """Batching workload: specimens with log-uniform particle counts in [2000, 50000], each with
+    its own Kn and friction (linear law, LM geometry), advanced together in one context, against
+    the same specimens each in its own context stepped one after the other."""
+    import torch
+
+    from sudodem_amd import configs, load_hip
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    lib = load_hip()
+    t0 = time.perf_counter()
+    specs = configs.config4(n_specimens=args.batch)
+    b = configs.batch(specs)
+    log("%d specimens, %d bodies, generated in %.1fs" % (len(specs), b.n, time.perf_counter() - t0))
+    nparts = sum(c.n - 6 for c in specs)
+    K, W = args.steps, args.warmup
+    bc = b.open(lib)
+    bc.step(W)
+    bc.sync()
+    t0 = time.perf_counter()
+    bc.step(K)
+    bc.sync()
+    tb = time.perf_counter() - t0
+    log("batched: %.3f ms/step" % (tb / K * 1e3))
+    bc.close()
+    ctxs = [c.open(lib) for c in specs]
+    for c in ctxs:
+        c.step(W)
+    for c in ctxs:
+        c.sync()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        for c in ctxs:
+            c.step(1)
+    for c in ctxs:
+        c.sync()
+    ts = time.perf_counter() - t0
+    log("one by one: %.3f ms/step" % (ts / K * 1e3))
+    print(json.dumps({
+        "metric": "aggregate particle-steps/s, %d independent specimens in one context" % len(specs),
+        "value": nparts * K / tb, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": tb / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%d box specimens (log-uniform 2000-50000 superellipsoids, own Kn and friction, "
+                               "linear law)" % len(specs), "particles": nparts, "parallelism": "single GPU"},
+        "one_by_one": {"value": nparts * K / ts, "ms_per_step": ts / K * 1e3},
+        "batched_over_one_by_one": ts / tb,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,11 +150,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--densify", type=int, default=150, help="iterations of the packing algorithm")
+    ap.add_argument("--batch", type=int, default=0, metavar="SPECIMENS",
+                    help="batching workload instead: this many box specimens in one context vs one by one")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of known size (the reference for rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.batch:
+        batch_bench(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",

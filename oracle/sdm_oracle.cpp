@@ -244,6 +244,7 @@ inline unsigned axisDOF(int axis, bool rot) { return 1u << (axis + (rot ? 3 : 0)
 struct Body {
   // state (core/State.hpp)
   double pos[3], ori[4], vel[3], angVel[3], angMom[3], mass, inertia[3];
+  int32_t specimen;  // batching attribute of the device library; the oracle runs one specimen
   int32_t blockedDOFs, isDamped;
   // shape
   int32_t kind;  // SDM_BODY_SUPERQUADRIC or SDM_BODY_WALL
@@ -1635,7 +1636,7 @@ const Field kFields[] = {
     BF("shape.r_max", r_max, SDM_F64, 1, true), BF("shape.isSphere", isSphere, SDM_I32, 1, true),
     BF("shape.wallAxis", wallAxis, SDM_I32, 1, true), BF("shape.wallSense", wallSense, SDM_I32, 1, true),
     BF("shape.rot_mat2local", rot_local, SDM_F64, 9, true), BF("shape.rot_mat2global", rot_global, SDM_F64, 9, true),
-    BF("body.material", material, SDM_I32, 1, true), BF("bound.exists", hasBound, SDM_I32, 1, true),
+    BF("body.material", material, SDM_I32, 1, true), BF("body.specimen", specimen, SDM_I32, 1, true), BF("bound.exists", hasBound, SDM_I32, 1, true),
     BF("bound.min", bmin, SDM_F64, 3, true), BF("bound.max", bmax, SDM_F64, 3, true),
     BF("bound.refPos", refPos, SDM_F64, 3, true), BF("bound.sweepLength", sweepLength, SDM_F64, 1, true),
     BF("bound.lastUpdateIter", lastUpdateIter, SDM_I32, 1, true), BF("forces.force", force, SDM_F64, 3, false),
@@ -1864,8 +1865,10 @@ int sdmcpu_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
 
 int sdmcpu_step(sdm_ctx* c, int32_t nsteps) {
   if (!c) return fail(c, "null context");
-  for (auto& b : c->bodies)
+  for (auto& b : c->bodies) {
     if (b.material < 0 || b.material >= (int)c->mats.size()) return fail(c, "body material index out of range");
+    if (b.specimen != 0) return fail(c, "the oracle steps one specimen (body.specimen must be 0)");
+  }
   for (int s = 0; s < nsteps; s++) one_step(c);
   return 0;
 }

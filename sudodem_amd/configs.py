@@ -369,3 +369,81 @@ def densify(cfg, lib, iterations=150, target=2e-6, growth=0.003, max_push=5e-5, 
     out.engines[1].verletDist = 0.1 * float(np.mean(B["r_max"][idx]))
     out.update_stats()
     return out
+
+
+def box_specimen(n_particles, rng, kn, mu, spacing=2.86e-3):
+    """A config-0-style specimen of any size: the 20x20x60 mm proportions of config0 (a
+    lattice nx x nx x 3nx of 2.86 mm), n_particles drawn sites, jitter 0.1 mm, config-0 shape
+    distributions, each particle scaled down to clear its lattice neighbours and the walls along
+    the lines of centres (vectorised, as config1).  Linear law: particle Kn = kn, Ks = 0.7 kn,
+    friction mu; walls frictionless with Kn 1e5, Ks 7e4 (config0)."""
+    nx = max(2, int(math.ceil((n_particles / 3.0) ** (1.0 / 3.0))))
+    nz = int(math.ceil(n_particles / float(nx * nx)))
+    box_lo = np.zeros(3)
+    box_hi = np.array([nx * spacing, nx * spacing, nz * spacing])
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(nx), np.arange(nz), indexing="ij"), axis=-1).reshape(-1, 3)
+    pick = np.sort(rng.choice(len(g), size=n_particles, replace=False))
+    pos = (g[pick] + 0.5) * spacing + rng.uniform(-0.1e-3, 0.1e-3, size=(n_particles, 3))
+    rxyz, eps = _draw_shapes(rng, n_particles)
+    quat = random_quaternions(rng, n_particles)
+    R = rot_matrices(quat)
+    scale = np.ones(n_particles)
+    ii, jj = _neighbour_pairs(pos, spacing * 1.8)
+    if len(ii):
+        d = pos[jj] - pos[ii]
+        L = np.linalg.norm(d, axis=1)
+        u = d / L[:, None]
+        hs = support_h(rxyz[ii], eps[ii], R[ii], u) + support_h(rxyz[jj], eps[jj], R[jj], -u)
+        lim = np.minimum(1.0, (L - 1e-6) / hs)
+        np.minimum.at(scale, ii, lim)
+        np.minimum.at(scale, jj, lim)
+    for axis in range(3):
+        e = np.zeros((n_particles, 3))
+        e[:, axis] = 1.0
+        hp = support_h(rxyz, eps, R, e)
+        hm = support_h(rxyz, eps, R, -e)
+        scale = np.minimum(scale, np.minimum((box_hi[axis] - pos[:, axis] - 1e-6) / hp,
+                                             (pos[:, axis] - box_lo[axis] - 1e-6) / hm))
+    rxyz = rxyz * scale[:, None]
+    p_mat = SuperquadricsMat(label="sq", Kn=kn, Ks=0.7 * kn, frictionAngle=math.atan(mu), density=2650.0)
+    w_mat = SuperquadricsMat(label="wall", Kn=1e5, Ks=7e4, frictionAngle=0.0, density=2650.0)
+    A = _empty_arrays(6 + n_particles)
+    _box_walls(A, box_lo, box_hi, wall_mat=1)
+    _fill_particles(A, 6, pos, rxyz, eps, quat, 0, p_mat.density)
+    verlet = 0.1 * float(np.mean(A["r_max"][6:]))
+    return Config(f"box specimen of {n_particles} superellipsoids", A, [p_mat, w_mat], linear_engines(verlet),
+                  dt=1e-6, steps=10000, box=(box_lo, box_hi))
+
+
+def batch(cfgs):
+    """Several independent specimens in one context: bodies concatenated specimen by specimen
+    (each with its six walls first), materials concatenated, `specimen` = the specimen index.
+    All specimens share the engine list and dt of the first (verletDist: the smallest)."""
+    n = sum(c.n for c in cfgs)
+    A = {k: np.concatenate([c.arrays[k] for c in cfgs]) for k in cfgs[0].arrays}
+    mats, spec, off = [], [], 0
+    for k, c in enumerate(cfgs):
+        A["material"][off:off + c.n] = c.arrays["material"] + len(mats)
+        mats.extend(c.materials)
+        spec.append(np.full(c.n, k, np.int32))
+        off += c.n
+    A["specimen"] = np.concatenate(spec)
+    engines = copy.deepcopy(cfgs[0].engines)
+    engines[1].verletDist = min(c.engines[1].verletDist for c in cfgs)
+    out = Config(f"batch of {len(cfgs)} specimens, {n} bodies", A, mats, engines, cfgs[0].dt, cfgs[0].steps,
+                 cfgs[0].box)
+    out.sizes = [c.n for c in cfgs]
+    return out
+
+
+def config4(n_specimens=256, seed_extra=0, n_min=2000, n_max=50000):
+    """Batching workload: n_specimens box specimens with particle counts log-uniform in
+    [n_min, n_max], each with its own Kn ~ U[5e4, 2e5] and friction coefficient ~ U[0.2, 0.8]."""
+    rng = rng_for(4, seed_extra)
+    sizes = np.exp(rng.uniform(math.log(n_min), math.log(n_max), n_specimens)).round().astype(int)
+    kns = rng.uniform(5e4, 2e5, n_specimens)
+    mus = rng.uniform(0.2, 0.8, n_specimens)
+    specs = [box_specimen(int(sizes[k]), rng, float(kns[k]), float(mus[k])) for k in range(n_specimens)]
+    for sp in specs:  # one verletDist for all: 0.1 x the mean r_max of the whole batch
+        sp.engines[1].verletDist = 0.1 * float(np.mean(np.concatenate([c.arrays["r_max"][6:] for c in specs])))
+    return specs

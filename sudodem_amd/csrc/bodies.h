@@ -83,7 +83,8 @@ __global__ __launch_bounds__(256) void k_geo_pack(BodyArrays B) {
   for (int k = 0; k < 3; k++) g[GEO_VEL + k] = B.vel[k * N + i];
   for (int k = 0; k < 3; k++) g[GEO_ANGVEL + k] = B.angVel[k * N + i];
   g[GEO_MASS] = B.mass[i];
-  const uint64_t fl = geo_flags(B.material[i], B.blocked[i], B.kind[i], B.isSphere[i], B.wallAxis[i], B.wallSense[i]);
+  const uint64_t fl =
+      geo_flags(B.material[i], B.blocked[i], B.kind[i], B.isSphere[i], B.wallAxis[i], B.wallSense[i], B.lid[i]);
   g[GEO_FLAGS] = __longlong_as_double((long long)fl);
 }
 

@@ -47,6 +47,16 @@ __global__ void k_bb_keys(BodyArrays B, const int32_t* order, int64_t n2, int ax
   vals[q] = (uint32_t)q;
 }
 
+// Specimen of the handle now at position p (several specimens in one context): a stable
+// sort by it after the coordinate sort keeps every specimen's bounds contiguous, so the
+// sweep and the crossings never leave a specimen.
+__global__ void k_spec_keys(const int32_t* specimen, const int32_t* oldOrder, const uint32_t* qs, int64_t n2,
+                            uint32_t* keys) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n2) return;
+  keys[p] = (uint32_t)specimen[oldOrder[qs[p]] >> 1];
+}
+
 __global__ void k_bb_apply(const int32_t* oldOrder, const uint32_t* qs, int64_t n2, int32_t* newOrder) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n2) return;

@@ -81,6 +81,7 @@ struct BodyView {
   V3<double> pos, vel, angVel, bmin, bmax;
   double mass;
   int blocked, material, kind, isSphere, wallAxis, wallSense;
+  bool low6;  // one of the first six bodies of its specimen (the reference's ids 0-5)
   Shape<double> s;
   double rmax;
 };
@@ -119,6 +120,7 @@ __device__ __forceinline__ BodyView load_body(const BodyArrays& B, int id) {
   b.isSphere = (int)((fl >> 42) & 1);
   b.wallAxis = (int)((fl >> 43) & 3);
   b.wallSense = (int)((fl >> 45) & 3) - 1;
+  b.low6 = ((fl >> 47) & 1) != 0;
   return b;
 }
 
@@ -202,8 +204,10 @@ __device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
     const MaterialDev m2 = A.mats[b2.material];
     ph.kn = 0; ph.ks = 0; ph.tanPhi = 0; ph.nuab = 0; ph.Gab = 0; ph.mu1 = 0; ph.mu2 = 0;
     ph.fs = mk3(0.0, 0.0, 0.0);
+    // "id1 > 5" of Ip2 (Superquadrics.cpp:1100, 1134) and Law2 (:1394): id1 is not one of the
+    // first six bodies -- of its specimen when several specimens share the context
     if (LAW == 0) {
-      if (id1 > 5) {
+      if (!b1.low6) {
         ph.tanPhi = tan(fmin(m1.frictionAngle, m2.frictionAngle));
         ph.kn = 2.0 * m1.Kn * m2.Kn / (m1.Kn + m2.Kn);
         ph.ks = 2.0 * m1.Ks * m2.Ks / (m1.Ks + m2.Ks);
@@ -213,7 +217,7 @@ __device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
         ph.ks = m1.Ks;
       }
     } else {
-      if (id1 > 5) {
+      if (!b1.low6) {
         ph.nuab = 0.5 * (m1.nu + m2.nu);
         ph.Gab = 1.0 / ((1 - m1.nu) / m1.G + (1 - m2.nu) / m2.G);
         ph.mu1 = 1. / m1.G + 1. / m2.G;
@@ -240,7 +244,7 @@ __device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
     ph.fn = mk3(0.0, 0.0, 0.0);
     ph.fs = ph.fn;
   } else {
-    if (LAW == 1 && id1 > 5) {
+    if (LAW == 1 && !b1.low6) {
       const double depth = g.depth, cs = g.curvSum;
       if (g.gSphere) {
         double tmp = sqrt(depth / cs);
@@ -395,6 +399,11 @@ __device__ __forceinline__ void early_store(const StepArgs& A, int64_t p, int id
   b1.blocked = b2.blocked = 0;
   b1.material = hadPhys ? 0 : B.material[id1];
   b2.material = hadPhys ? 0 : B.material[id2];
+  b1.low6 = b2.low6 = false;
+  if (!hadPhys) {
+    const uint64_t f1 = (uint64_t)__double_as_longlong(B.geo[(int64_t)id1 * GEO_STRIDE + GEO_FLAGS]);
+    b1.low6 = ((f1 >> 47) & 1) != 0;
+  }
   GeomState g;
   g.depth = 0;
   g.isShearNew = true;

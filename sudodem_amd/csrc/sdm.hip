@@ -110,6 +110,7 @@ struct sdm_ctx {
   bool matChecked = false;
   bool solverRan = false;
   bool geoDirty = true;
+  int32_t nSpec = 1;  // specimens (body.specimen), contiguous ranges of body ids
   bool pairsWritten = true;  // host wrote interaction data since the last step
   uint32_t* stage = nullptr;  // attribute staging buffer (32-bit words)
   int64_t stageCap = 0;
@@ -140,9 +141,30 @@ void alloc_bodies(sdm_ctx* c, int32_t n) {
   c->smin = dalloc<uint32_t>(2 * (int64_t)n + 1);
   c->csrOff = dalloc<int32_t>((int64_t)n + 1);
   c->B.geo = dalloc<double>((int64_t)n * GEO_STRIDE);
+  c->B.lid = dalloc<int32_t>(std::max<int64_t>(1, n));
 }
 
 void free_pairs(PairArrays& P) { dfree(P.r); }
+
+// body.specimen: ids 0,0,..,1,1,.. (non-decreasing); each body's index within its specimen.
+void set_specimens(sdm_ctx* c, const int32_t* spec) {
+  const int64_t N = c->B.n;
+  std::vector<int32_t> lid(std::max<int64_t>(1, N));
+  int32_t cur = 0;
+  int64_t start = 0;
+  for (int64_t i = 0; i < N; i++) {
+    if (spec[i] < cur || spec[i] < 0) throw HipError{"body.specimen must be non-decreasing and >= 0"};
+    if (spec[i] != cur) {
+      cur = spec[i];
+      start = i;
+    }
+    lid[i] = (int32_t)(i - start);
+  }
+  const int32_t nSpec = N > 0 ? spec[N - 1] + 1 : 1;
+  if (N > 0) HIPCHK(hipMemcpy(c->B.lid, lid.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (nSpec != c->nSpec) c->bbValid = false;  // the bounds are re-sorted specimen by specimen
+  c->nSpec = nSpec;
+}
 
 uint32_t* staging(sdm_ctx* c, int64_t words) {
   if (words > c->stageCap) {
@@ -345,6 +367,13 @@ void collider_run(sdm_ctx* c) {
   for (int a = 0; a < 3; a++) {
     hipLaunchKernelGGL(k_bb_keys, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, c->bb[a], n2, a, c->k0, c->v0);
     sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, n2, 64, c->rsScratch, c->st);
+    if (c->nSpec > 1) {
+      uint32_t* sk = (uint32_t*)c->k0;
+      hipLaunchKernelGGL(k_spec_keys, dim3(nblk(n2)), dim3(256), 0, c->st, c->B.specimen, c->bb[a], c->v0, n2, sk);
+      int bits = 1;
+      while ((1ll << bits) < c->nSpec) bits++;
+      sdmrs::sort_pairs<uint32_t, uint32_t>(sk, c->v0, (uint32_t*)c->k1, c->v1, n2, bits, c->rsScratch, c->st);
+    }
     hipLaunchKernelGGL(k_bb_apply, dim3(nblk(n2)), dim3(256), 0, c->st, c->bb[a], c->v0, n2, c->bbTmp);
     HIPCHK(hipMemcpyAsync(c->qs[a], c->v0, n2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->st));
     std::swap(c->bb[a], c->bbTmp);
@@ -726,6 +755,9 @@ sdm_ctx* sdm_create(const sdm_params* params, int32_t n_bodies, int32_t n_materi
       std::vector<double> one(N, 1.0);
       HIPCHK(hipMemcpy(c->B.ori, one.data(), N * 8, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(c->B.damped, onei.data(), N * 4, hipMemcpyHostToDevice));
+      std::vector<int32_t> iota(N);
+      for (int64_t i = 0; i < N; i++) iota[i] = (int32_t)i;
+      HIPCHK(hipMemcpy(c->B.lid, iota.data(), N * 4, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(c->B.refPos, nan.data(), N * 24, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(c->B.bmin, nan.data(), N * 24, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(c->B.bmax, nan.data(), N * 24, hipMemcpyHostToDevice));
@@ -759,6 +791,7 @@ void sdm_destroy(sdm_ctx* c) {
   SDM_BODY_FIELDS(SDM_FB)
 #undef SDM_FB
   dfree(c->B.geo);
+  dfree(c->B.lid);
   free_pairs(c->P);
   free_queue(c->S);
   dfree(c->retryList);
@@ -951,6 +984,7 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
       }
       c->matChecked = false;
       c->geoDirty = true;
+      if (std::strcmp(name, "body.specimen") == 0) set_specimens(c, (const int32_t*)src);
     } else {
       switch (f->off) {
         case 0: c->iter = *(const int64_t*)src; break;

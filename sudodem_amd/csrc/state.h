@@ -29,6 +29,7 @@
   X(double, rotL, 9, "shape.rot_mat2local")              \
   X(double, rotG, 9, "shape.rot_mat2global")             \
   X(int32_t, material, 1, "body.material")               \
+  X(int32_t, specimen, 1, "body.specimen")               \
   X(int32_t, hasBound, 1, "bound.exists")                \
   X(double, bmin, 3, "bound.min")                        \
   X(double, bmax, 3, "bound.max")                        \
@@ -117,6 +118,8 @@ struct BodyArrays {
   // reads.  The first line alone (pos, r_max, bounds) settles the bounding-sphere test and
   // the law's erase test, which is all most interactions need.
   double* geo;
+  // index of the body within its specimen (body.specimen; = the body id with one specimen)
+  int32_t* lid;
 };
 
 enum {
@@ -124,11 +127,13 @@ enum {
   GEO_VEL = 24, GEO_ANGVEL = 27, GEO_MASS = 30, GEO_FLAGS = 31, GEO_STRIDE = 32
 };
 // Static per-body integers in one 64-bit word: material (low 32 bits), then blockedDOFs (8),
-// kind (2), isSphere (1), wallAxis (2), wallSense + 1 (2).
+// kind (2), isSphere (1), wallAxis (2), wallSense + 1 (2), "among the first six bodies of
+// its specimen" (1: the wall ids 0-5 the reference's Ip2/Law2 functors test for).
 __host__ __device__ inline uint64_t geo_flags(int material, int blocked, int kind, int isSphere, int wallAxis,
-                                              int wallSense) {
+                                              int wallSense, int lid) {
   return (uint64_t)(uint32_t)material | ((uint64_t)(blocked & 0xFF) << 32) | ((uint64_t)(kind & 3) << 40) |
-         ((uint64_t)(isSphere & 1) << 42) | ((uint64_t)(wallAxis & 3) << 43) | ((uint64_t)((wallSense + 1) & 3) << 45);
+         ((uint64_t)(isSphere & 1) << 42) | ((uint64_t)(wallAxis & 3) << 43) |
+         ((uint64_t)((wallSense + 1) & 3) << 45) | ((uint64_t)(lid <= 5 ? 1 : 0) << 47);
 }
 
 struct PairArrays {

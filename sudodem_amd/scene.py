@@ -279,6 +279,8 @@ def upload_arrays(ctx: nat.Context, arrays: dict):
     rl = rot_matrices(conj)
     ctx.set("shape.rot_mat2global", rg)
     ctx.set("shape.rot_mat2local", rl)
+    if "specimen" in arrays:  # several independent specimens in one context (batching)
+        ctx.set("body.specimen", np.asarray(arrays["specimen"], dtype=np.int32))
 
 
 def rot_matrices(q):
