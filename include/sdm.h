@@ -128,6 +128,50 @@ int sdm_reset_counters(sdm_ctx* ctx);
 int sdm_set_timing(sdm_ctx* ctx, int32_t on);
 int sdm_get_timers(sdm_ctx* ctx, double out[8]);
 
+/* ---------------------------------------------------------------------------------------
+ * Differentiable rollout over a frozen contact set (linear law + NewtonIntegrator).
+ * No counterpart in the reference: it differentiates, over a window of `steps` steps, the
+ * force law of SuperquadricsLaw::go (pkg/dem/Superquadrics.cpp:1181-1363) and the leapfrog of
+ * NewtonIntegrator (pkg/dem/NewtonIntegrator.cpp:99-360) with the contacts and their geometry
+ * held at the window start (the narrowphase is not differentiated).  Loss = mean, over the
+ * last `loss_steps` steps, of the summed normal-force magnitudes of the flagged contacts.
+ * Every pointer in the problem and every array argument is DEVICE memory; the caller owns
+ * all of it (checkpoint and work buffers of the sizes returned below), nothing is
+ * allocated.  Calls run on the null stream and return after completion.
+ * ------------------------------------------------------------------------------------- */
+typedef struct {
+  int32_t n_bodies, n_contacts;
+  const double* pos0;         /* [n][3] positions at the window start */
+  const double* mass;         /* [n] */
+  const double* inv_inertia;  /* [n][9] world-frame inverse inertia at the window start (row-major) */
+  const int32_t* dynamic;     /* [n] 1: integrated; 0: moves with wall_vel (walls) */
+  const double* wall_vel;     /* [n][3] velocity of the non-dynamic bodies */
+  const int32_t* id_a;        /* [m] first body (the wall for wall contacts) */
+  const int32_t* id_b;        /* [m] */
+  const double* normal;       /* [m][3] unit normal from a to b */
+  const double* ra;           /* [m][3] contact point - centre of a */
+  const double* rb;           /* [m][3] contact point - centre of b */
+  const double* depth0;       /* [m] penetration depth at the window start */
+  const double* fs0;          /* [m][3] shear force at the window start */
+  const int32_t* in_loss;     /* [m] 1: the contact's normal force enters the loss */
+  const int32_t* csr_off;     /* [n+1] per-body contact lists */
+  const int32_t* csr_ent;     /* [2m] contact << 1 | side (0: body is a, 1: body is b), in contact order */
+  double dt, damping, gravity[3];
+  int32_t steps, loss_steps, checkpoint_every;
+} sdm_frozen_problem;
+
+/* Sizes (in doubles) of the checkpoint and work buffers. */
+int64_t sdm_frozen_checkpoint_size(const sdm_frozen_problem* p);
+int64_t sdm_frozen_work_size(const sdm_frozen_problem* p);
+/* params = {kn, ks, mu} (device, 3 doubles); v0, w0: [n][3] start velocities and angular
+ * velocities (entries of non-dynamic bodies are ignored).  Writes the loss (host). */
+int sdm_frozen_forward(const sdm_frozen_problem* p, const double* params, const double* v0, const double* w0,
+                       double* checkpoints, double* work, double* loss);
+/* After sdm_frozen_forward with the same arguments: dLoss/d{kn, ks, mu} (host, 3 doubles)
+ * and dLoss/dv0 ([n][3], device; zero for non-dynamic bodies). */
+int sdm_frozen_backward(const sdm_frozen_problem* p, const double* params, const double* checkpoints, double* work,
+                        double* grad_params, double* grad_v0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@
 
 #include "../../include/sdm.h"
 #include "bodies.h"
+#include "frozen.h"
 #include "collider.h"
 #include "radix.h"
 
@@ -1078,3 +1079,151 @@ int sdm_reset_counters(sdm_ctx* c) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Frozen-contact rollout and its adjoint (frozen.h)
+// ---------------------------------------------------------------------------------------
+namespace {
+
+int64_t fz_state(const sdm_frozen_problem* p) { return 9 * (int64_t)p->n_bodies + 3 * (int64_t)p->n_contacts; }
+int64_t fz_nck(const sdm_frozen_problem* p) { return (p->steps + p->checkpoint_every - 1) / p->checkpoint_every; }
+
+struct FzState {  // views into one state slot [x | v | w | fs]
+  double *x, *v, *w, *fs;
+};
+FzState fz_slot(const sdm_frozen_problem* p, double* base) {
+  const int64_t n3 = 3 * (int64_t)p->n_bodies;
+  return FzState{base, base + n3, base + 2 * n3, base + 3 * n3};
+}
+
+// work layout: [two forward slots][segment slots (checkpoint_every + 1)][adjoint slot]
+//              [fc 3m][lF 3n][lT 3n][contrib 18m][acc 8]
+struct FzWork {
+  double *fwd[2], *seg, *adj, *fc, *lF, *lT, *contrib, *acc;
+};
+FzWork fz_work(const sdm_frozen_problem* p, double* w) {
+  const int64_t S = fz_state(p), n = p->n_bodies, m = p->n_contacts;
+  FzWork W;
+  W.fwd[0] = w;
+  W.fwd[1] = w + S;
+  W.seg = w + 2 * S;
+  W.adj = W.seg + (p->checkpoint_every + 1) * S;
+  W.fc = W.adj + S;
+  W.lF = W.fc + 3 * m;
+  W.lT = W.lF + 3 * n;
+  W.contrib = W.lT + 3 * n;
+  W.acc = W.contrib + 18 * m;
+  return W;
+}
+
+int fz_check(const sdm_frozen_problem* p) {
+  if (!p || p->n_bodies <= 0 || p->n_contacts < 0 || p->steps <= 0 || p->checkpoint_every <= 0 ||
+      p->loss_steps <= 0 || p->loss_steps > p->steps)
+    return fail(nullptr, "sdm_frozen: invalid problem sizes");
+  return 0;
+}
+
+__global__ void k_fz_init(sdm_frozen_problem P, const double* v0, const double* w0, double* x, double* v, double* w,
+                          double* fs) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P.n_bodies) {
+    for (int k = 0; k < 3; k++) {
+      x[3 * i + k] = P.pos0[3 * i + k];
+      v[3 * i + k] = P.dynamic[i] ? v0[3 * i + k] : P.wall_vel[3 * i + k];
+      w[3 * i + k] = P.dynamic[i] ? w0[3 * i + k] : 0.0;
+    }
+  }
+  if (i < P.n_contacts)
+    for (int k = 0; k < 3; k++) fs[3 * i + k] = P.fs0[3 * i + k];
+}
+
+// One forward step from slot `a` into slot `b`; the loss is added into *loss.
+void fz_step(const sdm_frozen_problem& P, const double* par, FzState a, FzState b, double* fc, int t, double* loss) {
+  if (P.n_contacts > 0)
+    hipLaunchKernelGGL(fz::k_contacts, dim3(nblk(P.n_contacts)), dim3(256), 0, 0, P, par, a.x, a.v, a.w, a.fs, b.fs,
+                       fc, t, loss);
+  hipLaunchKernelGGL(fz::k_newton, dim3(nblk(P.n_bodies)), dim3(256), 0, 0, P, a.x, a.v, a.w, fc, b.x, b.v, b.w);
+}
+
+}  // namespace
+
+int64_t sdm_frozen_checkpoint_size(const sdm_frozen_problem* p) {
+  if (fz_check(p)) return -1;
+  return fz_nck(p) * fz_state(p);
+}
+
+int64_t sdm_frozen_work_size(const sdm_frozen_problem* p) {
+  if (fz_check(p)) return -1;
+  return (p->checkpoint_every + 4) * fz_state(p) + 3 * (int64_t)p->n_contacts + 6 * (int64_t)p->n_bodies +
+         18 * (int64_t)p->n_contacts + 8;
+}
+
+int sdm_frozen_forward(const sdm_frozen_problem* p, const double* params, const double* v0, const double* w0,
+                       double* checkpoints, double* work, double* loss) {
+  if (fz_check(p) || !params || !v0 || !w0 || !checkpoints || !work || !loss)
+    return fail(nullptr, "sdm_frozen_forward: invalid arguments");
+  try {
+    const sdm_frozen_problem& P = *p;
+    FzWork W = fz_work(p, work);
+    const int64_t S = fz_state(p);
+    HIPCHK(hipMemsetAsync(W.acc, 0, 8 * sizeof(double), 0));
+    FzState cur = fz_slot(p, W.fwd[0]), nxt = fz_slot(p, W.fwd[1]);
+    const int64_t nmax = std::max<int64_t>(P.n_bodies, P.n_contacts);
+    hipLaunchKernelGGL(k_fz_init, dim3(nblk(nmax)), dim3(256), 0, 0, P, v0, w0, cur.x, cur.v, cur.w, cur.fs);
+    for (int t = 0; t < P.steps; t++) {
+      if (t % P.checkpoint_every == 0)
+        HIPCHK(hipMemcpyAsync(checkpoints + (t / P.checkpoint_every) * S, cur.x, S * sizeof(double),
+                              hipMemcpyDeviceToDevice, 0));
+      fz_step(P, params, cur, nxt, W.fc, t, W.acc);
+      std::swap(cur, nxt);
+    }
+    HIPCHK(hipMemcpy(loss, W.acc, sizeof(double), hipMemcpyDeviceToHost));
+  } catch (HipError& e) {
+    return fail(nullptr, e.msg);
+  }
+  return 0;
+}
+
+int sdm_frozen_backward(const sdm_frozen_problem* p, const double* params, const double* checkpoints, double* work,
+                        double* grad_params, double* grad_v0) {
+  if (fz_check(p) || !params || !checkpoints || !work || !grad_params || !grad_v0)
+    return fail(nullptr, "sdm_frozen_backward: invalid arguments");
+  try {
+    const sdm_frozen_problem& P = *p;
+    FzWork W = fz_work(p, work);
+    const int64_t S = fz_state(p);
+    HIPCHK(hipMemsetAsync(W.adj, 0, S * sizeof(double), 0));
+    HIPCHK(hipMemsetAsync(W.acc, 0, 8 * sizeof(double), 0));
+    FzState L = fz_slot(p, W.adj);
+    double* dummyLoss = W.acc + 4;
+    double* gpar = W.acc + 1;
+    const int ck = P.checkpoint_every;
+    for (int64_t seg = fz_nck(p) - 1; seg >= 0; seg--) {
+      const int t0 = (int)(seg * ck), t1 = std::min(t0 + ck, P.steps);
+      // recompute the segment's states from its checkpoint
+      HIPCHK(hipMemcpyAsync(W.seg, checkpoints + seg * S, S * sizeof(double), hipMemcpyDeviceToDevice, 0));
+      for (int t = t0; t < t1; t++)
+        fz_step(P, params, fz_slot(p, W.seg + (t - t0) * S), fz_slot(p, W.seg + (t - t0 + 1) * S), W.fc, t, dummyLoss);
+      for (int t = t1 - 1; t >= t0; t--) {
+        const FzState s = fz_slot(p, W.seg + (t - t0) * S);
+        const FzState s1 = fz_slot(p, W.seg + (t - t0 + 1) * S);
+        // the step's contact forces (the shear output goes to a slot that is not read again)
+        if (P.n_contacts > 0)
+          hipLaunchKernelGGL(fz::k_contacts, dim3(nblk(P.n_contacts)), dim3(256), 0, 0, P, params, s.x, s.v, s.w,
+                             s.fs, s1.fs, W.fc, t, dummyLoss);
+        hipLaunchKernelGGL(fz::k_back_newton, dim3(nblk(P.n_bodies)), dim3(256), 0, 0, P, s.x, s.v, s.w, W.fc, L.x,
+                           L.v, L.w, W.lF, W.lT);
+        if (P.n_contacts > 0) {
+          hipLaunchKernelGGL(fz::k_back_contacts, dim3(nblk(P.n_contacts)), dim3(256), 0, 0, P, params, s.x, s.v,
+                             s.w, s.fs, W.lF, W.lT, L.fs, W.contrib, t, gpar);
+          hipLaunchKernelGGL(fz::k_back_gather, dim3(nblk(P.n_bodies)), dim3(256), 0, 0, P, W.contrib, L.x, L.v, L.w);
+        }
+      }
+    }
+    HIPCHK(hipMemcpyAsync(grad_v0, L.v, 3 * (size_t)P.n_bodies * sizeof(double), hipMemcpyDeviceToDevice, 0));
+    HIPCHK(hipMemcpy(grad_params, gpar, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  } catch (HipError& e) {
+    return fail(nullptr, e.msg);
+  }
+  return 0;
+}
