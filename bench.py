@@ -91,6 +91,69 @@ def calibrate_copy(nbytes=1 << 30, reps=4):
                       "dispatches": reps}))
 
 
+def adjoint_bench(args, n=20000, steps=500, loss_steps=100, checkpoint_every=50, wall_speed=0.05):
+    """Differentiation workload: n superellipsoids (the config-0 size and shape distributions)
+    densified into a bed, the top wall lowered onto the tallest particles and moving down at
+    wall_speed; a `steps`-step frozen-contact rollout whose loss is the mean top-wall normal
+    force over the last `loss_steps` steps; gradients w.r.t. (kn, ks, mu) and the 3n start
+    velocities, checkpoints every `checkpoint_every` steps.  The kn gradient is checked against
+    a central difference of the device forward."""
+    import torch
+
+    from sudodem_amd import configs, load_hip
+    from sudodem_amd.adjoint import FrozenProblem, rollout
+    from sudodem_amd.scene import rot_matrices
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    lib = load_hip()
+    t0 = time.perf_counter()
+    cfg = configs.densify(configs.config1(n_particles=n, box_xy=40e-3), lib, iterations=100)
+    A = cfg.arrays
+    part = A["kind"] == 0
+    ez = np.zeros((int(part.sum()), 3))
+    ez[:, 2] = 1.0
+    tops = A["pos"][part, 2] + configs.support_h(A["rxyz"][part], A["eps"][part], rot_matrices(A["ori"][part]), ez)
+    A["pos"][5, 2] = float(np.sort(tops)[-40]) - 1e-6  # the 40 tallest particles touch the top wall
+    ctx = cfg.open(lib)
+    ctx.step(20)
+    arr = FrozenProblem.arrays_from_context(ctx, loss_body=5, wall_velocity={5: (0.0, 0.0, -wall_speed)})
+    ctx.close()
+    P = FrozenProblem(arr, cfg.dt, 0.3, (0.0, 0.0, -9.81), steps, loss_steps, checkpoint_every)
+    log("window: %d bodies, %d contacts (%d on the top wall), prepared in %.1fs" % (
+        P.n, P.m, int(arr["in_loss"].sum()), time.perf_counter() - t0))
+    params0 = (1e5, 7e4, 0.5)
+    p = torch.tensor(params0, dtype=torch.float64, device="cuda", requires_grad=True)
+    v = torch.tensor(arr["v0"], dtype=torch.float64, device="cuda", requires_grad=True)
+    rollout(P, p.detach(), v.detach())  # first call: allocation and code load
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = rollout(P, p, v)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    h = 1e-6 * params0[0]
+    lp = float(rollout(P, torch.tensor((params0[0] + h,) + params0[1:], dtype=torch.float64, device="cuda"), v.detach()))
+    lm = float(rollout(P, torch.tensor((params0[0] - h,) + params0[1:], dtype=torch.float64, device="cuda"), v.detach()))
+    fd = (lp - lm) / (2 * h)
+    g = p.grad.cpu().numpy()
+    nb = int(arr["dynamic"].sum())
+    print(json.dumps({
+        "metric": "frozen-contact rollout, forward + adjoint, particle-steps/s",
+        "value": nb * steps / (tf + tb), "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": (tf + tb) / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%d superellipsoids, %d-step window, top wall at %.2f m/s, loss over the last %d steps, "
+                               "checkpoints every %d steps" % (nb, steps, wall_speed, loss_steps, checkpoint_every),
+                   "contacts": P.m},
+        "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "loss": float(loss.detach()),
+        "grad_kn_ks_mu": g.tolist(), "grad_v0_norm": float(v.grad.norm()),
+        "grad_kn_central_difference": fd, "grad_kn_rel_err": abs(g[0] - fd) / max(abs(fd), 1e-300),
+    }))
+
+
 def batch_bench(args):
     """Batching workload: specimens with log-uniform particle counts in [2000, 50000], each with
     its own Kn and friction (linear law, LM geometry), advanced together in one context, against
@@ -152,6 +215,8 @@ def main():
     ap.add_argument("--densify", type=int, default=150, help="iterations of the packing algorithm")
     ap.add_argument("--batch", type=int, default=0, metavar="SPECIMENS",
                     help="batching workload instead: this many box specimens in one context vs one by one")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="differentiation workload instead: 20k-body frozen-contact rollout, forward + adjoint")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of known size (the reference for rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -160,6 +225,9 @@ def main():
         return
     if args.batch:
         batch_bench(args)
+        return
+    if args.adjoint:
+        adjoint_bench(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
