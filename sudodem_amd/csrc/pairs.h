@@ -82,6 +82,7 @@ struct BodyView {
   double mass;
   int blocked, material, kind, isSphere, wallAxis, wallSense;
   bool low6;  // one of the first six bodies of its specimen (the reference's ids 0-5)
+  M3<double> G;  // rot_mat2global
   Shape<double> s;
   double rmax;
 };
@@ -108,6 +109,8 @@ __device__ __forceinline__ BodyView load_body(const BodyArrays& B, int id) {
   b.s.rx = v[GEO_R]; b.s.ry = v[GEO_R + 1]; b.s.rz = v[GEO_R + 2];
   b.s.eps1 = v[GEO_E]; b.s.eps2 = v[GEO_E + 1];
   b.rmax = v[GEO_RMAX];
+#pragma unroll
+  for (int k = 0; k < 9; k++) b.G.m[k] = v[GEO_G + k];
   b.bmin = mk3(v[GEO_BMIN], v[GEO_BMIN + 1], v[GEO_BMIN + 2]);
   b.bmax = mk3(v[GEO_BMAX], v[GEO_BMAX + 1], v[GEO_BMAX + 2]);
   b.vel = mk3(v[GEO_VEL], v[GEO_VEL + 1], v[GEO_VEL + 2]);
@@ -606,7 +609,7 @@ __global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* m
           g.depth = depth;
         }
       } else {
-        const M3<double> G2 = ld9(B.rotG, id2, N);
+        const M3<double>& G2 = b2.G;
         const V3<double> point2 = support(b2.s, G2, -wn) + b2.pos;
         nev = 1;
         V3<double> q1 = point2;
@@ -791,7 +794,8 @@ __global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, con
       g.depth = S.f[w];
       // curvature direction convention of :2463-2465
       const M3<double> gC = crotation_mat(b1.pos, b2.pos);
-      const M3<double> G1 = ld9(B.rotG, id1, N), G2 = ld9(B.rotG, id2, N);
+      const M3<double>& G1 = b1.G;
+      const M3<double>& G2 = b2.G;
       const V3<double> contact = mul(gC, mk3(cos(g.a0) * sin(g.a1), sin(g.a0) * sin(g.a1), cos(g.a1)));
       const V3<double> c1 = mulT(G1, contact), c2 = mulT(G2, contact);
       V3<double> dmax1 = mk3(0.0, 0.0, 0.0), dmax2 = dmax1, dmin1, dmin2;
