@@ -60,6 +60,8 @@ typedef struct sdm_params {
   int32_t sort_axis;            /* InsertionSortCollider.sortAxis (InsertionSortCollider.hpp:211) */
   int32_t law;                  /* SDM_LAW_* : which Ig2/Ip2/Law2 functor family the InteractionLoop holds */
   int32_t precision;            /* SDM_FP64 or SDM_MIXED */
+  int32_t periodic;             /* Scene::isPeriodic: an axis-aligned periodic cell [0, cell) (core/Scene.hpp) */
+  double cell[3];               /* Cell size (Cell::getSize, core/Cell.hpp); used when periodic */
 } sdm_params;
 
 /* SuperquadricsMat / SuperquadricsMat2 (pkg/dem/Superquadrics.hpp:247-277) + Material.density */

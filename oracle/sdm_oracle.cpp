@@ -277,6 +277,7 @@ struct Pair {
   // narrowphase diagnostics: evaluations and solver stop code of the last step
   int32_t nEval, stopCode;
   int64_t iterMadeReal, iterBorn;
+  int32_t cellDist[3];  // Interaction::cellDist (core/Interaction.hpp): period of id2 relative to id1
 };
 
 void geom_reset(Pair& p) {
@@ -313,6 +314,7 @@ struct Bounds {  // InsertionSortCollider::Bounds (InsertionSortCollider.hpp:90-
   double coord;
   int32_t id;
   int32_t isMin, hasBB;
+  int32_t period;  // periodic cell coordinate of the bound
   bool operator<(const Bounds& b) const {
     if (id == b.id && coord == b.coord) return isMin;
     return coord < b.coord;
@@ -339,6 +341,9 @@ struct sdm_ctx {
   bool found_dirty = false;
   // InsertionSortCollider state
   std::vector<Bounds> BB[3];
+  long loIdx[3] = {0, 0, 0};  // VecBounds::loIdx: the element before which each ring wraps
+  bool periodicState = false;  // InsertionSortCollider::periodic
+  bool periErr = false;        // spatialOverlapPeri met a body larger than half the cell
   std::vector<double> minima, maxima;
   int32_t strideActive = 0;
   double fastestBodyMaxDist = -1;
@@ -761,10 +766,18 @@ void rotate_shear(const Pair& g, double* fs) {
 
 const double zero3[3] = {0, 0, 0};
 
+// shift2 = Cell::hSize * cellDist (InteractionLoop.cpp:103); the cell is axis-aligned, so
+// component i is size_i * cellDist_i.  Zero outside a periodic scene (:101).
+void pair_shift2(const sdm_ctx* c, const Pair& I, double* s) {
+  for (int i = 0; i < 3; i++) s[i] = c->prm.periodic ? c->prm.cell[i] * (double)I.cellDist[i] : 0.0;
+}
+
 // Ig2_Superquadrics_Superquadrics_SuperquadricsGeom::go (Superquadrics_Ig2.cpp:1880-2317).
 void ig2_lm(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
+  double shift2[3];
+  pair_shift2(c, I, shift2);
   const double* position1 = b1.pos;
-  const double* position2 = b2.pos;  // + shift2 (zero: the periodic cell is not built)
+  const double position2[3] = {b2.pos[0] + shift2[0], b2.pos[1] + shift2[1], b2.pos[2] + shift2[2]};
   SQ A = make_sq(b1), B = make_sq(b2);
   bool isNew = !I.hasGeom;
   double para[2] = {1e-5, 1e-5};
@@ -865,7 +878,7 @@ void ig2_lm(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
     I.depth = pDepth;
     normalize3(contactNormal);
   }
-  precompute(I, b1, b2, c->prm.dt, contactNormal, I.isShearNew, zero3);
+  precompute(I, b1, b2, c->prm.dt, contactNormal, I.isShearNew, shift2);
   copy3(I.normal, contactNormal);
   I.contactAngle[0] = para[0];
   I.contactAngle[1] = para[1];
@@ -875,7 +888,9 @@ void ig2_lm(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
 // Ig2_Superquadrics_Superquadrics_SuperquadricsGeom2::go (Superquadrics_Ig2.cpp:2322-2517).
 void ig2_nm(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
   const double* position1 = b1.pos;
-  const double* position2 = b2.pos;  // QUIRK: no shift2 (:2328)
+  const double* position2 = b2.pos;  // QUIRK: no shift2 (:2328); precompute gets it (:2511)
+  double shift2[3];
+  pair_shift2(c, I, shift2);
   SQ A = make_sq(b1), B = make_sq(b2);
   bool isNew = !I.hasGeom;
   double dels[2] = {0.01, 0.01};
@@ -984,7 +999,7 @@ void ig2_nm(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
     I.K = hm[2];
     I.E = hm[3];
   }
-  precompute(I, b1, b2, c->prm.dt, contactNormal, I.isShearNew, zero3);
+  precompute(I, b1, b2, c->prm.dt, contactNormal, I.isShearNew, shift2);
   I.contactAngle[0] = para[0];
   I.contactAngle[1] = para[1];
   I.nEval = nev;
@@ -1098,9 +1113,21 @@ void ip2(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
 bool law(sdm_ctx* c, Pair& I, const Body& A, const Body& B) {
   const bool hertz = c->prm.law == SDM_LAW_HERTZ;
   I.contributed = 0;
-  // AABB separation test, strict (:1203, :1381)
-  if (A.bmin[0] >= B.bmax[0] || B.bmin[0] >= A.bmax[0] || A.bmin[1] >= B.bmax[1] || B.bmin[1] >= A.bmax[1] ||
-      A.bmin[2] >= B.bmax[2] || B.bmin[2] >= A.bmax[2]) {
+  // AABB separation test, strict (:1203, :1381); SuperquadricsLaw shifts B's box by shift2
+  // in a periodic scene (:1196-1201), QUIRK: SuperquadricsLaw2 does not (:1381)
+  double bmn[3], bmx[3];
+  copy3(bmn, B.bmin);
+  copy3(bmx, B.bmax);
+  if (c->prm.periodic && !hertz) {
+    double shift2[3];
+    pair_shift2(c, I, shift2);
+    for (int i = 0; i < 3; i++) {
+      bmn[i] += shift2[i];
+      bmx[i] += shift2[i];
+    }
+  }
+  if (A.bmin[0] >= bmx[0] || bmn[0] >= A.bmax[0] || A.bmin[1] >= bmx[1] || bmn[1] >= A.bmax[1] ||
+      A.bmin[2] >= bmx[2] || bmn[2] >= A.bmax[2]) {
     set3(I.normalForce, 0., 0., 0.);
     set3(I.shearForce, 0., 0., 0.);
     return false;
@@ -1304,10 +1331,12 @@ inline bool spatial_overlap(const sdm_ctx* c, int id1, int id2) {
          (mi[3 * id1 + 2] <= ma[3 * id2 + 2]) && (ma[3 * id1 + 2] >= mi[3 * id2 + 2]);
 }
 
-void insert_pair(sdm_ctx* c, int id1, int id2) {
+void insert_pair(sdm_ctx* c, int id1, int id2, const int32_t* cellDist = nullptr) {
   // InteractionContainer::insert (core/InteractionContainer.cpp:9-34)
   if (!c->found_set.insert(pair_key(id1, id2)).second) return;
   Pair p = new_pair(id1, id2);
+  if (cellDist)
+    for (int k = 0; k < 3; k++) p.cellDist[k] = cellDist[k];
   p.iterBorn = c->iter;
   c->intrs.push_back(p);
   c->cnt.pairs_inserted++;
@@ -1335,6 +1364,94 @@ void insertion_sort(sdm_ctx* c, std::vector<Bounds>& v) {
       j--;
     }
     v[j + 1] = viInit;
+  }
+}
+
+// ---- periodic cell (InsertionSortCollider.cpp:404-543) ----
+// cellWrap: x wrapped into [x0, x1), the period number returned in `period`.
+double cell_wrap(double x, double x0, double x1, int& period) {
+  const double xNorm = (x - x0) / (x1 - x0);
+  period = (int)std::floor(xNorm);
+  return x0 + (xNorm - period) * (x1 - x0);
+}
+// cellWrapRel: x wrapped into [0, x1 - x0) relative to x0.
+double cell_wrap_rel(double x, double x0, double x1) {
+  const double xNorm = (x - x0) / (x1 - x0);
+  return (xNorm - std::floor(xNorm)) * (x1 - x0);
+}
+
+// spatialOverlapPeri (:491-543) with allowBiggerThanPeriod = false: per axis, both boxes
+// are wrapped into the period that starts at the minimum giving the smaller gap; a box
+// whose maximum lands in another period than its minimum raises the error flag (the
+// reference throws).  `periods` receives the period of id2 relative to id1.
+bool spatial_overlap_peri(sdm_ctx* c, int id1, int id2, int32_t periods[3]) {
+  const double* mi = c->minima.data();
+  const double* ma = c->maxima.data();
+  for (int axis = 0; axis < 3; axis++) {
+    const double dim = c->prm.cell[axis];
+    const double m1 = mi[3 * id1 + axis], m2 = mi[3 * id2 + axis];
+    const double wMn = (cell_wrap_rel(m1, m2, m2 + dim) < cell_wrap_rel(m2, m1, m1 + dim)) ? m2 : m1;
+    int pmn1, pmx1, pmn2, pmx2;
+    const double mn1 = cell_wrap(m1, wMn, wMn + dim, pmn1), mx1 = cell_wrap(ma[3 * id1 + axis], wMn, wMn + dim, pmx1);
+    const double mn2 = cell_wrap(m2, wMn, wMn + dim, pmn2), mx2 = cell_wrap(ma[3 * id2 + axis], wMn, wMn + dim, pmx2);
+    if (pmn1 != pmx1 || pmn2 != pmx2) {
+      c->periErr = true;
+      return false;
+    }
+    periods[axis] = pmn1 - pmn2;
+    if (!(mn1 <= mx2 && mx1 >= mn2)) return false;
+  }
+  return true;
+}
+
+// handleBoundInversionPeri (:466-475).
+void handle_bound_inversion_peri(sdm_ctx* c, int id1, int id2) {
+  int32_t periods[3] = {0, 0, 0};
+  const bool overlap = spatial_overlap_peri(c, id1, id2, periods);
+  if (overlap && !c->found_set.count(pair_key(id1, id2))) insert_pair(c, id1, id2, periods);
+}
+
+// insertionSortPeri (:416-463): the bounds of one axis form a ring sorted from loIdx; a
+// bound that moves across the split changes period and coordinate by one cell.
+void insertion_sort_peri(sdm_ctx* c, std::vector<Bounds>& v, long& loIdx, double cellDim, bool doCollide) {
+  const long size = (long)v.size();
+  auto norm = [size](long i) {
+    if (i < 0) i += size;
+    return i % size;
+  };
+  for (long ii = 0; ii < size; ii++) {
+    const long i = norm(ii);
+    const long i_1 = norm(i - 1);
+    if (i == loIdx && v[i].coord < 0) {
+      v[i].period -= 1;
+      v[i].coord += cellDim;
+      loIdx = norm(loIdx + 1);
+    }
+    const double iCmpCoord = v[i].coord + (i == loIdx ? cellDim : 0);
+    if (v[i_1].coord <= iCmpCoord) continue;
+    long j = i_1;
+    Bounds vi = v[i];
+    const bool viHasBB = vi.hasBB;
+    const bool isMin = v[i].isMin;
+    while (v[j].coord > vi.coord + (norm(j + 1) == loIdx ? cellDim : 0)) {
+      const long j1 = norm(j + 1);
+      Bounds& vNew = v[j1];
+      vNew = v[j];
+      if (j == loIdx && vi.coord < 0) {
+        vi.period -= 1;
+        vi.coord += cellDim;
+        loIdx = norm(loIdx + 1);
+      } else if (j1 == loIdx) {
+        vNew.period += 1;
+        vNew.coord -= cellDim;
+        loIdx = norm(loIdx - 1);
+      }
+      if (isMin && !v[j].isMin && (doCollide && viHasBB && v[j].hasBB)) {
+        if (vi.id != vNew.id) handle_bound_inversion_peri(c, vi.id, vNew.id);
+      }
+      j = norm(j - 1);
+    }
+    v[norm(j + 1)] = vi;
   }
 }
 
@@ -1391,11 +1508,16 @@ bool collider_is_activated(sdm_ctx* c) {
   return false;
 }
 
-// InsertionSortCollider::action (InsertionSortCollider.cpp:191-400), non-periodic.
+// InsertionSortCollider::action (InsertionSortCollider.cpp:191-400).
 void collider_action(sdm_ctx* c) {
   c->cnt.collider_runs++;
   if (c->found_dirty) build_found(c);
   const long nBodies = (long)c->bodies.size();
+  const bool periodic = c->prm.periodic != 0;
+  if (periodic != c->periodicState) {  // periodicity changed: reinitialise (:202-206)
+    for (int i = 0; i < 3; i++) c->BB[i].clear();
+    c->periodicState = periodic;
+  }
   bool doInitSort = false;
   if ((long)c->BB[0].size() != 2 * nBodies) {
     long BBsize = (long)c->BB[0].size();
@@ -1424,7 +1546,9 @@ void collider_action(sdm_ctx* c) {
       Bounds& bb = V[i];
       const Body& b = c->bodies[bb.id];
       bb.hasBB = b.hasBound;
-      bb.coord = b.hasBound ? (bb.isMin ? b.bmin[j] : b.bmax[j]) : b.pos[j];
+      bb.coord = (b.hasBound ? (bb.isMin ? b.bmin[j] : b.bmax[j]) : b.pos[j]) -
+                 (periodic ? c->prm.cell[j] * bb.period : 0.);
+      if (doInitSort && periodic) bb.coord = cell_wrap(bb.coord, 0, c->prm.cell[j], bb.period);
     }
   }
   for (long id = 0; id < nBodies; id++) {
@@ -1438,7 +1562,9 @@ void collider_action(sdm_ctx* c) {
   // conditionalyEraseNonReal (core/InteractionContainer.hpp:104-112)
   for (size_t linPos = 0; linPos < c->intrs.size();) {
     Pair& I = c->intrs[linPos];
-    if (!(I.hasGeom && I.hasPhys) && !spatial_overlap(c, I.id1, I.id2)) {
+    int32_t periods[3];
+    if (!(I.hasGeom && I.hasPhys) &&
+        !(periodic ? spatial_overlap_peri(c, I.id1, I.id2, periods) : spatial_overlap(c, I.id1, I.id2))) {
       c->found_set.erase(pair_key(I.id1, I.id2));
       if (linPos < c->intrs.size() - 1) c->intrs[linPos] = c->intrs.back();
       c->intrs.pop_back();
@@ -1447,9 +1573,13 @@ void collider_action(sdm_ctx* c) {
       linPos++;
   }
   if (!doInitSort) {
-    for (int i = 0; i < 3; i++) insertion_sort(c, c->BB[i]);
+    if (!periodic)
+      for (int i = 0; i < 3; i++) insertion_sort(c, c->BB[i]);
+    else
+      for (int i = 0; i < 3; i++) insertion_sort_peri(c, c->BB[i], c->loIdx[i], c->prm.cell[i], true);
   } else {
     for (int i = 0; i < 3; i++) {
+      c->loIdx[i] = 0;
       if (c->initSortStable)
         std::stable_sort(c->BB[i].begin(), c->BB[i].end());
       else
@@ -1457,6 +1587,19 @@ void collider_action(sdm_ctx* c) {
     }
     c->numReinit++;
     const std::vector<Bounds>& V = c->BB[c->prm.sort_axis];
+    const long n2 = 2 * nBodies;
+    if (periodic) {
+      // the sweep wraps over the end of the ring (:384-397)
+      for (long i = 0; i < n2; i++) {
+        if (!(V[i].isMin && V[i].hasBB)) continue;
+        const int iid = V[i].id;
+        for (long j = (i + 1) % n2; V[j].id != iid; j = (j + 1) % n2) {
+          if (!(V[j].isMin && V[j].hasBB)) continue;
+          handle_bound_inversion_peri(c, iid, V[j].id);
+        }
+      }
+      return;
+    }
     for (long i = 0; i < 2 * nBodies; i++) {
       if (!(V[i].isMin && V[i].hasBB)) continue;
       const int iid = V[i].id;
@@ -1644,7 +1787,7 @@ const Field kFields[] = {
     PF("intr.id1", id1, SDM_I32, 1, true), PF("intr.id2", id2, SDM_I32, 1, true),
     PF("intr.hasGeom", hasGeom, SDM_I32, 1, true), PF("intr.hasPhys", hasPhys, SDM_I32, 1, true),
     PF("intr.functorSet", functorSet, SDM_I32, 1, true), PF("intr.geomExists", geomExists, SDM_I32, 1, true),
-    PF("intr.iterMadeReal", iterMadeReal, SDM_I64, 1, true),
+    PF("intr.iterMadeReal", iterMadeReal, SDM_I64, 1, true), PF("intr.cellDist", cellDist, SDM_I32, 3, true),
     PF("geom.PenetrationDepth", depth, SDM_F64, 1, true), PF("geom.contactAngle", contactAngle, SDM_F64, 2, true),
     PF("geom.contactPoint", contactPoint, SDM_F64, 3, true), PF("geom.shearInc", shearInc, SDM_F64, 3, true),
     PF("geom.relativeVn", relVn, SDM_F64, 3, true), PF("geom.relativeVs", relVs, SDM_F64, 3, true),
@@ -1726,6 +1869,11 @@ sdm_ctx* sdmcpu_create(const sdm_params* params, int32_t n_bodies, int32_t n_mat
   }
   if (params->sort_axis < 0 || params->sort_axis > 2) {
     g_err = "sdmcpu_create: sortAxis must be 0, 1 or 2";
+    return nullptr;
+  }
+  if (params->periodic && !(params->cell[0] > 0 && params->cell[1] > 0 && params->cell[2] > 0 &&
+                            std::isfinite(params->cell[0] + params->cell[1] + params->cell[2]))) {
+    g_err = "sdmcpu_create: a periodic cell needs three positive finite sizes";
     return nullptr;
   }
   sdm_ctx* c = new sdm_ctx();
@@ -1868,8 +2016,15 @@ int sdmcpu_step(sdm_ctx* c, int32_t nsteps) {
   for (auto& b : c->bodies) {
     if (b.material < 0 || b.material >= (int)c->mats.size()) return fail(c, "body material index out of range");
     if (b.specimen != 0) return fail(c, "the oracle steps one specimen (body.specimen must be 0)");
+    if (c->prm.periodic && b.kind == SDM_BODY_WALL) return fail(c, "walls are not supported in a periodic cell");
   }
-  for (int s = 0; s < nsteps; s++) one_step(c);
+  for (int s = 0; s < nsteps; s++) {
+    one_step(c);
+    if (c->periErr) {
+      c->periErr = false;
+      return fail(c, "body larger than half of the cell size encountered");
+    }
+  }
   return 0;
 }
 
@@ -1899,6 +2054,21 @@ int sdmcpu_set_threads(sdm_ctx* c, int32_t n) {
   if (n != 1) return fail(c, "oracle built without OpenMP");
 #endif
   c->nthreads = n;
+  return 0;
+}
+
+// The collider's bound ring of one axis (coordinates, ids, min flags, periods) and loIdx.
+int sdmcpu_dump_bounds(sdm_ctx* c, int32_t axis, double* coord, int32_t* id, int32_t* isMin, int32_t* period,
+                       int64_t* loIdx) {
+  if (!c || axis < 0 || axis > 2) return -1;
+  const std::vector<Bounds>& v = c->BB[axis];
+  for (size_t i = 0; i < v.size(); i++) {
+    coord[i] = v[i].coord;
+    id[i] = v[i].id;
+    isMin[i] = v[i].isMin;
+    period[i] = v[i].period;
+  }
+  *loIdx = c->loIdx[axis];
   return 0;
 }
 

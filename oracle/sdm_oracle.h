@@ -45,6 +45,9 @@ int sdmcpu_set_threads(sdm_ctx* ctx, int32_t nthreads);
  * InsertionSortCollider.cpp:339), 1 = std::stable_sort with the same comparator.  The two
  * differ only in the placement of bounds that compare equal (the +-inf bounds of walls). */
 int sdmcpu_set_init_sort(sdm_ctx* ctx, int32_t stable);
+/* The collider's bound ring of one axis: coordinates, ids, min flags, periods (2N each), loIdx. */
+int sdmcpu_dump_bounds(sdm_ctx* ctx, int32_t axis, double* coord, int32_t* id, int32_t* isMin, int32_t* period,
+                       int64_t* loIdx);
 
 /* Distance evaluation used by the narrowphase tests: the support-map point of one
  * superellipsoid for a local unit direction (Normal2Phi + getSurface). */

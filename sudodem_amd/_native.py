@@ -34,6 +34,8 @@ class sdm_params(ctypes.Structure):
         ("sort_axis", ctypes.c_int32),
         ("law", ctypes.c_int32),
         ("precision", ctypes.c_int32),
+        ("periodic", ctypes.c_int32),
+        ("cell", ctypes.c_double * 3),
     ]
 
 

@@ -120,16 +120,17 @@ def _fill_particles(A, start, pos, rxyz, eps, quat, mat_index, density):
 class Config:
     """A synthetic scene: body arrays, materials and the engine list."""
 
-    def __init__(self, name, arrays, materials, engines, dt, steps, box):
+    def __init__(self, name, arrays, materials, engines, dt, steps, box, cell=None):
         self.name, self.arrays, self.materials, self.engines = name, arrays, materials, engines
         self.dt, self.steps, self.box = dt, steps, box
+        self.cell = cell  # periodic cell size, or None
 
     @property
     def n(self):
         return len(self.arrays["kind"])
 
     def params(self, precision=nat.SDM_FP64):
-        return engines_to_params(self.engines, self.dt, precision)
+        return engines_to_params(self.engines, self.dt, precision, self.cell)
 
     def open(self, lib=None, precision=nat.SDM_FP64):
         from .scene import open_arrays
@@ -140,6 +141,12 @@ class Config:
         the highest bounding sphere); mean_scale: mean semi-axis scale against the drawn sizes."""
         A = self.arrays
         part = A["kind"] == nat.SDM_BODY_SUPERQUADRIC
+        if self.cell is not None:  # periodic: particle volume over the cell volume
+            vol = float((A["mass"][part] / self.materials[A["material"][part][0]].density).sum())
+            self.solids_fraction = vol / float(np.prod(self.cell))
+            drawn = getattr(self, "rxyz_drawn", None)
+            self.mean_scale = float((A["rxyz"][part, 0] / drawn[part, 0]).mean()) if drawn is not None else 1.0
+            return
         lo, hi = self.box
         top = float((A["pos"][part, 2] + A["r_max"][part]).max()) - lo[2]
         vol = float((A["mass"][part] / self.materials[A["material"][part][0]].density).sum())
@@ -364,11 +371,70 @@ def densify(cfg, lib, iterations=150, target=2e-6, growth=0.003, max_push=5e-5, 
     B["r_max"][idx] = rmax
     B["mass"][idx] = rho * vol
     B["inertia"][idx] = inert * rho
-    out = Config(cfg.name + ", densified", B, cfg.materials, copy.deepcopy(cfg.engines), cfg.dt, cfg.steps, cfg.box)
+    out = Config(cfg.name + ", densified", B, cfg.materials, copy.deepcopy(cfg.engines), cfg.dt, cfg.steps, cfg.box,
+                 cfg.cell)
     out.rxyz_drawn = cfg.rxyz_drawn
     out.engines[1].verletDist = 0.1 * float(np.mean(B["r_max"][idx]))
     out.update_stats()
     return out
+
+
+def config2(n_particles=1000000, seed_extra=0, fraction=0.62):
+    """Periodic workload: n_particles superellipsoids in a cubic periodic cell, linear law.
+
+    Shapes: rx = ry = r with r lognormal of mean 1 mm and log-sigma 0.25, rz = U[0.5, 1.5] r,
+    eps1, eps2 U[0.3, 1.9].  The cell is sized so that the drawn particles fill `fraction` of
+    it.  Centres: a face-centred cubic lattice over the cell (n sites drawn from its sites),
+    jittered by 2% of the spacing; every particle is then scaled by the largest factor <= 1
+    that keeps its support along each line of centres to a lattice neighbour (nearest periodic
+    image) inside the gap, so the lattice has no contacts; `densify` grows it towards the
+    drawn sizes.  No walls, no gravity.  The drawn semi-axes are kept as `cfg.rxyz_drawn`."""
+    from scipy.spatial import cKDTree
+    rng = rng_for(2, seed_extra)
+    n = int(n_particles)
+    sig = 0.25
+    r = rng.lognormal(mean=math.log(1e-3) - 0.5 * sig * sig, sigma=sig, size=n)
+    rxyz = np.stack([r, r, r * rng.uniform(0.5, 1.5, n)], axis=1)
+    eps = rng.uniform(0.3, 1.9, size=(n, 2))
+    _, vol, _ = superquadric_props(rxyz, eps)
+    L = float((vol.sum() / fraction) ** (1.0 / 3.0))
+    m = int(math.ceil((n / 4.0) ** (1.0 / 3.0)))
+    cc = L / m
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), axis=-1).reshape(-1, 3)
+    sites = ((g[:, None, :] + basis[None, :, :]).reshape(-1, 3) + 0.25) * cc
+    pick = np.sort(rng.choice(len(sites), size=n, replace=False))
+    a = cc / math.sqrt(2.0)  # nearest-neighbour distance
+    pos = sites[pick] + rng.uniform(-0.02 * a, 0.02 * a, size=(n, 3))
+    quat = random_quaternions(rng, n)
+    R = rot_matrices(quat)
+    tree = cKDTree(np.mod(pos, L), boxsize=L)
+    pairs = tree.query_pairs(a * 1.5, output_type="ndarray")
+    ii, jj = pairs[:, 0], pairs[:, 1]
+    d = pos[jj] - pos[ii]
+    d -= L * np.round(d / L)
+    Ld = np.linalg.norm(d, axis=1)
+    u = d / Ld[:, None]
+    hs = support_h(rxyz[ii], eps[ii], R[ii], u) + support_h(rxyz[jj], eps[jj], R[jj], -u)
+    lim = np.minimum(1.0, (Ld - 1e-7) / hs)
+    scale = np.ones(n)
+    np.minimum.at(scale, ii, lim)
+    np.minimum.at(scale, jj, lim)
+    drawn = rxyz.copy()
+    rxyz = rxyz * scale[:, None]
+    p_mat = SuperquadricsMat(label="sq", Kn=1e5, Ks=7e4, frictionAngle=math.atan(0.5), density=2650.0)
+    A = _empty_arrays(n)
+    _fill_particles(A, 0, pos, rxyz, eps, quat, 0, p_mat.density)
+    verlet = 0.1 * float(np.mean(A["r_max"]))
+    engines = linear_engines(verlet)
+    engines[1].boundFunctors = [Bo1_Superquadrics_Aabb()]
+    engines[2].geomFunctors = [Ig2_Superquadrics_Superquadrics_SuperquadricsGeom()]
+    engines[3].gravity = (0.0, 0.0, 0.0)
+    cfg = Config(f"C2: {n} superellipsoids, eps in [0.3,1.9], periodic cell {L * 1e3:.1f} mm, linear law", A, [p_mat],
+                 engines, dt=1e-6, steps=1000, box=(np.zeros(3), np.full(3, L)), cell=(L, L, L))
+    cfg.rxyz_drawn = drawn
+    cfg.update_stats()
+    return cfg
 
 
 def box_specimen(n_particles, rng, kn, mu, spacing=2.86e-3):

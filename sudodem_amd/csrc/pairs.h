@@ -32,7 +32,16 @@ struct StepArgs {
   unsigned long long* counters;  // [0] contact evals, [1] distance evaluations
   int fullWrite;                 // store every value (first step after host writes)
   int32_t* csr;                  // per-body interaction lists (entries carry the active bit)
+  int periodic;                  // Scene::isPeriodic
+  double cell[3];                // cell size (axis-aligned cell: hSize = diag(cell))
 };
+
+// shift2 = hSize * cellDist (InteractionLoop.cpp:100-105): zero outside a periodic cell.
+__device__ __forceinline__ V3<double> pair_shift(const StepArgs& A, int64_t p) {
+  if (!A.periodic) return mk3(0.0, 0.0, 0.0);
+  const PairRec& R = A.P.r[p];
+  return mk3(A.cell[0] * (double)R.cellDist[0], A.cell[1] * (double)R.cellDist[1], A.cell[2] * (double)R.cellDist[2]);
+}
 
 __device__ __forceinline__ int32_t csr_entry(int64_t p, int side, bool active) {
   return (int32_t)((p << 2) | (side << 1) | (active ? 1 : 0));
@@ -75,6 +84,7 @@ struct GeomState {
   bool isShearNew;
   double c10, c11, c20, c21, curvAngle, alpha, K, E, curvSum;
   bool gSphere;
+  V3<double> shift2;  // periodic shift of body 2 (precompute, SuperquadricsLaw's box test)
 };
 
 struct BodyView {
@@ -161,7 +171,7 @@ __device__ __forceinline__ void precompute(GeomState& g, const BodyView& b1, con
   }
   g.normal = currentNormal;
   V3<double> c1x = g.cpt - b1.pos;
-  V3<double> c2x = (g.cpt - b2.pos) - mk3(0.0, 0.0, 0.0);
+  V3<double> c2x = (g.cpt - b2.pos) - g.shift2;
   V3<double> rv = (b2.vel + cross(b2.angVel, c2x)) - (b1.vel + cross(b1.angVel, c1x));
   double vn = dot(g.normal, rv);
   g.relVn = vn * g.normal;
@@ -240,8 +250,15 @@ __device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
   bool keep = true;
   bool contributed = false;
   V3<double> F = mk3(0.0, 0.0, 0.0);
-  if (b1.bmin.x >= b2.bmax.x || b2.bmin.x >= b1.bmax.x || b1.bmin.y >= b2.bmax.y || b2.bmin.y >= b1.bmax.y ||
-      b1.bmin.z >= b2.bmax.z || b2.bmin.z >= b1.bmax.z) {
+  // SuperquadricsLaw shifts B's box by shift2 in a periodic cell (Superquadrics.cpp:1196-1201);
+  // SuperquadricsLaw2 does not (:1381)
+  V3<double> bmn2 = b2.bmin, bmx2 = b2.bmax;
+  if (LAW == 0 && A.periodic) {
+    bmn2 = bmn2 + g.shift2;
+    bmx2 = bmx2 + g.shift2;
+  }
+  if (b1.bmin.x >= bmx2.x || bmn2.x >= b1.bmax.x || b1.bmin.y >= bmx2.y || bmn2.y >= b1.bmax.y ||
+      b1.bmin.z >= bmx2.z || bmn2.z >= b1.bmax.z) {
     keep = false;  // requestErase: the interaction is reset below
   } else if (!(g.depth > 1E-18)) {
     ph.fn = mk3(0.0, 0.0, 0.0);
@@ -413,6 +430,7 @@ __device__ __forceinline__ void early_store(const StepArgs& A, int64_t p, int id
   g.a0 = a0;
   g.a1 = a1;
   g.gSphere = false;
+  g.shift2 = pair_shift(A, p);
   phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, nev, stopCode);
 }
 
@@ -466,7 +484,9 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
           const bool isNew = !hadGeom;
           ce++;
           const GeoRec r1 = load_rec(B.geo, id1), r2 = load_rec(B.geo, id2);
-          const V3<double> x1 = r1.pos, x2 = r2.pos;
+          // Geom (LM) works on position2 = pos + shift2 (Superquadrics_Ig2.cpp:1886), Geom2
+          // (NM) on the unshifted position (:2328)
+          const V3<double> x1 = r1.pos, x2 = LAW == 0 ? r2.pos + pair_shift(A, p) : r2.pos;
           const double rmax1 = r1.rmax, rmax2 = r2.rmax;
           double para0, para1;
           if (LAW == 0 && isNew) {
@@ -505,9 +525,9 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
             const V3<double> n1 = mulT(r1.G, c), n2 = mulT(r2.G, c);
             const double hb1 = r1.s.rx * fabs(n1.x) + r1.s.ry * fabs(n1.y) + r1.s.rz * fabs(n1.z);
             const double hb2 = r2.s.rx * fabs(n2.x) + r2.s.ry * fabs(n2.y) + r2.s.rz * fabs(n2.z);
-            const V3<double> dx = r2.pos - r1.pos;
+            const V3<double> dx = x2 - x1;
             const double gap = dot(c, dx) - hb1 - hb2;
-            if (gap > 1e-12 * (1.0 + norm(r1.pos) + norm(r2.pos) + hb1 + hb2)) {
+            if (gap > 1e-12 * (1.0 + norm(x1) + norm(x2) + hb1 + hb2)) {
               outcome = GEO_EARLY_ANG;
               stopCode = (LAW == 0) ? 9 : 1;
             }
@@ -541,10 +561,12 @@ __global__ __launch_bounds__(256) void k_queue_fill(StepArgs A, SolveIO S, const
   const int id1 = R.id1[0], id2 = R.id2[0];
   const bool isNew = !R.hasGeom[0];
   double a0, a1;
+  const V3<double> sh = pair_shift(A, p);
+  if (LAW == 0 && S.sh) st3s(S.sh, slot, S.cap, sh);
   if (LAW == 0 && isNew) {
     const double* g1 = A.B.geo + (int64_t)id1 * GEO_STRIDE;
     const double* g2 = A.B.geo + (int64_t)id2 * GEO_STRIDE;
-    const V3<double> d0 = mk3(g2[GEO_POS], g2[GEO_POS + 1], g2[GEO_POS + 2]) -
+    const V3<double> d0 = (mk3(g2[GEO_POS], g2[GEO_POS + 1], g2[GEO_POS + 2]) + sh) -
                           mk3(g1[GEO_POS], g1[GEO_POS + 1], g1[GEO_POS + 2]);
     a0 = atan2(d0.y, d0.x) + 1e-5;
     a1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y)) + 1e-5;
@@ -587,6 +609,9 @@ __global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* m
     const BodyView b2 = load_body(B, id2);
     GeomState g;
     load_geom_prev(P, p, hadGeom, g);
+    g.shift2 = pair_shift(A, p);
+    // position2 of the superquadric functors: shifted in Geom (LM), not in Geom2 (NM)
+    const V3<double> x2 = LAW == 0 ? b2.pos + g.shift2 : b2.pos;
     int outcome = GEO_FULL, nev = 0;
     V3<double> cn = mk3(0.0, 0.0, 0.0);
     if (b1.kind == SDM_KIND_WALL) {
@@ -630,7 +655,7 @@ __global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* m
       ce++;
       double para0, para1;
       if (LAW == 0 && isNew) {
-        const V3<double> d0 = b2.pos - b1.pos;
+        const V3<double> d0 = x2 - b1.pos;
         para0 = atan2(d0.y, d0.x);
         para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
         para0 += 1e-5;
@@ -640,7 +665,7 @@ __global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* m
         para1 = g.a1;
       }
       if (b1.isSphere && b2.isSphere) {
-        V3<double> dist = b2.pos - b1.pos;
+        V3<double> dist = x2 - b1.pos;
         const double r_sum = b1.rmax + b2.rmax;
         const double depth = r_sum - norm(dist);
         if (depth < 1E-18) {
@@ -648,7 +673,7 @@ __global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* m
         } else {
           dist = normalized(dist);
           const V3<double> q1 = b1.pos + b1.rmax * dist;
-          const V3<double> q2 = b2.pos - b2.rmax * dist;
+          const V3<double> q2 = x2 - b2.rmax * dist;
           cn = normalized(q1 - q2);
           g.point1 = q1; g.point2 = q2;
           g.cpt = mk3(0.5 * (q1.x + q2.x), 0.5 * (q1.y + q2.y), 0.5 * (q1.z + q2.z));
@@ -725,7 +750,8 @@ __global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int3
       restart = isnan(S.a0[w]) || isnan(S.a1[w]) || S.stop[w] == 7;
     } else {
       const V3<double> p1 = ld3s(S.p1, w, S.cap), p2 = ld3s(S.p2, w, S.cap);
-      const V3<double> x1 = ld3(A.B.pos, id1, N), x2 = ld3(A.B.pos, id2, N);
+      const V3<double> x1 = ld3(A.B.pos, id1, N);
+      const V3<double> x2 = S.sh ? ld3(A.B.pos, id2, N) + ld3s(S.sh, w, S.cap) : ld3(A.B.pos, id2, N);
       if (10.0 * norm(p2 - p1) > A.B.rmax[id1]) {
         Shape<double> s1, s2;
         s1.rx = A.B.rxyz[id1]; s1.ry = A.B.rxyz[N + id1]; s1.rz = A.B.rxyz[2 * N + id1];
@@ -737,7 +763,8 @@ __global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int3
       }
     }
     if (restart) {
-      const V3<double> d0 = ld3(A.B.pos, id2, N) - ld3(A.B.pos, id1, N);
+      const V3<double> sh = S.sh ? ld3s(S.sh, w, S.cap) : mk3(0.0, 0.0, 0.0);
+      const V3<double> d0 = (ld3(A.B.pos, id2, N) + sh) - ld3(A.B.pos, id1, N);
       double a0 = atan2(d0.y, d0.x);
       double a1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
       S.a0[w] = a0 + 1e-5;
@@ -767,6 +794,7 @@ __global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, con
     const BodyView b2 = load_body(B, id2);
     GeomState g;
     load_geom_prev(P, p, hadGeom, g);
+    g.shift2 = pair_shift(A, p);
     const int nev = S.nev[w];
     de = nev;
     g.a0 = S.a0[w];

@@ -1,0 +1,315 @@
+// periodic.h -- broadphase of a periodic cell.
+//
+// In a periodic scene InsertionSortCollider keeps each axis as a ring of bounds with a
+// period per bound and insertion-sorts the ring (insertionSortPeri,
+// pkg/common/InsertionSortCollider.cpp:416-463); a pair is created when the boxes overlap
+// in the sense of spatialOverlapPeri (:491-543), which also fixes the pair's cellDist.
+// Between collider runs the container only loses pairs at the collider's own erase step,
+// so after every run it holds exactly the real interactions plus every pair whose boxes
+// overlap now.  This file finds that set without the ring: a cell list over the periodic
+// cell (each body listed in every cell its box touches, sorted by cell), where every pair
+// sharing cells is tested once, in the first cell of the cyclic intersection of the two
+// cell ranges, with spatialOverlapPeri's own arithmetic.
+//
+// Orientation and order of the new pairs:
+//  * first run (std::sort + sweep, :336-397): body A scans, along the sort axis, the lower
+//    bounds that follow its own lower bound in the wrapped ring until its upper bound; the
+//    pair is (A, B) and is inserted in the order (ring rank of A's lower bound, distance
+//    to B's lower bound).  Both come from the ranks of the sorted wrapped bounds here.
+//  * later runs: a pair is created by a lower bound crossing an upper bound, oriented
+//    (owner of the lower bound, other); the crossing axis is the first axis on which the
+//    boxes were apart at the previous run, and the crossed lower bound is the one that now
+//    lies inside the other box.  These pairs are appended in (axis, id1, id2) order, which
+//    is not the ring traversal order of insertionSortPeri.
+#pragma once
+#include "collider.h"
+
+struct PeriCell {
+  double L[3];   // cell size
+  double h[3];   // grid cell size (L / n)
+  int32_t n[3];  // grid cells per axis
+  double eps;    // absolute widening of every box before it is listed (covers rounding)
+};
+
+// cellWrap / cellWrapRel (InsertionSortCollider.cpp:404-414), same operations.
+__device__ __forceinline__ double cell_wrap(double x, double x0, double x1, int& period) {
+  const double xNorm = (x - x0) / (x1 - x0);
+  period = (int)floor(xNorm);
+  return x0 + (xNorm - (double)period) * (x1 - x0);
+}
+__device__ __forceinline__ double cell_wrap_rel(double x, double x0, double x1) {
+  const double xNorm = (x - x0) / (x1 - x0);
+  return (xNorm - floor(xNorm)) * (x1 - x0);
+}
+
+// One axis of spatialOverlapPeri.  Returns 1 overlap, 0 apart, -1 a box that spans into
+// the next period (the reference throws "Body larger than half of the cell size").
+__device__ __forceinline__ int axis_overlap_peri(double m1, double M1, double m2, double M2, double dim, int& per) {
+  const double wMn = (cell_wrap_rel(m1, m2, m2 + dim) < cell_wrap_rel(m2, m1, m1 + dim)) ? m2 : m1;
+  int pmn1, pmx1, pmn2, pmx2;
+  const double mn1 = cell_wrap(m1, wMn, wMn + dim, pmn1), mx1 = cell_wrap(M1, wMn, wMn + dim, pmx1);
+  const double mn2 = cell_wrap(m2, wMn, wMn + dim, pmn2), mx2 = cell_wrap(M2, wMn, wMn + dim, pmx2);
+  if (pmn1 != pmx1 || pmn2 != pmx2) return -1;
+  per = pmn1 - pmn2;
+  return (mn1 <= mx2 && mx1 >= mn2) ? 1 : 0;
+}
+
+// spatialOverlapPeri (:491-543): axes in order, the first apart axis ends the test.
+__device__ __forceinline__ bool spatial_overlap_peri(const BodyArrays& B, const PeriCell& pc, int a, int b,
+                                                     int32_t per[3], unsigned long long* err) {
+  const int64_t N = B.n;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    int p = 0;
+    const int r = axis_overlap_peri(B.bmin[k * N + a], B.bmax[k * N + a], B.bmin[k * N + b], B.bmax[k * N + b],
+                                    pc.L[k], p);
+    if (r < 0) {
+      atomicOr(err, 1ull);
+      return false;
+    }
+    per[k] = p;
+    if (r == 0) return false;
+  }
+  return true;
+}
+
+// conditionalyEraseNonReal with shouldBeErased = !spatialOverlapPeri (InsertionSortCollider.hpp:179-182).
+__global__ void k_erase_flags_peri(BodyArrays B, PairArrays P, int64_t np, PeriCell pc, uint32_t* flag,
+                                   unsigned long long* err) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const bool real = P.r[p].hasGeom[0] && P.r[p].hasPhys[0];
+  int32_t per[3];
+  flag[p] = (!real && !spatial_overlap_peri(B, pc, P.r[p].id1[0], P.r[p].id2[0], per, err)) ? 1u : 0u;
+}
+
+// Box widths: sum and maximum (for the grid size); a body without finite bounds (a wall)
+// cannot live in a periodic cell and sets err bit 2.
+__global__ void k_width_stats(BodyArrays B, double* sum, unsigned long long* maxBits, unsigned long long* err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t N = B.n;
+  double w = 0;
+  if (i < N) {
+    for (int k = 0; k < 3; k++) {
+      const double d = B.bmax[k * N + i] - B.bmin[k * N + i];
+      if (!isfinite(d)) {
+        atomicOr(err, 2ull);
+        w = 0;
+        break;
+      }
+      w = fmax(w, d);
+    }
+  }
+  __shared__ double ps[16];
+  __shared__ double pm[16];
+  double s = w, m = w;
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_down(s, off);
+    m = fmax(m, __shfl_down(m, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ps[threadIdx.x >> 6] = s;
+    pm[threadIdx.x >> 6] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ts = 0, tm = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+      ts += ps[k];
+      tm = fmax(tm, pm[k]);
+    }
+    atomicAdd(sum, ts);
+    atomicMax(maxBits, (unsigned long long)__double_as_longlong(tm));  // non-negative: bit order = value order
+  }
+}
+
+// Cell range of every body: start cell s[k] and count c[k] per axis (c = n: the whole
+// axis), and the number of cells the box touches.
+__global__ void k_peri_ranges(BodyArrays B, PeriCell pc, int32_t* rng, uint32_t* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t N = B.n;
+  if (i >= N) return;
+  uint32_t total = 1;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int n = pc.n[k];
+    const double lo = B.bmin[k * N + i] - pc.eps;
+    const double w = (B.bmax[k * N + i] + pc.eps) - lo;
+    int per;
+    double lw = cell_wrap(lo, 0.0, pc.L[k], per);
+    int s = (int)floor(lw / pc.h[k]);
+    s = s < 0 ? 0 : (s >= n ? n - 1 : s);
+    int e = (int)floor((lw + w) / pc.h[k]);
+    int c = e - s + 1;
+    if (c < 1) c = 1;
+    if (c >= n) {
+      c = n;
+      s = 0;
+    }
+    rng[6 * i + k] = s;
+    rng[6 * i + 3 + k] = c;
+    total *= (uint32_t)c;
+  }
+  cnt[i] = total;
+}
+
+// Entries (cell, body) of every body at its offset.
+__global__ void k_peri_entries(int64_t N, PeriCell pc, const int32_t* rng, const uint32_t* offs, uint32_t* key,
+                               uint32_t* val) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int s0 = rng[6 * i], s1 = rng[6 * i + 1], s2 = rng[6 * i + 2];
+  const int c0 = rng[6 * i + 3], c1 = rng[6 * i + 4], c2 = rng[6 * i + 5];
+  uint32_t o = offs[i];
+  for (int z = 0; z < c2; z++) {
+    const int cz = (s2 + z) % pc.n[2];
+    for (int y = 0; y < c1; y++) {
+      const int cy = (s1 + y) % pc.n[1];
+      for (int x = 0; x < c0; x++) {
+        const int cx = (s0 + x) % pc.n[0];
+        key[o] = (uint32_t)(cx + pc.n[0] * (cy + pc.n[1] * cz));
+        val[o] = (uint32_t)i;
+        o++;
+      }
+    }
+  }
+}
+
+// End of the run of equal cells that starts at each sorted entry's cell.
+__global__ void k_cell_ends(const uint32_t* key, int64_t m, uint32_t* runEnd) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  if (e == m - 1 || key[e + 1] != key[e]) runEnd[key[e]] = (uint32_t)(e + 1);
+}
+
+__device__ __forceinline__ bool cyc_in(int x, int s, int c, int n) { return ((x - s + n) % n) < c; }
+
+// Smallest cell index of the intersection of two cyclic ranges (-1 if empty).
+__device__ __forceinline__ int first_shared(int sa, int ca, int sb, int cb, int n) {
+  int best = n;
+  if (cyc_in(0, sa, ca, n) && cyc_in(0, sb, cb, n)) best = 0;
+  if (cyc_in(sa, sb, cb, n) && sa < best) best = sa;
+  if (cyc_in(sb, sa, ca, n) && sb < best) best = sb;
+  return best == n ? -1 : best;
+}
+
+struct PeriOrder {
+  int init;                  // first run: orientation and order of the sorted-ring sweep
+  const uint32_t* rankMin;   // init: ring rank of each body's lower / upper bound on the sort axis
+  const uint32_t* rankMax;
+  uint32_t n2;
+  const double* pmin;        // later runs: bounds of the previous run, [3][N]
+  const double* pmax;
+};
+
+// Every pair sharing a cell, tested in the first shared cell; overlapping pairs not in the
+// container (hash `found`) become candidates with their orientation and order key.
+__global__ __launch_bounds__(256) void k_peri_pairs(BodyArrays B, PeriCell pc, PeriOrder po, const int32_t* rng,
+                                                    const uint32_t* key, const uint32_t* val, const uint32_t* runEnd,
+                                                    int64_t m, const uint64_t* found, uint64_t fmask, Cands C,
+                                                    unsigned long long* err) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  const uint32_t cell = key[e];
+  const int X[3] = {(int)(cell % (uint32_t)pc.n[0]), (int)((cell / (uint32_t)pc.n[0]) % (uint32_t)pc.n[1]),
+                    (int)(cell / ((uint32_t)pc.n[0] * (uint32_t)pc.n[1]))};
+  const int a = (int)val[e];
+  int ra[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) ra[k] = rng[6 * a + k];
+  const uint32_t end = runEnd[cell];
+  const int64_t N = B.n;
+  for (uint32_t f = (uint32_t)e + 1; f < end; f++) {
+    const int b = (int)val[f];
+    bool first = true;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if (first && first_shared(ra[k], ra[3 + k], rng[6 * b + k], rng[6 * b + 3 + k], pc.n[k]) != X[k]) first = false;
+    }
+    if (!first) continue;
+    // orientation (and the order key) before the overlap test: spatialOverlapPeri is
+    // evaluated as the reference does, with (id1, id2)
+    int id1, id2;
+    uint64_t okey;
+    if (po.init) {
+      const uint32_t n2 = po.n2;
+      const uint32_t dAB = (po.rankMin[b] + n2 - po.rankMin[a]) % n2, dAA = (po.rankMax[a] + n2 - po.rankMin[a]) % n2;
+      const uint32_t dBA = (po.rankMin[a] + n2 - po.rankMin[b]) % n2, dBB = (po.rankMax[b] + n2 - po.rankMin[b]) % n2;
+      const bool aScans = dAB < dAA, bScans = dBA < dBB;
+      if (!aScans && !bScans) continue;  // never met in the sweep
+      const uint64_t kA = ((uint64_t)po.rankMin[a] << 32) | dAB, kB = ((uint64_t)po.rankMin[b] << 32) | dBA;
+      if (aScans && (!bScans || kA < kB)) {
+        id1 = a; id2 = b; okey = kA;
+      } else {
+        id1 = b; id2 = a; okey = kB;
+      }
+    } else {
+      int axis = 3;
+      id1 = a < b ? a : b;
+      id2 = a < b ? b : a;
+      for (int k = 0; k < 3 && axis == 3; k++) {
+        int p;
+        const int was = axis_overlap_peri(po.pmin[k * N + a], po.pmax[k * N + a], po.pmin[k * N + b],
+                                          po.pmax[k * N + b], pc.L[k], p);
+        if (was != 0) continue;
+        const int now = axis_overlap_peri(B.bmin[k * N + a], B.bmax[k * N + a], B.bmin[k * N + b],
+                                          B.bmax[k * N + b], pc.L[k], p);
+        if (now != 1) continue;
+        axis = k;
+        const double mA = B.bmin[k * N + a], MA = B.bmax[k * N + a], mB = B.bmin[k * N + b], MB = B.bmax[k * N + b];
+        const bool bInA = cell_wrap_rel(mB, mA, mA + pc.L[k]) <= MA - mA;
+        const bool aInB = cell_wrap_rel(mA, mB, mB + pc.L[k]) <= MB - mB;
+        if (bInA != aInB) {
+          id1 = bInA ? b : a;
+          id2 = bInA ? a : b;
+        }
+      }
+      okey = ((uint64_t)axis << 62) | ((uint64_t)(uint32_t)id1 << 31) | (uint64_t)(uint32_t)id2;
+    }
+    int32_t per[3];
+    if (!spatial_overlap_peri(B, pc, id1, id2, per, err)) continue;
+    if (hash_contains(found, fmask, upair(id1, id2))) continue;
+    const unsigned long long slot = atomicAdd(C.count, 1ull);
+    if ((int64_t)slot < C.cap) {
+      C.a[slot] = id1;
+      C.b[slot] = id2;
+      C.key[slot] = okey;
+    }
+  }
+}
+
+// Sort keys of the wrapped bounds on the sort axis (first run: coord = cellWrap(bound, 0, L)).
+__global__ void k_peri_bound_keys(BodyArrays B, PeriCell pc, int axis, int64_t n2, uint64_t* keys, uint32_t* vals) {
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= n2) return;
+  const int64_t id = h >> 1;
+  const int64_t N = B.n;
+  const double x = (h & 1) ? B.bmax[axis * N + id] : B.bmin[axis * N + id];
+  int per;
+  keys[h] = coord_key(cell_wrap(x, 0.0, pc.L[axis], per));
+  vals[h] = (uint32_t)h;
+}
+
+__global__ void k_peri_ranks(const uint32_t* sortedVals, int64_t n2, uint32_t* rankMin, uint32_t* rankMax) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n2) return;
+  const uint32_t h = sortedVals[q];
+  if (h & 1)
+    rankMax[h >> 1] = (uint32_t)q;
+  else
+    rankMin[h >> 1] = (uint32_t)q;
+}
+
+// cellDist of the pairs appended at [start, start + count): the periods spatialOverlapPeri
+// returns for (id1, id2) (handleBoundInversionPeri, :466-475).
+__global__ void k_new_celldist(PairArrays P, int64_t start, int64_t count, BodyArrays B, PeriCell pc,
+                               unsigned long long* err) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  PairRec& R = P.r[start + k];
+  int32_t per[3] = {0, 0, 0};
+  spatial_overlap_peri(B, pc, R.id1[0], R.id2[0], per, err);
+  R.cellDist[0] = per[0];
+  R.cellDist[1] = per[1];
+  R.cellDist[2] = per[2];
+}

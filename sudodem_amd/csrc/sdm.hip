@@ -19,7 +19,7 @@
 #include "../../include/sdm.h"
 #include "bodies.h"
 #include "frozen.h"
-#include "collider.h"
+#include "periodic.h"
 #include "radix.h"
 
 #define SDM_VERSION "sdm-hip 1.0 (gfx950)"
@@ -115,6 +115,13 @@ struct sdm_ctx {
   bool pairsWritten = true;  // host wrote interaction data since the last step
   uint32_t* stage = nullptr;  // attribute staging buffer (32-bit words)
   int64_t stageCap = 0;
+  // periodic cell broadphase (periodic.h)
+  double *prevMin = nullptr, *prevMax = nullptr;  // bounds of the previous collider run
+  int32_t* cellRng = nullptr;                     // per body: start cell and count per axis
+  uint32_t *rankMin = nullptr, *rankMax = nullptr;
+  uint32_t *peCnt = nullptr, *peOff = nullptr;    // cells per body and their exclusive scan
+  uint32_t* runEnd = nullptr;
+  int64_t runCap = 0;
   // per-phase timing
   bool timing = false;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -250,7 +257,7 @@ void ensure_moves(sdm_ctx* c, int64_t n) {
 void free_queue(SolveIO& S) {
   dfree(S.pair); dfree(S.tag); dfree(S.id1); dfree(S.id2); dfree(S.a0); dfree(S.a1); dfree(S.del);
   dfree(S.f); dfree(S.la0); dfree(S.la1); dfree(S.p1); dfree(S.p2); dfree(S.ctr); dfree(S.ct);
-  dfree(S.nev); dfree(S.stop); dfree(S.touch);
+  dfree(S.nev); dfree(S.stop); dfree(S.touch); dfree(S.sh);
 }
 
 void ensure_queue(sdm_ctx* c, int64_t n) {
@@ -270,6 +277,7 @@ void ensure_queue(sdm_ctx* c, int64_t n) {
   S.la1 = dalloc<double>(cap); S.p1 = dalloc<double>(3 * cap); S.p2 = dalloc<double>(3 * cap);
   S.ctr = dalloc<double>(3 * cap); S.ct = dalloc<double>(3 * cap); S.nev = dalloc<int32_t>(cap);
   S.stop = dalloc<int32_t>(cap); S.touch = dalloc<int32_t>(cap);
+  S.sh = c->prm.periodic ? dalloc<double>(3 * cap) : nullptr;
   c->retryList = dalloc<int32_t>(cap);
   c->miscList = dalloc<int32_t>(cap);
   c->qflag = dalloc<uint32_t>(cap + 1);
@@ -321,8 +329,201 @@ void build_csr(sdm_ctx* c) {
   c->csrValid = true;
 }
 
+unsigned long long read_counter(sdm_ctx* c, int k) {
+  unsigned long long v = 0;
+  HIPCHK(hipMemcpyAsync(&v, c->dcnt + k, sizeof(v), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return v;
+}
+
+void zero_counter(sdm_ctx* c, int k) { HIPCHK(hipMemsetAsync(c->dcnt + k, 0, sizeof(unsigned long long), c->st)); }
+
+// Empty the found-set hash and insert the interactions of the container.
+void fill_found(sdm_ctx* c) {
+  const uint64_t mask = (uint64_t)c->hcap - 1;
+  hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->htab, c->hcap, SDM_HEMPTY);
+  hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->dkeys, c->hcap, SDM_HEMPTY);
+  hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->dvals, c->hcap, SDM_HEMPTY);
+  if (c->np > 0)
+    hipLaunchKernelGGL(k_hash_insert_pairs, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.r, c->np, c->htab, mask);
+}
+
+// The new candidates C[0, nc): drop the ones present, keep one per pair (smallest key),
+// append them in key order as fresh interactions.  Returns the number appended.
+int64_t append_candidates(sdm_ctx* c, int64_t nc) {
+  ensure_scratch(c, nc + 1);
+  const uint64_t mask = (uint64_t)c->hcap - 1;
+  uint32_t* alive = c->u0;
+  uint32_t* pos = c->u1;
+  hipLaunchKernelGGL(k_cand_dedup, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, c->htab, mask, c->dkeys, c->dvals,
+                     mask, alive);
+  hipLaunchKernelGGL(k_cand_keep, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, c->dkeys, c->dvals, mask, alive);
+  const int64_t ns = exclusive_total(c, alive, pos, nc);
+  if (ns > 0) {
+    hipLaunchKernelGGL(k_cand_compact, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, alive, pos, c->k0, c->v0);
+    sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, ns, 64, c->rsScratch, c->st);
+    ensure_pair_cap(c, c->np + ns);
+    hipLaunchKernelGGL(k_new_pairs, dim3(nblk(ns)), dim3(256), 0, c->st, c->P, c->np, ns, c->C, c->v0, c->iter);
+    c->np += ns;
+    c->cnt.pairs_inserted += ns;
+  }
+  return ns;
+}
+
+// Erase step of the collider (conditionalyEraseNonReal), given the per-interaction flags
+// in u0: the swap-with-last compaction of InteractionContainer::erase.
+void erase_flagged(sdm_ctx* c) {
+  uint32_t* flag = c->u0;
+  uint32_t* E = c->u1;
+  const uint32_t Etot = exclusive_total(c, flag, E, c->np);
+  if (Etot == 0) return;
+  const int64_t K = c->np - Etot;
+  hipLaunchKernelGGL(k_hole_pos, dim3(nblk(K)), dim3(256), 0, c->st, flag, E, K, c->holePos);
+  const int64_t tail = c->np - K;
+  hipLaunchKernelGGL(k_tail_moves, dim3(nblk(tail)), dim3(256), 0, c->st, flag, E, K, c->np, Etot, c->holePos,
+                     c->msrc, c->mdst);
+  // moves = holes below K = erased entries below K = E[K] (= kept entries at >= K)
+  const int64_t nmove = read_u32(c, E + K);
+  if (nmove > 0)
+    hipLaunchKernelGGL(k_move_pairs, dim3(nblk(nmove * (int64_t)(sizeof(PairRec) / 16))), dim3(256), 0, c->st,
+                       c->P.r, c->msrc, c->mdst, nmove);
+  c->np = K;
+  c->cnt.pairs_erased += Etot;
+}
+
+// InsertionSortCollider::action in a periodic cell (InsertionSortCollider.cpp:191-400 with
+// its periodic branches): bounds, erase by spatialOverlapPeri, then the cell-list search of
+// periodic.h for every pair that overlaps now and is not in the container.
+void collider_run_peri(sdm_ctx* c) {
+  c->cnt.collider_runs++;
+  const int64_t N = c->B.n;
+  const int64_t n2 = 2 * N;
+  const bool doInitSort = !c->bbValid;
+  if (doInitSort) {
+    // no bound ring is kept in a periodic cell: collider.bounds reads as the body order
+    std::vector<int32_t> h(n2);
+    for (int64_t i = 0; i < n2; i++) h[i] = (int32_t)i;
+    for (int a = 0; a < 3; a++) HIPCHK(hipMemcpy(c->bb[a], h.data(), n2 * sizeof(int32_t), hipMemcpyHostToDevice));
+    c->bbValid = true;
+    c->numReinit++;
+  }
+  if (!c->prevMin) {
+    c->prevMin = dalloc<double>(3 * N);
+    c->prevMax = dalloc<double>(3 * N);
+    c->cellRng = dalloc<int32_t>(6 * N);
+    c->rankMin = dalloc<uint32_t>(N);
+    c->rankMax = dalloc<uint32_t>(N);
+    c->peCnt = dalloc<uint32_t>(N + 1);
+    c->peOff = dalloc<uint32_t>(N + 1);
+  }
+  HIPCHK(hipMemcpyAsync(c->prevMin, c->B.bmin, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->prevMax, c->B.bmax, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  hipLaunchKernelGGL(k_bounds, dim3(nblk(N)), dim3(256), 0, c->st, c->B, c->iter, c->prm.verlet_dist,
+                     c->prm.min_sweep_dist_factor, c->prm.target_interv);
+  if (!c->strideActive && c->prm.verlet_dist > 0 && c->maxVelocitySq >= 0) c->strideActive = 1;
+  ensure_scratch(c, std::max<int64_t>({n2 + 1, c->np + 1, 2 * c->np + 1}));
+  ensure_moves(c, c->np + 1);
+
+  // box widths (grid size) and the wall check: dcnt[7] error bits, [8] width sum, [9] max
+  unsigned long long* err = c->dcnt + 7;
+  HIPCHK(hipMemsetAsync(c->dcnt + 7, 0, 3 * sizeof(unsigned long long), c->st));
+  hipLaunchKernelGGL(k_width_stats, dim3(nblk(N)), dim3(256), 0, c->st, c->B, (double*)(c->dcnt + 8), c->dcnt + 9,
+                     err);
+  unsigned long long st[3] = {0, 0, 0};
+  HIPCHK(hipMemcpyAsync(st, c->dcnt + 7, sizeof(st), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (st[0] & 2) throw HipError{"walls (unbounded boxes) are not supported in a periodic cell"};
+  double wsum, wmax;
+  std::memcpy(&wsum, &st[1], 8);
+  std::memcpy(&wmax, &st[2], 8);
+  PeriCell pc;
+  const double wmean = N > 0 ? wsum / (double)N : 0.0;
+  const double ht = std::max(2.0 * wmean, 0.5 * wmax);
+  int64_t ncell = 1;
+  for (int k = 0; k < 3; k++) {
+    pc.L[k] = c->prm.cell[k];
+    int64_t n = ht > 0 ? (int64_t)std::floor(pc.L[k] / ht) : 1;
+    pc.n[k] = (int32_t)std::min<int64_t>(std::max<int64_t>(n, 1), 1024);
+  }
+  while ((int64_t)pc.n[0] * pc.n[1] * pc.n[2] > (1ll << 22)) {
+    int k = (pc.n[0] >= pc.n[1] && pc.n[0] >= pc.n[2]) ? 0 : (pc.n[1] >= pc.n[2] ? 1 : 2);
+    pc.n[k] = std::max(1, pc.n[k] / 2);
+  }
+  for (int k = 0; k < 3; k++) pc.h[k] = pc.L[k] / pc.n[k];
+  ncell = (int64_t)pc.n[0] * pc.n[1] * pc.n[2];
+  pc.eps = 1e-9 * std::max(pc.L[0], std::max(pc.L[1], pc.L[2]));
+
+  // ---- conditionalyEraseNonReal ----
+  if (c->np > 0) {
+    hipLaunchKernelGGL(k_erase_flags_peri, dim3(nblk(c->np)), dim3(256), 0, c->st, c->B, c->P, c->np, pc, c->u0, err);
+    erase_flagged(c);
+  }
+
+  // ---- first run: ring ranks of the wrapped bounds on the sort axis ----
+  PeriOrder po;
+  po.init = doInitSort ? 1 : 0;
+  po.rankMin = c->rankMin;
+  po.rankMax = c->rankMax;
+  po.n2 = (uint32_t)n2;
+  po.pmin = c->prevMin;
+  po.pmax = c->prevMax;
+  if (doInitSort) {
+    hipLaunchKernelGGL(k_peri_bound_keys, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, pc, c->prm.sort_axis, n2, c->k0,
+                       c->v0);
+    sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, n2, 64, c->rsScratch, c->st);
+    hipLaunchKernelGGL(k_peri_ranks, dim3(nblk(n2)), dim3(256), 0, c->st, c->v0, n2, c->rankMin, c->rankMax);
+  }
+
+  // ---- cell lists ----
+  hipLaunchKernelGGL(k_peri_ranges, dim3(nblk(N)), dim3(256), 0, c->st, c->B, pc, c->cellRng, c->peCnt);
+  const int64_t m = exclusive_total(c, c->peCnt, c->peOff, N);
+  ensure_scratch(c, m + 1);
+  uint32_t* ckey = (uint32_t*)c->k0;
+  uint32_t* cval = c->v0;
+  hipLaunchKernelGGL(k_peri_entries, dim3(nblk(N)), dim3(256), 0, c->st, N, pc, c->cellRng, c->peOff, ckey, cval);
+  int bits = 1;
+  while ((1ll << bits) < ncell) bits++;
+  sdmrs::sort_pairs<uint32_t, uint32_t>(ckey, cval, (uint32_t*)c->k1, c->v1, m, bits, c->rsScratch, c->st);
+  if (ncell > c->runCap) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(c->runEnd);
+    c->runCap = ncell;
+    c->runEnd = dalloc<uint32_t>(ncell);
+  }
+  hipLaunchKernelGGL(k_cell_ends, dim3(nblk(m)), dim3(256), 0, c->st, ckey, m, c->runEnd);
+
+  // ---- candidates: overlapping pairs not in the container ----
+  ensure_hash(c, c->np + 1024);
+  fill_found(c);
+  int64_t nc = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    HIPCHK(hipMemsetAsync(c->dcnt + 2, 0, sizeof(unsigned long long), c->st));
+    c->C.count = c->dcnt + 2;
+    hipLaunchKernelGGL(k_peri_pairs, dim3(nblk(m)), dim3(256), 0, c->st, c->B, pc, po, c->cellRng, ckey, cval,
+                       c->runEnd, m, c->htab, (uint64_t)c->hcap - 1, c->C, err);
+    nc = (int64_t)read_counter(c, 2);
+    if (nc <= c->C.cap) break;
+    ensure_cands(c, nc);  // rerun with room for all
+  }
+  if ((read_counter(c, 7) & 1) != 0) throw HipError{"body larger than half of the cell size encountered"};
+  if (nc > 0) {
+    const int64_t hc = c->hcap;
+    ensure_hash(c, c->np + nc);
+    if (c->hcap != hc) fill_found(c);
+    const int64_t start = c->np;
+    const int64_t ns = append_candidates(c, nc);
+    if (ns > 0)
+      hipLaunchKernelGGL(k_new_celldist, dim3(nblk(ns)), dim3(256), 0, c->st, c->P, start, ns, c->B, pc, err);
+  }
+  build_csr(c);
+}
+
 // InsertionSortCollider::action (InsertionSortCollider.cpp:191-400), non-periodic.
 void collider_run(sdm_ctx* c) {
+  if (c->prm.periodic) {
+    collider_run_peri(c);
+    return;
+  }
   c->cnt.collider_runs++;
   const int64_t N = c->B.n;
   const int64_t n2 = 2 * N;
@@ -344,24 +545,8 @@ void collider_run(sdm_ctx* c) {
 
   // ---- conditionalyEraseNonReal ----
   if (c->np > 0) {
-    uint32_t* flag = c->u0;
-    uint32_t* E = c->u1;
-    hipLaunchKernelGGL(k_erase_flags, dim3(nblk(c->np)), dim3(256), 0, c->st, c->B, c->P, c->np, flag);
-    const uint32_t Etot = exclusive_total(c, flag, E, c->np);
-    if (Etot > 0) {
-      const int64_t K = c->np - Etot;
-      hipLaunchKernelGGL(k_hole_pos, dim3(nblk(K)), dim3(256), 0, c->st, flag, E, K, c->holePos);
-      const int64_t tail = c->np - K;
-      hipLaunchKernelGGL(k_tail_moves, dim3(nblk(tail)), dim3(256), 0, c->st, flag, E, K, c->np, Etot, c->holePos,
-                         c->msrc, c->mdst);
-      // moves = holes below K = erased entries below K = E[K] (= kept entries at >= K)
-      const int64_t nmove = read_u32(c, E + K);
-      if (nmove > 0)
-        hipLaunchKernelGGL(k_move_pairs, dim3(nblk(nmove * (int64_t)(sizeof(PairRec) / 16))), dim3(256), 0, c->st,
-                           c->P.r, c->msrc, c->mdst, nmove);
-      c->np = K;
-      c->cnt.pairs_erased += Etot;
-    }
+    hipLaunchKernelGGL(k_erase_flags, dim3(nblk(c->np)), dim3(256), 0, c->st, c->B, c->P, c->np, c->u0);
+    erase_flagged(c);
   }
 
   // ---- sort the bounds of each axis ----
@@ -412,40 +597,12 @@ void collider_run(sdm_ctx* c) {
 
   if (nc > 0) {
     ensure_hash(c, c->np + nc);
-    ensure_scratch(c, nc + 1);
-    const uint64_t mask = (uint64_t)c->hcap - 1;
-    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->htab, c->hcap, SDM_HEMPTY);
-    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->dkeys, c->hcap, SDM_HEMPTY);
-    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->dvals, c->hcap, SDM_HEMPTY);
-    if (c->np > 0)
-      hipLaunchKernelGGL(k_hash_insert_pairs, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.r, c->np,
-                         c->htab, mask);
-    uint32_t* alive = c->u0;
-    uint32_t* pos = c->u1;
-    hipLaunchKernelGGL(k_cand_dedup, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, c->htab, mask, c->dkeys, c->dvals,
-                       mask, alive);
-    hipLaunchKernelGGL(k_cand_keep, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, c->dkeys, c->dvals, mask, alive);
-    const int64_t ns = exclusive_total(c, alive, pos, nc);
-    if (ns > 0) {
-      hipLaunchKernelGGL(k_cand_compact, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, alive, pos, c->k0, c->v0);
-      sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, ns, 64, c->rsScratch, c->st);
-      ensure_pair_cap(c, c->np + ns);
-      hipLaunchKernelGGL(k_new_pairs, dim3(nblk(ns)), dim3(256), 0, c->st, c->P, c->np, ns, c->C, c->v0, c->iter);
-      c->np += ns;
-      c->cnt.pairs_inserted += ns;
-    }
+    fill_found(c);
+    append_candidates(c, nc);
   }
   build_csr(c);
 }
 
-unsigned long long read_counter(sdm_ctx* c, int k) {
-  unsigned long long v = 0;
-  HIPCHK(hipMemcpyAsync(&v, c->dcnt + k, sizeof(v), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  return v;
-}
-
-void zero_counter(sdm_ctx* c, int k) { HIPCHK(hipMemsetAsync(c->dcnt + k, 0, sizeof(unsigned long long), c->st)); }
 
 // Grid of a persistent solver: the blocks that are resident at once (occupancy x CUs), or
 // fewer when the list is short.  Each lane then works through several pairs, so the lanes
@@ -595,6 +752,10 @@ void one_step(sdm_ctx* c) {
   A.counters = c->dcnt;
   A.csr = c->csrPair;
   A.fullWrite = c->pairsWritten ? 1 : 0;
+  A.periodic = c->prm.periodic ? 1 : 0;
+  A.cell[0] = c->prm.cell[0];
+  A.cell[1] = c->prm.cell[1];
+  A.cell[2] = c->prm.cell[2];
   c->pairsWritten = false;
   if (c->prm.law == SDM_LAW_LINEAR)
     interaction_loop<0>(c, A);
@@ -736,6 +897,11 @@ sdm_ctx* sdm_create(const sdm_params* params, int32_t n_bodies, int32_t n_materi
     g_err = "sdm_create: sortAxis must be 0, 1 or 2";
     return nullptr;
   }
+  if (params->periodic && !(params->cell[0] > 0 && params->cell[1] > 0 && params->cell[2] > 0 &&
+                            std::isfinite(params->cell[0] + params->cell[1] + params->cell[2]))) {
+    g_err = "sdm_create: a periodic cell needs three positive finite sizes";
+    return nullptr;
+  }
   sdm_ctx* c = new sdm_ctx();
   try {
     c->prm = *params;
@@ -805,6 +971,8 @@ void sdm_destroy(sdm_ctx* c) {
   dfree(c->htab); dfree(c->dkeys); dfree(c->dvals);
   dfree(c->C.a); dfree(c->C.b); dfree(c->C.key);
   dfree(c->msrc); dfree(c->mdst); dfree(c->holePos); dfree(c->stage);
+  dfree(c->prevMin); dfree(c->prevMax); dfree(c->cellRng); dfree(c->rankMin); dfree(c->rankMax);
+  dfree(c->peCnt); dfree(c->peOff); dfree(c->runEnd);
   dfree(c->dmats); dfree(c->dcnt); dfree(c->dMaxDisp);
   for (int k = 0; k < 6; k++)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
@@ -1016,6 +1184,7 @@ int sdm_step(sdm_ctx* c, int32_t nsteps) {
         if (m < 0 || m >= (int)c->mats.size()) return fail(c, "body material index out of range");
       c->matChecked = true;
     }
+    if (c->prm.periodic && c->nSpec > 1) return fail(c, "several specimens in one periodic cell are not supported");
     for (int s = 0; s < nsteps; s++) one_step(c);
   } catch (HipError& e) {
     return fail(c, e.msg);

@@ -37,6 +37,7 @@ struct SolveIO {  // indexed by work slot
   int32_t* nev;    // distance evaluations (accumulated over solver passes)
   int32_t* stop;   // LM stop code / NM nfunc
   int32_t* touch;  // 1 = touching
+  double* sh;      // LM: [3][cap] periodic shift of body 2 (shift2); nullptr outside a periodic cell
   int64_t cap;
 };
 
@@ -465,7 +466,9 @@ __global__ __launch_bounds__(256) void k_solve_lm(SolveArgs A) {
         done = true;
       } else {
         w = A.list ? A.list[got] : got;
-        pair_geo<double>(load_rec(A.geo, A.S.id1[w]), load_rec(A.geo, A.S.id2[w]), false, g);
+        GeoRec r2 = load_rec(A.geo, A.S.id2[w]);
+        if (A.S.sh) r2.pos = r2.pos + ld3s(A.S.sh, w, A.S.cap);  // position2 = pos + shift2 (:1886)
+        pair_geo<double>(load_rec(A.geo, A.S.id1[w]), r2, false, g);
         p0 = A.S.a0[w];
         p1v = A.S.a1[w];
         ct = ld3s(A.S.ct, w, A.S.cap);

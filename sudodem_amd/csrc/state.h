@@ -58,6 +58,7 @@
   X(int32_t, active, 1, "contact.active")                    \
   X(int32_t, nEval, 1, "diag.nEval")                         \
   X(int32_t, stop, 1, "diag.stop")                           \
+  X(int32_t, cellDist, 3, "intr.cellDist")                   \
   X(int64_t, iterMadeReal, 1, "intr.iterMadeReal")           \
   X(double, depth, 1, "geom.PenetrationDepth")               \
   X(double, ang, 2, "geom.contactAngle")                     \
@@ -102,7 +103,7 @@ struct PairRec {
 #undef SDM_PREC
   // positions of the interaction's two entries in the per-body lists (rebuilt with them)
   int32_t csrPos[2];
-  double pad_[9];
+  int32_t pad_[16];
 };
 static_assert(sizeof(PairRec) == 640, "interaction record size");
 

@@ -213,8 +213,25 @@ class NewtonIntegrator:
         self.__dict__.update(kw)
 
 
-def engines_to_params(engines, dt, precision=nat.SDM_FP64):
-    """Validate the engine list and map its attributes onto sdm_params."""
+class Cell:
+    """Axis-aligned periodic cell (core/Cell.hpp): only its size is used (no velGrad)."""
+
+    def __init__(self):
+        self.size = (1.0, 1.0, 1.0)
+
+    def setBox(self, x, y=None, z=None):
+        if y is None:
+            x, y, z = x
+        self.size = (float(x), float(y), float(z))
+
+    @property
+    def hSize(self):
+        return np.diag(self.size)
+
+
+def engines_to_params(engines, dt, precision=nat.SDM_FP64, cell=None):
+    """Validate the engine list and map its attributes onto sdm_params; `cell` (three sizes)
+    makes the scene periodic (Scene::isPeriodic with an axis-aligned Cell)."""
     kinds = [type(e).__name__ for e in engines]
     if kinds != ["ForceResetter", "InsertionSortCollider", "InteractionLoop", "NewtonIntegrator"]:
         raise ValueError("engine list must be [ForceResetter, InsertionSortCollider, InteractionLoop, "
@@ -249,6 +266,9 @@ def engines_to_params(engines, dt, precision=nat.SDM_FP64):
     p.sort_axis = col.sortAxis
     p.law = lawid
     p.precision = precision
+    if cell is not None:
+        p.periodic = 1
+        p.cell[:] = [float(v) for v in cell]
     return p
 
 
@@ -309,6 +329,8 @@ class Omega:
         self.engines = []
         self.dt = 1e-8
         self.ctx = None
+        self.periodic = False  # Scene::isPeriodic
+        self.cell = Cell()
 
     # -- lifecycle --
     def _arrays(self):
@@ -348,7 +370,8 @@ class Omega:
         if self.ctx is not None:
             return
         lib = self.lib or nat.load_hip()
-        params = engines_to_params(self.engines, self.dt, self.precision)
+        params = engines_to_params(self.engines, self.dt, self.precision,
+                                   self.cell.size if self.periodic else None)
         self.ctx = nat.Context(lib, params, len(self.bodies), [m._native() for m in self.materials])
         upload_arrays(self.ctx, self._arrays())
         for b in self.bodies:

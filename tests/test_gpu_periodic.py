@@ -1,0 +1,170 @@
+"""Periodic cell on the GPU (libsdm_hip.so) against the oracle and against brute force.
+
+Bars:
+ * first collider run: the interaction list (ids, orientation, container order) and every
+   pair's cellDist are bit-exact against the oracle's sorted-ring sweep;
+ * every later run: the device set is exactly the brute-force set of periodically overlapping
+   boxes plus real contacts (the oracle's insertionSortPeri may create a pair one run late,
+   see test_periodic.py), and the pairs both create from identical states carry the same
+   orientation and cellDist;
+ * one step from identical states (linear law, FP64): depth, normal and force per contact,
+   with the bars and the reason for them given in _contact_agreement.
+"""
+import numpy as np
+import pytest
+
+from sudodem_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _pairs(ctx):
+    return np.stack([ctx.get("intr.id1"), ctx.get("intr.id2")], axis=1)
+
+
+def _contact_agreement(g, c, what, frac):
+    """Per-contact depth / normal / force agreement between the device and the oracle.
+
+    The Levenberg-Marquardt solve of the Geom functor takes its forward-difference Jacobian
+    with the step set to the last increment (lmder2, Superquadrics_Ig2.cpp:1170); once the
+    increments are small, that Jacobian carries the last-ulp differences of pow / atan2 /
+    sin / cos between the device math library and glibc at O(1) relative weight, and the loose
+    stop test |f.c| > 0.95 (:1150-1166) ends at an iterate that shows them, even when both
+    sides take the same path (stop code and evaluation count).  Bars: the same path for 99%
+    of the contacts, a fraction `frac` of them within 1e-10 relative, every one within 5%,
+    and contacts across a face (cellDist != 0) no worse than interior ones."""
+    dg, dc = g.get("geom.PenetrationDepth"), c.get("geom.PenetrationDepth")
+    act = (dc > 0) & (dg > 0)
+    assert act.sum() > 300 and act.sum() >= 0.98 * (dc > 0).sum()
+    path = act & (g.get("diag.stop") == c.get("diag.stop")) & (g.get("diag.nEval") == c.get("diag.nEval"))
+    F = c.get("contact.force").reshape(-1, 3)
+    Fs = np.maximum(np.max(np.abs(F), axis=1), 1e-300)
+    rel = np.stack([np.abs(dg - dc) / np.maximum(dc, 1e-300),
+                    np.max(np.abs(g.get("geom.normal").reshape(-1, 3) - c.get("geom.normal").reshape(-1, 3)), axis=1),
+                    np.max(np.abs(g.get("contact.force").reshape(-1, 3) - F), axis=1) / Fs], axis=1).max(axis=1)
+    cross = (c.get("intr.cellDist").reshape(-1, 3) != 0).any(axis=1)
+    fr = {}
+    for name, sel in (("interior", act & ~cross), ("across a face", act & cross)):
+        fr[name] = np.mean(rel[path & sel] <= 1e-10)
+        print("%s, %s: %d contacts, same path %.4f, within 1e-10 %.4f, max rel %.3g"
+              % (what, name, sel.sum(), (path & sel).sum() / max(1, sel.sum()), fr[name], rel[sel].max(initial=0)))
+    assert path.sum() >= 0.99 * act.sum()
+    assert np.mean(rel[path] <= 1e-10) >= frac
+    assert rel[act].max() <= 0.05
+    assert (act & cross).sum() > 20
+    assert fr["across a face"] >= fr["interior"] - 0.05
+
+
+def _periodic_overlaps(bmin, bmax, L):
+    c = 0.5 * (bmin + bmax)
+    w = bmax - bmin
+    out = set()
+    for i in range(len(c)):
+        d = c[i + 1:] - c[i]
+        d -= L * np.round(d / L)
+        ok = np.all(np.abs(d) <= 0.5 * (w[i + 1:] + w[i]), axis=1)
+        out.update((i, i + 1 + int(j)) for j in np.nonzero(ok)[0])
+    return out
+
+
+@pytest.fixture(scope="module")
+def c2_small(gpu):
+    # 2,000 particles of the periodic workload's distributions, densified into contact
+    cfg = configs.densify(configs.config2(n_particles=2000), gpu, iterations=80, growth=0.01, relax=20)
+    print("c2_small: cell %.2f mm, solids fraction %.3f" % (cfg.cell[0] * 1e3, cfg.solids_fraction), flush=True)
+    return cfg
+
+
+def test_periodic_first_run_exact(gpu, oracle, c2_small):
+    g = c2_small.open(gpu)
+    c = c2_small.open(oracle)
+    oracle.set_init_sort(c.ptr, 1)
+    g.step(1)
+    c.step(1)
+    assert np.array_equal(_pairs(g), _pairs(c))
+    assert np.array_equal(g.get("intr.cellDist"), c.get("intr.cellDist"))
+    cd = g.get("intr.cellDist").reshape(-1, 3)
+    assert (cd != 0).any(axis=1).sum() > 50  # pairs across the faces
+    for name in ("bound.min", "bound.max"):
+        assert np.array_equal(g.get(name), c.get(name)), name
+    _contact_agreement(g, c, "first step", 0.9)
+
+
+def test_periodic_pair_set_every_run_is_brute_force(gpu, c2_small):
+    cfg = c2_small
+    rng = np.random.default_rng(3)
+    g = cfg.open(gpu)
+    g.set("state.vel", rng.normal(scale=20.0, size=(cfg.n, 3)))
+    L = np.array(cfg.cell)
+    g.step(1)
+    for k in range(30):
+        i1, i2 = g.get("intr.id1"), g.get("intr.id2")
+        real = (g.get("intr.hasGeom") != 0) & (g.get("intr.hasPhys") != 0)
+        real_before = {(min(a, b), max(a, b)) for a, b, r in zip(i1, i2, real) if r}
+        runs = g.counters()["collider_runs"]
+        g.set("collider.strideActive", np.array([0], np.int32))
+        g.step(1)
+        assert g.counters()["collider_runs"] == runs + 1
+        bmin, bmax = g.get("bound.min").reshape(-1, 3), g.get("bound.max").reshape(-1, 3)
+        i1, i2 = g.get("intr.id1"), g.get("intr.id2")
+        have = {(min(a, b), max(a, b)) for a, b in zip(i1, i2)}
+        assert len(have) == len(i1)  # no duplicates
+        expect = _periodic_overlaps(bmin, bmax, L)
+        assert expect <= have, k
+        assert have - expect <= real_before, k
+    pos = g.get("state.pos").reshape(-1, 3)
+    assert ((pos < 0) | (pos >= L)).any(axis=1).sum() > 10
+
+
+def test_periodic_new_pairs_orientation_from_identical_states(gpu, oracle, c2_small):
+    """From the oracle's state after some motion, one forced collider run on both sides: every
+    pair the oracle creates is created by the device with the same (id1, id2) and cellDist
+    (except a pair that already overlapped at the previous run, which the oracle's ring pass
+    created late and whose crossing the device cannot see), and the device's extra pairs (the
+    ones insertionSortPeri creates one run late) all overlap."""
+    cfg = c2_small
+    rng = np.random.default_rng(4)
+    c = cfg.open(oracle)
+    oracle.set_init_sort(c.ptr, 1)
+    c.set("state.vel", rng.normal(scale=1.0, size=(cfg.n, 3)))
+    c.step(8)
+    g = cfg.open(gpu)
+    g.restore(c.snapshot())
+    L = np.array(cfg.cell)
+    prev = _periodic_overlaps(c.get("bound.min").reshape(-1, 3), c.get("bound.max").reshape(-1, 3), L)
+    before = {(min(a, b), max(a, b)) for a, b in _pairs(c)}
+    for ctx in (g, c):
+        ctx.set("collider.strideActive", np.array([0], np.int32))
+        ctx.step(1)
+
+    def new(ctx):
+        cd = ctx.get("intr.cellDist").reshape(-1, 3)
+        return {(min(a, b), max(a, b)): (int(a), int(b), tuple(cd[k])) for k, (a, b) in enumerate(_pairs(ctx))
+                if (min(a, b), max(a, b)) not in before}
+    ng, nc = new(g), new(c)
+    assert len(nc) > 20
+    assert set(nc) <= set(ng)
+    bad = [k for k in nc if k not in prev and ng[k] != nc[k]]
+    assert not bad, bad[:5]
+    bmin, bmax = g.get("bound.min").reshape(-1, 3), g.get("bound.max").reshape(-1, 3)
+    extra = set(ng) - set(nc)
+    assert extra <= _periodic_overlaps(bmin, bmax, L)
+    assert len(extra) <= max(3, len(nc) // 10)
+
+
+def test_periodic_step_physics_from_identical_state(gpu, oracle, c2_small):
+    cfg = c2_small
+    c = cfg.open(oracle)
+    oracle.set_init_sort(c.ptr, 1)
+    c.step(3)
+    g = cfg.open(gpu)
+    g.restore(c.snapshot())
+    g.step(1)
+    c.step(1)
+    pg, pc = _pairs(g), _pairs(c)
+    if not np.array_equal(pg, pc):  # a pair created late by the oracle's ring pass
+        assert len(pg) >= len(pc)
+        pytest.skip("container differs by late pairs; covered by the orientation test")
+    _contact_agreement(g, c, "step from restored state", 0.97)
+    xg, xc = g.get("state.pos"), c.get("state.pos")
+    assert np.max(np.abs(xg - xc)) <= 1e-9 * max(1.0, np.abs(xc).max())
