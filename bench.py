@@ -204,6 +204,120 @@ def batch_bench(args):
     }))
 
 
+def _sample_pair_set_check(ctx, cfg, n_sample=2000, seed=0):
+    """Exactness of the periodic pair set on a sample of bodies: for each sampled body, the
+    brute-force set of partners whose boxes overlap it in the periodic cell (closed intervals;
+    candidates from a periodic k-d tree in the max norm), against the interactions of that
+    body in the container.  Returns counts of missing and extra (non-real) pairs."""
+    from scipy.spatial import cKDTree
+    L = np.array(cfg.cell)
+    bmin, bmax = ctx.get("bound.min").reshape(-1, 3), ctx.get("bound.max").reshape(-1, 3)
+    c = 0.5 * (bmin + bmax)
+    w = bmax - bmin
+    wmax = float(w.max())
+    tree = cKDTree(np.mod(c, L), boxsize=L)
+    rng = np.random.default_rng(seed)
+    sample = np.sort(rng.choice(len(c), size=min(n_sample, len(c)), replace=False))
+    i1, i2 = ctx.get("intr.id1"), ctx.get("intr.id2")
+    real = (ctx.get("intr.hasGeom") != 0) & (ctx.get("intr.hasPhys") != 0)
+    insample = np.zeros(len(c), bool)
+    insample[sample] = True
+    have, have_real = {}, {}
+    for a, b, r in zip(i1, i2, real):
+        for x, y in ((a, b), (b, a)):
+            if insample[x]:
+                have.setdefault(int(x), set()).add(int(y))
+                if r:
+                    have_real.setdefault(int(x), set()).add(int(y))
+    missing = extra = checked = 0
+    for i in sample:
+        cand = tree.query_ball_point(np.mod(c[i], L), r=0.5 * (w[i].max() + wmax) * (1 + 1e-9), p=np.inf)
+        cand = np.array([j for j in cand if j != i], dtype=np.int64)
+        d = c[cand] - c[i]
+        d -= L * np.round(d / L)
+        ok = np.all(np.abs(d) <= 0.5 * (w[cand] + w[i]), axis=1)
+        expect = set(int(j) for j in cand[ok])
+        got = have.get(int(i), set())
+        missing += len(expect - got)
+        extra += len((got - expect) - have_real.get(int(i), set()))
+        checked += len(expect)
+    return {"bodies_sampled": int(len(sample)), "overlapping_pairs_checked": checked, "missing": missing,
+            "extra_non_real": extra}
+
+
+def periodic_bench(args):
+    """Periodic workload: args.periodic superellipsoids (r lognormal mean 1 mm / log-sigma 0.25,
+    rz = U[0.5,1.5] r, eps U[0.3,1.9]) in a cubic periodic cell, linear law (Levenberg-Marquardt
+    Geom with the periodic shift), FP64, no gravity.  Seeded lattice + grow-and-relax packing,
+    W warm-up steps, K timed steps; pair-set exactness checked on a sample of bodies after the
+    timed run; the CPU oracle runs a bounded sample on a 20k-particle cell of the same
+    distributions (its initial sweep over a million bodies alone would take hours)."""
+    import torch
+
+    from sudodem_amd import SDM_FP64, configs, load_hip
+    from sudodem_amd._native import Context
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    lib = load_hip()
+    t0 = time.perf_counter()
+    lattice = configs.config2(n_particles=args.periodic)
+    log("periodic lattice of %d bodies in %.1fs (cell %.1f mm, solids fraction %.3f)" % (
+        lattice.n, time.perf_counter() - t0, lattice.cell[0] * 1e3, lattice.solids_fraction))
+    t0 = time.perf_counter()
+    cfg = configs.densify(lattice, lib, iterations=args.densify, growth=0.01, relax=20, log=log)
+    log("densified in %.1fs (solids fraction %.3f, mean scale %.3f)" % (time.perf_counter() - t0,
+                                                                        cfg.solids_fraction, cfg.mean_scale))
+    rng = np.random.default_rng(7)
+    ctx = cfg.open(lib, precision=SDM_FP64)
+    ctx.set("state.vel", rng.normal(scale=0.05, size=(cfg.n, 3)))  # agitation: the collider keeps running
+    t0 = time.perf_counter()
+    ctx.step(args.warmup)
+    ctx.sync()
+    log("warm-up %d steps in %.1fs" % (args.warmup, time.perf_counter() - t0))
+    K = args.steps
+    wall, cnt, tm = timed_run(ctx, K)
+    N = cfg.n
+    depth = ctx.get("geom.PenetrationDepth")
+    check = _sample_pair_set_check(ctx, cfg)
+    log("pair-set check: %s" % check)
+    ctx.close()
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import load_oracle
+        orc = load_oracle()
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        small = configs.densify(configs.config2(n_particles=20000), lib, iterations=args.densify, growth=0.01,
+                                relax=20)
+        cctx = small.open(orc)
+        orc.check(orc.set_threads(cctx.ptr, threads), cctx.ptr)
+        cctx.step(1)  # initial sort and sweep outside the sample
+        t0 = time.perf_counter()
+        cctx.step(args.cpu_steps)
+        cw = time.perf_counter() - t0
+        cctx.close()
+        cpu = {"value": small.n * args.cpu_steps / cw, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": "%d steps of a 20000-particle periodic cell of the same distributions, FP64 oracle "
+                         "(restatement of the reference loop), OpenMP %d threads, %.1fs wall" % (
+                             args.cpu_steps, threads, cw)}
+        log("cpu baseline: %.0f particle-steps/s" % cpu["value"])
+    print(json.dumps({
+        "metric": "particle-steps/s and contact-evals/s, periodic cell, %d superellipsoids" % N,
+        "value": N * K / wall, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%d superellipsoids (eps U[0.3,1.9], rz/rx U[0.5,1.5], lognormal sizes) in a "
+                               "%.1f mm periodic cell, linear law (LM Geom), dt=1e-6 s" % (N, cfg.cell[0] * 1e3),
+                   "n_particles": N, "parallelism": "single GPU"},
+        "contact_evals_per_s": cnt["contact_evals"] / wall, "dist_evals_per_s": cnt["dist_evals"] / wall,
+        "collider_runs": cnt["collider_runs"], "interactions": int(len(depth)), "contacts": int((depth > 0).sum()),
+        "solids_fraction": cfg.solids_fraction,
+        "phases_ms_per_step": {k: tm[k] / K for k in ("collider_ms", "interactions_ms", "newton_ms", "solver_ms")},
+        "pair_set_check": check,
+        "cpu_baseline": cpu,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +331,8 @@ def main():
                     help="batching workload instead: this many box specimens in one context vs one by one")
     ap.add_argument("--adjoint", action="store_true",
                     help="differentiation workload instead: 20k-body frozen-contact rollout, forward + adjoint")
+    ap.add_argument("--periodic", type=int, default=0, metavar="PARTICLES",
+                    help="periodic workload instead: this many superellipsoids in a periodic cell (1000000)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of known size (the reference for rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -228,6 +344,9 @@ def main():
         return
     if args.adjoint:
         adjoint_bench(args)
+        return
+    if args.periodic:
+        periodic_bench(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",

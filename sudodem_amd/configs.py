@@ -357,7 +357,7 @@ def densify(cfg, lib, iterations=150, target=2e-6, growth=0.003, max_push=5e-5, 
             np.maximum.at(dm, i2, d)
             gr = part & (dm < 2 * target) & (lam < cap) & (it < iterations - relax)
             lam[gr] = np.minimum(cap[gr], lam[gr] * (1 + growth))
-            if log and (it % 25 == 0 or it == iterations - 1):
+            if log and (it % 10 == 0 or it == iterations - 1):
                 log("densify %d: %d contacts, %d growing" % (it, int((d > 0).sum()), int(gr.sum())))
     finally:
         ctx.close()
