@@ -204,11 +204,12 @@ def batch_bench(args):
     }))
 
 
-def _sample_pair_set_check(ctx, cfg, n_sample=2000, seed=0):
-    """Exactness of the periodic pair set on a sample of bodies: for each sampled body, the
-    brute-force set of partners whose boxes overlap it in the periodic cell (closed intervals;
-    candidates from a periodic k-d tree in the max norm), against the interactions of that
-    body in the container.  Returns counts of missing and extra (non-real) pairs."""
+def _sample_pair_set_check(ctx, cfg, real_before, n_sample=2000, seed=0):
+    """Exactness of the periodic pair set on a sample of bodies, right after a collider run:
+    for each sampled body, the brute-force set of partners whose boxes overlap it in the
+    periodic cell (closed intervals; candidates from a periodic k-d tree in the max norm),
+    against the interactions of that body in the container.  Returns the number of missing
+    pairs and of extra pairs that were not real contacts when the collider ran."""
     from scipy.spatial import cKDTree
     L = np.array(cfg.cell)
     bmin, bmax = ctx.get("bound.min").reshape(-1, 3), ctx.get("bound.max").reshape(-1, 3)
@@ -219,16 +220,16 @@ def _sample_pair_set_check(ctx, cfg, n_sample=2000, seed=0):
     rng = np.random.default_rng(seed)
     sample = np.sort(rng.choice(len(c), size=min(n_sample, len(c)), replace=False))
     i1, i2 = ctx.get("intr.id1"), ctx.get("intr.id2")
-    real = (ctx.get("intr.hasGeom") != 0) & (ctx.get("intr.hasPhys") != 0)
     insample = np.zeros(len(c), bool)
     insample[sample] = True
     have, have_real = {}, {}
-    for a, b, r in zip(i1, i2, real):
+    for a, b in zip(i1.tolist(), i2.tolist()):
+        r = (a, b) in real_before or (b, a) in real_before
         for x, y in ((a, b), (b, a)):
             if insample[x]:
-                have.setdefault(int(x), set()).add(int(y))
+                have.setdefault(x, set()).add(y)
                 if r:
-                    have_real.setdefault(int(x), set()).add(int(y))
+                    have_real.setdefault(x, set()).add(y)
     missing = extra = checked = 0
     for i in sample:
         cand = tree.query_ball_point(np.mod(c[i], L), r=0.5 * (w[i].max() + wmax) * (1 + 1e-9), p=np.inf)
@@ -251,7 +252,8 @@ def periodic_bench(args):
     Geom with the periodic shift), FP64, no gravity.  Seeded lattice + grow-and-relax packing,
     W warm-up steps, K timed steps; pair-set exactness checked on a sample of bodies after the
     timed run; the CPU oracle runs a bounded sample on a 20k-particle cell of the same
-    distributions (its initial sweep over a million bodies alone would take hours)."""
+    distributions (its initial sweep over a million bodies alone would take hours).  The dense
+    packing is an 8000-particle cell compressed under dynamics and tiled (configs.config2_dense)."""
     import torch
 
     from sudodem_amd import SDM_FP64, configs, load_hip
@@ -260,13 +262,10 @@ def periodic_bench(args):
     torch.cuda.set_device(0)
     lib = load_hip()
     t0 = time.perf_counter()
-    lattice = configs.config2(n_particles=args.periodic)
-    log("periodic lattice of %d bodies in %.1fs (cell %.1f mm, solids fraction %.3f)" % (
-        lattice.n, time.perf_counter() - t0, lattice.cell[0] * 1e3, lattice.solids_fraction))
-    t0 = time.perf_counter()
-    cfg = configs.densify(lattice, lib, iterations=args.densify, growth=0.01, relax=20, log=log)
-    log("densified in %.1fs (solids fraction %.3f, mean scale %.3f)" % (time.perf_counter() - t0,
-                                                                        cfg.solids_fraction, cfg.mean_scale))
+    k = max(1, int(round((args.periodic / 8000.0) ** (1.0 / 3.0))))
+    cfg = configs.config2_dense(lib, n_particles=args.periodic, base=args.periodic // k ** 3, log=log)
+    log("dense periodic packing of %d bodies in %.1fs (cell %.1f mm, solids fraction %.3f)" % (
+        cfg.n, time.perf_counter() - t0, cfg.cell[0] * 1e3, cfg.solids_fraction))
     rng = np.random.default_rng(7)
     ctx = cfg.open(lib, precision=SDM_FP64)
     ctx.set("state.vel", rng.normal(scale=0.05, size=(cfg.n, 3)))  # agitation: the collider keeps running
@@ -278,7 +277,12 @@ def periodic_bench(args):
     wall, cnt, tm = timed_run(ctx, K)
     N = cfg.n
     depth = ctx.get("geom.PenetrationDepth")
-    check = _sample_pair_set_check(ctx, cfg)
+    # pair-set exactness right after a collider run (forced on the next step)
+    real = (ctx.get("intr.hasGeom") != 0) & (ctx.get("intr.hasPhys") != 0)
+    real_before = set(zip(ctx.get("intr.id1")[real].tolist(), ctx.get("intr.id2")[real].tolist()))
+    ctx.set("collider.strideActive", np.array([0], np.int32))
+    ctx.step(1)
+    check = _sample_pair_set_check(ctx, cfg, real_before=real_before)
     log("pair-set check: %s" % check)
     ctx.close()
     cpu = None
@@ -287,19 +291,22 @@ def periodic_bench(args):
         from oracle_lib import load_oracle
         orc = load_oracle()
         threads = max(1, min(os.cpu_count() or 1, 16))
-        small = configs.densify(configs.config2(n_particles=20000), lib, iterations=args.densify, growth=0.01,
-                                relax=20)
+        small = cfg.base
         cctx = small.open(orc)
         orc.check(orc.set_threads(cctx.ptr, threads), cctx.ptr)
+        cctx.set("state.vel", np.random.default_rng(7).normal(scale=0.05, size=(small.n, 3)))
         cctx.step(1)  # initial sort and sweep outside the sample
         t0 = time.perf_counter()
-        cctx.step(args.cpu_steps)
+        csteps = 0
+        while time.perf_counter() - t0 < 10.0 and csteps < 5000:  # ~10 s of CPU work
+            cctx.step(10)
+            csteps += 10
         cw = time.perf_counter() - t0
         cctx.close()
-        cpu = {"value": small.n * args.cpu_steps / cw, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": "%d steps of a 20000-particle periodic cell of the same distributions, FP64 oracle "
-                         "(restatement of the reference loop), OpenMP %d threads, %.1fs wall" % (
-                             args.cpu_steps, threads, cw)}
+        cpu = {"value": small.n * csteps / cw, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": "%d steps of the %d-particle base cell of the same packing, same agitation, FP64 "
+                         "oracle (restatement of the reference loop), OpenMP %d threads, %.1fs wall" % (
+                             csteps, small.n, threads, cw)}
         log("cpu baseline: %.0f particle-steps/s" % cpu["value"])
     print(json.dumps({
         "metric": "particle-steps/s and contact-evals/s, periodic cell, %d superellipsoids" % N,
@@ -307,7 +314,8 @@ def periodic_bench(args):
         "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "%d superellipsoids (eps U[0.3,1.9], rz/rx U[0.5,1.5], lognormal sizes) in a "
-                               "%.1f mm periodic cell, linear law (LM Geom), dt=1e-6 s" % (N, cfg.cell[0] * 1e3),
+                               "%.1f mm periodic cell (a %d-particle compressed cell tiled %dx%dx%d), linear law "
+                               "(LM Geom), dt=1e-6 s" % (N, cfg.cell[0] * 1e3, cfg.base.n, k, k, k),
                    "n_particles": N, "parallelism": "single GPU"},
         "contact_evals_per_s": cnt["contact_evals"] / wall, "dist_evals_per_s": cnt["dist_evals"] / wall,
         "collider_runs": cnt["collider_runs"], "interactions": int(len(depth)), "contacts": int((depth > 0).sum()),

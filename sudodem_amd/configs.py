@@ -346,8 +346,10 @@ def densify(cfg, lib, iterations=150, target=2e-6, growth=0.003, max_push=5e-5, 
             ctx.step(1)
             i1 = ctx.get("intr.id1")
             i2 = ctx.get("intr.id2")
-            d = ctx.get("geom.PenetrationDepth")
-            nrm = ctx.get("geom.normal").reshape(-1, 3)
+            # a Levenberg-Marquardt solve that ends non-finite after its restarts leaves a NaN depth
+            # (the reference stores it too); such a pair pushes nothing
+            d = np.nan_to_num(ctx.get("geom.PenetrationDepth"), nan=0.0)
+            nrm = np.nan_to_num(ctx.get("geom.normal").reshape(-1, 3), nan=0.0)
             ex = np.minimum(np.where(d > target, d - target, 0.0), max_push)
             w = np.where(A["kind"][i1] == nat.SDM_BODY_WALL, 0.7, 0.35) * ex
             for k in range(3):
@@ -435,6 +437,108 @@ def config2(n_particles=1000000, seed_extra=0, fraction=0.62):
     cfg.rxyz_drawn = drawn
     cfg.update_stats()
     return cfg
+
+
+def compress(cfg, lib, target=0.62, growth=1e-3, every=50, max_steps=8000, max_depth=1e-5, relax=200, damping=0.7,
+             log=None):
+    """Growth under dynamics for a periodic packing: the engine runs with its own dt, local
+    damping `damping` and frictionless copies of the materials; every `every` steps all
+    particles grow by the factor 1 + `growth` (semi-axes, r_max, mass and inertia scaled as
+    size^1, ^1, ^3, ^5) and all velocities are zeroed, until the solids fraction reaches `target`, the mean contact overlap exceeds
+    `max_depth` in the median (jammed), or `max_steps`; then `relax` steps without growth and all
+    velocities zeroed.  Returns a new Config with the final positions, orientations and sizes
+    (the original materials)."""
+    from .scene import SuperquadricsMat as _M
+    assert cfg.cell is not None, "compress() is for periodic cells"
+    A = cfg.arrays
+    n = cfg.n
+    mats = []
+    for m in cfg.materials:
+        c = copy.copy(m)
+        c.frictionAngle = 0.0
+        mats.append(c)
+    eng = copy.deepcopy(cfg.engines)
+    eng[3].damping = damping
+    ctx = Config(cfg.name, A, mats, eng, cfg.dt, cfg.steps, cfg.box, cfg.cell).open(lib)
+    zero = np.zeros((n, 3))
+    vol = A["mass"] / np.array([cfg.materials[k].density for k in A["material"]])
+    cellvol = float(np.prod(cfg.cell))
+    scale, steps, frac = 1.0, 0, float(vol.sum()) / cellvol
+    try:
+        while steps < max_steps and frac < target:
+            ctx.step(every)
+            steps += every
+            g = 1.0 + growth
+            scale *= g
+            ctx.set("shape.rxyz", A["rxyz"] * scale)
+            ctx.set("shape.r_max", A["r_max"] * scale)
+            ctx.set("state.mass", A["mass"] * scale ** 3)
+            ctx.set("state.inertia", A["inertia"] * scale ** 5)
+            for k in ("state.vel", "state.angVel", "state.angMom"):  # quench: a slow, cold compression
+                ctx.set(k, zero)
+            frac = float(vol.sum()) * scale ** 3 / cellvol
+            if steps % (10 * every) == 0:
+                d = np.nan_to_num(ctx.get("geom.PenetrationDepth"), nan=0.0)
+                md = float(np.median(d[d > 0])) if (d > 0).any() else 0.0
+                if log:
+                    log("compress %d steps: solids fraction %.3f, %d contacts, median overlap %.2e" % (
+                        steps, frac, int((d > 0).sum()), md))
+                if md > max_depth:
+                    break
+        ctx.step(relax)
+        B = {k: v.copy() for k, v in A.items()}
+        B["pos"] = ctx.get("state.pos").reshape(-1, 3).copy()
+        B["ori"] = ctx.get("state.ori").reshape(-1, 4).copy()
+        B["rxyz"] = A["rxyz"] * scale
+        B["r_max"] = A["r_max"] * scale
+        B["mass"] = A["mass"] * scale ** 3
+        B["inertia"] = A["inertia"] * scale ** 5
+        for k in ("vel", "angVel", "angMom"):
+            B[k] = np.zeros_like(A[k])
+    finally:
+        ctx.close()
+    out = Config(cfg.name + ", compressed", B, cfg.materials, copy.deepcopy(cfg.engines), cfg.dt, cfg.steps, cfg.box,
+                 cfg.cell)
+    out.rxyz_drawn = getattr(cfg, "rxyz_drawn", None)
+    out.update_stats()
+    return out
+
+
+def tile(cfg, k):
+    """k x k x k copies of a periodic configuration (positions wrapped into the cell first) in a
+    cell k times larger: the same packing, solids fraction and contacts, k^3 times the bodies."""
+    assert cfg.cell is not None
+    L0 = np.array(cfg.cell)
+    A = cfg.arrays
+    pos = np.mod(A["pos"], L0)
+    offs = np.array([[i, j, l] for l in range(k) for j in range(k) for i in range(k)], float) * L0
+    B = {key: np.concatenate([v] * len(offs)) for key, v in A.items()}
+    B["pos"] = np.concatenate([pos + o for o in offs])
+    out = Config(cfg.name + ", tiled %dx%dx%d" % (k, k, k), B, cfg.materials, copy.deepcopy(cfg.engines), cfg.dt,
+                 cfg.steps, (np.zeros(3), L0 * k), tuple(float(x) for x in L0 * k))
+    drawn = getattr(cfg, "rxyz_drawn", None)
+    out.rxyz_drawn = np.concatenate([drawn] * len(offs)) if drawn is not None else None
+    out.update_stats()
+    return out
+
+
+def config2_dense(lib, n_particles=1000000, base=8000, log=None):
+    """The periodic workload as a dense packing: a `base`-particle cell of config2's
+    distributions, densified (grow-and-relax) and compressed under dynamics to ~0.6 solids
+    fraction, then tiled k x k x k (k^3 base = n_particles) -- compressing a million bodies
+    directly would take tens of minutes of engine time."""
+    k = int(round((n_particles / float(base)) ** (1.0 / 3.0)))
+    assert k >= 1 and k ** 3 * base == n_particles, "n_particles must be k^3 x base"
+    c = config2(n_particles=base)
+    c = densify(c, lib, iterations=60, growth=0.01, relax=20)
+    if log:
+        log("base cell: %d particles densified to solids fraction %.3f" % (base, c.solids_fraction))
+    c = compress(c, lib, log=log)
+    if log:
+        log("base cell compressed to solids fraction %.3f" % c.solids_fraction)
+    out = tile(c, k) if k > 1 else c
+    out.base = c
+    return out
 
 
 def box_specimen(n_particles, rng, kn, mu, spacing=2.86e-3):
