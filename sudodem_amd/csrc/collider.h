@@ -149,6 +149,32 @@ __global__ void k_fill_u64(uint64_t* a, int64_t n, uint64_t v) {
   if (i < n) a[i] = v;
 }
 
+// ---- device-side counts: the step's sizes stay on the device, no host round trip ----
+// Total of an exclusive scan: dst = out[n-1] + in[n-1].
+__global__ void k_scan_total(const uint32_t* in, const uint32_t* out, int64_t n, unsigned long long* dst) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = n > 0 ? (unsigned long long)out[n - 1] + in[n - 1] : 0ull;
+}
+// Sort padding past a device-side count: the largest key, so that a stable sort of all n
+// entries leaves the counted ones first, in their order.
+__global__ void k_pad_keys(uint64_t* keys, uint32_t* vals, const unsigned long long* count, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && i >= (int64_t)*count) {
+    keys[i] = ~0ull;
+    vals[i] = 0;
+  }
+}
+__global__ void k_copy_u32(uint32_t* dst, const uint32_t* src, const unsigned long long* count, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && i < (int64_t)*count) dst[i] = src[i];
+}
+// Clear counters with a kernel (a memset node misbehaves under graph replay).
+__global__ void k_zero_u64(unsigned long long* a, int n) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < n) a[threadIdx.x] = 0ull;
+}
+__global__ void k_zero_u32(uint32_t* a, int n) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < n) a[threadIdx.x] = 0u;
+}
+
 __global__ void k_hash_insert_pairs(const PairRec* R, int64_t np, uint64_t* tab, uint64_t mask) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;

@@ -593,10 +593,11 @@ __global__ __launch_bounds__(256) void k_queue_fill(StepArgs A, SolveIO S, const
 // Superquadrics_Ig2.cpp:2522-2703) and sphere-sphere pairs (the sphere branches of both Ig2
 // superquadric functors, :1950-1970, :2331-2350), listed by k_pairs_prep.
 template <int LAW>
-__global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* mlist, int64_t nm) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* mlist,
+                                                    const unsigned long long* count) {
+  const int64_t nm = (int64_t)*count;
   unsigned long long ce = 0, de = 0;
-  if (k < nm) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += (int64_t)gridDim.x * blockDim.x) {
     const PairArrays& P = A.P;
     const BodyArrays& B = A.B;
     const int64_t N = B.n;
@@ -703,9 +704,10 @@ __global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* m
 template <typename T>
 __global__ __launch_bounds__(256) void k_nm_first(StepArgs A, SolveArgs SA, uint32_t* tflag, uint64_t* tkey) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nw = work_n(SA);
   bool go = false;
   uint32_t key = 0;
-  if (w < SA.nwork) {
+  if (w < nw) {
     const PairArrays& P = A.P;
     const int64_t p = SA.S.pair[w];
     const bool hadGeom = P.r[p].hasGeom[0] != 0;
@@ -717,13 +719,13 @@ __global__ __launch_bounds__(256) void k_nm_first(StepArgs A, SolveArgs SA, uint
                      GEO_EARLY_ANG, 1, 1);
     }
   }
-  if (w < SA.nwork) {
+  if (w < SA.nwork) {  // flags up to the upper bound: zero past the count, for the scan
     tflag[w] = go ? 1u : 0u;
     tkey[w] = key;
   }
-  add_evals(SA.evals, w < SA.nwork ? 1ull : 0ull);
+  add_evals(SA.evals, w < nw ? 1ull : 0ull);
   // listed pairs are counted when they finish; the others are complete here
-  add_counters(A.counters, 0, (w < SA.nwork && !go) ? 1ull : 0ull);
+  add_counters(A.counters, 0, (w < nw && !go) ? 1ull : 0ull);
 }
 
 // Compaction of the touching slots (in slot order) with their sort keys.
@@ -743,7 +745,7 @@ __global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int3
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool restart = false;
   const SolveIO& S = SA.S;
-  if (w < SA.nwork && S.touch[w]) {
+  if (w < work_n(SA) && S.touch[w]) {
     const int64_t N = A.B.n;
     const int id1 = S.id1[w], id2 = S.id2[w];
     if (KIND == 1) {
@@ -778,8 +780,10 @@ __global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int3
 
 // Queued pairs after the solvers: geometry outputs, curvature / Hertz terms, precompute, law.
 template <int LAW>
-__global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, const uint32_t* list, int64_t nq) {
+__global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, const uint32_t* list, int64_t nmax,
+                                                      const unsigned long long* count) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nq = count ? ((int64_t)*count < nmax ? (int64_t)*count : nmax) : nmax;
   unsigned long long de = 0;
   if (k < nq) {
     const int64_t w = list ? (int64_t)list[k] : k;

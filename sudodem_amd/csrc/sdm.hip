@@ -336,7 +336,18 @@ unsigned long long read_counter(sdm_ctx* c, int k) {
   return v;
 }
 
-void zero_counter(sdm_ctx* c, int k) { HIPCHK(hipMemsetAsync(c->dcnt + k, 0, sizeof(unsigned long long), c->st)); }
+void zero_counter(sdm_ctx* c, int k) { hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->st, c->dcnt + k, 1); }
+
+// Device counter slots: [0] contact evals, [1] distance evals, [2] collider candidates,
+// [3] solver batch counter, [4] solver evaluations, [5] other evaluations, [6] wall/sphere
+// list, [7]-[9] periodic collider, [10] solver queue, [11] touching list, [12] LM restarts.
+enum { DC_NEXT = 3, DC_MISC = 6, DC_NQ = 10, DC_NT = 11, DC_NR = 12 };
+
+// Exclusive scan of in[0..n) into out[0..n) with the total left on the device in dcnt[slot].
+void scan_to_counter(sdm_ctx* c, const uint32_t* in, uint32_t* out, int64_t n, int slot) {
+  if (n > 0) sdmrs::scan<sdmrs::OpAdd, false, false>(in, out, n, c->rsScratch, c->st);
+  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->st, in, out, n, c->dcnt + slot);
+}
 
 // Empty the found-set hash and insert the interactions of the container.
 void fill_found(sdm_ctx* c) {
@@ -622,28 +633,32 @@ unsigned resident_grid(sdm_ctx* c, K kernel, int& cache, int64_t nwork) {
 // pair (dense, one pair per lane), then the persistent solver over the touching ones, longest
 // expected solves first, the mixed form's double refinement, and the finish kernel.
 template <typename T>
-void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA, int64_t nq) {
-  ensure_scratch(c, nq + 1);
+void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA) {
+  // every size below is known only on the device; launches cover the upper bound np
+  const int64_t nmax = A.np;
+  ensure_scratch(c, nmax + 1);
   SA.evals = c->dcnt + 5;
-  hipLaunchKernelGGL(k_nm_first<T>, dim3(nblk(nq)), dim3(256), 0, c->st, A, SA, c->qflag, c->k1);
-  const int64_t n2 = exclusive_total(c, c->qflag, c->qpos, nq);
-  if (n2 == 0) return;
-  hipLaunchKernelGGL(k_list_fill, dim3(nblk(nq)), dim3(256), 0, c->st, c->qflag, c->qpos, c->k1, nq, c->k0, c->v0);
+  hipLaunchKernelGGL(k_nm_first<T>, dim3(nblk(nmax)), dim3(256), 0, c->st, A, SA, c->qflag, c->k1);
+  scan_to_counter(c, c->qflag, c->qpos, nmax, DC_NT);
+  const unsigned long long* nt = c->dcnt + DC_NT;
+  hipLaunchKernelGGL(k_list_fill, dim3(nblk(nmax)), dim3(256), 0, c->st, c->qflag, c->qpos, c->k1, nmax, c->k0, c->v0);
+  hipLaunchKernelGGL(k_pad_keys, dim3(nblk(nmax)), dim3(256), 0, c->st, c->k0, c->v0, nt, nmax);
   // the list in queue order (increasing interaction index) for the finish kernel, whose
   // record reads and writes then follow memory order; a copy sorted by expected cost for
   // the solver
   uint32_t* byPair = c->u0;
-  HIPCHK(hipMemcpyAsync(byPair, c->v0, n2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->st));
-  sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, n2, 8, c->rsScratch, c->st);
+  hipLaunchKernelGGL(k_copy_u32, dim3(nblk(nmax)), dim3(256), 0, c->st, byPair, c->v0, nt, nmax);
+  sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, nmax, 8, c->rsScratch, c->st);
   const uint32_t* list = c->v0;
   SolveArgs S2 = SA;
   S2.list = reinterpret_cast<const int32_t*>(list);
-  S2.nwork = n2;
+  S2.nwork = nmax;
+  S2.count = nt;
   S2.evals = c->dcnt + 4;
-  S2.next = c->dcnt + 3;
-  zero_counter(c, 3);
+  S2.next = c->dcnt + DC_NEXT;
+  zero_counter(c, DC_NEXT);
   const int ti = sizeof(T) == 4 ? 0 : 1;
-  const unsigned grid = resident_grid(c, k_solve_nm<T>, c->residentNM[ti], n2);
+  const unsigned grid = resident_grid(c, k_solve_nm<T>, c->residentNM[ti], nmax);
   if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
   hipLaunchKernelGGL(k_solve_nm<T>, dim3(grid), dim3(256), 0, c->st, S2);
   if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
@@ -652,71 +667,63 @@ void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA, int64_t nq) {
     SolveArgs RF = S2;
     RF.list = reinterpret_cast<const int32_t*>(byPair);
     RF.evals = c->dcnt + 5;
-    hipLaunchKernelGGL(k_nm_refine, dim3(nblk(n2)), dim3(256), 0, c->st, RF);
+    hipLaunchKernelGGL(k_nm_refine, dim3(nblk(nmax)), dim3(256), 0, c->st, RF);
   }
-  hipLaunchKernelGGL((k_pairs_finish<1>), dim3(nblk(n2)), dim3(256), 0, c->st, A, c->S, byPair, n2);
+  hipLaunchKernelGGL((k_pairs_finish<1>), dim3(nblk(nmax)), dim3(256), 0, c->st, A, c->S, byPair, nmax, nt);
 }
 
 // InteractionLoop::action: prep (every interaction), wall / sphere pairs, solvers (queued
-// pairs), finish.
+// pairs), finish.  No host synchronisation: the queue sizes stay on the device.
 template <int LAW>
 void interaction_loop(sdm_ctx* c, const StepArgs& A) {
+  zero_counter(c, DC_NQ);
   if (A.np <= 0) return;
   ensure_queue(c, A.np);
   ensure_scratch(c, A.np + 1);
-  HIPCHK(hipMemsetAsync(c->dcnt + 6, 0, sizeof(unsigned long long), c->st));
+  zero_counter(c, DC_MISC);
   hipLaunchKernelGGL((k_pairs_prep<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->qflag, c->miscList,
-                     c->dcnt + 6);
+                     c->dcnt + DC_MISC);
   // the solver queue: scan of the flags (no shared counter), then the slots in interaction order
-  const int64_t nq = exclusive_total(c, c->qflag, c->qpos, A.np);
-  const int64_t nm = (int64_t)read_counter(c, 6);
-  if (nq > 0)
-    hipLaunchKernelGGL((k_queue_fill<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, c->qflag, c->qpos, A.np);
-  if (nm > 0)
-    hipLaunchKernelGGL((k_pairs_misc<LAW>), dim3(nblk(nm)), dim3(256), 0, c->st, A, c->miscList, nm);
-  if (c->timing) c->tms[7] += (double)nq;
-  if (nq == 0) return;
+  scan_to_counter(c, c->qflag, c->qpos, A.np, DC_NQ);
+  const unsigned long long* nq = c->dcnt + DC_NQ;
+  hipLaunchKernelGGL((k_queue_fill<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, c->qflag, c->qpos, A.np);
+  hipLaunchKernelGGL((k_pairs_misc<LAW>), dim3(std::max(1, c->numCU) * 2), dim3(256), 0, c->st, A, c->miscList,
+                     c->dcnt + DC_MISC);
   SolveArgs SA;
   SA.geo = c->B.geo;
   SA.N = c->B.n;
   SA.S = c->S;
   SA.list = nullptr;
-  SA.nwork = nq;
-  SA.next = c->dcnt + 3;
+  SA.nwork = A.np;
+  SA.count = nq;
+  SA.next = c->dcnt + DC_NEXT;
   SA.evals = c->dcnt + 4;
   if (LAW == 0) {
-    zero_counter(c, 3);
-    const unsigned grid = resident_grid(c, k_solve_lm, c->residentLM, nq);
+    zero_counter(c, DC_NEXT);
+    const unsigned grid = resident_grid(c, k_solve_lm, c->residentLM, A.np);
     if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
     hipLaunchKernelGGL(k_solve_lm, dim3(grid), dim3(256), 0, c->st, SA);
     if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
     c->solverRan = true;
     // the two restarts of Ig2_..._SuperquadricsGeom::go, each a further pass over its list
     for (int kind = 1; kind <= 2; kind++) {
-      SolveArgs CA = SA;
-      CA.list = nullptr;
-      CA.nwork = nq;
-      zero_counter(c, 3);
+      zero_counter(c, DC_NR);
       if (kind == 1)
-        hipLaunchKernelGGL(k_lm_check<1>, dim3(nblk(nq)), dim3(256), 0, c->st, A, CA, c->retryList, c->dcnt + 3);
+        hipLaunchKernelGGL(k_lm_check<1>, dim3(nblk(A.np)), dim3(256), 0, c->st, A, SA, c->retryList, c->dcnt + DC_NR);
       else
-        hipLaunchKernelGGL(k_lm_check<2>, dim3(nblk(nq)), dim3(256), 0, c->st, A, CA, c->retryList, c->dcnt + 3);
-      const int64_t nr = (int64_t)read_counter(c, 3);
-      if (nr > 0) {
-        SolveArgs RA = SA;
-        RA.list = c->retryList;
-        RA.nwork = nr;
-        RA.evals = c->dcnt + 5;
-        zero_counter(c, 3);
-        hipLaunchKernelGGL(k_solve_lm, dim3(resident_grid(c, k_solve_lm, c->residentLM, nr)), dim3(256), 0, c->st,
-                           RA);
-      }
+        hipLaunchKernelGGL(k_lm_check<2>, dim3(nblk(A.np)), dim3(256), 0, c->st, A, SA, c->retryList, c->dcnt + DC_NR);
+      SolveArgs RA = SA;
+      RA.list = c->retryList;
+      RA.count = c->dcnt + DC_NR;
+      RA.evals = c->dcnt + 5;
+      zero_counter(c, DC_NEXT);
+      hipLaunchKernelGGL(k_solve_lm, dim3(resident_grid(c, k_solve_lm, c->residentLM, A.np)), dim3(256), 0, c->st, RA);
     }
-    hipLaunchKernelGGL((k_pairs_finish<0>), dim3(nblk(nq)), dim3(256), 0, c->st, A, c->S, nullptr, nq);
+    hipLaunchKernelGGL((k_pairs_finish<0>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, nullptr, A.np, nq);
   } else if (c->prm.precision == SDM_MIXED) {
-    nm_pass<float>(c, A, SA, nq);
+    nm_pass<float>(c, A, SA);
   } else {
-    nm_pass<double>(c, A, SA, nq);
+    nm_pass<double>(c, A, SA);
   }
 }
 
@@ -774,14 +781,17 @@ void one_step(sdm_ctx* c) {
   NA.g[2] = c->prm.gravity[2];
   NA.exactAsph = c->prm.exact_aspherical_rot;
   NA.maxDisp = c->dMaxDisp;
-  HIPCHK(hipMemsetAsync(c->dMaxDisp, 0, sizeof(int), c->st));
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, (uint32_t*)c->dMaxDisp, 1);
   hipLaunchKernelGGL(k_force_sum, dim3((unsigned)((c->B.n + 15) / 16)), dim3(256), 0, c->st, c->B, c->P, c->csrOff,
                      c->csrPair);
   hipLaunchKernelGGL(k_newton, dim3(nblk(c->B.n)), dim3(256), 0, c->st, NA);
   if (c->timing) HIPCHK(hipEventRecord(c->ev[3], c->st));
   int code = 0;
+  unsigned long long nqueued = 0;
   HIPCHK(hipMemcpyAsync(&code, c->dMaxDisp, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  if (c->timing) HIPCHK(hipMemcpyAsync(&nqueued, c->dcnt + DC_NQ, sizeof(nqueued), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  if (c->timing) c->tms[7] += (double)nqueued;
   if (c->timing) {
     float a = 0, b = 0, d = 0;
     HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));

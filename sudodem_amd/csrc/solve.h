@@ -46,10 +46,19 @@ struct SolveArgs {
   int64_t N;
   SolveIO S;
   const int32_t* list;  // work slots to process (nullptr: slots 0..nwork-1)
-  int64_t nwork;
+  int64_t nwork;        // number of work items, or their upper bound when `count` is set
+  const unsigned long long* count;  // device-side number of work items (nullptr: nwork)
   unsigned long long* next;   // work counter (64-entry batches), zero at launch
   unsigned long long* evals;  // distance evaluations done by this launch (added)
 };
+
+// Work items of a launch: the device-side count (written by an earlier kernel of the same
+// step, so the host never waits for it) capped by the launch's upper bound.
+__device__ __forceinline__ int64_t work_n(const SolveArgs& A) {
+  if (!A.count) return A.nwork;
+  const int64_t c = (int64_t)*A.count;
+  return c < A.nwork ? c : A.nwork;
+}
 
 // Add per-thread values a, b into c[0], c[1] with one atomic per block: a device-scope
 // atomic on one address from every wavefront serialises across the chip.  Every thread of
@@ -220,9 +229,10 @@ __global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
   bool done = false;
   unsigned long long tot = 0;
   WaveBatch batch;
+  const int64_t nwork = work_n(A);
   for (;;) {
     const bool idle = (w < 0) && !done;
-    const int64_t got = take_work(idle, batch, A.next, A.nwork);
+    const int64_t got = take_work(idle, batch, A.next, nwork);
     if (idle) {
       if (got < 0) {
         done = true;
@@ -416,7 +426,7 @@ __device__ __forceinline__ bool nm_first_eval(const SolveArgs& A, int64_t w) {
 // reference takes them (Superquadrics_Ig2.cpp:2439-2454).  One work item per touching pair.
 __global__ __launch_bounds__(256) void k_nm_refine(SolveArgs A) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= A.nwork) return;
+  if (k >= work_n(A)) return;
   const int64_t w = A.list ? A.list[k] : k;
   if (!A.S.touch[w]) return;
   NMGeo<double> n;
@@ -458,9 +468,10 @@ __global__ __launch_bounds__(256) void k_solve_lm(SolveArgs A) {
   bool done = false;
   unsigned long long tot = 0;
   WaveBatch batch;
+  const int64_t nwork = work_n(A);
   for (;;) {
     const bool idle = (w < 0) && !done;
-    const int64_t got = take_work(idle, batch, A.next, A.nwork);
+    const int64_t got = take_work(idle, batch, A.next, nwork);
     if (idle) {
       if (got < 0) {
         done = true;
