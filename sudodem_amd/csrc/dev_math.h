@@ -125,6 +125,14 @@ SDM_HD double xatan2(double a, double b) { return atan2(a, b); }
 SDM_HD float xatan2(float a, float b) { return atan2f(a, b); }
 SDM_HD double xfabs(double a) { return fabs(a); }
 SDM_HD float xfabs(float a) { return fabsf(a); }
+SDM_HD double xsqrt(double a) { return sqrt(a); }
+SDM_HD float xsqrt(float a) { return sqrtf(a); }
+SDM_HD double xlog(double a) { return log(a); }
+SDM_HD float xlog(float a) { return logf(a); }
+SDM_HD double xlog10(double a) { return log10(a); }
+SDM_HD float xlog10(float a) { return log10f(a); }
+SDM_HD double xacos(double a) { return acos(a); }
+SDM_HD float xacos(float a) { return acosf(a); }
 
 template <typename T, typename S>
 SDM_HD V3<T> cast3(V3<S> a) { return mk3(T(a.x), T(a.y), T(a.z)); }

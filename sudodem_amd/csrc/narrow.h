@@ -181,53 +181,55 @@ SDM_HD T nm_eval(const NMGeo<T>& n, T a0, T a1, V3<T>& p1, V3<T>& p2, bool& touc
 }
 
 // Curvatures (Superquadrics.cpp:131-255).  Keeps the reference's expressions: K ignores the
-// M term, and R11/R22 use the coefficient choices of :172-178.
-SDM_HD void sq_curvatures(const Shape<double>& s, const M3<double>& Rg, double ph0, double ph1, double& k1o,
-                          double& k2o, V3<double>& dir_max, V3<double>& dir_min, bool& max_flag) {
+// M term, and R11/R22 use the coefficient choices of :172-178.  T = double is the reference
+// arithmetic; the mixed form evaluates it in float (the force law's inputs).
+template <typename T>
+SDM_HD void sq_curvatures(const Shape<T>& s, const M3<T>& Rg, T ph0, T ph1, T& k1o, T& k2o, V3<T>& dir_max,
+                          V3<T>& dir_min, bool& max_flag) {
   max_flag = true;
-  if (fabs(ph0) < 1E-20) ph0 = 1E-20;
-  if (fabs(ph1) < 1E-20) ph1 = 1E-20;
-  const double rx = s.rx, ry = s.ry, rz = s.rz, eps1 = s.eps1, eps2 = s.eps2;
-  double s1 = sin(ph0), s2 = sin(ph1), c1 = cos(ph0), c2 = cos(ph1);
-  double s_s1 = sgn(s1), s_s2 = sgn(s2), s_c1 = sgn(c1), s_c2 = sgn(c2);
-  double abs_s1 = fabs(s1), abs_s2 = fabs(s2), abs_c1 = fabs(c1), abs_c2 = fabs(c2);
-  double s10 = s_s1 * pow(abs_s1, eps1);
-  double s20 = s_s2 * pow(abs_s2, eps2);
-  double c10 = s_c1 * pow(abs_c1, eps1);
-  double c20 = s_c2 * pow(abs_c2, eps2);
-  double s12 = s_s1 * pow(abs_s1, eps1 - 2.0);
-  double s22 = s_s2 * pow(abs_s2, eps2 - 2.0);
-  double c12 = s_c1 * pow(abs_c1, eps1 - 2.0);
-  double c22 = s_c2 * pow(abs_c2, eps2 - 2.0);
-  V3<double> R1 = mk3(-rx * eps1 * c12 * c1 * c20 * s1, ry * eps1 * c1 * c20 * s12 * s1, 0.0);
-  V3<double> R2 = mk3(-rx * eps2 * c10 * c22 * c2 * s2, -ry * eps2 * c22 * c2 * s10 * s2, rz * eps2 * c2 * s22 * s2);
-  V3<double> R11 = mk3(rx * eps1 * c12 * s20 * s1 * s1 * (eps1 - 1.0) - rx * eps1 * c10 * c20,
-                       ry * eps1 * c1 * c1 * c20 * s12 * (eps1 - 1.0) - ry * eps1 * c20 * s10, 0.0);
-  V3<double> R22 = mk3(rx * eps2 * c10 * c22 * s2 * s2 * (eps2 - 1.0) - rx * eps2 * c10 * c20,
-                       ry * eps2 * s10 * s1 * s1 * (eps2 - 1.0) - ry * eps2 * c20 * s10,
-                       rz * eps2 * c2 * c2 * s22 * (eps2 - 1.0) - rz * eps2 * s20);
-  V3<double> RR = cross(R1, R2);
-  double nr = norm(RR);
-  V3<double> n = mk3(RR.x / nr, RR.y / nr, RR.z / nr);
-  double L = dot(R11, n);
-  double N = dot(R22, n);
-  double E = dot(R1, R1);
-  double F = dot(R1, R2);
-  double G = dot(R2, R2);
-  double K = L * N / (E * G - F * F);
-  double H = 0.5 * (E * N + G * L) / (E * G - F * F);
-  double tmp0 = sqrt(fabs(H * H - K));
-  double k1 = H + tmp0;
-  double k2 = H - tmp0;
-  const double a1 = N - k1 * G, a2 = N - k2 * G;
-  V3<double> dir1 = mk3(R1.x * a1 + R2.x * k1 * F, R1.y * a1 + R2.y * k1 * F, R1.z * a1 + R2.z * k1 * F);
-  if (norm(dir1) == 0) {
+  if (xfabs(ph0) < T(1E-20)) ph0 = T(1E-20);
+  if (xfabs(ph1) < T(1E-20)) ph1 = T(1E-20);
+  const T rx = s.rx, ry = s.ry, rz = s.rz, eps1 = s.eps1, eps2 = s.eps2;
+  T s1 = xsin(ph0), s2 = xsin(ph1), c1 = xcos(ph0), c2 = xcos(ph1);
+  T s_s1 = sgn(s1), s_s2 = sgn(s2), s_c1 = sgn(c1), s_c2 = sgn(c2);
+  T abs_s1 = xfabs(s1), abs_s2 = xfabs(s2), abs_c1 = xfabs(c1), abs_c2 = xfabs(c2);
+  T s10 = s_s1 * xpow(abs_s1, eps1);
+  T s20 = s_s2 * xpow(abs_s2, eps2);
+  T c10 = s_c1 * xpow(abs_c1, eps1);
+  T c20 = s_c2 * xpow(abs_c2, eps2);
+  T s12 = s_s1 * xpow(abs_s1, eps1 - T(2.0));
+  T s22 = s_s2 * xpow(abs_s2, eps2 - T(2.0));
+  T c12 = s_c1 * xpow(abs_c1, eps1 - T(2.0));
+  T c22 = s_c2 * xpow(abs_c2, eps2 - T(2.0));
+  V3<T> R1 = mk3(-rx * eps1 * c12 * c1 * c20 * s1, ry * eps1 * c1 * c20 * s12 * s1, T(0.0));
+  V3<T> R2 = mk3(-rx * eps2 * c10 * c22 * c2 * s2, -ry * eps2 * c22 * c2 * s10 * s2, rz * eps2 * c2 * s22 * s2);
+  V3<T> R11 = mk3(rx * eps1 * c12 * s20 * s1 * s1 * (eps1 - T(1.0)) - rx * eps1 * c10 * c20,
+                  ry * eps1 * c1 * c1 * c20 * s12 * (eps1 - T(1.0)) - ry * eps1 * c20 * s10, T(0.0));
+  V3<T> R22 = mk3(rx * eps2 * c10 * c22 * s2 * s2 * (eps2 - T(1.0)) - rx * eps2 * c10 * c20,
+                  ry * eps2 * s10 * s1 * s1 * (eps2 - T(1.0)) - ry * eps2 * c20 * s10,
+                  rz * eps2 * c2 * c2 * s22 * (eps2 - T(1.0)) - rz * eps2 * s20);
+  V3<T> RR = cross(R1, R2);
+  T nr = norm(RR);
+  V3<T> n = mk3(RR.x / nr, RR.y / nr, RR.z / nr);
+  T L = dot(R11, n);
+  T N = dot(R22, n);
+  T E = dot(R1, R1);
+  T F = dot(R1, R2);
+  T G = dot(R2, R2);
+  T K = L * N / (E * G - F * F);
+  T H = T(0.5) * (E * N + G * L) / (E * G - F * F);
+  T tmp0 = xsqrt(xfabs(H * H - K));
+  T k1 = H + tmp0;
+  T k2 = H - tmp0;
+  const T a1 = N - k1 * G, a2 = N - k2 * G;
+  V3<T> dir1 = mk3(R1.x * a1 + R2.x * k1 * F, R1.y * a1 + R2.y * k1 * F, R1.z * a1 + R2.z * k1 * F);
+  if (norm(dir1) == T(0)) {
     max_flag = false;
   } else {
     dir1 = normalized(dir1);
     dir_max = mul(Rg, dir1);
   }
-  V3<double> dir2 = mk3(R1.x * a2 + R2.x * k2 * F, R1.y * a2 + R2.y * k2 * F, R1.z * a2 + R2.z * k2 * F);
+  V3<T> dir2 = mk3(R1.x * a2 + R2.x * k2 * F, R1.y * a2 + R2.y * k2 * F, R1.z * a2 + R2.z * k2 * F);
   dir2 = normalized(dir2);
   dir_min = mul(Rg, dir2);
   k1o = k1;
@@ -235,23 +237,26 @@ SDM_HD void sq_curvatures(const Shape<double>& s, const M3<double>& Rg, double p
 }
 
 // SuperquadricsGeom2::HertzMindlin (Superquadrics.cpp:974-1026).
-SDM_HD void hertz_mindlin(double c10, double c11, double c20, double c21, double theta, double& curvSum,
-                          double& alpha, double& Ko, double& Eo) {
-  double rho11 = fabs(c10), rho12 = fabs(c11), rho21 = fabs(c20), rho22 = fabs(c21);
-  if (rho11 * rho12 < 1E-18) rho11 = rho12 = 0.0;
-  if (rho21 * rho22 < 1E-18) rho21 = rho22 = 0.0;
-  double sum_rho = rho11 + rho12 + rho21 + rho22;
-  double a1 = rho11 - rho12, a2 = rho21 - rho22;
-  double F_rho = -sqrt(a1 * a1 + a2 * a2 + 2 * (fabs(rho11 - rho12)) * fabs(rho21 - rho22) * cos(2 * theta));
-  double A = (sum_rho + F_rho) / 4.;
-  double B = (sum_rho - F_rho) / 4.;
+template <typename T>
+SDM_HD void hertz_mindlin(T c10, T c11, T c20, T c21, T theta, T& curvSum, T& alpha, T& Ko, T& Eo) {
+  T rho11 = xfabs(c10), rho12 = xfabs(c11), rho21 = xfabs(c20), rho22 = xfabs(c21);
+  if (rho11 * rho12 < T(1E-18)) rho11 = rho12 = T(0.0);
+  if (rho21 * rho22 < T(1E-18)) rho21 = rho22 = T(0.0);
+  T sum_rho = rho11 + rho12 + rho21 + rho22;
+  T a1 = rho11 - rho12, a2 = rho21 - rho22;
+  T F_rho = -xsqrt(a1 * a1 + a2 * a2 + T(2) * (xfabs(rho11 - rho12)) * xfabs(rho21 - rho22) * xcos(T(2) * theta));
+  T A = (sum_rho + F_rho) / T(4.);
+  T B = (sum_rho - F_rho) / T(4.);
   curvSum = A + B;
-  double lg = log10(B / A);
-  double x_squ = lg * lg;
-  double gamma1 = 2 * (1 + x_squ * (0.40227436 + x_squ * (3.7491752e-2 + x_squ * (7.4855761e-4 + x_squ * 2.1667028e-6))));
-  double gamma2 = 3 * (1 + x_squ * (0.42678878 + x_squ * (4.2605401e-2 + x_squ * (9.0786922e-4 + x_squ * 2.7868927e-6))));
-  alpha = pow((B / A), gamma1 / gamma2);
-  double m1 = 1 / (alpha * alpha);
-  Ko = (1.3862944 + m1 * (0.1119723 + 0.0725296 * m1)) - log(m1) * (0.5 + m1 * (0.1213478 + 0.0288729 * m1));
-  Eo = (1 + m1 * (0.4630151 + 0.1077812 * m1)) - log(m1) * (0.2452727 + 0.0412496 * m1) * m1;
+  T lg = xlog10(B / A);
+  T x_squ = lg * lg;
+  T gamma1 = T(2) * (T(1) + x_squ * (T(0.40227436) + x_squ * (T(3.7491752e-2) + x_squ * (T(7.4855761e-4) +
+                                                                                       x_squ * T(2.1667028e-6)))));
+  T gamma2 = T(3) * (T(1) + x_squ * (T(0.42678878) + x_squ * (T(4.2605401e-2) + x_squ * (T(9.0786922e-4) +
+                                                                                       x_squ * T(2.7868927e-6)))));
+  alpha = xpow((B / A), gamma1 / gamma2);
+  T m1 = T(1) / (alpha * alpha);
+  Ko = (T(1.3862944) + m1 * (T(0.1119723) + T(0.0725296) * m1)) -
+       xlog(m1) * (T(0.5) + m1 * (T(0.1213478) + T(0.0288729) * m1));
+  Eo = (T(1) + m1 * (T(0.4630151) + T(0.1077812) * m1)) - xlog(m1) * (T(0.2452727) + T(0.0412496) * m1) * m1;
 }

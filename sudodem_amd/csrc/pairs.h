@@ -778,8 +778,39 @@ __global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int3
   if (restart) list[slot] = (int32_t)w;
 }
 
+// Curvature directions and the Hertz terms of a Geom2 contact (Superquadrics_Ig2.cpp:2459-2503)
+// in precision T; the mixed form takes them in float (they feed only the force law).
+template <typename T>
+__device__ __forceinline__ void contact_curvature(const BodyView& b1, const BodyView& b2, double a0, double a1,
+                                                  V3<double> cnd, double out[9]) {
+  const M3<T> gC = cast9<T>(crotation_mat(b1.pos, b2.pos));
+  const M3<T> G1 = cast9<T>(b1.G), G2 = cast9<T>(b2.G);
+  const Shape<T> s1 = cast_shape<T>(b1.s), s2 = cast_shape<T>(b2.s);
+  const V3<T> cn = cast3<T>(cnd);
+  const T ta0 = T(a0), ta1 = T(a1);
+  const V3<T> contact = mul(gC, mk3(xcos(ta0) * xsin(ta1), xsin(ta0) * xsin(ta1), xcos(ta1)));
+  const V3<T> c1 = mulT(G1, contact), c2 = mulT(G2, contact);
+  V3<T> dmax1 = mk3(T(0.0), T(0.0), T(0.0)), dmax2 = dmax1, dmin1, dmin2;
+  bool f1, f2;
+  T ph0, ph1, k10, k11, k20, k21;
+  normal2phi(s1, c1, ph0, ph1);
+  sq_curvatures(s1, G1, ph0, ph1, k10, k11, dmax1, dmin1, f1);
+  normal2phi(s2, mk3(T(-1.0) * c2.x, T(-1.0) * c2.y, T(-1.0) * c2.z), ph0, ph1);
+  sq_curvatures(s2, G2, ph0, ph1, k20, k21, dmax2, dmin2, f2);
+  const V3<T> cd1 = (f1 && f2) ? dmax1 : dmin1;
+  const V3<T> cd2 = (f1 && f2) ? dmax2 : dmin2;
+  const V3<T> dir1 = mk3(cd1.x * cn.x, cd1.y * cn.y, cd1.z * cn.z) - cd1;
+  const V3<T> dir2 = mk3(cd2.x * cn.x, cd2.y * cn.y, cd2.z * cn.z) - cd2;
+  T angle = xacos(dot(dir1, dir2) / (norm(dir1) * norm(dir2)));
+  if (isnan(angle)) angle = T(0.0);
+  T cs, al, K, E;
+  hertz_mindlin(k10, k11, k20, k21, angle, cs, al, K, E);
+  out[0] = k10; out[1] = k11; out[2] = k20; out[3] = k21; out[4] = angle;
+  out[5] = cs; out[6] = al; out[7] = K; out[8] = E;
+}
+
 // Queued pairs after the solvers: geometry outputs, curvature / Hertz terms, precompute, law.
-template <int LAW>
+template <int LAW, typename TC>
 __global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, const uint32_t* list, int64_t nmax,
                                                       const unsigned long long* count) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -825,27 +856,18 @@ __global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, con
       g.cpt = mk3(0.5 * (p1.x + p2.x), 0.5 * (p1.y + p2.y), 0.5 * (p1.z + p2.z));
       g.depth = S.f[w];
       // curvature direction convention of :2463-2465
-      const M3<double> gC = crotation_mat(b1.pos, b2.pos);
-      const M3<double>& G1 = b1.G;
-      const M3<double>& G2 = b2.G;
-      const V3<double> contact = mul(gC, mk3(cos(g.a0) * sin(g.a1), sin(g.a0) * sin(g.a1), cos(g.a1)));
-      const V3<double> c1 = mulT(G1, contact), c2 = mulT(G2, contact);
-      V3<double> dmax1 = mk3(0.0, 0.0, 0.0), dmax2 = dmax1, dmin1, dmin2;
-      bool f1, f2;
-      double ph0, ph1, k10, k11, k20, k21;
-      normal2phi(b1.s, c1, ph0, ph1);
-      sq_curvatures(b1.s, G1, ph0, ph1, k10, k11, dmax1, dmin1, f1);
-      normal2phi(b2.s, mk3(-1.0 * c2.x, -1.0 * c2.y, -1.0 * c2.z), ph0, ph1);
-      sq_curvatures(b2.s, G2, ph0, ph1, k20, k21, dmax2, dmin2, f2);
-      const V3<double> cd1 = (f1 && f2) ? dmax1 : dmin1;
-      const V3<double> cd2 = (f1 && f2) ? dmax2 : dmin2;
-      const V3<double> dir1 = mk3(cd1.x * cn.x, cd1.y * cn.y, cd1.z * cn.z) - cd1;
-      const V3<double> dir2 = mk3(cd2.x * cn.x, cd2.y * cn.y, cd2.z * cn.z) - cd2;
-      double angle = acos(dot(dir1, dir2) / (norm(dir1) * norm(dir2)));
-      if (isnan(angle)) angle = 0.0;
-      g.c10 = k10; g.c11 = k11; g.c20 = k20; g.c21 = k21;
-      g.curvAngle = angle;
-      hertz_mindlin(k10, k11, k20, k21, angle, g.curvSum, g.alpha, g.K, g.E);
+      double cv[9];
+      contact_curvature<TC>(b1, b2, g.a0, g.a1, cn, cv);
+      if (sizeof(TC) == 4) {
+        // a float overflow or cancellation that the double arithmetic survives: redo in double
+        bool ok = true;
+#pragma unroll
+        for (int q = 0; q < 9; q++) ok = ok && isfinite(cv[q]);
+        if (!ok) contact_curvature<double>(b1, b2, g.a0, g.a1, cn, cv);
+      }
+      g.c10 = cv[0]; g.c11 = cv[1]; g.c20 = cv[2]; g.c21 = cv[3];
+      g.curvAngle = cv[4];
+      g.curvSum = cv[5]; g.alpha = cv[6]; g.K = cv[7]; g.E = cv[8];
     }
     if (outcome == GEO_FULL) precompute(g, b1, b2, A.dt, cn, g.isShearNew);
     phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, nev, stopCode);

@@ -669,7 +669,11 @@ void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA) {
     RF.evals = c->dcnt + 5;
     hipLaunchKernelGGL(k_nm_refine, dim3(nblk(nmax)), dim3(256), 0, c->st, RF);
   }
-  hipLaunchKernelGGL((k_pairs_finish<1>), dim3(nblk(nmax)), dim3(256), 0, c->st, A, c->S, byPair, nmax, nt);
+  // the mixed form evaluates the curvature and Hertz terms of the force law in float
+  if (sizeof(T) == 4)
+    hipLaunchKernelGGL((k_pairs_finish<1, float>), dim3(nblk(nmax)), dim3(256), 0, c->st, A, c->S, byPair, nmax, nt);
+  else
+    hipLaunchKernelGGL((k_pairs_finish<1, double>), dim3(nblk(nmax)), dim3(256), 0, c->st, A, c->S, byPair, nmax, nt);
 }
 
 // InteractionLoop::action: prep (every interaction), wall / sphere pairs, solvers (queued
@@ -719,7 +723,7 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
       zero_counter(c, DC_NEXT);
       hipLaunchKernelGGL(k_solve_lm, dim3(resident_grid(c, k_solve_lm, c->residentLM, A.np)), dim3(256), 0, c->st, RA);
     }
-    hipLaunchKernelGGL((k_pairs_finish<0>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, nullptr, A.np, nq);
+    hipLaunchKernelGGL((k_pairs_finish<0, double>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, nullptr, A.np, nq);
   } else if (c->prm.precision == SDM_MIXED) {
     nm_pass<float>(c, A, SA);
   } else {
