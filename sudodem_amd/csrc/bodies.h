@@ -65,7 +65,11 @@ __global__ __launch_bounds__(256) void k_bounds(BodyArrays B, int64_t iter, doub
   B.sweep[i] = sweep;
   B.lastUpd[i] = (int32_t)iter;
   B.hasBound[i] = 1;
+  B.disp[i] = 0.0f;
 }
+
+// |d| rounded up to float (NaN stays NaN: no bound).
+__device__ __forceinline__ float disp_up(V3<double> d) { return __double2float_ru(norm(d)); }
 
 // Rebuild the packed records from the attribute arrays (after the host wrote attributes).
 __global__ __launch_bounds__(256) void k_geo_pack(BodyArrays B) {
@@ -86,6 +90,7 @@ __global__ __launch_bounds__(256) void k_geo_pack(BodyArrays B) {
   const uint64_t fl =
       geo_flags(B.material[i], B.blocked[i], B.kind[i], B.isSphere[i], B.wallAxis[i], B.wallSense[i], B.lid[i]);
   g[GEO_FLAGS] = __longlong_as_double((long long)fl);
+  B.disp[i] = B.hasBound[i] ? disp_up(ld3(B.pos, i, N) - ld3(B.refPos, i, N)) : __int_as_float(0x7fc00000);
 }
 
 // Per-body force and torque: the contributions of the body's interactions summed in
@@ -294,8 +299,10 @@ __global__ __launch_bounds__(256) void k_newton(NewtonArgs A) {
     gp[GEO_VEL] = vel.x; gp[GEO_VEL + 1] = vel.y; gp[GEO_VEL + 2] = vel.z;
     gp[GEO_ANGVEL] = angVel.x; gp[GEO_ANGVEL + 1] = angVel.y; gp[GEO_ANGVEL + 2] = angVel.z;
     // saveMaximaDisplacement (:72-85)
+    B.disp[i] = __int_as_float(0x7fc00000);
     if (B.hasBound[i]) {
       const V3<double> d = pos - ld3(B.refPos, i, N);
+      B.disp[i] = disp_up(d);
       const double ax = fabs(d.x), ay = fabs(d.y), az = fabs(d.z);
       const double myz = (ay < az) ? az : ay;
       const double md = (ax < myz) ? myz : ax;

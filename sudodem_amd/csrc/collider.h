@@ -327,10 +327,13 @@ __global__ void k_new_pairs(PairArrays P, int64_t start, int64_t count, const Ca
 }
 
 // Interaction endpoints for the per-body lists: entry 2p -> (id1, 2p), 2p+1 -> (id2, 2p+1).
-__global__ void k_csr_entries(const PairRec* R, int64_t np, uint32_t* key, uint32_t* val, int32_t* deg) {
+__global__ void k_csr_entries(const PairRec* R, int64_t np, uint32_t* key, uint32_t* val, int32_t* deg,
+                              int32_t* cid) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const int a = R[p].id1[0], b = R[p].id2[0];
+  cid[2 * p] = a;
+  cid[2 * p + 1] = b;
   key[2 * p] = (uint32_t)a;
   val[2 * p] = (uint32_t)(2 * p);
   key[2 * p + 1] = (uint32_t)b;

@@ -34,6 +34,7 @@ struct StepArgs {
   int32_t* csr;                  // per-body interaction lists (entries carry the active bit)
   int periodic;                  // Scene::isPeriodic
   double cell[3];                // cell size (axis-aligned cell: hSize = diag(cell))
+  int skip;                      // settled apart pairs may be skipped (P.gap, B.disp are current)
 };
 
 // shift2 = hSize * cellDist (InteractionLoop.cpp:100-105): zero outside a periodic cell.
@@ -197,7 +198,7 @@ __device__ __forceinline__ V3<double> rotate_shear(const GeomState& g, V3<double
 
 // Ip2 (first time only), the law, and the write-back of one interaction.
 template <int LAW>
-__device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1, const BodyView& b2, int id1,
+__device__ bool phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1, const BodyView& b2, int id1,
                                GeomState& g, int outcome, bool hadGeom, bool hadPhys, int nev, int stopCode) {
   const PairArrays& P = A.P;
   const int64_t S = P.cap;
@@ -352,7 +353,7 @@ __device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
     P.r[p].kn[0] = 0; P.r[p].ks[0] = 0; P.r[p].tanPhi[0] = __builtin_nan(""); P.r[p].betan[0] = 0; P.r[p].betas[0] = 0;
     stv3(P.r[p].fn, z); stv3(P.r[p].fs, z);
     P.r[p].nuab[0] = 0; P.r[p].Gab[0] = 0; P.r[p].mu1[0] = 0; P.r[p].mu2[0] = 0;
-    return;
+    return false;
   }
   if (!(hadGeom && hadPhys)) P.r[p].iterMadeReal[0] = A.iter;
   put(R.hasGeom[0], 1, lazy);
@@ -397,6 +398,7 @@ __device__ void phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
     stv3(P.r[p].cT1, -cross(b1.pos - g.cpt, F));
     stv3(P.r[p].cT2, cross(b2.pos - g.cpt, F));
   }
+  return true;
 }
 
 // Add the block's counters into the global ones (one atomic per block).
@@ -408,7 +410,7 @@ __device__ __forceinline__ void add_counters(unsigned long long* c, unsigned lon
 // shear state; EARLY leaves the stored angles, EARLY_ANG stores (a0, a1).  Only the bounds
 // (the law's erase test) and, for a first contact, the materials are read.
 template <int LAW>
-__device__ __forceinline__ void early_store(const StepArgs& A, int64_t p, int id1, int id2, bool hadGeom,
+__device__ __forceinline__ bool early_store(const StepArgs& A, int64_t p, int id1, int id2, bool hadGeom,
                                             bool hadPhys, double a0, double a1, int outcome, int nev, int stopCode) {
   const BodyArrays& B = A.B;
   BodyView b1, b2;
@@ -431,7 +433,7 @@ __device__ __forceinline__ void early_store(const StepArgs& A, int64_t p, int id
   g.a1 = a1;
   g.gSphere = false;
   g.shift2 = pair_shift(A, p);
-  phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, nev, stopCode);
+  return phys_law_store<LAW>(A, p, b1, b2, id1, g, outcome, hadGeom, hadPhys, nev, stopCode);
 }
 
 // Every interaction: functor dispatch; superquadric pairs that the bounding spheres or the
@@ -446,7 +448,25 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
   int id1 = 0, id2 = 0;
   const PairArrays& P = A.P;
   const BodyArrays& B = A.B;
-  if (p < A.np) {
+  // A settled apart pair: its last evaluation ended at the bounding-sphere test with depth 0
+  // and kept boxes, so its record holds exactly what that outcome stores, and P.gap[p] is
+  // its separation |x1 - x2| - (r_max1 + r_max2) minus both bodies' displacements since the
+  // last collider run at that step.  While gap exceeds the displacements now, the spheres
+  // are still apart (triangle inequality), the boxes are unchanged (they move only at
+  // collider runs), and the functors would store nothing: the interaction is skipped
+  // without reading its record or its bodies.
+  bool skipped = false;
+  if (A.skip && p < A.np) {
+    const float gp = P.gap[p];
+    if (gp > 0.0f) {
+      const float da = B.disp[P.cid[2 * p]], db = B.disp[P.cid[2 * p + 1]];
+      skipped = (double)gp - ((double)da + (double)db) > 1e-13;
+    }
+  }
+  float gapv = __int_as_float(0x7fc00000);
+  if (skipped) {
+    ce++;  // the reference's functors ran on it
+  } else if (p < A.np) {
     if (P.r[p].active[0]) {  // ForceResetter: last step's contribution is withdrawn
       P.r[p].active[0] = 0;
       A.csr[P.r[p].csrPos[0]] = csr_entry(p, 0, false);
@@ -468,6 +488,8 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
             int t = id1; id1 = id2; id2 = t;
             P.r[p].id1[0] = id1;
             P.r[p].id2[0] = id2;
+            P.cid[2 * p] = id1;
+            P.cid[2 * p + 1] = id2;
           }
         }
       }
@@ -503,7 +525,8 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
             para1 = P.r[p].ang[1];
           }
           int outcome = -1, stopCode = 0;
-          if (norm(x1 - x2) > rmax1 + rmax2) {
+          const double dist = norm(x1 - x2);
+          if (dist > rmax1 + rmax2) {
             outcome = GEO_EARLY;  // Superquadrics_Ig2.cpp:1975-1979, 2384-2388
           } else {
             // The first distance evaluation of either solver is at `para` along the contact
@@ -533,7 +556,9 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
             }
           }
           if (outcome >= 0) {
-            early_store<LAW>(A, p, id1, id2, hadGeom, hadPhys, para0, para1, outcome, 0, stopCode);
+            const bool keep = early_store<LAW>(A, p, id1, id2, hadGeom, hadPhys, para0, para1, outcome, 0, stopCode);
+            if (outcome == GEO_EARLY && keep)
+              gapv = __double2float_rd((dist - (rmax1 + rmax2)) - ((double)B.disp[id1] + (double)B.disp[id2]));
           } else {
             enqueue = true;
           }
@@ -541,7 +566,10 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
       }
     }
   }
-  if (p < A.np) qflag[p] = enqueue ? 1u : 0u;
+  if (p < A.np) {
+    qflag[p] = enqueue ? 1u : 0u;
+    if (!skipped) P.gap[p] = gapv;
+  }
   add_counters(A.counters, ce, 0);
   const int64_t ms = fetch_work(misc, mcount);  // wall / sphere pairs are few: rare atomics
   if (misc) mlist[ms] = (int32_t)p;

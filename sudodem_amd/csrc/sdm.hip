@@ -113,6 +113,7 @@ struct sdm_ctx {
   bool geoDirty = true;
   int32_t nSpec = 1;  // specimens (body.specimen), contiguous ranges of body ids
   bool pairsWritten = true;  // host wrote interaction data since the last step
+  bool skipValid = false;    // P.gap holds this container's settled gaps (no collider run or host write since)
   uint32_t* stage = nullptr;  // attribute staging buffer (32-bit words)
   int64_t stageCap = 0;
   // periodic cell broadphase (periodic.h)
@@ -150,9 +151,14 @@ void alloc_bodies(sdm_ctx* c, int32_t n) {
   c->csrOff = dalloc<int32_t>((int64_t)n + 1);
   c->B.geo = dalloc<double>((int64_t)n * GEO_STRIDE);
   c->B.lid = dalloc<int32_t>(std::max<int64_t>(1, n));
+  c->B.disp = dalloc<float>(std::max<int64_t>(1, n));
 }
 
-void free_pairs(PairArrays& P) { dfree(P.r); }
+void free_pairs(PairArrays& P) {
+  dfree(P.r);
+  dfree(P.cid);
+  dfree(P.gap);
+}
 
 // body.specimen: ids 0,0,..,1,1,.. (non-decreasing); each body's index within its specimen.
 void set_specimens(sdm_ctx* c, const int32_t* spec) {
@@ -192,6 +198,9 @@ void ensure_pair_cap(sdm_ctx* c, int64_t need) {
   PairArrays Q;
   Q.cap = cap;
   Q.r = dalloc<PairRec>(cap);
+  Q.cid = dalloc<int32_t>(2 * cap);
+  Q.gap = dalloc<float>(cap);
+  c->skipValid = false;
   if (c->P.r && c->np > 0)
     HIPCHK(hipMemcpyAsync(Q.r, c->P.r, c->np * sizeof(PairRec), hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -318,7 +327,7 @@ void build_csr(sdm_ctx* c) {
   if (c->np > 0) {
     uint32_t* key = c->u1;
     hipLaunchKernelGGL(k_csr_entries, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.r, c->np, key, c->v0,
-                       deg);
+                       deg, c->P.cid);
     int bits = 1;
     while ((1ll << bits) < N) bits++;
     // keys are 32-bit: reuse the 64-bit sort buffers through a 32-bit view
@@ -407,6 +416,7 @@ void erase_flagged(sdm_ctx* c) {
 // periodic.h for every pair that overlaps now and is not in the container.
 void collider_run_peri(sdm_ctx* c) {
   c->cnt.collider_runs++;
+  c->skipValid = false;
   const int64_t N = c->B.n;
   const int64_t n2 = 2 * N;
   const bool doInitSort = !c->bbValid;
@@ -536,6 +546,7 @@ void collider_run(sdm_ctx* c) {
     return;
   }
   c->cnt.collider_runs++;
+  c->skipValid = false;
   const int64_t N = c->B.n;
   const int64_t n2 = 2 * N;
   bool doInitSort = false;
@@ -764,6 +775,7 @@ void one_step(sdm_ctx* c) {
   A.csr = c->csrPair;
   A.fullWrite = c->pairsWritten ? 1 : 0;
   A.periodic = c->prm.periodic ? 1 : 0;
+  A.skip = (c->skipValid && !c->pairsWritten) ? 1 : 0;
   A.cell[0] = c->prm.cell[0];
   A.cell[1] = c->prm.cell[1];
   A.cell[2] = c->prm.cell[2];
@@ -812,6 +824,7 @@ void one_step(sdm_ctx* c) {
     }
   }
   c->maxVelocitySq = code == 2 ? 2.0 : (code == 1 ? 0.5 : 0.0);
+  c->skipValid = true;  // this step wrote every interaction's gap and every body's displacement
   c->iter++;
   c->time += c->prm.dt;
   c->cnt.steps++;
@@ -973,6 +986,7 @@ void sdm_destroy(sdm_ctx* c) {
 #undef SDM_FB
   dfree(c->B.geo);
   dfree(c->B.lid);
+  dfree(c->B.disp);
   free_pairs(c->P);
   free_queue(c->S);
   dfree(c->retryList);
@@ -997,6 +1011,7 @@ void sdm_destroy(sdm_ctx* c) {
 const char* sdm_last_error(const sdm_ctx* c) { return c ? c->err.c_str() : g_err.c_str(); }
 
 int sdm_set_materials(sdm_ctx* c, const sdm_material* mats, int32_t n) {
+  if (c) c->skipValid = false;
   if (!c || n < 0 || (n > 0 && !mats)) return fail(c, "sdm_set_materials: invalid arguments");
   try {
     c->matChecked = false;
@@ -1123,6 +1138,7 @@ int sdm_get(sdm_ctx* c, const char* name, void* dst, int64_t count) {
 int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
   const FieldInfo* f = find_field(name);
   if (!c || !f || (!src && count > 0)) return fail(c, std::string("sdm_set: bad attribute ") + (name ? name : "(null)"));
+  c->skipValid = false;
   try {
     HIPCHK(hipStreamSynchronize(c->st));
     if (f->scope == S_PAIR && std::strcmp(name, "intr.id1") == 0) {

@@ -121,6 +121,8 @@ struct BodyArrays {
   double* geo;
   // index of the body within its specimen (body.specimen; = the body id with one specimen)
   int32_t* lid;
+  // |pos - refPos| rounded up: how far the body moved since the last collider run
+  float* disp;
 };
 
 enum {
@@ -140,6 +142,10 @@ __host__ __device__ inline uint64_t geo_flags(int material, int blocked, int kin
 struct PairArrays {
   PairRec* r;   // [cap] records; the first np are the interaction container in linIntrs order
   int64_t cap;
+  // Per interaction, coalesced: the two ids (rebuilt with the per-body lists), and the
+  // separation margin of a settled apart pair (see k_pairs_prep), NaN when not settled.
+  int32_t* cid;
+  float* gap;
 };
 
 struct MaterialDev {
