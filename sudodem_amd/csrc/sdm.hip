@@ -990,6 +990,7 @@ void sdm_destroy(sdm_ctx* c) {
   free_pairs(c->P);
   free_queue(c->S);
   dfree(c->retryList);
+  dfree(c->miscList); dfree(c->qflag); dfree(c->qpos);
   for (int a = 0; a < 3; a++) {
     dfree(c->bb[a]);
     dfree(c->qs[a]);
