@@ -77,6 +77,7 @@ typedef struct sdm_counters {
   int64_t dist_evals;     /* Disfunction / Disfun evaluations inside the narrowphase solvers */
   int64_t pairs_inserted; /* interactions appended by the collider */
   int64_t pairs_erased;   /* interactions erased by the collider */
+  int64_t graph_steps;    /* steps replayed from the captured step graph (sdm_set_graph) */
 } sdm_counters;
 
 /* Version string of the library. */
@@ -129,6 +130,13 @@ int sdm_reset_counters(sdm_ctx* ctx);
  *   out[7] interactions handed to the solver kernel (summed over the timed steps) */
 int sdm_set_timing(sdm_ctx* ctx, int32_t on);
 int sdm_get_timers(sdm_ctx* ctx, double out[8]);
+
+/* Steady-state steps as a replayed graph.  No counterpart in the reference (its engines are
+ * called one by one).  When on (the default), a step without a collider run, host write or
+ * timing since the previous one replays a captured hipGraph of the interaction loop, the force
+ * sums and the integration instead of launching ~20 kernels; results are identical.  The
+ * graph is recaptured after any reallocation or parameter change. */
+int sdm_set_graph(sdm_ctx* ctx, int32_t on);
 
 /* ---------------------------------------------------------------------------------------
  * Differentiable rollout over a frozen contact set (linear law + NewtonIntegrator).

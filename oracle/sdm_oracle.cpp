@@ -2042,6 +2042,7 @@ int sdmcpu_reset_counters(sdm_ctx* c) {
 }
 
 int sdmcpu_set_timing(sdm_ctx* c, int32_t) { return c ? 0 : -1; }
+int sdmcpu_set_graph(sdm_ctx* c, int32_t) { return c ? 0 : -1; }
 int sdmcpu_get_timers(sdm_ctx* c, double out[8]) {
   if (!c || !out) return -1;
   for (int k = 0; k < 8; k++) out[k] = 0;

@@ -33,6 +33,7 @@ int sdmcpu_sync(sdm_ctx* ctx);
 int sdmcpu_get_counters(sdm_ctx* ctx, sdm_counters* out);
 int sdmcpu_reset_counters(sdm_ctx* ctx);
 int sdmcpu_set_timing(sdm_ctx* ctx, int32_t on);
+int sdmcpu_set_graph(sdm_ctx* ctx, int32_t on);
 int sdmcpu_get_timers(sdm_ctx* ctx, double out[8]);
 
 /* Number of OpenMP threads used by the engine loops (1 = the reference's default -j1,

@@ -45,7 +45,8 @@ class sdm_material(ctypes.Structure):
 
 class sdm_counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in
-                ("steps", "collider_runs", "contact_evals", "dist_evals", "pairs_inserted", "pairs_erased")]
+                ("steps", "collider_runs", "contact_evals", "dist_evals", "pairs_inserted", "pairs_erased",
+                 "graph_steps")]
 
 
 class SdmError(RuntimeError):
@@ -87,6 +88,7 @@ class SdmLibrary:
         self.get_counters = fn("get_counters", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_counters)])
         self.reset_counters = fn("reset_counters", ctypes.c_int, [c_ctx])
         self.set_timing = fn("set_timing", ctypes.c_int, [c_ctx, ctypes.c_int32])
+        self.set_graph = fn("set_graph", ctypes.c_int, [c_ctx, ctypes.c_int32])
         self.get_timers = fn("get_timers", ctypes.c_int, [c_ctx, ctypes.POINTER(ctypes.c_double)])
 
     def fields(self):
@@ -159,6 +161,10 @@ class Context:
 
     def reset_counters(self):
         self.lib.check(self.lib.reset_counters(self.ptr), self.ptr)
+
+    def set_graph(self, on=True):
+        """Replay steady-state steps as a captured graph (the default)."""
+        self.lib.check(self.lib.set_graph(self.ptr, 1 if on else 0), self.ptr)
 
     def set_timing(self, on=True):
         self.lib.check(self.lib.set_timing(self.ptr, 1 if on else 0), self.ptr)

@@ -174,6 +174,12 @@ __global__ void k_zero_u64(unsigned long long* a, int n) {
 __global__ void k_zero_u32(uint32_t* a, int n) {
   if (blockIdx.x == 0 && (int)threadIdx.x < n) a[threadIdx.x] = 0u;
 }
+__global__ void k_set_u64x2(unsigned long long* a, unsigned long long x, unsigned long long y) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a[0] = x;
+    a[1] = y;
+  }
+}
 
 __global__ void k_hash_insert_pairs(const PairRec* R, int64_t np, uint64_t* tab, uint64_t mask) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -35,7 +35,16 @@ struct StepArgs {
   int periodic;                  // Scene::isPeriodic
   double cell[3];                // cell size (axis-aligned cell: hSize = diag(cell))
   int skip;                      // settled apart pairs may be skipped (P.gap, B.disp are current)
+  // The container size and Scene::iter on the device (set before each step), so that a
+  // captured step stays valid while they change; np above is then the launch bound.
+  const unsigned long long* npDev;
+  const unsigned long long* iterDev;
 };
+
+__device__ __forceinline__ int64_t step_np(const StepArgs& A) {
+  const int64_t n = (int64_t)*A.npDev;
+  return n < A.np ? n : A.np;
+}
 
 // shift2 = hSize * cellDist (InteractionLoop.cpp:100-105): zero outside a periodic cell.
 __device__ __forceinline__ V3<double> pair_shift(const StepArgs& A, int64_t p) {
@@ -355,7 +364,7 @@ __device__ bool phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
     P.r[p].nuab[0] = 0; P.r[p].Gab[0] = 0; P.r[p].mu1[0] = 0; P.r[p].mu2[0] = 0;
     return false;
   }
-  if (!(hadGeom && hadPhys)) P.r[p].iterMadeReal[0] = A.iter;
+  if (!(hadGeom && hadPhys)) P.r[p].iterMadeReal[0] = (int64_t)*A.iterDev;
   put(R.hasGeom[0], 1, lazy);
   put(R.hasPhys[0], 1, lazy);
   put(R.depth[0], g.depth, lazy);
@@ -456,7 +465,8 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
   // collider runs), and the functors would store nothing: the interaction is skipped
   // without reading its record or its bodies.
   bool skipped = false;
-  if (A.skip && p < A.np) {
+  const int64_t np = step_np(A);
+  if (A.skip && p < np) {
     const float gp = P.gap[p];
     if (gp > 0.0f) {
       const float da = B.disp[P.cid[2 * p]], db = B.disp[P.cid[2 * p + 1]];
@@ -466,7 +476,7 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
   float gapv = __int_as_float(0x7fc00000);
   if (skipped) {
     ce++;  // the reference's functors ran on it
-  } else if (p < A.np) {
+  } else if (p < np) {
     if (P.r[p].active[0]) {  // ForceResetter: last step's contribution is withdrawn
       P.r[p].active[0] = 0;
       A.csr[P.r[p].csrPos[0]] = csr_entry(p, 0, false);
@@ -583,7 +593,7 @@ template <int LAW>
 __global__ __launch_bounds__(256) void k_queue_fill(StepArgs A, SolveIO S, const uint32_t* qflag,
                                                     const uint32_t* qpos, int64_t np) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np || !qflag[p]) return;
+  if (p >= np || p >= step_np(A) || !qflag[p]) return;
   const int64_t slot = qpos[p];
   const PairRec& R = A.P.r[p];
   const int id1 = R.id1[0], id2 = R.id2[0];
