@@ -138,6 +138,32 @@ int sdm_get_timers(sdm_ctx* ctx, double out[8]);
  * graph is recaptured after any reallocation or parameter change. */
 int sdm_set_graph(sdm_ctx* ctx, int32_t on);
 
+/* Observables of the current state, reduced on the device.
+ * sdm_mean_coordination: Shop__meanCoordinationNumber (py/_utils.cpp:522-539): twice the real
+ *   interactions with a non-zero normal force over the bodies that are not walls.
+ * sdm_stress: Shop::getStress (pkg/dem/Shop_02.cpp:350-377), the Love-Weber stress
+ *   sum over real interactions of (normalForce + shearForce) (x) (pos1 - pos2 - hSize cellDist)
+ *   over `volume` (row-major 3x3; volume 0 = the cell volume if periodic, else 1).  Summed in a
+ *   fixed block tree: deterministic, equal to the reference's serial sum to rounding.
+ * sdm_body_forces: ForceContainer::getForce (core/ForceContainer.hpp) of `n` bodies after the
+ *   last step (x, y, z each): the wall forces CompressionEngine turns into wall stresses
+ *   (pkg/dem/Compression.cpp:270-275) without reading back every body. */
+int sdm_mean_coordination(sdm_ctx* ctx, double* cn);
+int sdm_stress(sdm_ctx* ctx, double volume, double sigma[9]);
+int sdm_body_forces(sdm_ctx* ctx, const int32_t* ids, int32_t n, double* forces);
+
+/* Launch shapes of the step (no counterpart in the reference).  None changes a result: they
+ * only schedule the same per-interaction work differently.  sudodem_amd/autotune.py picks
+ * them by timing a workload's own steps. */
+typedef struct sdm_tuning {
+  int32_t solver_blocks_per_cu; /* persistent solvers: blocks per CU (0 = the occupancy maximum) */
+  int32_t sort_by_cost;         /* Nelder-Mead list sorted by expected cost, longest first (1) or not (0) */
+  int32_t misc_blocks_per_cu;   /* wall / sphere kernel: blocks per CU (>= 1) */
+  int32_t graph;                /* steady-state steps replayed from a captured graph (sdm_set_graph) */
+} sdm_tuning;
+int sdm_set_tuning(sdm_ctx* ctx, const sdm_tuning* t);
+int sdm_get_tuning(sdm_ctx* ctx, sdm_tuning* t);
+
 /* ---------------------------------------------------------------------------------------
  * Differentiable rollout over a frozen contact set (linear law + NewtonIntegrator).
  * No counterpart in the reference: it differentiates, over a window of `steps` steps, the

@@ -2030,6 +2030,58 @@ int sdmcpu_step(sdm_ctx* c, int32_t nsteps) {
 
 int sdmcpu_sync(sdm_ctx*) { return 0; }
 
+// Shop__meanCoordinationNumber (py/_utils.cpp:522-539).
+int sdmcpu_mean_coordination(sdm_ctx* c, double* cn) {
+  if (!c || !cn) return fail(c, "bad arguments");
+  int num_contacts = 0, num_particles = 0;
+  for (const Pair& I : c->intrs) {
+    if (!(I.hasGeom && I.hasPhys)) continue;
+    const double* f = I.normalForce;
+    if (std::sqrt(f[0] * f[0] + (f[1] * f[1] + f[2] * f[2])) == 0) continue;  // normalForce.norm() == 0
+    num_contacts += 1;
+  }
+  for (const Body& b : c->bodies) {
+    if (b.kind == SDM_BODY_WALL) continue;
+    num_particles += 1;
+  }
+  *cn = 2.0 * num_contacts / num_particles;
+  return 0;
+}
+
+// Shop::getStress (pkg/dem/Shop_02.cpp:350-377), serial over the container.
+int sdmcpu_stress(sdm_ctx* c, double volume, double sigma[9]) {
+  if (!c || !sigma) return fail(c, "bad arguments");
+  const bool isPeriodic = c->prm.periodic != 0;
+  if (volume == 0) volume = isPeriodic ? c->prm.cell[0] * c->prm.cell[1] * c->prm.cell[2] : 1;
+  double st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (const Pair& I : c->intrs) {
+    if (!(I.hasGeom && I.hasPhys)) continue;
+    const Body& b1 = c->bodies[I.id1];
+    const Body& b2 = c->bodies[I.id2];
+    double branch[3], f[3];
+    for (int k = 0; k < 3; k++) branch[k] = b1.pos[k] - b2.pos[k];
+    if (isPeriodic) {
+      double shift2[3];
+      pair_shift2(c, I, shift2);
+      for (int k = 0; k < 3; k++) branch[k] -= shift2[k];
+    }
+    for (int k = 0; k < 3; k++) f[k] = I.normalForce[k] + I.shearForce[k];
+    for (int r = 0; r < 3; r++)
+      for (int q = 0; q < 3; q++) st[3 * r + q] += f[r] * branch[q];
+  }
+  for (int k = 0; k < 9; k++) sigma[k] = st[k] / volume;
+  return 0;
+}
+
+int sdmcpu_body_forces(sdm_ctx* c, const int32_t* ids, int32_t n, double* forces) {
+  if (!c || n < 0 || (n > 0 && (!ids || !forces))) return fail(c, "bad arguments");
+  for (int32_t i = 0; i < n; i++) {
+    if (ids[i] < 0 || ids[i] >= (int32_t)c->bodies.size()) return fail(c, "body id out of range");
+    for (int k = 0; k < 3; k++) forces[3 * i + k] = c->bodies[ids[i]].force[k];
+  }
+  return 0;
+}
+
 int sdmcpu_get_counters(sdm_ctx* c, sdm_counters* out) {
   if (!c || !out) return fail(c, "bad arguments");
   *out = c->cnt;
@@ -2043,6 +2095,16 @@ int sdmcpu_reset_counters(sdm_ctx* c) {
 
 int sdmcpu_set_timing(sdm_ctx* c, int32_t) { return c ? 0 : -1; }
 int sdmcpu_set_graph(sdm_ctx* c, int32_t) { return c ? 0 : -1; }
+// Launch shapes have no meaning on the CPU: accepted and reported as the device defaults.
+int sdmcpu_set_tuning(sdm_ctx* c, const sdm_tuning* t) { return (c && t) ? 0 : -1; }
+int sdmcpu_get_tuning(sdm_ctx* c, sdm_tuning* t) {
+  if (!c || !t) return -1;
+  t->solver_blocks_per_cu = 0;
+  t->sort_by_cost = 1;
+  t->misc_blocks_per_cu = 2;
+  t->graph = 1;
+  return 0;
+}
 int sdmcpu_get_timers(sdm_ctx* c, double out[8]) {
   if (!c || !out) return -1;
   for (int k = 0; k < 8; k++) out[k] = 0;

@@ -34,6 +34,11 @@ int sdmcpu_get_counters(sdm_ctx* ctx, sdm_counters* out);
 int sdmcpu_reset_counters(sdm_ctx* ctx);
 int sdmcpu_set_timing(sdm_ctx* ctx, int32_t on);
 int sdmcpu_set_graph(sdm_ctx* ctx, int32_t on);
+int sdmcpu_set_tuning(sdm_ctx* ctx, const sdm_tuning* t);
+int sdmcpu_mean_coordination(sdm_ctx* ctx, double* cn);
+int sdmcpu_stress(sdm_ctx* ctx, double volume, double sigma[9]);
+int sdmcpu_body_forces(sdm_ctx* ctx, const int32_t* ids, int32_t n, double* forces);
+int sdmcpu_get_tuning(sdm_ctx* ctx, sdm_tuning* t);
 int sdmcpu_get_timers(sdm_ctx* ctx, double out[8]);
 
 /* Number of OpenMP threads used by the engine loops (1 = the reference's default -j1,

@@ -49,6 +49,11 @@ class sdm_counters(ctypes.Structure):
                  "graph_steps")]
 
 
+class sdm_tuning(ctypes.Structure):
+    _fields_ = [("solver_blocks_per_cu", ctypes.c_int32), ("sort_by_cost", ctypes.c_int32),
+                ("misc_blocks_per_cu", ctypes.c_int32), ("graph", ctypes.c_int32)]
+
+
 class SdmError(RuntimeError):
     pass
 
@@ -89,6 +94,12 @@ class SdmLibrary:
         self.reset_counters = fn("reset_counters", ctypes.c_int, [c_ctx])
         self.set_timing = fn("set_timing", ctypes.c_int, [c_ctx, ctypes.c_int32])
         self.set_graph = fn("set_graph", ctypes.c_int, [c_ctx, ctypes.c_int32])
+        d = ctypes.POINTER(ctypes.c_double)
+        self.mean_coordination = fn("mean_coordination", ctypes.c_int, [c_ctx, d])
+        self.stress = fn("stress", ctypes.c_int, [c_ctx, ctypes.c_double, d])
+        self.body_forces = fn("body_forces", ctypes.c_int, [c_ctx, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, d])
+        self.set_tuning = fn("set_tuning", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_tuning)])
+        self.get_tuning = fn("get_tuning", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_tuning)])
         self.get_timers = fn("get_timers", ctypes.c_int, [c_ctx, ctypes.POINTER(ctypes.c_double)])
 
     def fields(self):
@@ -161,6 +172,36 @@ class Context:
 
     def reset_counters(self):
         self.lib.check(self.lib.reset_counters(self.ptr), self.ptr)
+
+    def mean_coordination(self) -> float:
+        """Shop__meanCoordinationNumber (getMeanCN) of the current state."""
+        v = ctypes.c_double()
+        self.lib.check(self.lib.mean_coordination(self.ptr, ctypes.byref(v)), self.ptr)
+        return v.value
+
+    def stress(self, volume=0.0) -> np.ndarray:
+        """Shop::getStress (Love-Weber stress tensor, 3x3) of the current state."""
+        out = np.zeros(9)
+        self.lib.check(self.lib.stress(self.ptr, float(volume), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                       self.ptr)
+        return out.reshape(3, 3)
+
+    def body_forces(self, ids) -> np.ndarray:
+        """Forces on the given bodies after the last step (n x 3)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        out = np.zeros((len(ids), 3))
+        self.lib.check(self.lib.body_forces(self.ptr, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(ids),
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), self.ptr)
+        return out
+
+    def tuning(self) -> dict:
+        t = sdm_tuning()
+        self.lib.check(self.lib.get_tuning(self.ptr, ctypes.byref(t)), self.ptr)
+        return {k: int(getattr(t, k)) for k, _ in sdm_tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = sdm_tuning(**{**self.tuning(), **kw})
+        self.lib.check(self.lib.set_tuning(self.ptr, ctypes.byref(t)), self.ptr)
 
     def set_graph(self, on=True):
         """Replay steady-state steps as a captured graph (the default)."""

@@ -19,6 +19,7 @@
 #include "../../include/sdm.h"
 #include "bodies.h"
 #include "frozen.h"
+#include "observe.h"
 #include "periodic.h"
 #include "radix.h"
 
@@ -117,6 +118,7 @@ struct sdm_ctx {
   // Captured steady-state step (hipGraph): rebuilt when any buffer it uses is reallocated or a
   // parameter it bakes in changes (graphGen moves on).
   bool useGraph = true;
+  sdm_tuning tune = {0, 1, 2, 1};
   hipGraphExec_t graphExec = nullptr;
   uint64_t graphGen = 1, graphBuiltGen = 0;
   int64_t graphs = 0;
@@ -647,6 +649,9 @@ unsigned resident_grid(sdm_ctx* c, K kernel, int& cache, int64_t nwork) {
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0));
     cache = std::max(1, nb) * std::max(1, c->numCU);
   }
+  if (c->tune.solver_blocks_per_cu > 0)
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((nwork + 255) / 256,
+                                                            (int64_t)c->tune.solver_blocks_per_cu * std::max(1, c->numCU)));
   const int64_t b = (nwork + 255) / 256;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, cache));
 }
@@ -670,7 +675,7 @@ void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA) {
   // the solver
   uint32_t* byPair = c->u0;
   hipLaunchKernelGGL(k_copy_u32, dim3(nblk(nmax)), dim3(256), 0, c->st, byPair, c->v0, nt, nmax);
-  sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, nmax, 8, c->rsScratch, c->st);
+  if (c->tune.sort_by_cost) sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, nmax, 8, c->rsScratch, c->st);
   const uint32_t* list = c->v0;
   SolveArgs S2 = SA;
   S2.list = reinterpret_cast<const int32_t*>(list);
@@ -713,7 +718,8 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
   scan_to_counter(c, c->qflag, c->qpos, A.np, DC_NQ);
   const unsigned long long* nq = c->dcnt + DC_NQ;
   hipLaunchKernelGGL((k_queue_fill<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, c->qflag, c->qpos, A.np);
-  hipLaunchKernelGGL((k_pairs_misc<LAW>), dim3(std::max(1, c->numCU) * 2), dim3(256), 0, c->st, A, c->miscList,
+  hipLaunchKernelGGL((k_pairs_misc<LAW>), dim3(std::max(1, c->numCU) * std::max(1, c->tune.misc_blocks_per_cu)),
+                     dim3(256), 0, c->st, A, c->miscList,
                      c->dcnt + DC_MISC);
   SolveArgs SA;
   SA.geo = c->B.geo;
@@ -1318,6 +1324,85 @@ int sdm_set_timing(sdm_ctx* c, int32_t on) {
 int sdm_set_graph(sdm_ctx* c, int32_t on) {
   if (!c) return fail(c, "null context");
   c->useGraph = on != 0;
+  c->tune.graph = on != 0 ? 1 : 0;
+  return 0;
+}
+
+int sdm_set_tuning(sdm_ctx* c, const sdm_tuning* t) {
+  if (!c || !t) return fail(c, "sdm_set_tuning: bad arguments");
+  if (t->solver_blocks_per_cu < 0 || t->solver_blocks_per_cu > 64 || t->misc_blocks_per_cu < 1 ||
+      t->misc_blocks_per_cu > 64)
+    return fail(c, "sdm_set_tuning: blocks per CU out of range");
+  c->tune = *t;
+  c->useGraph = t->graph != 0;
+  c->graphGen++;
+  return 0;
+}
+
+int sdm_mean_coordination(sdm_ctx* c, double* cn) {
+  if (!c || !cn) return fail(c, "sdm_mean_coordination: bad arguments");
+  try {
+    if (c->geoDirty) {  // body kinds written by the host since the last step
+      hipLaunchKernelGGL(k_geo_pack, dim3(nblk(c->B.n)), dim3(256), 0, c->st, c->B);
+      c->geoDirty = false;
+    }
+    unsigned long long* buf = dalloc<unsigned long long>(2);
+    HIPCHK(hipMemsetAsync(buf, 0, 2 * sizeof(unsigned long long), c->st));
+    const int64_t n = std::max<int64_t>(c->np, c->B.n);
+    hipLaunchKernelGGL(k_cn_count, dim3(nblk(std::max<int64_t>(1, n))), dim3(256), 0, c->st, c->P, c->np, c->B,
+                       (int)SDM_KIND_WALL, buf);
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(h, buf, sizeof(h), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(buf);
+    *cn = 2.0 * (double)h[0] / (double)h[1];
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+int sdm_stress(sdm_ctx* c, double volume, double sigma[9]) {
+  if (!c || !sigma) return fail(c, "sdm_stress: bad arguments");
+  try {
+    if (volume == 0) volume = c->prm.periodic ? c->prm.cell[0] * c->prm.cell[1] * c->prm.cell[2] : 1.0;
+    const int64_t nb = (int64_t)nblk(std::max<int64_t>(1, c->np));
+    double* part = dalloc<double>(9 * nb + 9);
+    hipLaunchKernelGGL(k_stress_partial, dim3((unsigned)nb), dim3(256), 0, c->st, c->P, c->np, c->B,
+                       c->prm.periodic ? 1 : 0, c->prm.cell[0], c->prm.cell[1], c->prm.cell[2], part);
+    hipLaunchKernelGGL(k_stress_final, dim3(1), dim3(64), 0, c->st, part, nb, volume, part + 9 * nb);
+    HIPCHK(hipMemcpyAsync(sigma, part + 9 * nb, 9 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(part);
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+int sdm_body_forces(sdm_ctx* c, const int32_t* ids, int32_t n, double* forces) {
+  if (!c || n < 0 || (n > 0 && (!ids || !forces))) return fail(c, "sdm_body_forces: bad arguments");
+  if (n == 0) return 0;
+  try {
+    for (int32_t i = 0; i < n; i++)
+      if (ids[i] < 0 || ids[i] >= c->B.n) return fail(c, "sdm_body_forces: body id out of range");
+    int32_t* dids = dalloc<int32_t>(n);
+    double* dout = dalloc<double>(3 * (int64_t)n);
+    HIPCHK(hipMemcpyAsync(dids, ids, n * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_gather_forces, dim3(nblk(n)), dim3(256), 0, c->st, c->B, dids, n, dout);
+    HIPCHK(hipMemcpyAsync(forces, dout, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(dids);
+    dfree(dout);
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+int sdm_get_tuning(sdm_ctx* c, sdm_tuning* t) {
+  if (!c || !t) return fail(c, "sdm_get_tuning: bad arguments");
+  *t = c->tune;
   return 0;
 }
 

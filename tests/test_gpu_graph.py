@@ -53,3 +53,21 @@ def test_graph_batch(gpu):
     rng = np.random.default_rng(5)
     specs = [configs.box_specimen(n, rng, kn, mu) for n, kn, mu in ((300, 1e5, 0.5), (700, 6e4, 0.3), (200, 2e5, 0.7))]
     _run_twice(gpu, configs.batch(specs), 150)
+
+
+def test_tuning_changes_no_result(gpu):
+    """Launch shapes (sdm_set_tuning, chosen by sudodem_amd/autotune.py) only reschedule work."""
+    cfg = configs.densify(configs.config1(n_particles=2000, box_xy=20e-3), gpu, iterations=30)
+    ref = None
+    for tune in ({}, {"solver_blocks_per_cu": 1, "sort_by_cost": 0}, {"solver_blocks_per_cu": 4, "graph": 0,
+                                                                      "misc_blocks_per_cu": 1}):
+        ctx = cfg.open(gpu, precision=SDM_MIXED)
+        ctx.set_tuning(**tune)
+        ctx.step(60)
+        out = {k: ctx.get(k) for k in FIELDS}
+        ctx.close()
+        if ref is None:
+            ref = out
+        else:
+            for k in FIELDS:
+                assert np.array_equal(out[k], ref[k], equal_nan=True), (tune, k)
