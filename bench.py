@@ -175,11 +175,17 @@ def batch_bench(args):
     bc = b.open(lib)
     bc.step(W)
     bc.sync()
+    from sudodem_amd.autotune import autotune
+    t0 = time.perf_counter()
+    tuning, trials = autotune(bc, steps=20, log=log)
+    log("autotuned in %.1fs: %s" % (time.perf_counter() - t0, tuning))
+    bc.reset_counters()
     t0 = time.perf_counter()
     bc.step(K)
     bc.sync()
     tb = time.perf_counter() - t0
-    log("batched: %.3f ms/step" % (tb / K * 1e3))
+    graph_steps = bc.counters()["graph_steps"]
+    log("batched: %.3f ms/step (%d of %d steps replayed from the step graph)" % (tb / K * 1e3, graph_steps, K))
     bc.close()
     ctxs = [c.open(lib) for c in specs]
     for c in ctxs:
@@ -203,6 +209,8 @@ def batch_bench(args):
                                "linear law)" % len(specs), "particles": nparts, "parallelism": "single GPU"},
         "one_by_one": {"value": nparts * K / ts, "ms_per_step": ts / K * 1e3},
         "batched_over_one_by_one": ts / tb,
+        "tuning": tuning, "graph_steps": graph_steps,
+        "autotune_trials": [dict(t, ms_per_step=ms * 1e3) for t, ms in trials],
     }))
 
 

@@ -12,12 +12,12 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("which", ["walls", "periodic"])
 def test_observables_match_oracle(gpu, oracle, which):
     if which == "walls":
-        cfg = configs.config0(n_particles=300)
+        cfg = configs.densify(configs.config1(n_particles=2000, box_xy=20e-3), gpu, iterations=30)
     else:
         cfg = configs.densify(configs.config2(n_particles=1500), gpu, iterations=40, growth=0.01, relax=10)
     c = cfg.open(oracle)
     oracle.set_init_sort(c.ptr, 1)
-    c.step(40)
+    c.step(5)
     g = cfg.open(gpu)
     g.restore(c.snapshot())
     assert g.mean_coordination() == c.mean_coordination()
