@@ -181,10 +181,11 @@ __global__ void k_set_u64x2(unsigned long long* a, unsigned long long x, unsigne
   }
 }
 
-__global__ void k_hash_insert_pairs(const PairRec* R, int64_t np, uint64_t* tab, uint64_t mask) {
+__global__ void k_hash_insert_pairs(const int32_t* cid, int64_t np, uint64_t* tab, uint64_t mask) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const uint64_t key = upair(R[p].id1[0], R[p].id2[0]);
+  const int2 ab = reinterpret_cast<const int2*>(cid)[p];
+  const uint64_t key = upair(ab.x, ab.y);
   uint64_t h = hmix(key) & mask;
   while (true) {
     const unsigned long long prev = atomicCAS((unsigned long long*)&tab[h], SDM_HEMPTY, (unsigned long long)key);
@@ -272,7 +273,7 @@ __global__ void k_tail_moves(const uint32_t* flag, const uint32_t* E, int64_t K,
 
 // Move whole interaction records: record msrc[k] to slot mdst[k] (the swap-with-last
 // compaction of InteractionContainer::erase).  One 16-byte word per lane, 40 lanes per record.
-__global__ void k_move_pairs(PairRec* R, const int32_t* msrc, const int32_t* mdst, int64_t nm) {
+__global__ void k_move_pairs(PairRec* R, int32_t* cid, const int32_t* msrc, const int32_t* mdst, int64_t nm) {
   constexpr int W = (int)(sizeof(PairRec) / 16);
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t k = t / W;
@@ -280,6 +281,14 @@ __global__ void k_move_pairs(PairRec* R, const int32_t* msrc, const int32_t* mds
   if (k >= nm) return;
   const int64_t s = msrc[k], d = mdst[k];
   reinterpret_cast<uint4*>(R + d)[w] = reinterpret_cast<const uint4*>(R + s)[w];
+  if (w == 0) reinterpret_cast<int2*>(cid)[d] = reinterpret_cast<const int2*>(cid)[s];
+}
+
+// (id1, id2) of every interaction into the packed endpoint array.
+__global__ void k_pair_ids(const PairRec* R, int64_t np, int32_t* cid) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  reinterpret_cast<int2*>(cid)[p] = make_int2(R[p].id1[0], R[p].id2[0]);
 }
 
 // List entry e (sorted by body, interaction order within a body) -> its interaction: the
@@ -326,6 +335,7 @@ __global__ void k_new_pairs(PairArrays P, int64_t start, int64_t count, const Ca
 #undef SDM_ZERO
   R.id1[0] = C.a[c];
   R.id2[0] = C.b[c];
+  reinterpret_cast<int2*>(P.cid)[p] = make_int2(C.a[c], C.b[c]);
   P.r[p].geomExists[0] = 1;
   P.r[p].isShearNew[0] = 1;
   P.r[p].iterMadeReal[0] = -1;
@@ -333,13 +343,11 @@ __global__ void k_new_pairs(PairArrays P, int64_t start, int64_t count, const Ca
 }
 
 // Interaction endpoints for the per-body lists: entry 2p -> (id1, 2p), 2p+1 -> (id2, 2p+1).
-__global__ void k_csr_entries(const PairRec* R, int64_t np, uint32_t* key, uint32_t* val, int32_t* deg,
-                              int32_t* cid) {
+__global__ void k_csr_entries(const int32_t* cid, int64_t np, uint32_t* key, uint32_t* val, int32_t* deg) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const int a = R[p].id1[0], b = R[p].id2[0];
-  cid[2 * p] = a;
-  cid[2 * p + 1] = b;
+  const int2 ab = reinterpret_cast<const int2*>(cid)[p];
+  const int a = ab.x, b = ab.y;
   key[2 * p] = (uint32_t)a;
   val[2 * p] = (uint32_t)(2 * p);
   key[2 * p + 1] = (uint32_t)b;

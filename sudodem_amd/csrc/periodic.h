@@ -29,7 +29,14 @@ struct PeriCell {
   double h[3];   // grid cell size (L / n)
   int32_t n[3];  // grid cells per axis
   double eps;    // absolute widening of every box before it is listed (covers rounding)
+  // single-precision screen of box pairs (k_peri_pairs): cell size, its inverse and the
+  // margin that covers the screen's rounding
+  float Lf[3], iLf[3], mf[3];
 };
+
+// Boxes wider than this fraction of the cell on some axis are never screened: only the
+// exact test (which reports a box spanning into the next period) decides their pairs.
+#define SDM_PERI_WIDE 0.45
 
 // cellWrap / cellWrapRel (InsertionSortCollider.cpp:404-414), same operations.
 __device__ __forceinline__ double cell_wrap(double x, double x0, double x1, int& period) {
@@ -124,15 +131,25 @@ __global__ void k_width_stats(BodyArrays B, double* sum, unsigned long long* max
 }
 
 // Cell range of every body: start cell s[k] and count c[k] per axis (c = n: the whole
-// axis), and the number of cells the box touches.
-__global__ void k_peri_ranges(BodyArrays B, PeriCell pc, int32_t* rng, uint32_t* cnt) {
+// axis), and the number of cells the box touches.  Also the box in single precision for the
+// pair screen: box[2i] = wrapped centre (w: 1 for a wide box), box[2i+1] = half widths.
+__global__ void k_peri_ranges(BodyArrays B, PeriCell pc, int32_t* rng, uint32_t* cnt, float4* box) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t N = B.n;
   if (i >= N) return;
   uint32_t total = 1;
+  float cen[3], hw[3];
+  bool wide = false;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     const int n = pc.n[k];
+    {
+      const double m = B.bmin[k * N + i], M = B.bmax[k * N + i];
+      int per;
+      cen[k] = (float)cell_wrap(0.5 * (m + M), 0.0, pc.L[k], per);
+      hw[k] = (float)(0.5 * (M - m));
+      wide = wide || !(M - m < SDM_PERI_WIDE * pc.L[k]);
+    }
     const double lo = B.bmin[k * N + i] - pc.eps;
     const double w = (B.bmax[k * N + i] + pc.eps) - lo;
     int per;
@@ -151,6 +168,8 @@ __global__ void k_peri_ranges(BodyArrays B, PeriCell pc, int32_t* rng, uint32_t*
     total *= (uint32_t)c;
   }
   cnt[i] = total;
+  box[2 * i] = make_float4(cen[0], cen[1], cen[2], wide ? 1.0f : 0.0f);
+  box[2 * i + 1] = make_float4(hw[0], hw[1], hw[2], 0.0f);
 }
 
 // Entries (cell, body) of every body at its offset.
@@ -175,14 +194,20 @@ __global__ void k_peri_entries(int64_t N, PeriCell pc, const int32_t* rng, const
   }
 }
 
-// End of the run of equal cells that starts at each sorted entry's cell.
-__global__ void k_cell_ends(const uint32_t* key, int64_t m, uint32_t* runEnd) {
+// Start and end of the run of equal cells in the sorted entries, per cell.
+__global__ void k_cell_ends(const uint32_t* key, int64_t m, uint32_t* runStart, uint32_t* runEnd) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m) return;
+  if (e == 0 || key[e - 1] != key[e]) runStart[key[e]] = (uint32_t)e;
   if (e == m - 1 || key[e + 1] != key[e]) runEnd[key[e]] = (uint32_t)(e + 1);
 }
 
-__device__ __forceinline__ bool cyc_in(int x, int s, int c, int n) { return ((x - s + n) % n) < c; }
+// x in the cyclic range [s, s + c) of 0..n-1 (x, s in [0, n)).
+__device__ __forceinline__ bool cyc_in(int x, int s, int c, int n) {
+  int d = x - s;
+  if (d < 0) d += n;
+  return d < c;
+}
 
 // Smallest cell index of the intersection of two cyclic ranges (-1 if empty).
 __device__ __forceinline__ int first_shared(int sa, int ca, int sb, int cb, int n) {
@@ -204,9 +229,18 @@ struct PeriOrder {
 
 // Every pair sharing a cell, tested in the first shared cell; overlapping pairs not in the
 // container (hash `found`) become candidates with their orientation and order key.
+//
+// The entries of a cell (k of them) are paired round-robin: entry i meets the entries
+// i+1 .. i+(k-1)/2 after it, cyclically, and for even k the one at distance k/2 when
+// i < k/2, so every unordered pair is met once and all lanes of a cell loop equally long.
+// A pair of narrow boxes first passes a single-precision screen (minimum-image distance of
+// the box centres against the half widths plus a margin far above the screen's rounding, so
+// it never rejects a pair the exact test accepts) and the container lookup; only the pairs
+// that survive both (the new ones) are oriented and given spatialOverlapPeri's exact test.
 __global__ __launch_bounds__(256) void k_peri_pairs(BodyArrays B, PeriCell pc, PeriOrder po, const int32_t* rng,
-                                                    const uint32_t* key, const uint32_t* val, const uint32_t* runEnd,
-                                                    int64_t m, const uint64_t* found, uint64_t fmask, Cands C,
+                                                    const float4* box, const uint32_t* key, const uint32_t* val,
+                                                    const uint32_t* runStart, const uint32_t* runEnd, int64_t m,
+                                                    const uint64_t* found, uint64_t fmask, Cands C,
                                                     unsigned long long* err) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m) return;
@@ -217,16 +251,37 @@ __global__ __launch_bounds__(256) void k_peri_pairs(BodyArrays B, PeriCell pc, P
   int ra[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) ra[k] = rng[6 * a + k];
-  const uint32_t end = runEnd[cell];
+  const float4 ca = box[2 * a], ha = box[2 * a + 1];
+  const uint32_t s0 = runStart[cell];
+  const int kc = (int)(runEnd[cell] - s0);
+  const int i = (int)((uint32_t)e - s0);
+  const int iters = (kc - 1) / 2 + ((kc % 2 == 0 && i < kc / 2) ? 1 : 0);
   const int64_t N = B.n;
-  for (uint32_t f = (uint32_t)e + 1; f < end; f++) {
-    const int b = (int)val[f];
+  for (int t = 1; t <= iters; t++) {
+    int j = i + t;
+    if (j >= kc) j -= kc;
+    const int b = (int)val[s0 + j];
     bool first = true;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       if (first && first_shared(ra[k], ra[3 + k], rng[6 * b + k], rng[6 * b + 3 + k], pc.n[k]) != X[k]) first = false;
     }
     if (!first) continue;
+    const float4 cb = box[2 * b];
+    const bool screened = ca.w == 0.0f && cb.w == 0.0f;
+    if (screened) {
+      const float4 hb = box[2 * b + 1];
+      const float d[3] = {ca.x - cb.x, ca.y - cb.y, ca.z - cb.z};
+      const float h[3] = {ha.x + hb.x, ha.y + hb.y, ha.z + hb.z};
+      bool apart = false;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const float dk = d[k] - pc.Lf[k] * rintf(d[k] * pc.iLf[k]);
+        apart = apart || fabsf(dk) > h[k] + pc.mf[k];
+      }
+      if (apart) continue;
+      if (hash_contains(found, fmask, upair(a, b))) continue;
+    }
     // orientation (and the order key) before the overlap test: spatialOverlapPeri is
     // evaluated as the reference does, with (id1, id2)
     int id1, id2;
@@ -268,7 +323,7 @@ __global__ __launch_bounds__(256) void k_peri_pairs(BodyArrays B, PeriCell pc, P
     }
     int32_t per[3];
     if (!spatial_overlap_peri(B, pc, id1, id2, per, err)) continue;
-    if (hash_contains(found, fmask, upair(id1, id2))) continue;
+    if (!screened && hash_contains(found, fmask, upair(id1, id2))) continue;
     const unsigned long long slot = atomicAdd(C.count, 1ull);
     if ((int64_t)slot < C.cap) {
       C.a[slot] = id1;

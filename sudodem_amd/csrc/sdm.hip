@@ -81,6 +81,7 @@ struct sdm_ctx {
   int32_t* csrPair = nullptr;
   int64_t csrCap = 0;
   bool csrValid = false;
+  bool cidValid = false;  // P.cid holds (id1, id2) of every interaction
   // scratch
   uint64_t *k0 = nullptr, *k1 = nullptr;
   uint32_t *v0 = nullptr, *v1 = nullptr, *u0 = nullptr, *u1 = nullptr;
@@ -129,8 +130,9 @@ struct sdm_ctx {
   int32_t* cellRng = nullptr;                     // per body: start cell and count per axis
   uint32_t *rankMin = nullptr, *rankMax = nullptr;
   uint32_t *peCnt = nullptr, *peOff = nullptr;    // cells per body and their exclusive scan
-  uint32_t* runEnd = nullptr;
+  uint32_t* runEnd = nullptr;                     // [runCap] end, then [runCap] start of each cell's entries
   int64_t runCap = 0;
+  float4* pbox = nullptr;                          // [2N] single-precision boxes of the pair screen
   // per-phase timing
   bool timing = false;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -210,8 +212,10 @@ void ensure_pair_cap(sdm_ctx* c, int64_t need) {
   Q.gap = dalloc<float>(cap);
   c->skipValid = false;
   c->graphGen++;
-  if (c->P.r && c->np > 0)
+  if (c->P.r && c->np > 0) {
     HIPCHK(hipMemcpyAsync(Q.r, c->P.r, c->np * sizeof(PairRec), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(Q.cid, c->P.cid, c->np * 2 * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+  }
   HIPCHK(hipStreamSynchronize(c->st));
   free_pairs(c->P);
   c->P = Q;
@@ -322,8 +326,17 @@ uint32_t exclusive_total(sdm_ctx* c, const uint32_t* in, uint32_t* out, int64_t 
   return a + b;
 }
 
+// P.cid from the records, after the host wrote interactions.
+void refresh_cid(sdm_ctx* c) {
+  if (c->cidValid) return;
+  if (c->np > 0)
+    hipLaunchKernelGGL(k_pair_ids, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.r, c->np, c->P.cid);
+  c->cidValid = true;
+}
+
 // Per-body interaction lists, each in interaction order (stable sort by body id).
 void build_csr(sdm_ctx* c) {
+  refresh_cid(c);
   const int64_t N = c->B.n;
   const int64_t m = 2 * c->np;
   if (m > c->csrCap) {
@@ -338,8 +351,7 @@ void build_csr(sdm_ctx* c) {
   HIPCHK(hipMemsetAsync(deg, 0, (N + 1) * sizeof(int32_t), c->st));
   if (c->np > 0) {
     uint32_t* key = c->u1;
-    hipLaunchKernelGGL(k_csr_entries, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.r, c->np, key, c->v0,
-                       deg, c->P.cid);
+    hipLaunchKernelGGL(k_csr_entries, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.cid, c->np, key, c->v0, deg);
     int bits = 1;
     while ((1ll << bits) < N) bits++;
     // keys are 32-bit: reuse the 64-bit sort buffers through a 32-bit view
@@ -378,7 +390,7 @@ void fill_found(sdm_ctx* c) {
   hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->dkeys, c->hcap, SDM_HEMPTY);
   hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->dvals, c->hcap, SDM_HEMPTY);
   if (c->np > 0)
-    hipLaunchKernelGGL(k_hash_insert_pairs, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.r, c->np, c->htab, mask);
+    hipLaunchKernelGGL(k_hash_insert_pairs, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.cid, c->np, c->htab, mask);
 }
 
 // The new candidates C[0, nc): drop the ones present, keep one per pair (smallest key),
@@ -419,7 +431,7 @@ void erase_flagged(sdm_ctx* c) {
   const int64_t nmove = read_u32(c, E + K);
   if (nmove > 0)
     hipLaunchKernelGGL(k_move_pairs, dim3(nblk(nmove * (int64_t)(sizeof(PairRec) / 16))), dim3(256), 0, c->st,
-                       c->P.r, c->msrc, c->mdst, nmove);
+                       c->P.r, c->P.cid, c->msrc, c->mdst, nmove);
   c->np = K;
   c->cnt.pairs_erased += Etot;
 }
@@ -430,6 +442,7 @@ void erase_flagged(sdm_ctx* c) {
 void collider_run_peri(sdm_ctx* c) {
   c->cnt.collider_runs++;
   c->skipValid = false;
+  refresh_cid(c);
   const int64_t N = c->B.n;
   const int64_t n2 = 2 * N;
   const bool doInitSort = !c->bbValid;
@@ -449,6 +462,7 @@ void collider_run_peri(sdm_ctx* c) {
     c->rankMax = dalloc<uint32_t>(N);
     c->peCnt = dalloc<uint32_t>(N + 1);
     c->peOff = dalloc<uint32_t>(N + 1);
+    c->pbox = dalloc<float4>(2 * N);
   }
   HIPCHK(hipMemcpyAsync(c->prevMin, c->B.bmin, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->prevMax, c->B.bmax, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
@@ -483,7 +497,12 @@ void collider_run_peri(sdm_ctx* c) {
     int k = (pc.n[0] >= pc.n[1] && pc.n[0] >= pc.n[2]) ? 0 : (pc.n[1] >= pc.n[2] ? 1 : 2);
     pc.n[k] = std::max(1, pc.n[k] / 2);
   }
-  for (int k = 0; k < 3; k++) pc.h[k] = pc.L[k] / pc.n[k];
+  for (int k = 0; k < 3; k++) {
+    pc.h[k] = pc.L[k] / pc.n[k];
+    pc.Lf[k] = (float)pc.L[k];
+    pc.iLf[k] = (float)(1.0 / pc.L[k]);
+    pc.mf[k] = (float)(1e-5 * pc.L[k]);  // the screen's rounding stays below 1e-6 L
+  }
   ncell = (int64_t)pc.n[0] * pc.n[1] * pc.n[2];
   pc.eps = 1e-9 * std::max(pc.L[0], std::max(pc.L[1], pc.L[2]));
 
@@ -509,7 +528,7 @@ void collider_run_peri(sdm_ctx* c) {
   }
 
   // ---- cell lists ----
-  hipLaunchKernelGGL(k_peri_ranges, dim3(nblk(N)), dim3(256), 0, c->st, c->B, pc, c->cellRng, c->peCnt);
+  hipLaunchKernelGGL(k_peri_ranges, dim3(nblk(N)), dim3(256), 0, c->st, c->B, pc, c->cellRng, c->peCnt, c->pbox);
   const int64_t m = exclusive_total(c, c->peCnt, c->peOff, N);
   ensure_scratch(c, m + 1);
   uint32_t* ckey = (uint32_t*)c->k0;
@@ -522,9 +541,10 @@ void collider_run_peri(sdm_ctx* c) {
     HIPCHK(hipStreamSynchronize(c->st));
     dfree(c->runEnd);
     c->runCap = ncell;
-    c->runEnd = dalloc<uint32_t>(ncell);
+    c->runEnd = dalloc<uint32_t>(2 * ncell);
   }
-  hipLaunchKernelGGL(k_cell_ends, dim3(nblk(m)), dim3(256), 0, c->st, ckey, m, c->runEnd);
+  uint32_t* runStart = c->runEnd + c->runCap;
+  hipLaunchKernelGGL(k_cell_ends, dim3(nblk(m)), dim3(256), 0, c->st, ckey, m, runStart, c->runEnd);
 
   // ---- candidates: overlapping pairs not in the container ----
   ensure_hash(c, c->np + 1024);
@@ -533,8 +553,8 @@ void collider_run_peri(sdm_ctx* c) {
   for (int pass = 0; pass < 2; pass++) {
     HIPCHK(hipMemsetAsync(c->dcnt + 2, 0, sizeof(unsigned long long), c->st));
     c->C.count = c->dcnt + 2;
-    hipLaunchKernelGGL(k_peri_pairs, dim3(nblk(m)), dim3(256), 0, c->st, c->B, pc, po, c->cellRng, ckey, cval,
-                       c->runEnd, m, c->htab, (uint64_t)c->hcap - 1, c->C, err);
+    hipLaunchKernelGGL(k_peri_pairs, dim3(nblk(m)), dim3(256), 0, c->st, c->B, pc, po, c->cellRng, c->pbox, ckey,
+                       cval, runStart, c->runEnd, m, c->htab, (uint64_t)c->hcap - 1, c->C, err);
     nc = (int64_t)read_counter(c, 2);
     if (nc <= c->C.cap) break;
     ensure_cands(c, nc);  // rerun with room for all
@@ -560,6 +580,7 @@ void collider_run(sdm_ctx* c) {
   }
   c->cnt.collider_runs++;
   c->skipValid = false;
+  refresh_cid(c);
   const int64_t N = c->B.n;
   const int64_t n2 = 2 * N;
   bool doInitSort = false;
@@ -1058,7 +1079,7 @@ void sdm_destroy(sdm_ctx* c) {
   dfree(c->C.a); dfree(c->C.b); dfree(c->C.key);
   dfree(c->msrc); dfree(c->mdst); dfree(c->holePos); dfree(c->stage);
   dfree(c->prevMin); dfree(c->prevMax); dfree(c->cellRng); dfree(c->rankMin); dfree(c->rankMax);
-  dfree(c->peCnt); dfree(c->peOff); dfree(c->runEnd);
+  dfree(c->peCnt); dfree(c->peOff); dfree(c->runEnd); dfree(c->pbox);
   dfree(c->dmats); dfree(c->dcnt); dfree(c->dMaxDisp);
   for (int k = 0; k < 6; k++)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
@@ -1209,6 +1230,7 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
       c->np = count;
       init_fresh_pairs(c, old, count);
       c->csrValid = false;
+      c->cidValid = false;
     }
     if (f->scope == S_BOUNDS) {
       const int64_t n2 = 2 * (int64_t)c->B.n;
@@ -1230,6 +1252,7 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
         HIPCHK(hipStreamSynchronize(c->st));
       }
       c->csrValid = false;
+      c->cidValid = false;
       c->pairsWritten = true;
     } else if (f->scope == S_BODY) {
       const int64_t rows = c->B.n;
