@@ -287,11 +287,16 @@ def periodic_bench(args):
     wall, cnt, tm = timed_run(ctx, K)
     N = cfg.n
     depth = ctx.get("geom.PenetrationDepth")
-    # pair-set exactness right after a collider run (forced on the next step)
-    real = (ctx.get("intr.hasGeom") != 0) & (ctx.get("intr.hasPhys") != 0)
-    real_before = set(zip(ctx.get("intr.id1")[real].tolist(), ctx.get("intr.id2")[real].tolist()))
-    ctx.set("collider.strideActive", np.array([0], np.int32))
-    ctx.step(1)
+    # pair-set exactness right after the next collider run the steps trigger themselves
+    runs = ctx.counters()["collider_runs"]
+    for _ in range(100):
+        real = (ctx.get("intr.hasGeom") != 0) & (ctx.get("intr.hasPhys") != 0)
+        real_before = set(zip(ctx.get("intr.id1")[real].tolist(), ctx.get("intr.id2")[real].tolist()))
+        ctx.step(1)
+        if ctx.counters()["collider_runs"] > runs:
+            break
+    else:
+        raise RuntimeError("no collider run within 100 steps")
     check = _sample_pair_set_check(ctx, cfg, real_before=real_before)
     log("pair-set check: %s" % check)
     ctx.close()

@@ -292,16 +292,32 @@ __global__ void k_pair_ids(const PairRec* R, int64_t np, int32_t* cid) {
 }
 
 // List entry e (sorted by body, interaction order within a body) -> its interaction: the
-// entry holds p << 2 | side << 1 | active, and the record remembers where its two entries
-// are, so the law can mark a contributing contact in both lists.
-__global__ void k_csr_finalize(PairRec* R, const uint32_t* sorted, int64_t m, int32_t* csr) {
+// entry holds p << 2 | side << 1 | active, and cpos remembers where the two entries of each
+// interaction are, so the law can mark a contributing contact in both lists.  The lists are
+// rebuilt only before the first interaction pass of a step, which withdraws every active
+// interaction's entries before any are read, so they start with the bit clear.
+__global__ void k_csr_finalize(const uint32_t* sorted, int64_t m, int32_t* cpos, int32_t* csr) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m) return;
   const uint32_t v = sorted[e];
-  const int64_t p = v >> 1;
-  const int side = (int)(v & 1u);
-  R[p].csrPos[side] = (int32_t)e;
-  csr[e] = csr_entry(p, side, R[p].active[0] != 0);
+  cpos[v] = (int32_t)e;
+  csr[e] = csr_entry(v >> 1, (int)(v & 1u), false);
+}
+
+// List offsets from the sorted body keys: off[b] = first entry whose body is >= b (a
+// binary search per body; off[N] = m).
+__global__ void k_csr_offsets(const uint32_t* key, int64_t m, int64_t N, int32_t* off) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > N) return;
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)key[mid] < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  off[b] = (int32_t)lo;
 }
 
 // One attribute of every interaction between the records and a row-major buffer
@@ -343,7 +359,7 @@ __global__ void k_new_pairs(PairArrays P, int64_t start, int64_t count, const Ca
 }
 
 // Interaction endpoints for the per-body lists: entry 2p -> (id1, 2p), 2p+1 -> (id2, 2p+1).
-__global__ void k_csr_entries(const int32_t* cid, int64_t np, uint32_t* key, uint32_t* val, int32_t* deg) {
+__global__ void k_csr_entries(const int32_t* cid, int64_t np, uint32_t* key, uint32_t* val) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const int2 ab = reinterpret_cast<const int2*>(cid)[p];
@@ -352,6 +368,4 @@ __global__ void k_csr_entries(const int32_t* cid, int64_t np, uint32_t* key, uin
   val[2 * p] = (uint32_t)(2 * p);
   key[2 * p + 1] = (uint32_t)b;
   val[2 * p + 1] = (uint32_t)(2 * p + 1);
-  atomicAdd(&deg[a], 1);
-  atomicAdd(&deg[b], 1);
 }

@@ -401,8 +401,8 @@ __device__ bool phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
     stv3(R.fs, ph.fs);
   }
   if (contributed) {
-    A.csr[R.csrPos[0]] = csr_entry(p, 0, true);
-    A.csr[R.csrPos[1]] = csr_entry(p, 1, true);
+    A.csr[P.cpos[2 * p]] = csr_entry(p, 0, true);
+    A.csr[P.cpos[2 * p + 1]] = csr_entry(p, 1, true);
     stv3(P.r[p].cF, F);
     stv3(P.r[p].cT1, -cross(b1.pos - g.cpt, F));
     stv3(P.r[p].cT2, cross(b2.pos - g.cpt, F));
@@ -479,8 +479,8 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
   } else if (p < np) {
     if (P.r[p].active[0]) {  // ForceResetter: last step's contribution is withdrawn
       P.r[p].active[0] = 0;
-      A.csr[P.r[p].csrPos[0]] = csr_entry(p, 0, false);
-      A.csr[P.r[p].csrPos[1]] = csr_entry(p, 1, false);
+      A.csr[P.cpos[2 * p]] = csr_entry(p, 0, false);
+      A.csr[P.cpos[2 * p + 1]] = csr_entry(p, 1, false);
     }
     if (P.r[p].geomExists[0]) {
       id1 = P.r[p].id1[0];

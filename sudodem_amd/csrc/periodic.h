@@ -130,25 +130,35 @@ __global__ void k_width_stats(BodyArrays B, double* sum, unsigned long long* max
   }
 }
 
+// Single-precision box of the pair screen: wrapped centre and half widths.
+__device__ __forceinline__ void screen_box(double m, double M, double L, float& cen, float& hw, bool& wide) {
+  int per;
+  cen = (float)cell_wrap(0.5 * (m + M), 0.0, L, per);
+  hw = (float)(0.5 * (M - m));
+  wide = wide || !(M - m < SDM_PERI_WIDE * L);
+}
+
 // Cell range of every body: start cell s[k] and count c[k] per axis (c = n: the whole
-// axis), and the number of cells the box touches.  Also the box in single precision for the
-// pair screen: box[2i] = wrapped centre (w: 1 for a wide box), box[2i+1] = half widths.
-__global__ void k_peri_ranges(BodyArrays B, PeriCell pc, int32_t* rng, uint32_t* cnt, float4* box) {
+// axis), and the number of cells the box touches.  Also the boxes of the pair screen,
+// box[4i .. 4i+3] = centre and half widths now, then at the previous collider run (`prev`
+// set; a copy of the current ones otherwise); box[4i].w is 1 for a wide box.
+__global__ void k_peri_ranges(BodyArrays B, PeriCell pc, const double* pmin, const double* pmax, int prev,
+                              int32_t* rng, uint32_t* cnt, float4* box) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t N = B.n;
   if (i >= N) return;
   uint32_t total = 1;
-  float cen[3], hw[3];
+  float cen[3], hw[3], pcen[3], phw[3];
   bool wide = false;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     const int n = pc.n[k];
-    {
-      const double m = B.bmin[k * N + i], M = B.bmax[k * N + i];
-      int per;
-      cen[k] = (float)cell_wrap(0.5 * (m + M), 0.0, pc.L[k], per);
-      hw[k] = (float)(0.5 * (M - m));
-      wide = wide || !(M - m < SDM_PERI_WIDE * pc.L[k]);
+    screen_box(B.bmin[k * N + i], B.bmax[k * N + i], pc.L[k], cen[k], hw[k], wide);
+    if (prev) {
+      screen_box(pmin[k * N + i], pmax[k * N + i], pc.L[k], pcen[k], phw[k], wide);
+    } else {
+      pcen[k] = cen[k];
+      phw[k] = hw[k];
     }
     const double lo = B.bmin[k * N + i] - pc.eps;
     const double w = (B.bmax[k * N + i] + pc.eps) - lo;
@@ -168,8 +178,10 @@ __global__ void k_peri_ranges(BodyArrays B, PeriCell pc, int32_t* rng, uint32_t*
     total *= (uint32_t)c;
   }
   cnt[i] = total;
-  box[2 * i] = make_float4(cen[0], cen[1], cen[2], wide ? 1.0f : 0.0f);
-  box[2 * i + 1] = make_float4(hw[0], hw[1], hw[2], 0.0f);
+  box[4 * i] = make_float4(cen[0], cen[1], cen[2], wide ? 1.0f : 0.0f);
+  box[4 * i + 1] = make_float4(hw[0], hw[1], hw[2], 0.0f);
+  box[4 * i + 2] = make_float4(pcen[0], pcen[1], pcen[2], 0.0f);
+  box[4 * i + 3] = make_float4(phw[0], phw[1], phw[2], 0.0f);
 }
 
 // Entries (cell, body) of every body at its offset.
@@ -227,110 +239,196 @@ struct PeriOrder {
   const double* pmax;
 };
 
-// Every pair sharing a cell, tested in the first shared cell; overlapping pairs not in the
-// container (hash `found`) become candidates with their orientation and order key.
+// Membership in the container as it was before this run's erase step: with `cover` through
+// the per-body lists of the last run (a scan of the shorter of the two bodies' lists),
+// otherwise through a hash of all its pairs.
+struct PeriFound {
+  int cover;
+  const int32_t* off;   // per-body list offsets [N + 1]
+  const int32_t* csr;   // list entries (interaction << 2 | ...)
+  const int32_t* cid;   // endpoints of every interaction
+  const uint64_t* tab;  // hash of upair keys (without cover)
+  uint64_t mask;
+};
+
+__device__ __forceinline__ bool peri_in_container(const PeriFound& F, int a, int b) {
+  if (!F.cover) return hash_contains(F.tab, F.mask, upair(a, b));
+  int x = a, y = b;
+  if (F.off[b + 1] - F.off[b] < F.off[a + 1] - F.off[a]) {
+    x = b;
+    y = a;
+  }
+  const int e1 = F.off[x + 1];
+  for (int e = F.off[x]; e < e1; e++) {
+    const int2 ab = reinterpret_cast<const int2*>(F.cid)[F.csr[e] >> 2];
+    if (ab.x == y || ab.y == y) return true;
+  }
+  return false;
+}
+
+// A pair that survived the screens: orientation and order key, then spatialOverlapPeri's
+// exact test with (id1, id2) as the reference evaluates it, then (for an unscreened pair) the
+// container lookup; a new overlapping pair becomes a candidate.
+__device__ __forceinline__ void peri_candidate(const BodyArrays& B, const PeriCell& pc, const PeriOrder& po, int a, int b,
+                                               bool screened, const PeriFound& F, const Cands& C,
+                                               unsigned long long* err) {
+  const int64_t N = B.n;
+  int id1, id2;
+  uint64_t okey;
+  if (po.init) {
+    const uint32_t n2 = po.n2;
+    const uint32_t dAB = (po.rankMin[b] + n2 - po.rankMin[a]) % n2, dAA = (po.rankMax[a] + n2 - po.rankMin[a]) % n2;
+    const uint32_t dBA = (po.rankMin[a] + n2 - po.rankMin[b]) % n2, dBB = (po.rankMax[b] + n2 - po.rankMin[b]) % n2;
+    const bool aScans = dAB < dAA, bScans = dBA < dBB;
+    if (!aScans && !bScans) return;  // never met in the sweep
+    const uint64_t kA = ((uint64_t)po.rankMin[a] << 32) | dAB, kB = ((uint64_t)po.rankMin[b] << 32) | dBA;
+    if (aScans && (!bScans || kA < kB)) {
+      id1 = a; id2 = b; okey = kA;
+    } else {
+      id1 = b; id2 = a; okey = kB;
+    }
+  } else {
+    int axis = 3;
+    id1 = a < b ? a : b;
+    id2 = a < b ? b : a;
+    for (int k = 0; k < 3 && axis == 3; k++) {
+      int p;
+      const int was = axis_overlap_peri(po.pmin[k * N + a], po.pmax[k * N + a], po.pmin[k * N + b],
+                                        po.pmax[k * N + b], pc.L[k], p);
+      if (was != 0) continue;
+      const int now = axis_overlap_peri(B.bmin[k * N + a], B.bmax[k * N + a], B.bmin[k * N + b],
+                                        B.bmax[k * N + b], pc.L[k], p);
+      if (now != 1) continue;
+      axis = k;
+      const double mA = B.bmin[k * N + a], MA = B.bmax[k * N + a], mB = B.bmin[k * N + b], MB = B.bmax[k * N + b];
+      const bool bInA = cell_wrap_rel(mB, mA, mA + pc.L[k]) <= MA - mA;
+      const bool aInB = cell_wrap_rel(mA, mB, mB + pc.L[k]) <= MB - mB;
+      if (bInA != aInB) {
+        id1 = bInA ? b : a;
+        id2 = bInA ? a : b;
+      }
+    }
+    okey = ((uint64_t)axis << 62) | ((uint64_t)(uint32_t)id1 << 31) | (uint64_t)(uint32_t)id2;
+  }
+  int32_t per[3];
+  if (!spatial_overlap_peri(B, pc, id1, id2, per, err)) return;
+  if (!screened && peri_in_container(F, id1, id2)) return;
+  const unsigned long long slot = atomicAdd(C.count, 1ull);
+  if ((int64_t)slot < C.cap) {
+    C.a[slot] = id1;
+    C.b[slot] = id2;
+    C.key[slot] = okey;
+  }
+}
+
+// Entries of a cell staged in LDS (the rest of a longer cell is read from global memory).
+#define SDM_PERI_TILE 128
+
+// One block per grid cell: every pair of the cell's entries whose first shared cell is this
+// one, found overlapping and not in the container, becomes a candidate.  Each unordered pair
+// is met in exactly one cell, so the candidates are distinct.  The container is the one
+// before this run's erase step, which removes only pairs that do not overlap now, so the
+// candidates are the same as after it.
 //
-// The entries of a cell (k of them) are paired round-robin: entry i meets the entries
-// i+1 .. i+(k-1)/2 after it, cyclically, and for even k the one at distance k/2 when
-// i < k/2, so every unordered pair is met once and all lanes of a cell loop equally long.
-// A pair of narrow boxes first passes a single-precision screen (minimum-image distance of
-// the box centres against the half widths plus a margin far above the screen's rounding, so
-// it never rejects a pair the exact test accepts) and the container lookup; only the pairs
-// that survive both (the new ones) are oriented and given spatialOverlapPeri's exact test.
-__global__ __launch_bounds__(256) void k_peri_pairs(BodyArrays B, PeriCell pc, PeriOrder po, const int32_t* rng,
-                                                    const float4* box, const uint32_t* key, const uint32_t* val,
-                                                    const uint32_t* runStart, const uint32_t* runEnd, int64_t m,
-                                                    const uint64_t* found, uint64_t fmask, Cands C,
-                                                    unsigned long long* err) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  const uint32_t cell = key[e];
-  const int X[3] = {(int)(cell % (uint32_t)pc.n[0]), (int)((cell / (uint32_t)pc.n[0]) % (uint32_t)pc.n[1]),
-                    (int)(cell / ((uint32_t)pc.n[0] * (uint32_t)pc.n[1]))};
-  const int a = (int)val[e];
-  int ra[6];
-#pragma unroll
-  for (int k = 0; k < 6; k++) ra[k] = rng[6 * a + k];
-  const float4 ca = box[2 * a], ha = box[2 * a + 1];
+// The k entries are paired round-robin: entry i meets the entries i+1 .. i+(k-1)/2 after
+// it, cyclically, and for even k the one at distance k/2 when i < k/2, so every unordered
+// pair is met once and all lanes loop equally long.  A pair of narrow boxes is screened in
+// single precision (minimum-image distance of the centres against the half widths plus a
+// margin far above the screen's rounding, so no pair the exact test accepts is dropped).
+// With F.cover set, the container holds every pair whose boxes overlapped at the previous
+// collider run (that run appended all of them, and pairs leave the container only at the
+// collider's erase), so a pair that certainly overlapped then (screen with the margin taken
+// off) is present.  The remaining narrow pairs are looked up in the container; only new
+// ones reach the exact test.
+__global__ __launch_bounds__(64) void k_peri_cell_pairs(BodyArrays B, PeriCell pc, PeriOrder po, const int32_t* rng,
+                                                        const float4* box, const uint32_t* val,
+                                                        const uint32_t* runStart, const uint32_t* runEnd, PeriFound F,
+                                                        Cands C, unsigned long long* err) {
+  __shared__ int32_t sId[SDM_PERI_TILE];
+  __shared__ uint32_t sRg[3][SDM_PERI_TILE];
+  __shared__ float4 sBox[4][SDM_PERI_TILE];
+  const uint32_t cell = blockIdx.x;
   const uint32_t s0 = runStart[cell];
   const int kc = (int)(runEnd[cell] - s0);
-  const int i = (int)((uint32_t)e - s0);
-  const int iters = (kc - 1) / 2 + ((kc % 2 == 0 && i < kc / 2) ? 1 : 0);
-  const int64_t N = B.n;
-  for (int t = 1; t <= iters; t++) {
-    int j = i + t;
-    if (j >= kc) j -= kc;
-    const int b = (int)val[s0 + j];
-    bool first = true;
+  if (kc < 2) return;  // the whole block
+  const int X[3] = {(int)(cell % (uint32_t)pc.n[0]), (int)((cell / (uint32_t)pc.n[0]) % (uint32_t)pc.n[1]),
+                    (int)(cell / ((uint32_t)pc.n[0] * (uint32_t)pc.n[1]))};
+  const int nl = kc < SDM_PERI_TILE ? kc : SDM_PERI_TILE;
+  for (int j = threadIdx.x; j < nl; j += blockDim.x) {
+    const int a = (int)val[s0 + j];
+    sId[j] = a;
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      if (first && first_shared(ra[k], ra[3 + k], rng[6 * b + k], rng[6 * b + 3 + k], pc.n[k]) != X[k]) first = false;
-    }
-    if (!first) continue;
-    const float4 cb = box[2 * b];
-    const bool screened = ca.w == 0.0f && cb.w == 0.0f;
-    if (screened) {
-      const float4 hb = box[2 * b + 1];
-      const float d[3] = {ca.x - cb.x, ca.y - cb.y, ca.z - cb.z};
-      const float h[3] = {ha.x + hb.x, ha.y + hb.y, ha.z + hb.z};
-      bool apart = false;
+    for (int k = 0; k < 3; k++) sRg[k][j] = (uint32_t)rng[6 * a + k] | ((uint32_t)rng[6 * a + 3 + k] << 16);
+#pragma unroll
+    for (int q = 0; q < 4; q++) sBox[q][j] = box[4 * a + q];
+  }
+  __syncthreads();
+  auto ent_id = [&](int j) -> int { return j < SDM_PERI_TILE ? sId[j] : (int)val[s0 + j]; };
+  auto ent_rg = [&](int j, int k) -> uint32_t {
+    if (j < SDM_PERI_TILE) return sRg[k][j];
+    const int a = (int)val[s0 + j];
+    return (uint32_t)rng[6 * a + k] | ((uint32_t)rng[6 * a + 3 + k] << 16);
+  };
+  auto ent_box = [&](int j, int q) -> float4 { return j < SDM_PERI_TILE ? sBox[q][j] : box[4 * (int)val[s0 + j] + q]; };
+  for (int i = threadIdx.x; i < kc; i += blockDim.x) {
+    const int a = ent_id(i);
+    uint32_t ra[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) ra[k] = ent_rg(i, k);
+    const float4 ca = ent_box(i, 0), ha = ent_box(i, 1);
+    const int iters = (kc - 1) / 2 + ((kc % 2 == 0 && i < kc / 2) ? 1 : 0);
+    for (int t = 1; t <= iters; t++) {
+      int j = i + t;
+      if (j >= kc) j -= kc;
+      bool first = true;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        const float dk = d[k] - pc.Lf[k] * rintf(d[k] * pc.iLf[k]);
-        apart = apart || fabsf(dk) > h[k] + pc.mf[k];
-      }
-      if (apart) continue;
-      if (hash_contains(found, fmask, upair(a, b))) continue;
-    }
-    // orientation (and the order key) before the overlap test: spatialOverlapPeri is
-    // evaluated as the reference does, with (id1, id2)
-    int id1, id2;
-    uint64_t okey;
-    if (po.init) {
-      const uint32_t n2 = po.n2;
-      const uint32_t dAB = (po.rankMin[b] + n2 - po.rankMin[a]) % n2, dAA = (po.rankMax[a] + n2 - po.rankMin[a]) % n2;
-      const uint32_t dBA = (po.rankMin[a] + n2 - po.rankMin[b]) % n2, dBB = (po.rankMax[b] + n2 - po.rankMin[b]) % n2;
-      const bool aScans = dAB < dAA, bScans = dBA < dBB;
-      if (!aScans && !bScans) continue;  // never met in the sweep
-      const uint64_t kA = ((uint64_t)po.rankMin[a] << 32) | dAB, kB = ((uint64_t)po.rankMin[b] << 32) | dBA;
-      if (aScans && (!bScans || kA < kB)) {
-        id1 = a; id2 = b; okey = kA;
-      } else {
-        id1 = b; id2 = a; okey = kB;
-      }
-    } else {
-      int axis = 3;
-      id1 = a < b ? a : b;
-      id2 = a < b ? b : a;
-      for (int k = 0; k < 3 && axis == 3; k++) {
-        int p;
-        const int was = axis_overlap_peri(po.pmin[k * N + a], po.pmax[k * N + a], po.pmin[k * N + b],
-                                          po.pmax[k * N + b], pc.L[k], p);
-        if (was != 0) continue;
-        const int now = axis_overlap_peri(B.bmin[k * N + a], B.bmax[k * N + a], B.bmin[k * N + b],
-                                          B.bmax[k * N + b], pc.L[k], p);
-        if (now != 1) continue;
-        axis = k;
-        const double mA = B.bmin[k * N + a], MA = B.bmax[k * N + a], mB = B.bmin[k * N + b], MB = B.bmax[k * N + b];
-        const bool bInA = cell_wrap_rel(mB, mA, mA + pc.L[k]) <= MA - mA;
-        const bool aInB = cell_wrap_rel(mA, mB, mB + pc.L[k]) <= MB - mB;
-        if (bInA != aInB) {
-          id1 = bInA ? b : a;
-          id2 = bInA ? a : b;
+        if (first) {
+          const uint32_t rb = ent_rg(j, k);
+          first = first_shared((int)(ra[k] & 0xffffu), (int)(ra[k] >> 16), (int)(rb & 0xffffu), (int)(rb >> 16),
+                               pc.n[k]) == X[k];
         }
       }
-      okey = ((uint64_t)axis << 62) | ((uint64_t)(uint32_t)id1 << 31) | (uint64_t)(uint32_t)id2;
-    }
-    int32_t per[3];
-    if (!spatial_overlap_peri(B, pc, id1, id2, per, err)) continue;
-    if (!screened && hash_contains(found, fmask, upair(id1, id2))) continue;
-    const unsigned long long slot = atomicAdd(C.count, 1ull);
-    if ((int64_t)slot < C.cap) {
-      C.a[slot] = id1;
-      C.b[slot] = id2;
-      C.key[slot] = okey;
+      if (!first) continue;
+      const int b = ent_id(j);
+      const float4 cb = ent_box(j, 0);
+      const bool screened = ca.w == 0.0f && cb.w == 0.0f;
+      if (screened) {
+        const float4 hb = ent_box(j, 1);
+        const float d[3] = {ca.x - cb.x, ca.y - cb.y, ca.z - cb.z};
+        const float h[3] = {ha.x + hb.x, ha.y + hb.y, ha.z + hb.z};
+        bool apart = false;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const float dk = d[k] - pc.Lf[k] * rintf(d[k] * pc.iLf[k]);
+          apart = apart || fabsf(dk) > h[k] + pc.mf[k];
+        }
+        if (apart) continue;
+        if (F.cover) {
+          const float4 pa = ent_box(i, 2), pha = ent_box(i, 3), pb = ent_box(j, 2), phb = ent_box(j, 3);
+          const float e[3] = {pa.x - pb.x, pa.y - pb.y, pa.z - pb.z};
+          const float g[3] = {pha.x + phb.x, pha.y + phb.y, pha.z + phb.z};
+          bool was = true;
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            const float dk = e[k] - pc.Lf[k] * rintf(e[k] * pc.iLf[k]);
+            was = was && fabsf(dk) <= g[k] - pc.mf[k];
+          }
+          if (was) continue;
+        }
+        if (peri_in_container(F, a, b)) continue;
+      }
+      peri_candidate(B, pc, po, a, b, screened, F, C, err);
     }
   }
+}
+
+// The periodic candidates are distinct: all of them, with their order keys and indices.
+__global__ void k_cand_all(const Cands C, int64_t nc, uint64_t* okey, uint32_t* oidx) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  okey[c] = C.key[c];
+  oidx[c] = (uint32_t)c;
 }
 
 // Sort keys of the wrapped bounds on the sort axis (first run: coord = cellWrap(bound, 0, L)).

@@ -132,7 +132,8 @@ struct sdm_ctx {
   uint32_t *peCnt = nullptr, *peOff = nullptr;    // cells per body and their exclusive scan
   uint32_t* runEnd = nullptr;                     // [runCap] end, then [runCap] start of each cell's entries
   int64_t runCap = 0;
-  float4* pbox = nullptr;                          // [2N] single-precision boxes of the pair screen
+  float4* pbox = nullptr;                          // [4N] single-precision boxes of the pair screen
+  bool periCover = false;  // the container holds every pair overlapping at the last collider run's bounds
   // per-phase timing
   bool timing = false;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -167,6 +168,7 @@ void alloc_bodies(sdm_ctx* c, int32_t n) {
 void free_pairs(PairArrays& P) {
   dfree(P.r);
   dfree(P.cid);
+  dfree(P.cpos);
   dfree(P.gap);
 }
 
@@ -209,6 +211,7 @@ void ensure_pair_cap(sdm_ctx* c, int64_t need) {
   Q.cap = cap;
   Q.r = dalloc<PairRec>(cap);
   Q.cid = dalloc<int32_t>(2 * cap);
+  Q.cpos = dalloc<int32_t>(2 * cap);
   Q.gap = dalloc<float>(cap);
   c->skipValid = false;
   c->graphGen++;
@@ -347,18 +350,16 @@ void build_csr(sdm_ctx* c) {
     c->csrPair = dalloc<int32_t>(c->csrCap);
   }
   ensure_scratch(c, std::max<int64_t>(m, N + 1));
-  int32_t* deg = (int32_t*)c->u0;
-  HIPCHK(hipMemsetAsync(deg, 0, (N + 1) * sizeof(int32_t), c->st));
+  uint32_t* key = c->u1;
   if (c->np > 0) {
-    uint32_t* key = c->u1;
-    hipLaunchKernelGGL(k_csr_entries, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.cid, c->np, key, c->v0, deg);
+    hipLaunchKernelGGL(k_csr_entries, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.cid, c->np, key, c->v0);
     int bits = 1;
     while ((1ll << bits) < N) bits++;
     // keys are 32-bit: reuse the 64-bit sort buffers through a 32-bit view
     sdmrs::sort_pairs<uint32_t, uint32_t>(key, c->v0, (uint32_t*)c->k1, c->v1, m, bits, c->rsScratch, c->st);
-    hipLaunchKernelGGL(k_csr_finalize, dim3(nblk(m)), dim3(256), 0, c->st, c->P.r, c->v0, m, c->csrPair);
+    hipLaunchKernelGGL(k_csr_finalize, dim3(nblk(m)), dim3(256), 0, c->st, c->v0, m, c->P.cpos, c->csrPair);
   }
-  sdmrs::scan<sdmrs::OpAdd, false, false>((uint32_t*)deg, (uint32_t*)c->csrOff, N + 1, c->rsScratch, c->st);
+  hipLaunchKernelGGL(k_csr_offsets, dim3(nblk(N + 1)), dim3(256), 0, c->st, key, m, N, c->csrOff);
   c->csrValid = true;
 }
 
@@ -437,8 +438,9 @@ void erase_flagged(sdm_ctx* c) {
 }
 
 // InsertionSortCollider::action in a periodic cell (InsertionSortCollider.cpp:191-400 with
-// its periodic branches): bounds, erase by spatialOverlapPeri, then the cell-list search of
-// periodic.h for every pair that overlaps now and is not in the container.
+// its periodic branches): bounds; the cell-list search of periodic.h for every pair that
+// overlaps now and is not in the container; the erase by spatialOverlapPeri (which removes
+// only pairs that do not overlap now, so it may follow the search); the new pairs appended.
 void collider_run_peri(sdm_ctx* c) {
   c->cnt.collider_runs++;
   c->skipValid = false;
@@ -462,7 +464,7 @@ void collider_run_peri(sdm_ctx* c) {
     c->rankMax = dalloc<uint32_t>(N);
     c->peCnt = dalloc<uint32_t>(N + 1);
     c->peOff = dalloc<uint32_t>(N + 1);
-    c->pbox = dalloc<float4>(2 * N);
+    c->pbox = dalloc<float4>(4 * N);
   }
   HIPCHK(hipMemcpyAsync(c->prevMin, c->B.bmin, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->prevMax, c->B.bmax, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
@@ -506,12 +508,6 @@ void collider_run_peri(sdm_ctx* c) {
   ncell = (int64_t)pc.n[0] * pc.n[1] * pc.n[2];
   pc.eps = 1e-9 * std::max(pc.L[0], std::max(pc.L[1], pc.L[2]));
 
-  // ---- conditionalyEraseNonReal ----
-  if (c->np > 0) {
-    hipLaunchKernelGGL(k_erase_flags_peri, dim3(nblk(c->np)), dim3(256), 0, c->st, c->B, c->P, c->np, pc, c->u0, err);
-    erase_flagged(c);
-  }
-
   // ---- first run: ring ranks of the wrapped bounds on the sort axis ----
   PeriOrder po;
   po.init = doInitSort ? 1 : 0;
@@ -528,7 +524,12 @@ void collider_run_peri(sdm_ctx* c) {
   }
 
   // ---- cell lists ----
-  hipLaunchKernelGGL(k_peri_ranges, dim3(nblk(N)), dim3(256), 0, c->st, c->B, pc, c->cellRng, c->peCnt, c->pbox);
+  // cover: the container and its per-body lists are those the last run left (no host write
+  // since), so they hold every pair that overlapped at that run's bounds
+  const int cover = (c->periCover && !doInitSort && c->csrValid) ? 1 : 0;
+  c->periCover = false;
+  hipLaunchKernelGGL(k_peri_ranges, dim3(nblk(N)), dim3(256), 0, c->st, c->B, pc, c->prevMin, c->prevMax, cover,
+                     c->cellRng, c->peCnt, c->pbox);
   const int64_t m = exclusive_total(c, c->peCnt, c->peOff, N);
   ensure_scratch(c, m + 1);
   uint32_t* ckey = (uint32_t*)c->k0;
@@ -544,32 +545,59 @@ void collider_run_peri(sdm_ctx* c) {
     c->runEnd = dalloc<uint32_t>(2 * ncell);
   }
   uint32_t* runStart = c->runEnd + c->runCap;
+  HIPCHK(hipMemsetAsync(c->runEnd, 0, ncell * sizeof(uint32_t), c->st));
+  HIPCHK(hipMemsetAsync(runStart, 0, ncell * sizeof(uint32_t), c->st));
   hipLaunchKernelGGL(k_cell_ends, dim3(nblk(m)), dim3(256), 0, c->st, ckey, m, runStart, c->runEnd);
 
-  // ---- candidates: overlapping pairs not in the container ----
-  ensure_hash(c, c->np + 1024);
-  fill_found(c);
+  // ---- candidates: overlapping pairs not in the container (before its erase step) ----
+  PeriFound F;
+  F.cover = cover;
+  F.off = c->csrOff;
+  F.csr = c->csrPair;
+  F.cid = c->P.cid;
+  F.tab = nullptr;
+  F.mask = 0;
+  if (!cover) {
+    ensure_hash(c, c->np + 1024);
+    const uint64_t mask = (uint64_t)c->hcap - 1;
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->htab, c->hcap, SDM_HEMPTY);
+    if (c->np > 0)
+      hipLaunchKernelGGL(k_hash_insert_pairs, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.cid, c->np, c->htab, mask);
+    F.tab = c->htab;
+    F.mask = mask;
+  }
   int64_t nc = 0;
   for (int pass = 0; pass < 2; pass++) {
     HIPCHK(hipMemsetAsync(c->dcnt + 2, 0, sizeof(unsigned long long), c->st));
     c->C.count = c->dcnt + 2;
-    hipLaunchKernelGGL(k_peri_pairs, dim3(nblk(m)), dim3(256), 0, c->st, c->B, pc, po, c->cellRng, c->pbox, ckey,
-                       cval, runStart, c->runEnd, m, c->htab, (uint64_t)c->hcap - 1, c->C, err);
+    hipLaunchKernelGGL(k_peri_cell_pairs, dim3((unsigned)ncell), dim3(64), 0, c->st, c->B, pc, po, c->cellRng,
+                       c->pbox, cval, runStart, c->runEnd, F, c->C, err);
     nc = (int64_t)read_counter(c, 2);
     if (nc <= c->C.cap) break;
     ensure_cands(c, nc);  // rerun with room for all
   }
+
+  // ---- conditionalyEraseNonReal ----
+  if (c->np > 0) {
+    hipLaunchKernelGGL(k_erase_flags_peri, dim3(nblk(c->np)), dim3(256), 0, c->st, c->B, c->P, c->np, pc, c->u0, err);
+    erase_flagged(c);
+  }
   if ((read_counter(c, 7) & 1) != 0) throw HipError{"body larger than half of the cell size encountered"};
+
+  // ---- append the new pairs in order-key order ----
   if (nc > 0) {
-    const int64_t hc = c->hcap;
-    ensure_hash(c, c->np + nc);
-    if (c->hcap != hc) fill_found(c);
     const int64_t start = c->np;
-    const int64_t ns = append_candidates(c, nc);
-    if (ns > 0)
-      hipLaunchKernelGGL(k_new_celldist, dim3(nblk(ns)), dim3(256), 0, c->st, c->P, start, ns, c->B, pc, err);
+    ensure_scratch(c, nc + 1);
+    hipLaunchKernelGGL(k_cand_all, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, c->k0, c->v0);
+    sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, nc, 64, c->rsScratch, c->st);
+    ensure_pair_cap(c, c->np + nc);
+    hipLaunchKernelGGL(k_new_pairs, dim3(nblk(nc)), dim3(256), 0, c->st, c->P, c->np, nc, c->C, c->v0, c->iter);
+    c->np += nc;
+    c->cnt.pairs_inserted += nc;
+    hipLaunchKernelGGL(k_new_celldist, dim3(nblk(nc)), dim3(256), 0, c->st, c->P, start, nc, c->B, pc, err);
   }
   build_csr(c);
+  c->periCover = true;
 }
 
 // InsertionSortCollider::action (InsertionSortCollider.cpp:191-400), non-periodic.
@@ -1221,6 +1249,7 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
   const FieldInfo* f = find_field(name);
   if (!c || !f || (!src && count > 0)) return fail(c, std::string("sdm_set: bad attribute ") + (name ? name : "(null)"));
   c->skipValid = false;
+  c->periCover = false;
   c->graphGen++;
   try {
     HIPCHK(hipStreamSynchronize(c->st));

@@ -101,9 +101,7 @@ struct PairRec {
 #define SDM_PREC(T, m, k, name) T m[k];
   SDM_PAIR_FIELDS(SDM_PREC)
 #undef SDM_PREC
-  // positions of the interaction's two entries in the per-body lists (rebuilt with them)
-  int32_t csrPos[2];
-  int32_t pad_[16];
+  int32_t pad_[18];
 };
 static_assert(sizeof(PairRec) == 640, "interaction record size");
 
@@ -142,9 +140,11 @@ __host__ __device__ inline uint64_t geo_flags(int material, int blocked, int kin
 struct PairArrays {
   PairRec* r;   // [cap] records; the first np are the interaction container in linIntrs order
   int64_t cap;
-  // Per interaction, coalesced: the two ids (rebuilt with the per-body lists), and the
+  // Per interaction, coalesced: the two ids (kept through every change of the container),
+  // the positions of its two entries in the per-body lists (rebuilt with them), and the
   // separation margin of a settled apart pair (see k_pairs_prep), NaN when not settled.
   int32_t* cid;
+  int32_t* cpos;
   float* gap;
 };
 
