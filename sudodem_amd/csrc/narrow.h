@@ -62,43 +62,56 @@ SDM_HD V3<T> support(const Shape<T>& s, const M3<T>& G, V3<T> dir) {
 //  * x^y for x >= 0, y > 0 as 2^(y log2 x): hardware log2, one Newton step on 2^L = x for
 //    the logarithm, y L as an exact product (fma residual), and hardware exp2 with a first
 //    order correction for the low part -- a few ulp, against ~150 instructions for powf.
+//  * sgn(x) * p for p >= 0 as copysign(p, x) (equal but for the sign of a zero result).
+//  * the exponents 1/(2 - eps) and 2 - eps of a body are computed once per pair (SupF).
+// The evaluation has no branches: special cases are selects after the arithmetic.
 // ---------------------------------------------------------------------------------------
 SDM_HD float fpow(float x, float y) {
-  if (!(x > 0.0f)) return 0.0f;
   const float L0 = __builtin_amdgcn_logf(x);               // log2 x, hardware
   const float r = fmaf(x, __builtin_amdgcn_exp2f(-L0), -1.0f);  // x 2^-L0 - 1, |r| ~ 1e-7
   const float Llo = r * 1.44269504088896341f;              // r / ln 2: Newton correction of L0
   const float p = y * L0;
   const float plo = fmaf(y, L0, -p) + y * Llo;             // y L = p + plo
   const float z = __builtin_amdgcn_exp2f(p);
-  return fmaf(z, plo * 0.693147180559945309f, z);          // 2^p (1 + plo ln 2)
+  const float v = fmaf(z, plo * 0.693147180559945309f, z);  // 2^p (1 + plo ln 2)
+  return x > 0.0f ? v : 0.0f;                              // 0 for x <= 0 (and NaN)
 }
 
-// (cos, sin) of atan2(a, b) without the angle.
+// (cos, sin) of atan2(a, b) without the angle: (b, a) divided by the larger magnitude,
+// then normalised.
 SDM_HD void unit_angle(float a, float b, float& c, float& s) {
   const float m = fmaxf(fabsf(a), fabsf(b));
-  if (!(m > 0.0f)) {
-    c = 1.0f;
-    s = 0.0f;
-    return;
-  }
-  const float im = 1.0f / m;
+  const bool ok = m > 0.0f;
+  const float im = 1.0f / (ok ? m : 1.0f);
   const float an = a * im, bn = b * im;
-  const float ih = rsqrtf(an * an + bn * bn);
-  c = bn * ih;
-  s = an * ih;
+  const float ih = __builtin_amdgcn_rsqf(an * an + bn * bn);  // argument in [1, 2]: no rescaling
+  c = ok ? bn * ih : 1.0f;
+  s = ok ? an * ih : 0.0f;
 }
 
-SDM_HD V3<float> support_f(const Shape<float>& s, const M3<float>& G, V3<float> dir) {
+// A body's shape with the exponents of the float support map.
+struct SupF {
+  float rx, ry, rz, eps1, eps2;
+  float e1, e2, t1;  // 1 / (2 - eps1), 1 / (2 - eps2), 2 - eps1
+};
+SDM_HD SupF sup_f(const Shape<float>& s) {
+  SupF f;
+  f.rx = s.rx; f.ry = s.ry; f.rz = s.rz; f.eps1 = s.eps1; f.eps2 = s.eps2;
+  f.e1 = 1.0f / (2.0f - s.eps1);
+  f.e2 = 1.0f / (2.0f - s.eps2);
+  f.t1 = 2.0f - s.eps1;
+  return f;
+}
+
+SDM_HD V3<float> support_f(const SupF& s, const M3<float>& G, V3<float> dir) {
   const V3<float> n = mulT(G, dir);
-  const float e1 = 1.0f / (2.0f - s.eps1), e2 = 1.0f / (2.0f - s.eps2);
   const float ax = fabsf(s.rx * n.x);
   float c0, s0, c1, s1;
-  unit_angle(sgn(n.y) * fpow(fabsf(s.ry * n.y), e1), sgn(n.x) * fpow(ax, e1), c0, s0);
-  unit_angle(sgn(n.z) * fpow(fabsf(s.rz * n.z * fpow(fabsf(c0), 2.0f - s.eps1)), e2), fpow(ax, e2), c1, s1);
+  unit_angle(copysignf(fpow(fabsf(s.ry * n.y), s.e1), n.y), copysignf(fpow(ax, s.e1), n.x), c0, s0);
+  unit_angle(copysignf(fpow(fabsf(s.rz * n.z * fpow(fabsf(c0), s.t1)), s.e2), n.z), fpow(ax, s.e2), c1, s1);
   const float pc1 = fpow(fabsf(c1), s.eps2);
-  const V3<float> q = mk3(sgn(c0) * s.rx * fpow(fabsf(c0), s.eps1) * pc1, sgn(s0) * s.ry * fpow(fabsf(s0), s.eps1) * pc1,
-                          sgn(s1) * s.rz * fpow(fabsf(s1), s.eps2));
+  const V3<float> q = mk3(copysignf(s.rx, c0) * fpow(fabsf(c0), s.eps1) * pc1,
+                          copysignf(s.ry, s0) * fpow(fabsf(s0), s.eps1) * pc1, copysignf(s.rz, s1) * fpow(fabsf(s1), s.eps2));
   return mul(G, q);
 }
 
@@ -158,26 +171,43 @@ template <typename T>
 struct NMGeo {
   PairGeo<T> g;
   M3<T> gC;  // CRotationMat frame
+  SupF f1, f2;  // float form: the shapes with their exponents (nm_prepare)
 };
+
+// Per-pair set-up of the float support map.
+template <typename T>
+SDM_HD void nm_prepare(NMGeo<T>& n) {
+  if constexpr (sizeof(T) == 4) {
+    n.f1 = sup_f(n.g.s1);
+    n.f2 = sup_f(n.g.s2);
+  }
+}
 
 // Disfun (Superquadrics_Ig2.cpp:1624-1678): distance |p2-p1| and the contact test.
 template <typename T>
 SDM_HD T nm_eval(const NMGeo<T>& n, T a0, T a1, V3<T>& p1, V3<T>& p2, bool& touching, int& nev) {
   nev++;
-  const V3<T> contact = mul(n.gC, mk3(xcos(a0) * xcos(a1), xsin(a0) * xcos(a1), xsin(a1)));
   if constexpr (sizeof(T) == 4) {
-    p1 = support_f(n.g.s1, n.g.G1, contact) + n.g.pos1;
-    p2 = support_f(n.g.s2, n.g.G2, mk3(-contact.x, -contact.y, -contact.z)) + n.g.pos2;
+    float s0, c0, s1, c1;  // one argument reduction per angle
+    sincosf(a0, &s0, &c0);
+    sincosf(a1, &s1, &c1);
+    const V3<T> contact = mul(n.gC, mk3(c0 * c1, s0 * c1, s1));
+    p1 = support_f(n.f1, n.g.G1, contact) + n.g.pos1;
+    p2 = support_f(n.f2, n.g.G2, mk3(-contact.x, -contact.y, -contact.z)) + n.g.pos2;
+    V3<T> d = p2 - p1;
+    if (dot(contact, d) > T(0)) touching = false;
+    return norm(d);
   } else {
+    const V3<T> contact = mul(n.gC, mk3(xcos(a0) * xcos(a1), xsin(a0) * xcos(a1), xsin(a1)));
     p1 = support(n.g.s1, n.g.G1, contact) + n.g.pos1;
     T ph0, ph1;
     V3<T> c2 = mulT(n.g.G2, contact);
     normal2phi(n.g.s2, mk3(T(-1) * c2.x, T(-1) * c2.y, T(-1) * c2.z), ph0, ph1);
     p2 = mul(n.g.G2, get_surface(n.g.s2, ph0, ph1)) + n.g.pos2;
+    V3<T> d = p2 - p1;
+    if (dot(contact, d) > T(0)) touching = false;
+    return norm(d);
   }
-  V3<T> d = p2 - p1;
-  if (dot(contact, d) > T(0)) touching = false;
-  return norm(d);
 }
 
 // Curvatures (Superquadrics.cpp:131-255).  Keeps the reference's expressions: K ignores the

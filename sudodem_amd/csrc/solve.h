@@ -241,6 +241,7 @@ __global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
         const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
         pair_geo<T>(r1, r2, kRel, n.g);
         n.gC = cast9<T>(crotation_mat(r1.pos, r2.pos));
+        nm_prepare(n);
         const double a0 = A.S.a0[w], a1 = A.S.a1[w], del = A.S.del[w];
         v[0][0] = a0; v[0][1] = a1;
         v[1][0] = a0 + del; v[1][1] = a1;
@@ -410,6 +411,7 @@ __device__ __forceinline__ bool nm_first_eval(const SolveArgs& A, int64_t w) {
   const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
   pair_geo<T>(r1, r2, kRel, n.g);
   n.gC = cast9<T>(crotation_mat(r1.pos, r2.pos));
+  nm_prepare(n);
   V3<T> q1, q2;
   bool touching = true;
   int nev = 0;
