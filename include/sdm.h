@@ -152,6 +152,47 @@ int sdm_mean_coordination(sdm_ctx* ctx, double* cn);
 int sdm_stress(sdm_ctx* ctx, double volume, double sigma[9]);
 int sdm_body_forces(sdm_ctx* ctx, const int32_t* ids, int32_t n, double* forces);
 
+/* Wall stress servo: CompressionEngine (pkg/dem/Compression.cpp) on a cubic specimen in its
+ * consolidation branch as the reference compiles it (NEW_SCHEME and SOLE_GAIN,
+ * Compression.hpp:26-29): action_cubic (:395-481) -> generalConsolidation (:324-345) ->
+ * servo (:1252-1316), get_gain (:1125-1220), consol_ss (:1330-1345), getStress (:764-841),
+ * Shop::unbalancedForce (pkg/dem/Shop_01.cpp:148-189).  Bodies 0-5 are the walls left, right,
+ * front, back, bottom, top (x-, x+, y-, y+, z-, z+), as the engine assumes.  It runs inside
+ * every step after the force sums and before the integrator (the engine order of
+ * examples/SudoDEM3D/example2_consol.py), with no host round trip: wall stresses, gains
+ * every iterate_num_max steps (stiffness summed in interaction order) and the stability
+ * check every solve_num_max steps.  When the check finds the target reached and the
+ * unbalanced force below unbf_tol, the integrator quiets the system in that step
+ * (NewtonIntegrator::quiet_system_flag: velocities zeroed, no move) and sdm_step returns
+ * after it (stopAtIter = iter + 1). */
+typedef struct sdm_servo {
+  double goal[3];           /* goalx, goaly, goalz: target stresses [Pa] */
+  double max_vel;           /* max_vel: wall speed limit [m/s] */
+  double gain_alpha;        /* gain_alpha */
+  double f_threshold;       /* f_threshold: |mean stress - goalx| < goalx f_threshold ... */
+  double fmin_threshold;    /* fmin_threshold: ... and deviator / mean < fmin_threshold ... */
+  double unbf_tol;          /* unbf_tol: ... and unbalanced force < unbf_tol: completed */
+  int32_t iterate_num_max;  /* iterate_num_max: steps between gain updates (100) */
+  int32_t solve_num_max;    /* solve_num_max: steps between stability checks (2000) */
+  int32_t activated[4];     /* left/right/front/back_wall_activated (the z walls follow z_servo) */
+  int32_t z_servo;          /* z_servo: stress control in z (the consolidation branch needs 1) */
+} sdm_servo;
+typedef struct sdm_servo_state {
+  double wall_stress[6];    /* wsxx_left, wsxx_right, wsyy_front, wsyy_back, wszz_bottom, wszz_top */
+  double stress[3];         /* wsxx, wsyy, wszz (means of the two walls of each axis) */
+  double gain[6];           /* gain_x, gain_x1, gain_y, gain_y1, gain_z, gain_z1 */
+  double stiffness[6];      /* avg_xstiff, avg_xstiff1, avg_ystiff, avg_ystiff1, avg_zstiff, avg_zstiff1 */
+  double size[3];           /* width, depth, height at the last wall-stress evaluation */
+  double mean_stress;       /* getStress() of the last stability check: mean ... */
+  double deviator_stress;   /* ... and deviatoric stress (0 before the first check) */
+  double unbalanced_force;  /* UnbalancedForce of the last check that reached the target (1 before) */
+  int32_t iterate_num, solve_num;
+  int64_t completed_iter;   /* Scene::iter of the step that completed consolidation, -1 if none */
+} sdm_servo_state;
+/* servo = NULL switches the servo off.  Switching it on starts it afresh (firstRun). */
+int sdm_set_servo(sdm_ctx* ctx, const sdm_servo* servo);
+int sdm_get_servo(sdm_ctx* ctx, sdm_servo_state* out);
+
 /* Launch shapes of the step (no counterpart in the reference).  None changes a result: they
  * only schedule the same per-interaction work differently.  sudodem_amd/autotune.py picks
  * them by timing a workload's own steps. */

@@ -351,6 +351,19 @@ struct sdm_ctx {
   int32_t initSortStable = 0;
   // NewtonIntegrator state
   double maxVelocitySq = NaN;
+  // CompressionEngine (consolidation branch, SOLE_GAIN + NEW_SCHEME)
+  struct Servo {
+    bool on = false, firstRun = true, quiet = false;
+    sdm_servo p;
+    int32_t iterate_num = 0, solve_num = 0;
+    double width = 0, depth = 0, height = 0, x_area = 0, y_area = 0, z_area = 0;
+    double ws[6] = {0, 0, 0, 0, 0, 0};  // wsxx_left, wsxx_right, wsyy_front, wsyy_back, wszz_bottom, wszz_top
+    double wsxx = 0, wsyy = 0, wszz = 0;
+    double stiff[6] = {0, 0, 0, 0, 0, 0}, gain[6] = {0, 0, 0, 0, 0, 0};
+    double mean = 0, dev = 0, unbf = 1;
+    int64_t completedIter = -1;
+    bool completedNow = false;
+  } servo;
   // Scene
   int64_t iter = 0;
   double time = 0;
@@ -1682,6 +1695,10 @@ double newton_body(sdm_ctx* c, Body& b) {
       }
     }
   }
+  if (c->servo.quiet) {  // quiet_system_flag (NewtonIntegrator.cpp:246-250): no move this step
+    set3(b.vel, 0, 0, 0);
+    set3(b.angVel, 0, 0, 0);
+  } else {
   bool sphereRotate = !useAspherical || b.isSphere;
   if (!sphereRotate) {
     // leapfrogSuperquadricsRotate (:391-429) -> leapfrogAsphericalRotate (:306-335)
@@ -1719,6 +1736,7 @@ double newton_body(sdm_ctx* c, Body& b) {
     quat_normalize(b.ori);
   }
   for (int i = 0; i < 3; i++) b.pos[i] += b.vel[i] * dt;  // leapfrogTranslate (:306-317)
+  }
   // saveMaximaDisplacement (:72-85)
   if (!b.hasBound) return 0;
   double disp[3] = {b.pos[0] - b.refPos[0], b.pos[1] - b.refPos[1], b.pos[2] - b.refPos[2]};
@@ -1741,6 +1759,162 @@ void newton(sdm_ctx* c) {
   c->maxVelocitySq = mx;
 }
 
+// ---------------------------------------------------------------------------------------
+// CompressionEngine, cubic specimen, consolidation (pkg/dem/Compression.cpp with SOLE_GAIN and
+// NEW_SCHEME defined, Compression.hpp:26-29).  Walls: bodies 0-5 = left, right, front, back,
+// bottom, top.
+// ---------------------------------------------------------------------------------------
+// updateBoxSize (:99-113)
+void servo_box(sdm_ctx* c) {
+  auto& S = c->servo;
+  const auto& B = c->bodies;
+  S.width = B[1].pos[0] - B[0].pos[0];
+  S.depth = B[3].pos[1] - B[2].pos[1];
+  S.height = B[5].pos[2] - B[4].pos[2];
+  S.x_area = S.depth * S.height;
+  S.y_area = S.width * S.height;
+  S.z_area = S.depth * S.width;
+}
+// consol_ss, SOLE_GAIN branch (:1330-1345)
+void servo_consol_ss(sdm_ctx* c) {
+  auto& S = c->servo;
+  const auto& B = c->bodies;
+  servo_box(c);
+  S.ws[1] = B[1].force[0] / S.x_area;
+  S.ws[0] = -B[0].force[0] / S.x_area;
+  S.wsxx = 0.5 * (S.ws[0] + S.ws[1]);
+  S.ws[3] = B[3].force[1] / S.y_area;
+  S.ws[2] = -B[2].force[1] / S.y_area;
+  S.wsyy = 0.5 * (S.ws[3] + S.ws[2]);
+  S.ws[5] = B[5].force[2] / S.z_area;
+  S.ws[4] = -B[4].force[2] / S.z_area;
+  S.wszz = 0.5 * (S.ws[5] + S.ws[4]);
+}
+// get_gain, SOLE_GAIN branch (:1125-1220): kn summed per wall over the real interactions in
+// container order (the wall is the smaller id).
+void servo_get_gain(sdm_ctx* c) {
+  auto& S = c->servo;
+  for (int j = 0; j < 6; j++) S.stiff[j] = 0.0;
+  for (const Pair& I : c->intrs) {
+    if (!(I.hasGeom && I.hasPhys)) continue;
+    const int id = I.id1 < I.id2 ? I.id1 : I.id2;
+    if (id < 6) S.stiff[id] += I.kn;
+  }
+  const double a = S.p.gain_alpha;
+  S.gain[0] = a * S.x_area / S.stiff[0];
+  S.gain[2] = a * S.y_area / S.stiff[2];
+  S.gain[4] = a * S.z_area / S.stiff[4];
+  S.gain[1] = a * S.x_area / S.stiff[1];
+  S.gain[3] = a * S.y_area / S.stiff[3];
+  S.gain[5] = a * S.z_area / S.stiff[5];
+}
+// Mathr::Sign(u) * std::min(fabs(u), lim)
+double servo_clamp(double u, double lim) {
+  const double a = std::abs(u);
+  return Sign(u) * std::min(a, lim);
+}
+// servo, SOLE_GAIN branch (:1252-1316)
+void servo_move(sdm_ctx* c, double dt) {
+  auto& S = c->servo;
+  auto& B = c->bodies;
+  servo_consol_ss(c);
+  const double lim = S.p.max_vel * dt;
+  if (S.p.activated[0]) B[0].pos[0] += -servo_clamp(S.gain[0] * (S.ws[0] - S.p.goal[0]), lim);
+  if (S.p.activated[1]) B[1].pos[0] += servo_clamp(S.gain[1] * (S.ws[1] - S.p.goal[0]), lim);
+  if (S.p.activated[2]) B[2].pos[1] += -servo_clamp(S.gain[2] * (S.ws[2] - S.p.goal[1]), lim);
+  if (S.p.activated[3]) B[3].pos[1] += servo_clamp(S.gain[3] * (S.ws[3] - S.p.goal[1]), lim);
+  if (S.p.z_servo) {
+    B[4].pos[2] += -servo_clamp(S.gain[4] * (S.ws[4] - S.p.goal[2]), lim);
+    B[5].pos[2] += servo_clamp(S.gain[5] * (S.ws[5] - S.p.goal[2]), lim);
+  }
+}
+// getStressTensor (:764-829) and getStress (:831-841): sum over real interactions with a
+// non-zero normal force of (fn + fs) (pos2 - pos1)^T over the box volume; mean = trace / 3,
+// deviator = |Sigma - mean I| sqrt(1.5).  Sums in the order of Eigen's unvectorised redux.
+void servo_get_stress(sdm_ctx* c, double& mean, double& dev) {
+  auto& S = c->servo;
+  servo_box(c);
+  const double vol = S.height * S.z_area;
+  double Sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // row-major Sigma(r, c)
+  for (const Pair& I : c->intrs) {
+    if (!(I.hasGeom && I.hasPhys)) continue;
+    if (norm3(I.normalForce) == 0) continue;
+    const double* x1 = c->bodies[I.id1].pos;
+    const double* x2 = c->bodies[I.id2].pos;
+    double f[3], l[3];
+    for (int k = 0; k < 3; k++) {
+      f[k] = I.normalForce[k] + I.shearForce[k];
+      l[k] = x2[k] - x1[k];
+    }
+    for (int r = 0; r < 3; r++)
+      for (int k = 0; k < 3; k++) Sg[3 * r + k] += f[r] * l[k];
+  }
+  for (int k = 0; k < 9; k++) Sg[k] /= vol;
+  mean = (Sg[0] + (Sg[4] + Sg[8])) / 3.0;
+  Sg[0] -= mean;
+  Sg[4] -= mean;
+  Sg[8] -= mean;
+  double q[9];  // squares in column-major order: (0,0), (1,0), (2,0), (0,1), ...
+  for (int col = 0; col < 3; col++)
+    for (int row = 0; row < 3; row++) q[3 * col + row] = Sg[3 * row + col] * Sg[3 * row + col];
+  const double sq = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + (q[7] + q[8])));
+  dev = std::sqrt(sq) * std::sqrt(1.5);
+}
+// Shop::unbalancedForce(false) (pkg/dem/Shop_01.cpp:148-189): sum over bodies that are not
+// fully blocked of |F + m g| over twice the sum over real interactions with a non-zero normal
+// force of |fn + fs|.
+double unbalanced_force(sdm_ctx* c) {
+  const double* g = c->prm.gravity;
+  double sumF = 0;
+  for (const Body& b : c->bodies) {
+    if ((unsigned)b.blockedDOFs == DOF_ALL) continue;
+    const double v[3] = {b.force[0] + b.mass * g[0], b.force[1] + b.mass * g[1], b.force[2] + b.mass * g[2]};
+    sumF += norm3(v);
+  }
+  double sumC = 0;
+  for (const Pair& I : c->intrs) {
+    if (!(I.hasGeom && I.hasPhys)) continue;
+    if (norm3(I.normalForce) == 0) continue;
+    const double v[3] = {I.normalForce[0] + I.shearForce[0], I.normalForce[1] + I.shearForce[1],
+                         I.normalForce[2] + I.shearForce[2]};
+    sumC += norm3(v);
+  }
+  sumC *= 2.0;
+  return sumF / sumC;
+}
+// action_cubic's first run (:397-480, the parts the servo uses) and generalConsolidation with
+// NEW_SCHEME (:324-345).
+void servo_action(sdm_ctx* c) {
+  auto& S = c->servo;
+  S.completedNow = false;
+  if (S.firstRun) {
+    servo_box(c);
+    S.iterate_num = 0;
+    S.solve_num = 0;
+    servo_get_gain(c);
+    S.firstRun = false;
+  }
+  S.iterate_num += 1;
+  S.solve_num += 1;
+  servo_move(c, c->prm.dt);
+  if (S.iterate_num >= S.p.iterate_num_max) {
+    S.iterate_num = 0;
+    servo_get_gain(c);
+  }
+  if (S.solve_num >= S.p.solve_num_max) {
+    S.solve_num = 0;
+    servo_get_stress(c, S.mean, S.dev);
+    if ((std::abs(S.mean - S.p.goal[0]) < S.p.goal[0] * S.p.f_threshold) && (S.dev / S.mean < S.p.fmin_threshold)) {
+      S.unbf = unbalanced_force(c);
+      if (S.unbf < S.p.unbf_tol) {  // stopAtIter = iter + 1; newton->quiet_system_flag = true
+        S.completedIter = c->iter;
+        S.completedNow = true;
+        S.quiet = true;
+      }
+    }
+  }
+}
+
 // Scene::moveToNextTimeStep (core/Scene.cpp:70-138) with the canonical engine list.
 void one_step(sdm_ctx* c) {
   for (auto& b : c->bodies) {  // ForceResetter (pkg/common/ForceResetter.hpp:9-12)
@@ -1749,7 +1923,9 @@ void one_step(sdm_ctx* c) {
   }
   if (collider_is_activated(c)) collider_action(c);
   interaction_loop(c);
+  if (c->servo.on) servo_action(c);
   newton(c);
+  c->servo.quiet = false;
   c->iter++;
   c->time += c->prm.dt;
   c->cnt.steps++;
@@ -2018,13 +2194,56 @@ int sdmcpu_step(sdm_ctx* c, int32_t nsteps) {
     if (b.specimen != 0) return fail(c, "the oracle steps one specimen (body.specimen must be 0)");
     if (c->prm.periodic && b.kind == SDM_BODY_WALL) return fail(c, "walls are not supported in a periodic cell");
   }
+  if (c->servo.on) {
+    if (c->prm.periodic) return fail(c, "the wall servo needs the six walls of a cubic specimen, not a periodic cell");
+    if (c->bodies.size() < 6) return fail(c, "the wall servo needs bodies 0-5 to be the walls");
+    for (int i = 0; i < 6; i++)
+      if (c->bodies[i].kind != SDM_BODY_WALL) return fail(c, "the wall servo needs bodies 0-5 to be the walls");
+  }
   for (int s = 0; s < nsteps; s++) {
     one_step(c);
     if (c->periErr) {
       c->periErr = false;
       return fail(c, "body larger than half of the cell size encountered");
     }
+    if (c->servo.completedNow) break;  // stopAtIter = iter + 1
   }
+  return 0;
+}
+
+int sdmcpu_set_servo(sdm_ctx* c, const sdm_servo* s) {
+  if (!c) return -1;
+  if (!s) {
+    c->servo.on = false;
+    return 0;
+  }
+  if (s->iterate_num_max < 1 || s->solve_num_max < 1) return fail(c, "sdm_set_servo: intervals must be >= 1");
+  c->servo = sdm_ctx::Servo();
+  c->servo.p = *s;
+  c->servo.on = true;
+  return 0;
+}
+
+int sdmcpu_get_servo(sdm_ctx* c, sdm_servo_state* o) {
+  if (!c || !o) return fail(c, "bad arguments");
+  const auto& S = c->servo;
+  for (int k = 0; k < 6; k++) {
+    o->wall_stress[k] = S.ws[k];
+    o->gain[k] = S.gain[k];
+    o->stiffness[k] = S.stiff[k];
+  }
+  o->stress[0] = S.wsxx;
+  o->stress[1] = S.wsyy;
+  o->stress[2] = S.wszz;
+  o->size[0] = S.width;
+  o->size[1] = S.depth;
+  o->size[2] = S.height;
+  o->mean_stress = S.mean;
+  o->deviator_stress = S.dev;
+  o->unbalanced_force = S.unbf;
+  o->iterate_num = S.iterate_num;
+  o->solve_num = S.solve_num;
+  o->completed_iter = S.completedIter;
   return 0;
 }
 

@@ -38,6 +38,8 @@ int sdmcpu_set_tuning(sdm_ctx* ctx, const sdm_tuning* t);
 int sdmcpu_mean_coordination(sdm_ctx* ctx, double* cn);
 int sdmcpu_stress(sdm_ctx* ctx, double volume, double sigma[9]);
 int sdmcpu_body_forces(sdm_ctx* ctx, const int32_t* ids, int32_t n, double* forces);
+int sdmcpu_set_servo(sdm_ctx* ctx, const sdm_servo* servo);
+int sdmcpu_get_servo(sdm_ctx* ctx, sdm_servo_state* out);
 int sdmcpu_get_tuning(sdm_ctx* ctx, sdm_tuning* t);
 int sdmcpu_get_timers(sdm_ctx* ctx, double out[8]);
 

@@ -54,6 +54,25 @@ class sdm_tuning(ctypes.Structure):
                 ("misc_blocks_per_cu", ctypes.c_int32), ("graph", ctypes.c_int32)]
 
 
+class sdm_servo(ctypes.Structure):
+    _fields_ = [("goal", ctypes.c_double * 3), ("max_vel", ctypes.c_double), ("gain_alpha", ctypes.c_double),
+                ("f_threshold", ctypes.c_double), ("fmin_threshold", ctypes.c_double), ("unbf_tol", ctypes.c_double),
+                ("iterate_num_max", ctypes.c_int32), ("solve_num_max", ctypes.c_int32),
+                ("activated", ctypes.c_int32 * 4), ("z_servo", ctypes.c_int32)]
+
+
+class sdm_servo_state(ctypes.Structure):
+    _fields_ = [("wall_stress", ctypes.c_double * 6), ("stress", ctypes.c_double * 3), ("gain", ctypes.c_double * 6),
+                ("stiffness", ctypes.c_double * 6), ("size", ctypes.c_double * 3), ("mean_stress", ctypes.c_double),
+                ("deviator_stress", ctypes.c_double), ("unbalanced_force", ctypes.c_double),
+                ("iterate_num", ctypes.c_int32), ("solve_num", ctypes.c_int32), ("completed_iter", ctypes.c_int64)]
+
+
+# CompressionEngine attribute defaults (pkg/dem/Compression.hpp:160-210) for the fields of sdm_servo
+SERVO_DEFAULTS = dict(goal=(0.0, 0.0, 0.0), max_vel=1.0, gain_alpha=0.5, f_threshold=0.01, fmin_threshold=0.01,
+                      unbf_tol=0.01, iterate_num_max=100, solve_num_max=2000, activated=(1, 1, 1, 1), z_servo=1)
+
+
 class SdmError(RuntimeError):
     pass
 
@@ -101,6 +120,8 @@ class SdmLibrary:
         self.set_tuning = fn("set_tuning", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_tuning)])
         self.get_tuning = fn("get_tuning", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_tuning)])
         self.get_timers = fn("get_timers", ctypes.c_int, [c_ctx, ctypes.POINTER(ctypes.c_double)])
+        self.set_servo = fn("set_servo", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_servo)])
+        self.get_servo = fn("get_servo", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_servo_state)])
 
     def fields(self):
         return self.field_list().decode().split()
@@ -192,6 +213,34 @@ class Context:
         out = np.zeros((len(ids), 3))
         self.lib.check(self.lib.body_forces(self.ptr, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(ids),
                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), self.ptr)
+        return out
+
+    def set_servo(self, enabled=True, **kw):
+        """Wall stress servo (CompressionEngine, cubic consolidation); keyword arguments are the
+        engine's attributes (goal=(goalx, goaly, goalz), max_vel, gain_alpha, ...)."""
+        if not enabled:
+            self.lib.check(self.lib.set_servo(self.ptr, None), self.ptr)
+            return
+        p = {**SERVO_DEFAULTS, **kw}
+        unknown = set(p) - set(SERVO_DEFAULTS)
+        if unknown:
+            raise SdmError("unknown servo attributes: %s" % sorted(unknown))
+        s = sdm_servo()
+        s.goal = (ctypes.c_double * 3)(*[float(x) for x in p["goal"]])
+        s.activated = (ctypes.c_int32 * 4)(*[int(x) for x in p["activated"]])
+        for k in ("max_vel", "gain_alpha", "f_threshold", "fmin_threshold", "unbf_tol"):
+            setattr(s, k, float(p[k]))
+        for k in ("iterate_num_max", "solve_num_max", "z_servo"):
+            setattr(s, k, int(p[k]))
+        self.lib.check(self.lib.set_servo(self.ptr, ctypes.byref(s)), self.ptr)
+
+    def servo_state(self) -> dict:
+        st = sdm_servo_state()
+        self.lib.check(self.lib.get_servo(self.ptr, ctypes.byref(st)), self.ptr)
+        out = {}
+        for k, t in sdm_servo_state._fields_:
+            v = getattr(st, k)
+            out[k] = np.array(list(v)) if hasattr(t, "_length_") else v
         return out
 
     def tuning(self) -> dict:

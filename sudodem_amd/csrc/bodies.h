@@ -163,6 +163,7 @@ struct NewtonArgs {
   double g[3];
   int exactAsph;
   int* maxDisp;  // 0: no bounded body, 1: all inside their sweep (0.5), 2: some outside (2)
+  const int32_t* quiet;  // quiet_system_flag (set by the wall servo), nullptr: never
 };
 
 __device__ __forceinline__ Q4 dotq(V3<double> w, Q4 Q) {  // NewtonIntegrator::DotQ (:470-477)
@@ -244,7 +245,12 @@ __global__ __launch_bounds__(256) void k_newton(NewtonArgs A) {
     }
     Q4 ori;
     ori.w = B.ori[i]; ori.x = B.ori[N + i]; ori.y = B.ori[2 * N + i]; ori.z = B.ori[3 * N + i];
-    if (useAsph && !isSphere) {
+    // quiet_system_flag (NewtonIntegrator.cpp:246-250): velocities zeroed, no rotation or move
+    const bool quiet = A.quiet && *A.quiet;
+    if (quiet) {
+      vel = mk3(0.0, 0.0, 0.0);
+      angVel = vel;
+    } else if (useAsph && !isSphere) {
       // leapfrogSuperquadricsRotate (:391-429) -> leapfrogAsphericalRotate (:306-335), with the
       // rotation matrix cached from the previous orientation.
       const M3<double> Am = ld9(B.rotL, i, N);
@@ -292,7 +298,7 @@ __global__ __launch_bounds__(256) void k_newton(NewtonArgs A) {
     st3(B.vel, i, N, vel);
     st3(B.angVel, i, N, angVel);
     V3<double> pos = ld3(B.pos, i, N);
-    pos = pos + vel * dt;  // leapfrogTranslate (:306-317)
+    if (!quiet) pos = pos + vel * dt;  // leapfrogTranslate (:306-317)
     st3(B.pos, i, N, pos);
     double* gp = B.geo + i * GEO_STRIDE;
     gp[GEO_POS] = pos.x; gp[GEO_POS + 1] = pos.y; gp[GEO_POS + 2] = pos.z;

@@ -500,6 +500,11 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
             P.r[p].id2[0] = id2;
             P.cid[2 * p] = id1;
             P.cid[2 * p + 1] = id2;
+            // the list entry of the new id1 is the one built for the old id2: the law marks
+            // entry cpos[2p] as side 0 (force +F) and cpos[2p + 1] as side 1 (-F)
+            const int32_t e = P.cpos[2 * p];
+            P.cpos[2 * p] = P.cpos[2 * p + 1];
+            P.cpos[2 * p + 1] = e;
           }
         }
       }

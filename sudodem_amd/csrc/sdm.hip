@@ -22,6 +22,7 @@
 #include "observe.h"
 #include "periodic.h"
 #include "radix.h"
+#include "servo.h"
 
 #define SDM_VERSION "sdm-hip 1.0 (gfx950)"
 
@@ -101,6 +102,12 @@ struct sdm_ctx {
   // [4] solver evaluations, [5] other evaluations, [6] misc list, [7] touching list
   unsigned long long* dcnt = nullptr;
   int* dMaxDisp = nullptr;
+  // wall stress servo (servo.h): on/off, its device state, the check's block partials
+  bool servoOn = false;
+  bool servoDone = false;  // the last step completed the consolidation (stopAtIter = iter + 1)
+  ServoDev* dServo = nullptr;
+  double* servoPart = nullptr;
+  int64_t servoPartCap = 0;
   // narrowphase work queue (indexed by queue slot)
   SolveIO S;
   int32_t* retryList = nullptr;
@@ -841,10 +848,26 @@ void step_body(sdm_ctx* c, const StepArgs& A) {
   NA.g[2] = c->prm.gravity[2];
   NA.exactAsph = c->prm.exact_aspherical_rot;
   NA.maxDisp = c->dMaxDisp;
+  NA.quiet = c->servoOn ? &c->dServo->quiet : nullptr;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, (uint32_t*)c->dMaxDisp, 1);
   hipLaunchKernelGGL(k_force_sum, dim3((unsigned)((c->B.n + 15) / 16)), dim3(256), 0, c->st, c->B, c->P, c->csrOff,
                      c->csrPair);
+  if (c->servoOn) {
+    // CompressionEngine between InteractionLoop and NewtonIntegrator; every decision on the device
+    ServoDev* S = c->dServo;
+    hipLaunchKernelGGL(k_servo_begin, dim3(1), dim3(64), 0, c->st, S, c->B);
+    hipLaunchKernelGGL(k_servo_gain, dim3(6), dim3(64), 0, c->st, S, c->P, c->csrOff, c->csrPair);
+    hipLaunchKernelGGL(k_servo_step, dim3(1), dim3(64), 0, c->st, S, c->B, c->prm.dt);
+    hipLaunchKernelGGL(k_servo_gain, dim3(6), dim3(64), 0, c->st, S, c->P, c->csrOff, c->csrPair);
+    const int64_t nb = (int64_t)nblk(std::max<int64_t>({(int64_t)1, A.np, (int64_t)c->B.n}));
+    hipLaunchKernelGGL(k_servo_check_partial, dim3((unsigned)nb), dim3(256), 0, c->st, S, c->P,
+                       (const unsigned long long*)(c->dcnt + DC_NP), c->B, c->prm.gravity[0], c->prm.gravity[1],
+                       c->prm.gravity[2], c->servoPart);
+    hipLaunchKernelGGL(k_servo_check_final, dim3(1), dim3(64), 0, c->st, S, c->B, c->servoPart, nb,
+                       (const unsigned long long*)(c->dcnt + DC_NP + 1));
+  }
   hipLaunchKernelGGL(k_newton, dim3(nblk(c->B.n)), dim3(256), 0, c->st, NA);
+  if (c->servoOn) hipLaunchKernelGGL(k_servo_end, dim3(1), dim3(64), 0, c->st, c->dServo);
 }
 
 void one_step(sdm_ctx* c) {
@@ -877,6 +900,16 @@ void one_step(sdm_ctx* c) {
   A.iterDev = c->dcnt + DC_NP + 1;
   hipLaunchKernelGGL(k_set_u64x2, dim3(1), dim3(64), 0, c->st, c->dcnt + DC_NP, (unsigned long long)c->np,
                      (unsigned long long)c->iter);
+  if (c->servoOn) {  // block partials of the servo's check, for launches up to the capacity
+    const int64_t need = 11 * (int64_t)nblk(std::max<int64_t>({(int64_t)1, c->P.cap, (int64_t)c->B.n}));
+    if (need > c->servoPartCap) {
+      HIPCHK(hipStreamSynchronize(c->st));
+      dfree(c->servoPart);
+      c->servoPart = dalloc<double>(need);
+      c->servoPartCap = need;
+      c->graphGen++;
+    }
+  }
   // Steady state (no collider run, no host write since the last step, no timing): replay the
   // captured step; its launches cover the container's capacity and read np and iter above.
   const bool graphable = c->useGraph && !c->timing && A.fullWrite == 0 && A.skip == 1 && c->np > 0;
@@ -907,9 +940,11 @@ void one_step(sdm_ctx* c) {
     step_body(c, A);
   }
   if (c->timing) HIPCHK(hipEventRecord(c->ev[3], c->st));
-  int code = 0;
+  int code = 0, servoDone = 0;
   unsigned long long nqueued = 0;
   HIPCHK(hipMemcpyAsync(&code, c->dMaxDisp, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  if (c->servoOn)
+    HIPCHK(hipMemcpyAsync(&servoDone, &c->dServo->completedNow, sizeof(int), hipMemcpyDeviceToHost, c->st));
   if (c->timing) HIPCHK(hipMemcpyAsync(&nqueued, c->dcnt + DC_NQ, sizeof(nqueued), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   if (c->timing) c->tms[7] += (double)nqueued;
@@ -929,6 +964,7 @@ void one_step(sdm_ctx* c) {
     }
   }
   c->maxVelocitySq = code == 2 ? 2.0 : (code == 1 ? 0.5 : 0.0);
+  c->servoDone = servoDone != 0;
   c->skipValid = true;  // this step wrote every interaction's gap and every body's displacement
   c->iter++;
   c->time += c->prm.dt;
@@ -1108,7 +1144,7 @@ void sdm_destroy(sdm_ctx* c) {
   dfree(c->msrc); dfree(c->mdst); dfree(c->holePos); dfree(c->stage);
   dfree(c->prevMin); dfree(c->prevMax); dfree(c->cellRng); dfree(c->rankMin); dfree(c->rankMax);
   dfree(c->peCnt); dfree(c->peOff); dfree(c->runEnd); dfree(c->pbox);
-  dfree(c->dmats); dfree(c->dcnt); dfree(c->dMaxDisp);
+  dfree(c->dmats); dfree(c->dcnt); dfree(c->dMaxDisp); dfree(c->dServo); dfree(c->servoPart);
   for (int k = 0; k < 6; k++)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -1329,7 +1365,19 @@ int sdm_step(sdm_ctx* c, int32_t nsteps) {
       c->matChecked = true;
     }
     if (c->prm.periodic && c->nSpec > 1) return fail(c, "several specimens in one periodic cell are not supported");
-    for (int s = 0; s < nsteps; s++) one_step(c);
+    if (c->servoOn) {
+      if (c->prm.periodic) return fail(c, "the wall servo needs the six walls of a cubic specimen, not a periodic cell");
+      if (c->nSpec > 1) return fail(c, "the wall servo steps one specimen");
+      if (c->B.n < 6) return fail(c, "the wall servo needs bodies 0-5 to be the walls");
+      int32_t kind[6];
+      HIPCHK(hipMemcpy(kind, c->B.kind, sizeof(kind), hipMemcpyDeviceToHost));
+      for (int i = 0; i < 6; i++)
+        if (kind[i] != SDM_KIND_WALL) return fail(c, "the wall servo needs bodies 0-5 to be the walls");
+    }
+    for (int s = 0; s < nsteps; s++) {
+      one_step(c);
+      if (c->servoOn && c->servoDone) break;  // stopAtIter = iter + 1
+    }
   } catch (HipError& e) {
     return fail(c, e.msg);
   }
@@ -1446,6 +1494,71 @@ int sdm_body_forces(sdm_ctx* c, const int32_t* ids, int32_t n, double* forces) {
     HIPCHK(hipStreamSynchronize(c->st));
     dfree(dids);
     dfree(dout);
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+int sdm_set_servo(sdm_ctx* c, const sdm_servo* sv) {
+  if (!c) return fail(c, "null context");
+  try {
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->graphGen++;
+    if (!sv) {
+      c->servoOn = false;
+      return 0;
+    }
+    if (sv->iterate_num_max < 1 || sv->solve_num_max < 1) return fail(c, "sdm_set_servo: intervals must be >= 1");
+    ServoDev h;
+    std::memset(&h, 0, sizeof(h));
+    for (int k = 0; k < 3; k++) h.goal[k] = sv->goal[k];
+    h.max_vel = sv->max_vel;
+    h.gain_alpha = sv->gain_alpha;
+    h.f_threshold = sv->f_threshold;
+    h.fmin_threshold = sv->fmin_threshold;
+    h.unbf_tol = sv->unbf_tol;
+    h.iterate_num_max = sv->iterate_num_max;
+    h.solve_num_max = sv->solve_num_max;
+    for (int k = 0; k < 4; k++) h.activated[k] = sv->activated[k];
+    h.z_servo = sv->z_servo;
+    h.first = 1;
+    h.completedIter = -1;
+    h.unbf = 1.0;  // CompressionEngine::UnbalancedForce default
+    if (!c->dServo) c->dServo = dalloc<ServoDev>(1);
+    HIPCHK(hipMemcpy(c->dServo, &h, sizeof(h), hipMemcpyHostToDevice));
+    c->servoOn = true;
+    c->servoDone = false;
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+int sdm_get_servo(sdm_ctx* c, sdm_servo_state* o) {
+  if (!c || !o) return fail(c, "sdm_get_servo: bad arguments");
+  if (!c->dServo) return fail(c, "sdm_get_servo: the servo was never set");
+  try {
+    ServoDev h;
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(&h, c->dServo, sizeof(h), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 6; k++) {
+      o->wall_stress[k] = h.ws[k];
+      o->gain[k] = h.gain[k];
+      o->stiffness[k] = h.stiff[k];
+    }
+    o->stress[0] = h.wsxx;
+    o->stress[1] = h.wsyy;
+    o->stress[2] = h.wszz;
+    o->size[0] = h.width;
+    o->size[1] = h.depth;
+    o->size[2] = h.height;
+    o->mean_stress = h.mean;
+    o->deviator_stress = h.dev;
+    o->unbalanced_force = h.unbf;
+    o->iterate_num = h.iterate_num;
+    o->solve_num = h.solve_num;
+    o->completed_iter = h.completedIter;
   } catch (HipError& e) {
     return fail(c, e.msg);
   }

@@ -106,3 +106,21 @@ def test_hertz_nm_first_steps(gpu, oracle, c1_small):
         b = c.get(name).reshape(-1, 3)[ok]
         err = np.max(np.abs(a - b), axis=1) / np.maximum(r, 1e-300)
         assert np.all(err < cond), (name, float(err.max()))
+
+
+def test_wall_contacts_on_every_axis(gpu, oracle):
+    """Walls resting on the outermost particles of all six faces.  The collider creates the
+    pairs of the walls whose planes lie on its sort axis as (particle, wall); the functor
+    dispatch swaps them to (wall, particle) (InteractionLoop: swapOrder), and the force sum must
+    then credit +F to the wall and -F to the particle: forces and positions follow the oracle."""
+    import test_servo as TS
+    g = TS.gentle_scene(gpu)[1]
+    c = TS.gentle_scene(oracle)[1]
+    oracle.set_init_sort(c.ptr, 1)
+    for k in range(5):
+        g.step(1)
+        c.step(1)
+        F = c.body_forces(np.arange(6))
+        assert (np.abs(F) > 0).any(axis=1).all(), k  # every wall touched
+        assert np.allclose(g.body_forces(np.arange(6)), F, rtol=1e-12, atol=1e-14), k
+        assert np.allclose(g.get("state.pos"), c.get("state.pos"), rtol=0, atol=1e-15), k
