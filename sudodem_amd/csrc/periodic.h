@@ -466,3 +466,179 @@ __global__ void k_new_celldist(PairArrays P, int64_t start, int64_t count, BodyA
   R.cellDist[1] = per[1];
   R.cellDist[2] = per[2];
 }
+
+// ---------------------------------------------------------------------------------------
+// Initial sort-and-sweep of an aperiodic scene (InsertionSortCollider.cpp:336-397) without
+// the sweep: the sweep creates (A, B) when B's lower bound lies strictly between A's lower
+// and upper bounds in the sorted order of the sort axis and the boxes overlap
+// (spatialOverlap), and inserts the pairs in the order (rank of A's lower bound, rank of B's
+// lower bound).  Every overlapping pair of bounded bodies is found here instead, the bodies
+// with finite boxes through a cell list (the pair met once, in its first shared cell) and
+// the walls (boxes infinite on two axes) against every body; each gets the sweep's
+// orientation and order key from the ranks, and pairs the sweep would not create (boxes
+// that only touch, with the tie broken the other way by the sort) are dropped.
+// ---------------------------------------------------------------------------------------
+
+// The sweep's decision for an overlapping pair (a, b): scanner s, scanned t, order key.
+__device__ __forceinline__ bool sweep_pair(const BodyArrays& B, const uint32_t* rankMin, const uint32_t* rankMax, int a,
+                                           int b, int& s, int& t, uint64_t& key) {
+  s = rankMin[a] < rankMin[b] ? a : b;
+  t = s == a ? b : a;
+  if (!(rankMin[t] < rankMax[s])) return false;
+  if (!spatial_overlap(B, s, t)) return false;
+  key = ((uint64_t)rankMin[s] << 32) | (uint64_t)rankMin[t];
+  return true;
+}
+
+__device__ __forceinline__ void push_cand(const Cands& C, int s, int t, uint64_t key) {
+  const unsigned long long slot = atomicAdd(C.count, 1ull);
+  if ((int64_t)slot < C.cap) {
+    C.a[slot] = s;
+    C.b[slot] = t;
+    C.key[slot] = key;
+  }
+}
+
+// Union box and width statistics of the bounded bodies with finite boxes (no walls), per
+// block: lo[3], hi[3], width sum, width max, count.
+__global__ __launch_bounds__(256) void k_ic_stats(BodyArrays B, double* partial) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t N = B.n;
+  const double inf = __builtin_huge_val();
+  double v[9] = {inf, inf, inf, -inf, -inf, -inf, 0.0, 0.0, 0.0};
+  if (i < N && B.kind[i] == SDM_KIND_SQ && B.hasBound[i]) {
+    double w = 0;
+    for (int k = 0; k < 3; k++) {
+      const double lo = B.bmin[k * N + i], hi = B.bmax[k * N + i];
+      v[k] = lo;
+      v[3 + k] = hi;
+      w = fmax(w, hi - lo);
+    }
+    v[6] = w;
+    v[7] = w;
+    v[8] = 1.0;
+  }
+  __shared__ double red[9][256];
+  for (int k = 0; k < 9; k++) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      for (int k = 0; k < 3; k++) red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+      for (int k = 3; k < 6; k++) red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+      red[6][threadIdx.x] += red[6][threadIdx.x + s];
+      red[7][threadIdx.x] = fmax(red[7][threadIdx.x], red[7][threadIdx.x + s]);
+      red[8][threadIdx.x] += red[8][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) partial[(int64_t)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_ic_stats_final(const double* partial, int64_t nb, double* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const double inf = __builtin_huge_val();
+  double v[9] = {inf, inf, inf, -inf, -inf, -inf, 0.0, 0.0, 0.0};
+  for (int64_t b = 0; b < nb; b++) {
+    const double* p = partial + b * 9;
+    for (int k = 0; k < 3; k++) v[k] = fmin(v[k], p[k]);
+    for (int k = 3; k < 6; k++) v[k] = fmax(v[k], p[k]);
+    v[6] += p[6];
+    v[7] = fmax(v[7], p[7]);
+    v[8] += p[8];
+  }
+  for (int k = 0; k < 9; k++) out[k] = v[k];
+}
+
+// Cell range of every body with a finite box (start cell and count per axis, clamped to the
+// grid), 0 cells for the others.
+__global__ void k_ic_ranges(BodyArrays B, PeriCell pc, double lo0, double lo1, double lo2, int32_t* rng,
+                            uint32_t* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t N = B.n;
+  if (i >= N) return;
+  if (!(B.kind[i] == SDM_KIND_SQ && B.hasBound[i])) {
+    for (int k = 0; k < 6; k++) rng[6 * i + k] = 0;  // no cells: k_peri_entries writes nothing
+    cnt[i] = 0;
+    return;
+  }
+  const double lo[3] = {lo0, lo1, lo2};
+  uint32_t total = 1;
+  for (int k = 0; k < 3; k++) {
+    const int n = pc.n[k];
+    int s = (int)floor((B.bmin[k * N + i] - lo[k]) / pc.h[k]);
+    int e = (int)floor((B.bmax[k * N + i] - lo[k]) / pc.h[k]);
+    s = s < 0 ? 0 : (s >= n ? n - 1 : s);
+    e = e < 0 ? 0 : (e >= n ? n - 1 : e);
+    if (e < s) e = s;
+    rng[6 * i + k] = s;
+    rng[6 * i + 3 + k] = e - s + 1;
+    total *= (uint32_t)(e - s + 1);
+  }
+  cnt[i] = total;
+}
+
+// One block per grid cell: every pair of the cell's entries whose first shared cell is this
+// one (the larger start cell on each axis) becomes a candidate if the sweep would create it.
+__global__ __launch_bounds__(64) void k_ic_cell_pairs(BodyArrays B, PeriCell pc, const int32_t* rng,
+                                                      const uint32_t* val, const uint32_t* runStart,
+                                                      const uint32_t* runEnd, const uint32_t* rankMin,
+                                                      const uint32_t* rankMax, Cands C) {
+  __shared__ int32_t sId[SDM_PERI_TILE];
+  __shared__ uint32_t sRg[3][SDM_PERI_TILE];
+  const uint32_t cell = blockIdx.x;
+  const uint32_t s0 = runStart[cell];
+  const int kc = (int)(runEnd[cell] - s0);
+  if (kc < 2) return;  // the whole block
+  const int X[3] = {(int)(cell % (uint32_t)pc.n[0]), (int)((cell / (uint32_t)pc.n[0]) % (uint32_t)pc.n[1]),
+                    (int)(cell / ((uint32_t)pc.n[0] * (uint32_t)pc.n[1]))};
+  const int nl = kc < SDM_PERI_TILE ? kc : SDM_PERI_TILE;
+  for (int j = threadIdx.x; j < nl; j += blockDim.x) {
+    const int a = (int)val[s0 + j];
+    sId[j] = a;
+#pragma unroll
+    for (int k = 0; k < 3; k++) sRg[k][j] = (uint32_t)rng[6 * a + k];
+  }
+  __syncthreads();
+  auto ent_id = [&](int j) -> int { return j < SDM_PERI_TILE ? sId[j] : (int)val[s0 + j]; };
+  auto ent_s = [&](int j, int k) -> int { return j < SDM_PERI_TILE ? (int)sRg[k][j] : rng[6 * (int)val[s0 + j] + k]; };
+  for (int i = threadIdx.x; i < kc; i += blockDim.x) {
+    const int a = ent_id(i);
+    int sa[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) sa[k] = ent_s(i, k);
+    const int iters = (kc - 1) / 2 + ((kc % 2 == 0 && i < kc / 2) ? 1 : 0);
+    for (int t = 1; t <= iters; t++) {
+      int j = i + t;
+      if (j >= kc) j -= kc;
+      bool first = true;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (first) {
+          const int sb = ent_s(j, k);
+          first = (sa[k] > sb ? sa[k] : sb) == X[k];
+        }
+      }
+      if (!first) continue;
+      const int b = ent_id(j);
+      int s, u;
+      uint64_t key;
+      if (sweep_pair(B, rankMin, rankMax, a, b, s, u, key)) push_cand(C, s, u, key);
+    }
+  }
+}
+
+// Walls (listed in `walls`) against every bounded body; a pair of walls once.
+__global__ __launch_bounds__(256) void k_ic_walls(BodyArrays B, const int32_t* walls, int nw, const uint32_t* rankMin,
+                                                  const uint32_t* rankMax, Cands C) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B.n || !B.hasBound[b]) return;
+  const bool bWall = B.kind[b] == SDM_KIND_WALL;
+  for (int k = 0; k < nw; k++) {
+    const int w = walls[k];
+    if (w == (int)b || !B.hasBound[w]) continue;
+    if (bWall && (int)b < w) continue;
+    int s, u;
+    uint64_t key;
+    if (sweep_pair(B, rankMin, rankMax, w, (int)b, s, u, key)) push_cand(C, s, u, key);
+  }
+}

@@ -607,6 +607,111 @@ void collider_run_peri(sdm_ctx* c) {
   c->periCover = true;
 }
 
+// The first run's pairs of a one-specimen aperiodic scene without the sweep (periodic.h,
+// "Initial sort-and-sweep"): ranks of the sorted bounds of the sort axis, a cell list of the
+// finite boxes, the walls against every body.  Candidates in C; returns their number.
+int64_t init_pairs_cells(sdm_ctx* c, const int32_t* V) {
+  const int64_t N = c->B.n;
+  const int64_t n2 = 2 * N;
+  if (!c->rankMin) {
+    c->rankMin = dalloc<uint32_t>(N);
+    c->rankMax = dalloc<uint32_t>(N);
+  }
+  if (!c->peCnt) {
+    c->peCnt = dalloc<uint32_t>(N + 1);
+    c->peOff = dalloc<uint32_t>(N + 1);
+  }
+  if (!c->cellRng) c->cellRng = dalloc<int32_t>(6 * N);
+  hipLaunchKernelGGL(k_peri_ranks, dim3(nblk(n2)), dim3(256), 0, c->st, (const uint32_t*)V, n2, c->rankMin, c->rankMax);
+  // union box of the finite boxes and their widths
+  const int64_t nb = (int64_t)nblk(std::max<int64_t>(1, N));
+  double* part = dalloc<double>(9 * nb + 9);
+  hipLaunchKernelGGL(k_ic_stats, dim3((unsigned)nb), dim3(256), 0, c->st, c->B, part);
+  hipLaunchKernelGGL(k_ic_stats_final, dim3(1), dim3(64), 0, c->st, part, nb, part + 9 * nb);
+  double st[9];
+  HIPCHK(hipMemcpyAsync(st, part + 9 * nb, sizeof(st), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  dfree(part);
+  // walls (infinite boxes): listed on the host, once
+  std::vector<int32_t> kind(N > 0 ? N : 1);
+  if (N > 0) HIPCHK(hipMemcpy(kind.data(), c->B.kind, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<int32_t> walls;
+  for (int64_t i = 0; i < N; i++)
+    if (kind[i] == SDM_KIND_WALL) walls.push_back((int32_t)i);
+  int32_t* dWalls = nullptr;
+  if (!walls.empty()) {
+    dWalls = dalloc<int32_t>((int64_t)walls.size());
+    HIPCHK(hipMemcpy(dWalls, walls.data(), walls.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  // grid over the union box
+  PeriCell pc;
+  std::memset(&pc, 0, sizeof(pc));
+  const bool finite = st[8] > 0;
+  int64_t ncell = 1;
+  if (finite) {
+    const double wmean = st[6] / st[8], wmax = st[7];
+    const double ht = std::max(2.0 * wmean, 0.5 * wmax);
+    for (int k = 0; k < 3; k++) {
+      const double ext = st[3 + k] - st[k];
+      int64_t n = (ht > 0 && ext > 0) ? (int64_t)std::floor(ext / ht) : 1;
+      pc.n[k] = (int32_t)std::min<int64_t>(std::max<int64_t>(n, 1), 1024);
+    }
+    while ((int64_t)pc.n[0] * pc.n[1] * pc.n[2] > (1ll << 22)) {
+      int k = (pc.n[0] >= pc.n[1] && pc.n[0] >= pc.n[2]) ? 0 : (pc.n[1] >= pc.n[2] ? 1 : 2);
+      pc.n[k] = std::max(1, pc.n[k] / 2);
+    }
+    for (int k = 0; k < 3; k++) {
+      const double ext = st[3 + k] - st[k];
+      pc.h[k] = ext > 0 ? ext / pc.n[k] : 1.0;
+      pc.L[k] = ext;
+    }
+    ncell = (int64_t)pc.n[0] * pc.n[1] * pc.n[2];
+  }
+  int64_t m = 0;
+  uint32_t* ckey = nullptr;
+  uint32_t* cval = nullptr;
+  uint32_t* runStart = nullptr;
+  if (finite) {
+    hipLaunchKernelGGL(k_ic_ranges, dim3(nblk(N)), dim3(256), 0, c->st, c->B, pc, st[0], st[1], st[2], c->cellRng,
+                       c->peCnt);
+    m = exclusive_total(c, c->peCnt, c->peOff, N);
+    ensure_scratch(c, m + 1);
+    ckey = (uint32_t*)c->k0;
+    cval = c->v0;
+    hipLaunchKernelGGL(k_peri_entries, dim3(nblk(N)), dim3(256), 0, c->st, N, pc, c->cellRng, c->peOff, ckey, cval);
+    int bits = 1;
+    while ((1ll << bits) < ncell) bits++;
+    sdmrs::sort_pairs<uint32_t, uint32_t>(ckey, cval, (uint32_t*)c->k1, c->v1, m, bits, c->rsScratch, c->st);
+    if (ncell > c->runCap) {
+      HIPCHK(hipStreamSynchronize(c->st));
+      dfree(c->runEnd);
+      c->runCap = ncell;
+      c->runEnd = dalloc<uint32_t>(2 * ncell);
+    }
+    runStart = c->runEnd + c->runCap;
+    HIPCHK(hipMemsetAsync(c->runEnd, 0, ncell * sizeof(uint32_t), c->st));
+    HIPCHK(hipMemsetAsync(runStart, 0, ncell * sizeof(uint32_t), c->st));
+    if (m > 0) hipLaunchKernelGGL(k_cell_ends, dim3(nblk(m)), dim3(256), 0, c->st, ckey, m, runStart, c->runEnd);
+  }
+  int64_t nc = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    HIPCHK(hipMemsetAsync(c->dcnt + 2, 0, sizeof(unsigned long long), c->st));
+    c->C.count = c->dcnt + 2;
+    if (m > 0)
+      hipLaunchKernelGGL(k_ic_cell_pairs, dim3((unsigned)ncell), dim3(64), 0, c->st, c->B, pc, c->cellRng, cval,
+                         runStart, c->runEnd, c->rankMin, c->rankMax, c->C);
+    if (!walls.empty())
+      hipLaunchKernelGGL(k_ic_walls, dim3(nblk(N)), dim3(256), 0, c->st, c->B, dWalls, (int)walls.size(), c->rankMin,
+                         c->rankMax, c->C);
+    nc = (int64_t)read_counter(c, 2);
+    if (nc <= c->C.cap) break;
+    ensure_cands(c, nc);  // rerun with room for all
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  dfree(dWalls);
+  return nc;
+}
+
 // InsertionSortCollider::action (InsertionSortCollider.cpp:191-400), non-periodic.
 void collider_run(sdm_ctx* c) {
   if (c->prm.periodic) {
@@ -676,14 +781,19 @@ void collider_run(sdm_ctx* c) {
   } else {
     c->numReinit++;
     const int32_t* V = c->bb[c->prm.sort_axis];
-    uint32_t* counts = c->u0;
-    uint32_t* offs = c->u1;
-    hipLaunchKernelGGL(k_init_sweep<false>, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, V, n2, counts,
-                       (const uint32_t*)nullptr, c->C);
-    nc = exclusive_total(c, counts, offs, n2);
-    ensure_cands(c, nc);
-    hipLaunchKernelGGL(k_init_sweep<true>, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, V, n2, (uint32_t*)nullptr, offs,
-                       c->C);
+    if (c->nSpec == 1) {
+      nc = init_pairs_cells(c, V);
+    } else {
+      // several specimens (each a few thousand bodies, spatially superposed): the sweep itself
+      uint32_t* counts = c->u0;
+      uint32_t* offs = c->u1;
+      hipLaunchKernelGGL(k_init_sweep<false>, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, V, n2, counts,
+                         (const uint32_t*)nullptr, c->C);
+      nc = exclusive_total(c, counts, offs, n2);
+      ensure_cands(c, nc);
+      hipLaunchKernelGGL(k_init_sweep<true>, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, V, n2, (uint32_t*)nullptr,
+                         offs, c->C);
+    }
   }
 
   if (nc > 0) {
