@@ -152,19 +152,29 @@ int sdm_mean_coordination(sdm_ctx* ctx, double* cn);
 int sdm_stress(sdm_ctx* ctx, double volume, double sigma[9]);
 int sdm_body_forces(sdm_ctx* ctx, const int32_t* ids, int32_t n, double* forces);
 
-/* Wall stress servo: CompressionEngine (pkg/dem/Compression.cpp) on a cubic specimen in its
- * consolidation branch as the reference compiles it (NEW_SCHEME and SOLE_GAIN,
- * Compression.hpp:26-29): action_cubic (:395-481) -> generalConsolidation (:324-345) ->
- * servo (:1252-1316), get_gain (:1125-1220), consol_ss (:1330-1345), getStress (:764-841),
- * Shop::unbalancedForce (pkg/dem/Shop_01.cpp:148-189).  Bodies 0-5 are the walls left, right,
+/* Wall stress servo: CompressionEngine (pkg/dem/Compression.cpp) on a cubic specimen as the
+ * reference compiles it (NEW_SCHEME and SOLE_GAIN, Compression.hpp:26-29): action_cubic
+ * (:395-604) with generalConsolidation (:324-345) or hydroConsolidation (:264-322) when z_servo,
+ * the triaxial shear branch (:498-603) otherwise; servo (:1252-1316), get_gain (:1125-1220),
+ * consol_ss (:1330-1345), getStress (:764-841), Shop::unbalancedForce
+ * (pkg/dem/Shop_01.cpp:148-189).  Bodies 0-5 are the walls left, right,
  * front, back, bottom, top (x-, x+, y-, y+, z-, z+), as the engine assumes.  It runs inside
  * every step after the force sums and before the integrator (the engine order of
- * examples/SudoDEM3D/example2_consol.py), with no host round trip: wall stresses, gains
- * every iterate_num_max steps (stiffness summed in interaction order) and the stability
- * check every solve_num_max steps.  When the check finds the target reached and the
- * unbalanced force below unbf_tol, the integrator quiets the system in that step
- * (NewtonIntegrator::quiet_system_flag: velocities zeroed, no move) and sdm_step returns
- * after it (stopAtIter = iter + 1). */
+ * examples/SudoDEM3D/example2_consol.py and example2_shear.py), with no host round trip:
+ *   consolidation (z_servo, not hydro_strain): wall stresses, each wall moved by its gain,
+ *     gains every iterate_num_max steps (stiffness summed in interaction order) and the
+ *     stability check every solve_num_max steps.  When the check finds the target reached
+ *     and the unbalanced force below unbf_tol, the integrator quiets the system in that step
+ *     (NewtonIntegrator::quiet_system_flag: velocities zeroed, no move);
+ *   hydroConsolidation (z_servo and hydro_strain): every wall still below its goal moves in
+ *     by min(0.1 gain deficit, hydro_strain_rate size dt); the check compares with the
+ *     UnbalancedForce of the last echo and does not quiet the system;
+ *   shear (z_servo = 0): the z walls close at the ramped rate, then at goalz height per unit
+ *     time; the x and y walls keep the lateral stress (shear_mode bit 1) or the volume;
+ *     the run ends once log(height0 / height) exceeds target_strain.
+ * Completion returns sdm_step after that step (stopAtIter = iter + 1).  Every echo_interval
+ * steps (Scene::iter a multiple of it) UnbalancedForce is recomputed and, in consolidation
+ * and in constant-volume shear, the wall stresses are re-evaluated after the moves. */
 typedef struct sdm_servo {
   double goal[3];           /* goalx, goaly, goalz: target stresses [Pa] */
   double max_vel;           /* max_vel: wall speed limit [m/s] */
@@ -175,7 +185,20 @@ typedef struct sdm_servo {
   int32_t iterate_num_max;  /* iterate_num_max: steps between gain updates (100) */
   int32_t solve_num_max;    /* solve_num_max: steps between stability checks (2000) */
   int32_t activated[4];     /* left/right/front/back_wall_activated (the z walls follow z_servo) */
-  int32_t z_servo;          /* z_servo: stress control in z (the consolidation branch needs 1) */
+  int32_t z_servo;          /* z_servo: 1 consolidation (stress control in z), 0 triaxial shear */
+  int32_t hydro_strain;     /* hydroStrain: consolidation by a bounded strain rate (hydroConsolidation) */
+  double hydro_strain_rate; /* hydroStrainRate */
+  int32_t shear_mode;       /* shearMode: bit 1 drained (lateral stress servo), else constant volume */
+  int32_t ramp_interval;    /* ramp_interval: steps over which the axial rate ramps up */
+  int32_t ramp_chunks;      /* ramp_chunks */
+  int32_t echo_interval;    /* echo_interval: every so many steps UnbalancedForce and the wall
+                               stresses are refreshed (the engine's console report) */
+  double target_strain;     /* target_strain: shear ends at log(height0 / height) above it */
+  int32_t keep_state;       /* 1: change the attributes of the running engine (as assigning them in
+                               Python does), 0: a new engine (firstRun) */
+  int32_t continue_flag;    /* continueFlag: the first run keeps size0 below instead of the box
+                               it finds, and shear skips the ramp */
+  double size0[3];          /* width0, depth0, height0: read when continue_flag */
 } sdm_servo;
 typedef struct sdm_servo_state {
   double wall_stress[6];    /* wsxx_left, wsxx_right, wsyy_front, wsyy_back, wszz_bottom, wszz_top */
@@ -185,11 +208,17 @@ typedef struct sdm_servo_state {
   double size[3];           /* width, depth, height at the last wall-stress evaluation */
   double mean_stress;       /* getStress() of the last stability check: mean ... */
   double deviator_stress;   /* ... and deviatoric stress (0 before the first check) */
-  double unbalanced_force;  /* UnbalancedForce of the last check that reached the target (1 before) */
+  double unbalanced_force;  /* UnbalancedForce: of the last check that reached the target or the
+                               last echo (1 before either) */
   int32_t iterate_num, solve_num;
-  int64_t completed_iter;   /* Scene::iter of the step that completed consolidation, -1 if none */
+  int64_t completed_iter;   /* Scene::iter of the step that completed consolidation or shear, -1 if none */
+  double size0[3];          /* width0, depth0, height0: the sizes of the first run */
+  int32_t ramp_num;         /* rampNum */
+  int32_t pad_;
 } sdm_servo_state;
-/* servo = NULL switches the servo off.  Switching it on starts it afresh (firstRun). */
+/* servo = NULL switches the servo off.  Switching it on starts it afresh (firstRun) unless
+ * keep_state asks to keep the state of the running one (e.g. z_servo = 0 after consolidation:
+ * shear with the sizes the engine recorded on its first run). */
 int sdm_set_servo(sdm_ctx* ctx, const sdm_servo* servo);
 int sdm_get_servo(sdm_ctx* ctx, sdm_servo_state* out);
 

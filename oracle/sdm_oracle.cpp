@@ -351,11 +351,12 @@ struct sdm_ctx {
   int32_t initSortStable = 0;
   // NewtonIntegrator state
   double maxVelocitySq = NaN;
-  // CompressionEngine (consolidation branch, SOLE_GAIN + NEW_SCHEME)
+  // CompressionEngine (cubic specimen, SOLE_GAIN + NEW_SCHEME)
   struct Servo {
     bool on = false, firstRun = true, quiet = false;
     sdm_servo p;
-    int32_t iterate_num = 0, solve_num = 0;
+    int32_t iterate_num = 0, solve_num = 0, rampNum = 0;
+    double width0 = 0, depth0 = 0, height0 = 0, initVol = 0;
     double width = 0, depth = 0, height = 0, x_area = 0, y_area = 0, z_area = 0;
     double ws[6] = {0, 0, 0, 0, 0, 0};  // wsxx_left, wsxx_right, wsyy_front, wsyy_back, wszz_bottom, wszz_top
     double wsxx = 0, wsyy = 0, wszz = 0;
@@ -1760,8 +1761,8 @@ void newton(sdm_ctx* c) {
 }
 
 // ---------------------------------------------------------------------------------------
-// CompressionEngine, cubic specimen, consolidation (pkg/dem/Compression.cpp with SOLE_GAIN and
-// NEW_SCHEME defined, Compression.hpp:26-29).  Walls: bodies 0-5 = left, right, front, back,
+// CompressionEngine, cubic specimen: consolidation, hydroConsolidation and triaxial shear
+// (pkg/dem/Compression.cpp with SOLE_GAIN and NEW_SCHEME defined, Compression.hpp:26-29).  Walls: bodies 0-5 = left, right, front, back,
 // bottom, top.
 // ---------------------------------------------------------------------------------------
 // updateBoxSize (:99-113)
@@ -1882,21 +1883,17 @@ double unbalanced_force(sdm_ctx* c) {
   sumC *= 2.0;
   return sumF / sumC;
 }
-// action_cubic's first run (:397-480, the parts the servo uses) and generalConsolidation with
-// NEW_SCHEME (:324-345).
-void servo_action(sdm_ctx* c) {
+// The servo's completion: stopAtIter = iter + 1 (sdmcpu_step returns after this step).
+void servo_complete(sdm_ctx* c) {
+  c->servo.completedIter = c->iter;
+  c->servo.completedNow = true;
+}
+// generalConsolidation with NEW_SCHEME (:324-353)
+void servo_general(sdm_ctx* c, double dt) {
   auto& S = c->servo;
-  S.completedNow = false;
-  if (S.firstRun) {
-    servo_box(c);
-    S.iterate_num = 0;
-    S.solve_num = 0;
-    servo_get_gain(c);
-    S.firstRun = false;
-  }
   S.iterate_num += 1;
   S.solve_num += 1;
-  servo_move(c, c->prm.dt);
+  servo_move(c, dt);
   if (S.iterate_num >= S.p.iterate_num_max) {
     S.iterate_num = 0;
     servo_get_gain(c);
@@ -1906,11 +1903,126 @@ void servo_action(sdm_ctx* c) {
     servo_get_stress(c, S.mean, S.dev);
     if ((std::abs(S.mean - S.p.goal[0]) < S.p.goal[0] * S.p.f_threshold) && (S.dev / S.mean < S.p.fmin_threshold)) {
       S.unbf = unbalanced_force(c);
-      if (S.unbf < S.p.unbf_tol) {  // stopAtIter = iter + 1; newton->quiet_system_flag = true
-        S.completedIter = c->iter;
-        S.completedNow = true;
+      if (S.unbf < S.p.unbf_tol) {  // and newton->quiet_system_flag = true
+        servo_complete(c);
         S.quiet = true;
       }
+    }
+  }
+}
+// hydroConsolidation (:264-322): each wall below its goal moves in by the smaller of a tenth
+// of its gain times the deficit and the strain-rate bound; the check uses the UnbalancedForce
+// of the last echo.  The wall stresses here are locals of the engine (left_wall_s ...), not the
+// exported wsxx_left ...
+void servo_hydro(sdm_ctx* c, double dt) {
+  auto& S = c->servo;
+  auto& B = c->bodies;
+  servo_box(c);
+  const double s[6] = {B[0].force[0] / S.x_area, B[1].force[0] / S.x_area, B[2].force[1] / S.y_area,
+                       B[3].force[1] / S.y_area, B[4].force[2] / S.z_area, B[5].force[2] / S.z_area};
+  const double size[3] = {S.width, S.depth, S.height};
+  const double rate = S.p.hydro_strain_rate;
+  for (int w = 0; w < 6; w++) {
+    const double delta = S.p.goal[w / 2] - std::abs(s[w]);
+    if (delta > 0) {
+      const double step = std::min(0.1 * S.gain[w] * delta, rate * size[w / 2] * dt);
+      if (w % 2 == 0) B[w].pos[w / 2] += step;
+      else B[w].pos[w / 2] -= step;
+    }
+  }
+  S.iterate_num += 1;
+  if (S.iterate_num >= S.p.iterate_num_max) {
+    S.iterate_num = 1;
+    servo_get_gain(c);
+  }
+  S.solve_num += 1;
+  if (S.solve_num >= S.p.solve_num_max) {
+    S.solve_num = 1;
+    servo_get_stress(c, S.mean, S.dev);
+    if ((std::abs(S.mean - S.p.goal[0]) / S.p.goal[0] < S.p.f_threshold) && (S.dev / S.mean < S.p.fmin_threshold)) {
+      if (S.unbf < S.p.unbf_tol) servo_complete(c);
+    }
+  }
+}
+// action_cubic's shear branch (:498-603): the z walls close at the ramped axial rate, then at
+// goalz; the lateral walls keep the stress (shearMode bit 1, servo without z) or the volume.
+void servo_shear(sdm_ctx* c, double dt) {
+  auto& S = c->servo;
+  auto& B = c->bodies;
+  const double goalz = S.p.goal[2];
+  if (!S.p.continue_flag && S.rampNum < S.p.ramp_interval) {
+    // (Flag_ForceTarget, which would re-take the sizes at the ramp's start, is only set by the
+    // consolidation scheme the reference compiles out)
+    S.rampNum++;
+    const double ramp_inter = 0.5 * (double)S.p.ramp_interval;
+    const double chunk = ramp_inter / (double)S.p.ramp_chunks;
+    double vel;
+    if ((double)S.rampNum < ramp_inter)
+      vel = (double)((float)((double)S.rampNum / chunk + 1) / (float)S.p.ramp_chunks) * goalz * 2.0;
+    else
+      vel = ((double)((float)((double)(S.p.ramp_interval - S.rampNum) / chunk + 1) / (float)S.p.ramp_chunks) + 1.0) *
+            goalz;
+    B[4].pos[2] += vel * dt * S.height;
+    B[5].pos[2] += -vel * dt * S.height;
+  } else {
+    if (S.p.shear_mode & 1) {
+      S.iterate_num += 1;
+      if (S.iterate_num >= S.p.iterate_num_max) {
+        S.iterate_num = 1;
+        servo_get_gain(c);
+      }
+    }
+    B[4].pos[2] += goalz * dt * S.height;
+    B[5].pos[2] += -goalz * dt * S.height;
+  }
+  if (S.p.shear_mode & 1) {
+    servo_move(c, dt);  // z_servo is off: the x and y walls only
+  } else {              // constant volume
+    S.width = B[1].pos[0] - B[0].pos[0];
+    S.depth = B[3].pos[1] - B[2].pos[1];
+    S.height = B[5].pos[2] - B[4].pos[2];
+    const double area = S.initVol / S.height;
+    const double dd = 0.5 * (area / S.width - S.depth);
+    const double dw = area / (S.depth + dd) - S.width;
+    B[0].pos[0] += -0.5 * dw;
+    B[1].pos[0] += 0.5 * dw;
+    B[2].pos[1] += -0.5 * dd;
+    B[3].pos[1] += 0.5 * dd;
+  }
+  if (std::log(S.height0 / S.height) > S.p.target_strain) servo_complete(c);
+}
+// action_cubic (:397-604): the first run (the parts the servo uses), the branch, the echo.
+void servo_action(sdm_ctx* c) {
+  auto& S = c->servo;
+  S.completedNow = false;
+  if (S.firstRun) {
+    servo_box(c);
+    if (!S.p.continue_flag) {
+      S.width0 = S.width;
+      S.depth0 = S.depth;
+      S.height0 = S.height;
+    }
+    S.initVol = S.height0 * S.width0 * S.depth0;
+    S.rampNum = 0;
+    S.firstRun = false;
+    S.iterate_num = 0;
+    S.solve_num = 0;
+    servo_get_gain(c);
+  }
+  const double dt = c->prm.dt;
+  const bool echo = c->iter % S.p.echo_interval == 0;
+  if (S.p.z_servo) {
+    if (S.p.hydro_strain) servo_hydro(c, dt);
+    else servo_general(c, dt);
+    if (echo) {
+      S.unbf = unbalanced_force(c);
+      servo_consol_ss(c);
+    }
+  } else {
+    servo_shear(c, dt);
+    if (echo) {
+      S.unbf = unbalanced_force(c);
+      if (S.p.shear_mode & 2) servo_consol_ss(c);
     }
   }
 }
@@ -2217,8 +2329,20 @@ int sdmcpu_set_servo(sdm_ctx* c, const sdm_servo* s) {
     c->servo.on = false;
     return 0;
   }
-  if (s->iterate_num_max < 1 || s->solve_num_max < 1) return fail(c, "sdm_set_servo: intervals must be >= 1");
-  c->servo = sdm_ctx::Servo();
+  if (s->iterate_num_max < 1 || s->solve_num_max < 1 || s->echo_interval < 1)
+    return fail(c, "sdm_set_servo: intervals must be >= 1");
+  if (s->ramp_interval < 0 || s->ramp_chunks < 1 || s->shear_mode < 0)
+    return fail(c, "sdm_set_servo: ramp_interval >= 0, ramp_chunks >= 1 and shear_mode >= 0 are needed");
+  if (s->keep_state) {
+    if (c->servo.firstRun && !c->servo.on) return fail(c, "sdm_set_servo: keep_state needs a servo that ran");
+  } else {
+    c->servo = sdm_ctx::Servo();
+    if (s->continue_flag) {
+      c->servo.width0 = s->size0[0];
+      c->servo.depth0 = s->size0[1];
+      c->servo.height0 = s->size0[2];
+    }
+  }
   c->servo.p = *s;
   c->servo.on = true;
   return 0;
@@ -2244,6 +2368,11 @@ int sdmcpu_get_servo(sdm_ctx* c, sdm_servo_state* o) {
   o->iterate_num = S.iterate_num;
   o->solve_num = S.solve_num;
   o->completed_iter = S.completedIter;
+  o->size0[0] = S.width0;
+  o->size0[1] = S.depth0;
+  o->size0[2] = S.height0;
+  o->ramp_num = S.rampNum;
+  o->pad_ = 0;
   return 0;
 }
 

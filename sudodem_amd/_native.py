@@ -58,19 +58,25 @@ class sdm_servo(ctypes.Structure):
     _fields_ = [("goal", ctypes.c_double * 3), ("max_vel", ctypes.c_double), ("gain_alpha", ctypes.c_double),
                 ("f_threshold", ctypes.c_double), ("fmin_threshold", ctypes.c_double), ("unbf_tol", ctypes.c_double),
                 ("iterate_num_max", ctypes.c_int32), ("solve_num_max", ctypes.c_int32),
-                ("activated", ctypes.c_int32 * 4), ("z_servo", ctypes.c_int32)]
+                ("activated", ctypes.c_int32 * 4), ("z_servo", ctypes.c_int32), ("hydro_strain", ctypes.c_int32),
+                ("hydro_strain_rate", ctypes.c_double), ("shear_mode", ctypes.c_int32), ("ramp_interval", ctypes.c_int32),
+                ("ramp_chunks", ctypes.c_int32), ("echo_interval", ctypes.c_int32), ("target_strain", ctypes.c_double),
+                ("keep_state", ctypes.c_int32), ("continue_flag", ctypes.c_int32), ("size0", ctypes.c_double * 3)]
 
 
 class sdm_servo_state(ctypes.Structure):
     _fields_ = [("wall_stress", ctypes.c_double * 6), ("stress", ctypes.c_double * 3), ("gain", ctypes.c_double * 6),
                 ("stiffness", ctypes.c_double * 6), ("size", ctypes.c_double * 3), ("mean_stress", ctypes.c_double),
                 ("deviator_stress", ctypes.c_double), ("unbalanced_force", ctypes.c_double),
-                ("iterate_num", ctypes.c_int32), ("solve_num", ctypes.c_int32), ("completed_iter", ctypes.c_int64)]
+                ("iterate_num", ctypes.c_int32), ("solve_num", ctypes.c_int32), ("completed_iter", ctypes.c_int64),
+                ("size0", ctypes.c_double * 3), ("ramp_num", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 # CompressionEngine attribute defaults (pkg/dem/Compression.hpp:160-210) for the fields of sdm_servo
 SERVO_DEFAULTS = dict(goal=(0.0, 0.0, 0.0), max_vel=1.0, gain_alpha=0.5, f_threshold=0.01, fmin_threshold=0.01,
-                      unbf_tol=0.01, iterate_num_max=100, solve_num_max=2000, activated=(1, 1, 1, 1), z_servo=1)
+                      unbf_tol=0.01, iterate_num_max=100, solve_num_max=2000, activated=(1, 1, 1, 1), z_servo=1,
+                      hydro_strain=0, hydro_strain_rate=0.0, shear_mode=1, ramp_interval=10000, ramp_chunks=400,
+                      echo_interval=1000, target_strain=0.15, keep_state=0, continue_flag=0, size0=(0.0, 0.0, 0.0))
 
 
 class SdmError(RuntimeError):
@@ -216,8 +222,10 @@ class Context:
         return out
 
     def set_servo(self, enabled=True, **kw):
-        """Wall stress servo (CompressionEngine, cubic consolidation); keyword arguments are the
-        engine's attributes (goal=(goalx, goaly, goalz), max_vel, gain_alpha, ...)."""
+        """Wall stress servo (CompressionEngine, cubic specimen: consolidation, hydroConsolidation,
+        triaxial shear); keyword arguments are the engine's attributes (goal=(goalx, goaly, goalz),
+        max_vel, gain_alpha, z_servo, shear_mode, ...).  keep_state=1 changes the attributes of the
+        running engine instead of starting a new one."""
         if not enabled:
             self.lib.check(self.lib.set_servo(self.ptr, None), self.ptr)
             return
@@ -228,9 +236,12 @@ class Context:
         s = sdm_servo()
         s.goal = (ctypes.c_double * 3)(*[float(x) for x in p["goal"]])
         s.activated = (ctypes.c_int32 * 4)(*[int(x) for x in p["activated"]])
-        for k in ("max_vel", "gain_alpha", "f_threshold", "fmin_threshold", "unbf_tol"):
+        s.size0 = (ctypes.c_double * 3)(*[float(x) for x in p["size0"]])
+        for k in ("max_vel", "gain_alpha", "f_threshold", "fmin_threshold", "unbf_tol", "hydro_strain_rate",
+                  "target_strain"):
             setattr(s, k, float(p[k]))
-        for k in ("iterate_num_max", "solve_num_max", "z_servo"):
+        for k in ("iterate_num_max", "solve_num_max", "z_servo", "hydro_strain", "shear_mode", "ramp_interval",
+                  "ramp_chunks", "echo_interval", "keep_state", "continue_flag"):
             setattr(s, k, int(p[k]))
         self.lib.check(self.lib.set_servo(self.ptr, ctypes.byref(s)), self.ptr)
 

@@ -965,16 +965,16 @@ void step_body(sdm_ctx* c, const StepArgs& A) {
   if (c->servoOn) {
     // CompressionEngine between InteractionLoop and NewtonIntegrator; every decision on the device
     ServoDev* S = c->dServo;
-    hipLaunchKernelGGL(k_servo_begin, dim3(1), dim3(64), 0, c->st, S, c->B);
+    const unsigned long long* iterDev = (const unsigned long long*)(c->dcnt + DC_NP + 1);
+    hipLaunchKernelGGL(k_servo_begin, dim3(1), dim3(64), 0, c->st, S, c->B, iterDev);
     hipLaunchKernelGGL(k_servo_gain, dim3(6), dim3(64), 0, c->st, S, c->P, c->csrOff, c->csrPair);
-    hipLaunchKernelGGL(k_servo_step, dim3(1), dim3(64), 0, c->st, S, c->B, c->prm.dt);
+    hipLaunchKernelGGL(k_servo_step, dim3(1), dim3(64), 0, c->st, S, c->B, c->prm.dt, iterDev);
     hipLaunchKernelGGL(k_servo_gain, dim3(6), dim3(64), 0, c->st, S, c->P, c->csrOff, c->csrPair);
     const int64_t nb = (int64_t)nblk(std::max<int64_t>({(int64_t)1, A.np, (int64_t)c->B.n}));
     hipLaunchKernelGGL(k_servo_check_partial, dim3((unsigned)nb), dim3(256), 0, c->st, S, c->P,
                        (const unsigned long long*)(c->dcnt + DC_NP), c->B, c->prm.gravity[0], c->prm.gravity[1],
                        c->prm.gravity[2], c->servoPart);
-    hipLaunchKernelGGL(k_servo_check_final, dim3(1), dim3(64), 0, c->st, S, c->B, c->servoPart, nb,
-                       (const unsigned long long*)(c->dcnt + DC_NP + 1));
+    hipLaunchKernelGGL(k_servo_check_final, dim3(1), dim3(64), 0, c->st, S, c->B, c->servoPart, nb, iterDev);
   }
   hipLaunchKernelGGL(k_newton, dim3(nblk(c->B.n)), dim3(256), 0, c->st, NA);
   if (c->servoOn) hipLaunchKernelGGL(k_servo_end, dim3(1), dim3(64), 0, c->st, c->dServo);
@@ -1619,9 +1619,25 @@ int sdm_set_servo(sdm_ctx* c, const sdm_servo* sv) {
       c->servoOn = false;
       return 0;
     }
-    if (sv->iterate_num_max < 1 || sv->solve_num_max < 1) return fail(c, "sdm_set_servo: intervals must be >= 1");
+    if (sv->iterate_num_max < 1 || sv->solve_num_max < 1 || sv->echo_interval < 1)
+      return fail(c, "sdm_set_servo: intervals must be >= 1");
+    if (sv->ramp_interval < 0 || sv->ramp_chunks < 1 || sv->shear_mode < 0)
+      return fail(c, "sdm_set_servo: ramp_interval >= 0, ramp_chunks >= 1 and shear_mode >= 0 are needed");
     ServoDev h;
     std::memset(&h, 0, sizeof(h));
+    if (sv->keep_state) {  // the running engine's state, new attributes
+      if (!c->dServo) return fail(c, "sdm_set_servo: keep_state needs a servo that ran");
+      HIPCHK(hipMemcpy(&h, c->dServo, sizeof(h), hipMemcpyDeviceToHost));
+    } else {
+      h.first = 1;
+      h.completedIter = -1;
+      h.unbf = 1.0;  // CompressionEngine::UnbalancedForce default
+      if (sv->continue_flag) {
+        h.width0 = sv->size0[0];
+        h.depth0 = sv->size0[1];
+        h.height0 = sv->size0[2];
+      }
+    }
     for (int k = 0; k < 3; k++) h.goal[k] = sv->goal[k];
     h.max_vel = sv->max_vel;
     h.gain_alpha = sv->gain_alpha;
@@ -1632,9 +1648,14 @@ int sdm_set_servo(sdm_ctx* c, const sdm_servo* sv) {
     h.solve_num_max = sv->solve_num_max;
     for (int k = 0; k < 4; k++) h.activated[k] = sv->activated[k];
     h.z_servo = sv->z_servo;
-    h.first = 1;
-    h.completedIter = -1;
-    h.unbf = 1.0;  // CompressionEngine::UnbalancedForce default
+    h.hydro = sv->hydro_strain;
+    h.hydroRate = sv->hydro_strain_rate;
+    h.shearMode = sv->shear_mode;
+    h.rampInterval = sv->ramp_interval;
+    h.rampChunks = sv->ramp_chunks;
+    h.echoInterval = sv->echo_interval;
+    h.targetStrain = sv->target_strain;
+    h.cont = sv->continue_flag;
     if (!c->dServo) c->dServo = dalloc<ServoDev>(1);
     HIPCHK(hipMemcpy(c->dServo, &h, sizeof(h), hipMemcpyHostToDevice));
     c->servoOn = true;
@@ -1669,6 +1690,11 @@ int sdm_get_servo(sdm_ctx* c, sdm_servo_state* o) {
     o->iterate_num = h.iterate_num;
     o->solve_num = h.solve_num;
     o->completed_iter = h.completedIter;
+    o->size0[0] = h.width0;
+    o->size0[1] = h.depth0;
+    o->size0[2] = h.height0;
+    o->ramp_num = h.rampNum;
+    o->pad_ = 0;
   } catch (HipError& e) {
     return fail(c, e.msg);
   }

@@ -1,4 +1,5 @@
-"""Wall stress servo on the GPU (libsdm_hip.so) against the oracle.
+"""Wall stress servo on the GPU (libsdm_hip.so) against the oracle: consolidation,
+hydroConsolidation, drained and constant-volume shear, the echo.
 
 Bars: the servo's state and the wall positions follow the oracle's through gain updates,
 stability checks and completion (FP64 step; wall forces are ordered sums, stiffness sums are in
@@ -89,3 +90,56 @@ def test_servo_in_step_graph(gpu):
     assert np.array_equal(pa, pb)
     for k in sa:
         assert np.array_equal(np.asarray(sa[k]), np.asarray(sb[k])), k
+
+
+def _agree_more(g, c, what):
+    _agree(g, c, what)
+    sg, sc = g.servo_state(), c.servo_state()
+    assert sg["ramp_num"] == sc["ramp_num"], what
+    assert np.array_equal(sg["size0"], sc["size0"]), what
+
+
+def test_shear_after_consolidation_follows_oracle(gpu, oracle):
+    # consolidation, then the same engine switched to drained shear (ramp of example2_shear.py)
+    g, c = _pair(gpu, oracle, **SERVO, echo_interval=7)
+    for ctx in (g, c):
+        ctx.step(12)
+    _agree_more(g, c, "consolidated")
+    dt = float(c.get("scene.dt")[0])
+    shear = {**SERVO, "goal": (2e4, 2e4, 2e-4 / dt), "z_servo": 0, "keep_state": 1, "ramp_interval": 10,
+             "ramp_chunks": 2, "echo_interval": 7, "iterate_num_max": 4}
+    for ctx in (g, c):
+        ctx.set_servo(**shear)
+    for k in range(4):
+        g.step(6)
+        c.step(6)
+        _agree_more(g, c, "shear, %d steps" % (6 * (k + 1)))
+    assert g.servo_state()["ramp_num"] == 10
+
+
+def test_constant_volume_shear_ends_at_same_iteration(gpu, oracle):
+    g, c = _pair(gpu, oracle, **SERVO)
+    dt = float(c.get("scene.dt")[0])
+    shear = {**SERVO, "goal": (2e4, 2e4, 1e-3 / dt), "z_servo": 0, "shear_mode": 2, "ramp_interval": 4,
+             "ramp_chunks": 2, "echo_interval": 3, "target_strain": 0.01}
+    for ctx in (g, c):
+        ctx.set_servo(**shear)
+        ctx.step(100)
+    assert int(g.get("scene.iter")[0]) == int(c.get("scene.iter")[0]) < 3 + 100
+    _agree_more(g, c, "end of shear")
+    assert g.servo_state()["completed_iter"] >= 0
+
+
+def test_hydro_follows_oracle(gpu, oracle):
+    hydro = {**SERVO, "hydro_strain": 1, "hydro_strain_rate": 5.0, "solve_num_max": 6, "iterate_num_max": 4,
+             "echo_interval": 5, "f_threshold": 1e9, "fmin_threshold": 1e9, "unbf_tol": 1e9}
+    g, c = _pair(gpu, oracle, **hydro)
+    for k in range(3):
+        g.step(4)
+        c.step(4)
+        _agree_more(g, c, "hydro, %d steps" % (4 * (k + 1)))
+    for ctx in (g, c):
+        ctx.step(100)
+    assert int(g.get("scene.iter")[0]) == int(c.get("scene.iter")[0]) < 3 + 112
+    _agree_more(g, c, "hydro completion")
+    assert np.any(g.get("state.vel") != 0)

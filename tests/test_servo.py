@@ -3,7 +3,7 @@
 Pinned by what the engine's definitions fix, from the exported state (pkg/dem/Compression.cpp):
  * with no wall contact every gain is infinite and each active wall moves inwards by exactly
    max_vel dt per step (servo :1252-1316 clamps |gain (stress - goal)| to max_vel dt);
-   inactive walls and, without z_servo, the z walls stay;
+   inactive walls stay;
  * wall stresses are the wall forces over the faces of the box the walls span (consol_ss
    :1330-1345, with sdm_body_forces as the independent read of the forces); each wall's
    stiffness is the sum of kn over its real contacts (get_gain :1125-1220, recomputed from the
@@ -76,7 +76,7 @@ def test_no_contact_walls_move_at_max_velocity(oracle):
     cfg, ctx = _scene(oracle, gravity=False, margin=-5e-3)
     dt = float(ctx.get("scene.dt")[0])
     p0 = ctx.get("state.pos").reshape(-1, 3)[:6].copy()
-    ctx.set_servo(goal=(1e5, 1e5, 1e5), max_vel=0.05, activated=(1, 0, 1, 1), z_servo=0)
+    ctx.set_servo(goal=(1e5, 1e5, 1e5), max_vel=0.05, activated=(1, 0, 1, 1))
     ctx.step(5)
     st = ctx.servo_state()
     assert np.all(st["stiffness"] == 0) and np.all(np.isinf(st["gain"]))
@@ -87,7 +87,9 @@ def test_no_contact_walls_move_at_max_velocity(oracle):
         want[0, 0] += lim   # left wall inwards (+x)
         want[2, 1] += lim   # front (+y)
         want[3, 1] -= lim   # back (-y)
-    assert np.array_equal(p, want)  # right wall inactive, z walls without z_servo
+        want[4, 2] += lim   # bottom (+z), under z_servo
+        want[5, 2] -= lim   # top (-z)
+    assert np.array_equal(p, want)  # right wall inactive
     assert st["iterate_num"] == 5 and st["solve_num"] == 5 and st["completed_iter"] == -1
 
 
@@ -177,3 +179,193 @@ def test_servo_off_and_argument_errors(oracle):
     assert np.array_equal(ctx.get("state.pos").reshape(-1, 3)[:6], p0)
     with pytest.raises(Exception):
         ctx.set_servo(goal=(1e5, 1e5, 1e5), solve_num_max=0)
+
+
+# --- hydroConsolidation, shear and the echo (Compression.cpp:264-322, :498-603, :483-497) ---
+
+def _walls(ctx):
+    return ctx.get("state.pos").reshape(-1, 3)[:6].copy()
+
+
+def test_shear_ramp_rates_and_lateral_walls(oracle):
+    # walls 5 mm clear of every particle: the ramp's axial rates of example2_shear.py
+    # (ramp_interval 10, ramp_chunks 2), then goalz; the lateral walls (drained, no contact) close
+    # at max_vel dt.  Rates for rampNum 1..10, worked out by hand from the engine's expression:
+    # first half float(r / 2.5 + 1) / 2 * 2 goalz, second half (float((10 - r) / 2.5 + 1) / 2 + 1) goalz.
+    f32 = lambda x: float(np.float32(x))
+    rates = [f32(1.4), f32(1.8), f32(2.2), f32(2.6), 2.5,
+             f32(2.6) / 2 + 1, f32(2.2) / 2 + 1, f32(1.8) / 2 + 1, f32(1.4) / 2 + 1, 1.5] + [1.0] * 4
+    cfg, ctx = _scene(oracle, gravity=False, margin=-5e-3)
+    dt = float(ctx.get("scene.dt")[0])
+    goalz = 0.01
+    ctx.set_servo(goal=(1e5, 1e5, goalz), max_vel=0.05, z_servo=0, ramp_interval=10, ramp_chunks=2,
+                  target_strain=0.4)
+    prev = _walls(ctx)
+    h0 = prev[5, 2] - prev[4, 2]
+    for k, m in enumerate(rates):
+        ctx.step(1)
+        w = _walls(ctx)
+        h = prev[5, 2] - prev[4, 2]  # the height of the previous step's wall stress evaluation
+        vel = m * goalz
+        tol = 2 * np.spacing(np.abs(prev[4:, 2]).max())  # the difference of two positions
+        assert w[4, 2] - prev[4, 2] == pytest.approx(vel * dt * h, rel=1e-12, abs=tol), k
+        assert w[5, 2] - prev[5, 2] == pytest.approx(-vel * dt * h, rel=1e-12, abs=tol), k
+        lim = 0.05 * dt
+        assert np.allclose(w[[0, 2], [0, 1]] - prev[[0, 2], [0, 1]], lim, rtol=1e-9, atol=0)
+        assert np.allclose(w[[1, 3], [0, 1]] - prev[[1, 3], [0, 1]], -lim, rtol=1e-9, atol=0)
+        prev = w
+    st = ctx.servo_state()
+    assert st["ramp_num"] == 10 and st["completed_iter"] == -1
+    assert st["size0"][2] == h0
+
+
+def test_shear_constant_volume_and_target_strain(oracle):
+    # shearMode 2 (undrained): the lateral walls keep width depth height at the first run's volume;
+    # shear ends in the first step whose axial log strain exceeds target_strain
+    cfg, ctx = _scene(oracle, gravity=False, margin=-5e-3)
+    w0 = _walls(ctx)
+    vol0 = (w0[1, 0] - w0[0, 0]) * (w0[3, 1] - w0[2, 1]) * (w0[5, 2] - w0[4, 2])
+    h0 = w0[5, 2] - w0[4, 2]
+    it0 = int(ctx.get("scene.iter")[0])
+    dt = float(ctx.get("scene.dt")[0])
+    ctx.set_servo(goal=(1e5, 1e5, 0.001 / dt), z_servo=0, shear_mode=2, ramp_interval=4, ramp_chunks=2,
+                  target_strain=0.02)  # 0.1 % axial strain per step after the ramp
+    done = None
+    for k in range(40):
+        ctx.step(1)
+        w = _walls(ctx)
+        width, depth, height = w[1, 0] - w[0, 0], w[3, 1] - w[2, 1], w[5, 2] - w[4, 2]
+        assert width * depth * height == pytest.approx(vol0, rel=1e-13)
+        assert (w[1, 0] + w[0, 0]) == pytest.approx(w0[1, 0] + w0[0, 0], abs=1e-15)  # symmetric moves
+        assert (w[3, 1] + w[2, 1]) == pytest.approx(w0[3, 1] + w0[2, 1], abs=1e-15)
+        if np.log(h0 / height) > 0.02:
+            done = it0 + k
+            break
+    assert done is not None
+    st = ctx.servo_state()
+    assert st["completed_iter"] == done
+    ctx.step(5)  # every later step ends the run again (stopAtIter = iter + 1)
+    assert int(ctx.get("scene.iter")[0]) == done + 2
+
+
+def test_hydro_moves_and_check_cadence(oracle):
+    # no contact: every gain is infinite and each wall closes by hydro_strain_rate size dt, the
+    # box size taken before any wall moves; the check counter restarts at 1, not 0
+    cfg, ctx = _scene(oracle, gravity=False, margin=-5e-3)
+    dt = float(ctx.get("scene.dt")[0])
+    rate = 0.3
+    ctx.set_servo(goal=(1e5, 1e5, 1e5), hydro_strain=1, hydro_strain_rate=rate, solve_num_max=4, iterate_num_max=3)
+    prev = _walls(ctx)
+    for k in range(1, 12):
+        ctx.step(1)
+        w = _walls(ctx)
+        size = [prev[1, 0] - prev[0, 0], prev[3, 1] - prev[2, 1], prev[5, 2] - prev[4, 2]]
+        for wall in range(6):
+            ax = wall // 2
+            want = rate * size[ax] * dt * (1 if wall % 2 == 0 else -1)
+            assert w[wall, ax] - prev[wall, ax] == pytest.approx(want, rel=1e-9, abs=0), (k, wall)
+        st = ctx.servo_state()
+        assert st["solve_num"] == (k if k < 4 else (k - 4) % 3 + 1), k
+        assert st["iterate_num"] == (k if k < 3 else (k - 3) % 2 + 1), k
+        prev = w
+
+
+def test_hydro_step_bounded_by_gain(oracle):
+    cfg, ctx = _scene(oracle, margin=8e-4)
+    ctx.step(3)
+    goal = 5e4
+    ctx.set_servo(goal=(goal,) * 3, hydro_strain=1, hydro_strain_rate=50.0, iterate_num_max=1000, echo_interval=10**6)
+    ctx.step(1)
+    dt = float(ctx.get("scene.dt")[0])
+    for k in range(3):
+        st = ctx.servo_state()
+        prev = _walls(ctx)
+        ctx.step(1)
+        F = ctx.body_forces(WALLS)
+        width, depth, height = prev[1, 0] - prev[0, 0], prev[3, 1] - prev[2, 1], prev[5, 2] - prev[4, 2]
+        area = [depth * height, width * height, depth * width]
+        size = [width, depth, height]
+        w = _walls(ctx)
+        for wall in range(6):
+            ax = wall // 2
+            delta = goal - abs(F[wall, ax] / area[ax])
+            step = min(0.1 * st["gain"][wall] * delta, 50.0 * size[ax] * dt) if delta > 0 else 0.0
+            want = step if wall % 2 == 0 else -step
+            assert w[wall, ax] - prev[wall, ax] == pytest.approx(want, rel=1e-9, abs=1e-20), (k, wall)
+    assert np.all(np.isfinite(st["gain"]))
+
+
+def test_hydro_completion_uses_echo_value(oracle):
+    loose = dict(goal=(1e3,) * 3, hydro_strain=1, hydro_strain_rate=1e-3, f_threshold=1e9, fmin_threshold=1e9,
+                 solve_num_max=4)
+    # no echo within the run: UnbalancedForce keeps its default 1, above unbf_tol 0.99
+    cfg, ctx = _scene(oracle, margin=8e-4)
+    ctx.step(3)
+    ctx.set_servo(**loose, unbf_tol=0.99, echo_interval=10**6)
+    ctx.step(12)
+    assert int(ctx.get("scene.iter")[0]) == 15 and ctx.servo_state()["completed_iter"] == -1
+    assert ctx.servo_state()["unbalanced_force"] == 1.0
+    # an echo every step: the check of the fourth step completes, and the system is not quieted
+    cfg, ctx = _scene(oracle, margin=8e-4)
+    ctx.step(3)
+    ctx.set_servo(**loose, unbf_tol=1e9, echo_interval=1)
+    ctx.step(12)
+    assert int(ctx.get("scene.iter")[0]) == 3 + 4
+    assert ctx.servo_state()["completed_iter"] == 3 + 3
+    assert np.any(ctx.get("state.vel") != 0)
+
+
+def test_echo_refreshes_unbalanced_force_and_wall_stress(oracle):
+    cfg, ctx = _scene(oracle, margin=8e-4)
+    ctx.step(3)
+    ctx.set_servo(goal=(2e4,) * 3, max_vel=0.2, echo_interval=2, solve_num_max=10**6)
+    for k in range(4):
+        ctx.step(1)
+        it = int(ctx.get("scene.iter")[0]) - 1  # Scene::iter during the step
+        st = ctx.servo_state()
+        w = _walls(ctx)
+        width, depth, height = w[1, 0] - w[0, 0], w[3, 1] - w[2, 1], w[5, 2] - w[4, 2]
+        if it % 2 == 0:  # consol_ss after the moves: the box at the end of the step
+            assert tuple(st["size"]) == (width, depth, height)
+            F = ctx.body_forces(WALLS)
+            assert st["wall_stress"][1] == pytest.approx(F[1, 0] / (depth * height), rel=1e-14)
+            assert st["wall_stress"][4] == pytest.approx(-F[4, 2] / (depth * width), rel=1e-14)
+            Fb = ctx.get("forces.force").reshape(-1, 3)
+            m, g = ctx.get("state.mass"), ctx.get("newton.gravity")
+            free = ctx.get("state.blockedDOFs") != 63
+            num = np.linalg.norm(Fb[free] + m[free, None] * g, axis=1).sum()
+            real = (ctx.get("intr.hasGeom") != 0) & (ctx.get("intr.hasPhys") != 0)
+            fn, fs = ctx.get("phys.normalForce").reshape(-1, 3), ctx.get("phys.shearForce").reshape(-1, 3)
+            act = real & (np.linalg.norm(fn, axis=1) != 0)
+            den = 2.0 * np.linalg.norm(fn[act] + fs[act], axis=1).sum()
+            assert st["unbalanced_force"] == pytest.approx(num / den, rel=1e-12)
+        else:  # the servo's own evaluation, before its moves
+            assert st["size"][0] != width or st["size"][2] != height
+
+
+def test_keep_state_switches_to_shear(oracle):
+    # consolidation, then z_servo = 0 on the same engine: counters, gains and the first run's
+    # sizes carry over (no new first run)
+    cfg, ctx = _scene(oracle, margin=8e-4)
+    ctx.step(3)
+    ctx.set_servo(goal=(2e4,) * 3, max_vel=0.2, iterate_num_max=7)
+    ctx.step(5)
+    a = ctx.servo_state()
+    ctx.set_servo(goal=(2e4, 2e4, 0.01), max_vel=0.2, iterate_num_max=7, z_servo=0, keep_state=1,
+                  ramp_interval=4, ramp_chunks=2)
+    ctx.step(1)
+    b = ctx.servo_state()
+    assert np.array_equal(a["size0"], b["size0"]) and b["ramp_num"] == 1
+    assert b["iterate_num"] == a["iterate_num"]  # no gain counter during the ramp
+    assert np.array_equal(a["gain"], b["gain"])
+    # a new engine would take the current box as its reference
+    ctx.set_servo(goal=(2e4, 2e4, 0.01), z_servo=0)
+    ctx.step(1)
+    assert ctx.servo_state()["size0"][2] != a["size0"][2]
+    # continueFlag: reference sizes given, no ramp
+    ctx.set_servo(goal=(2e4, 2e4, 0.01), z_servo=0, continue_flag=1, size0=(1.0, 2.0, 3.0))
+    ctx.step(1)
+    c = ctx.servo_state()
+    assert tuple(c["size0"]) == (1.0, 2.0, 3.0) and c["ramp_num"] == 0
+    with pytest.raises(Exception):
+        ctx.set_servo(goal=(1e5,) * 3, ramp_chunks=0)
