@@ -21,7 +21,8 @@ The same script runs here against the MI355X library::
     O.run(100, True)
 
 Only this engine list is accepted (anything else raises): it is the path the library
-implements.  Body data is held host-side until the first step, then lives on the device;
+implements, optionally with a ``CompressionEngine`` (the cubic specimen's wall servo) between
+the InteractionLoop and the NewtonIntegrator, as examples/SudoDEM3D/example2_consol.py has it.  Body data is held host-side until the first step, then lives on the device;
 ``O.bodies[i].state.pos`` and the bulk accessors ``O.get/O.set`` read or write it there.
 """
 from __future__ import annotations
@@ -213,6 +214,97 @@ class NewtonIntegrator:
         self.__dict__.update(kw)
 
 
+class CompressionEngine:
+    """Wall stress servo of a cubic specimen (pkg/dem/Compression.hpp:160-210), placed between
+    InteractionLoop and NewtonIntegrator (examples/SudoDEM3D/example2_consol.py:78-105).
+
+    Attributes carry the reference's names and defaults.  Assigning one while the scene runs
+    changes the running engine (``triax.z_servo = False`` starts shear with the state of the
+    consolidation, as in example2_shear.py:60-71); ``dead`` switches it off and on.  The
+    engine's state reads back as ``wsxx``, ``wsyy``, ``wszz``, ``width``, ``depth``,
+    ``height``, ``UnbalancedForce`` and, after the first run, ``width0``, ``depth0``,
+    ``height0``.  Assigning ``width0``/``depth0``/``height0`` takes effect at a first run with
+    ``continueFlag``.  File output (``file``, ``savedata_interval``) and the console echo text
+    are not produced."""
+
+    _DEFAULTS = dict(goalx=0.0, goaly=0.0, goalz=0.0, max_vel=1.0, gain_alpha=0.5, f_threshold=0.01,
+                     fmin_threshold=0.01, unbf_tol=0.01, iterate_num_max=100, solve_num_max=2000,
+                     left_wall_activated=True, right_wall_activated=True, front_wall_activated=True,
+                     back_wall_activated=True, bottom_wall_activated=True, top_wall_activated=True, z_servo=True,
+                     hydroStrain=False, hydroStrainRate=0.0, shearMode=1, ramp_interval=10000, ramp_chunks=400,
+                     echo_interval=1000, target_strain=0.15, continueFlag=False, width0=0.0, depth0=0.0,
+                     height0=0.0, cubic_specimen=True, dead=False, label="", savedata_interval=12,
+                     file="recordData", debug=False, interStressControl=False)
+    _STATE = ("wsxx", "wsyy", "wszz", "width", "depth", "height", "UnbalancedForce")
+
+    def __init__(self, **kw):
+        object.__setattr__(self, "_p", dict(self._DEFAULTS))
+        object.__setattr__(self, "_ctx", None)
+        unknown = set(kw) - set(self._DEFAULTS)
+        if unknown:
+            raise AttributeError("CompressionEngine has no attribute(s) %s" % sorted(unknown))
+        self._p.update(kw)
+
+    def _servo_kw(self):
+        p = self._p
+        if not p["cubic_specimen"]:
+            raise ValueError("only the cubic specimen is supported (cubic_specimen=True)")
+        if p["interStressControl"]:
+            raise ValueError("interStressControl is not supported (the SOLE_GAIN servo uses wall stresses)")
+        return dict(goal=(p["goalx"], p["goaly"], p["goalz"]), max_vel=p["max_vel"], gain_alpha=p["gain_alpha"],
+                    f_threshold=p["f_threshold"], fmin_threshold=p["fmin_threshold"], unbf_tol=p["unbf_tol"],
+                    iterate_num_max=p["iterate_num_max"], solve_num_max=p["solve_num_max"],
+                    activated=(p["left_wall_activated"], p["right_wall_activated"], p["front_wall_activated"],
+                               p["back_wall_activated"]),
+                    z_servo=p["z_servo"], hydro_strain=p["hydroStrain"], hydro_strain_rate=p["hydroStrainRate"],
+                    shear_mode=p["shearMode"], ramp_interval=p["ramp_interval"], ramp_chunks=p["ramp_chunks"],
+                    echo_interval=p["echo_interval"], target_strain=p["target_strain"],
+                    continue_flag=p["continueFlag"], size0=(p["width0"], p["depth0"], p["height0"]))
+
+    def _attach(self, ctx):
+        object.__setattr__(self, "_ctx", ctx)
+        object.__setattr__(self, "_set", False)
+        self._push()
+
+    def _push(self):
+        ctx = self._ctx
+        if ctx is None:
+            return
+        if self._p["dead"]:
+            if self._set:
+                ctx.set_servo(False)
+            return
+        ctx.set_servo(**self._servo_kw(), keep_state=1 if self._set else 0)
+        object.__setattr__(self, "_set", True)
+
+    def __getattr__(self, k):
+        p = object.__getattribute__(self, "_p")
+        ctx = object.__getattribute__(self, "_ctx")
+        live = ctx is not None and self.__dict__.get("_set", False)
+        if k in self._STATE or (live and k in ("width0", "depth0", "height0")):
+            if not live:
+                return 0.0 if k != "UnbalancedForce" else 1.0
+            st = ctx.servo_state()
+            if k == "UnbalancedForce":
+                return float(st["unbalanced_force"])
+            if k in ("wsxx", "wsyy", "wszz"):
+                return float(st["stress"]["xyz".index(k[2])])
+            if k in ("width", "depth", "height"):
+                return float(st["size"][("width", "depth", "height").index(k)])
+            return float(st["size0"][("width0", "depth0", "height0").index(k)])
+        if k in p:
+            return p[k]
+        raise AttributeError(k)
+
+    def __setattr__(self, k, v):
+        if k in self._STATE:
+            raise AttributeError("%s is read-only here (the engine's state)" % k)
+        if k not in self._p:
+            raise AttributeError("CompressionEngine has no attribute %s" % k)
+        self._p[k] = v
+        self._push()
+
+
 class Cell:
     """Axis-aligned periodic cell (core/Cell.hpp): only its size is used (no velGrad)."""
 
@@ -233,10 +325,14 @@ def engines_to_params(engines, dt, precision=nat.SDM_FP64, cell=None):
     """Validate the engine list and map its attributes onto sdm_params; `cell` (three sizes)
     makes the scene periodic (Scene::isPeriodic with an axis-aligned Cell)."""
     kinds = [type(e).__name__ for e in engines]
-    if kinds != ["ForceResetter", "InsertionSortCollider", "InteractionLoop", "NewtonIntegrator"]:
+    if kinds not in (["ForceResetter", "InsertionSortCollider", "InteractionLoop", "NewtonIntegrator"],
+                     ["ForceResetter", "InsertionSortCollider", "InteractionLoop", "CompressionEngine",
+                      "NewtonIntegrator"]):
         raise ValueError("engine list must be [ForceResetter, InsertionSortCollider, InteractionLoop, "
-                         f"NewtonIntegrator]; got {kinds}")
-    col, loop, newton = engines[1], engines[2], engines[3]
+                         f"(CompressionEngine,) NewtonIntegrator]; got {kinds}")
+    col, loop, newton = engines[1], engines[2], engines[-1]
+    if len(engines) == 5 and cell is not None:
+        raise ValueError("CompressionEngine needs the six walls of a cubic specimen, not a periodic cell")
     bo = sorted(type(f).__name__ for f in col.boundFunctors)
     if bo and not set(bo) <= {"Bo1_Superquadrics_Aabb", "Bo1_Wall_Aabb"}:
         raise ValueError(f"unsupported bound functors {bo}")
@@ -376,6 +472,9 @@ class Omega:
         upload_arrays(self.ctx, self._arrays())
         for b in self.bodies:
             b._scene = self
+        for e in self.engines:
+            if isinstance(e, CompressionEngine):
+                e._attach(self.ctx)
 
     def step(self):
         self._open()

@@ -369,3 +369,68 @@ def test_keep_state_switches_to_shear(oracle):
     assert tuple(c["size0"]) == (1.0, 2.0, 3.0) and c["ramp_num"] == 0
     with pytest.raises(Exception):
         ctx.set_servo(goal=(1e5,) * 3, ramp_chunks=0)
+
+
+# --- the scripting mirror (CompressionEngine between InteractionLoop and NewtonIntegrator) ---
+
+def _omega_box(oracle, triax, n=3, a=1e-3, gap=0.2e-3):
+    from sudodem_amd import scene as S
+    O = S.Omega(lib=oracle)
+    mat = S.SuperquadricsMat(Kn=1e5, Ks=7e4, frictionAngle=0.5, density=2650.0)
+    O.materials.append(mat)
+    side = n * 2 * a + (n + 1) * gap
+    for axis in range(3):  # left, right, front, back, bottom, top: ids 0-5
+        O.bodies.append(S.utils.wall(0.0, axis=axis, sense=1, material=mat))
+        O.bodies.append(S.utils.wall(side, axis=axis, sense=-1, material=mat))
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                b = S.NewSuperquadrics_rot2(a, 0.8 * a, 0.9 * a, 0.7, 1.2, mat, 1, 0, 0, 0, False)
+                b.state.pos = tuple(gap + a + (2 * a + gap) * np.array([i, j, k]))
+                O.bodies.append(b)
+    O.engines = [S.ForceResetter(), S.InsertionSortCollider([S.Bo1_Superquadrics_Aabb(), S.Bo1_Wall_Aabb()],
+                                                            verletDist=1e-4),
+                 S.InteractionLoop([S.Ig2_Wall_Superquadrics_SuperquadricsGeom(),
+                                    S.Ig2_Superquadrics_Superquadrics_SuperquadricsGeom()],
+                                   [S.Ip2_SuperquadricsMat_SuperquadricsMat_SuperquadricsPhys()],
+                                   [S.SuperquadricsLaw()]),
+                 triax,
+                 S.NewtonIntegrator(damping=0.3, gravity=(0.0, 0.0, 0.0), isSuperquadrics=1)]
+    O.dt = 1e-5
+    return O
+
+
+def test_compression_engine_in_the_engine_list(oracle):
+    from sudodem_amd import scene as S
+    triax = S.CompressionEngine(goalx=1e4, goaly=1e4, goalz=1e4, max_vel=0.5, echo_interval=2000,
+                                ramp_interval=6, ramp_chunks=2, file="record-con")
+    O = _omega_box(oracle, triax)
+    w0 = O.get("state.pos").reshape(-1, 3)[:6].copy()
+    O.run(20, True)
+    st = O.ctx.servo_state()
+    assert triax.wsxx == st["stress"][0] and triax.height == st["size"][2]
+    # (the echo of iteration 0 divided 0 by 0 contact forces: NaN, as the engine computes it)
+    assert np.array_equal([triax.UnbalancedForce], [st["unbalanced_force"]], equal_nan=True)
+    w1 = O.get("state.pos").reshape(-1, 3)[:6].copy()
+    assert np.all(w1[[0, 2, 4], [0, 1, 2]] > w0[[0, 2, 4], [0, 1, 2]])  # every wall closed in
+    assert np.all(w1[[1, 3, 5], [0, 1, 2]] < w0[[1, 3, 5], [0, 1, 2]])
+    h0 = triax.height0
+    assert h0 == w0[5, 2] - w0[4, 2]  # the first run's box
+    # shear on the running engine, as example2_shear.py assigns it
+    triax.z_servo = False
+    triax.goalz = 0.01
+    O.run(4, True)
+    st = O.ctx.servo_state()
+    assert st["ramp_num"] == 4 and triax.height0 == h0
+    # dead: the engine is skipped, the walls stay
+    triax.dead = True
+    w2 = O.get("state.pos").reshape(-1, 3)[:6].copy()
+    O.run(3, True)
+    assert np.array_equal(O.get("state.pos").reshape(-1, 3)[:6], w2)
+    triax.dead = False
+    O.run(1, True)
+    assert O.ctx.servo_state()["ramp_num"] == 5  # the state carried over
+    with pytest.raises(AttributeError):
+        triax.wsxx = 1.0
+    with pytest.raises(AttributeError):
+        S.CompressionEngine(goal_x=1.0)
