@@ -365,6 +365,7 @@ struct sdm_ctx {
     int64_t completedIter = -1;
     bool completedNow = false;
   } servo;
+  bool servoEverSet = false;  // a servo state exists (set once; kept while switched off)
   // Scene
   int64_t iter = 0;
   double time = 0;
@@ -2334,7 +2335,7 @@ int sdmcpu_set_servo(sdm_ctx* c, const sdm_servo* s) {
   if (s->ramp_interval < 0 || s->ramp_chunks < 1 || s->shear_mode < 0)
     return fail(c, "sdm_set_servo: ramp_interval >= 0, ramp_chunks >= 1 and shear_mode >= 0 are needed");
   if (s->keep_state) {
-    if (c->servo.firstRun && !c->servo.on) return fail(c, "sdm_set_servo: keep_state needs a servo that ran");
+    if (!c->servoEverSet) return fail(c, "sdm_set_servo: keep_state needs a servo that was set before");
   } else {
     c->servo = sdm_ctx::Servo();
     if (s->continue_flag) {
@@ -2345,6 +2346,7 @@ int sdmcpu_set_servo(sdm_ctx* c, const sdm_servo* s) {
   }
   c->servo.p = *s;
   c->servo.on = true;
+  c->servoEverSet = true;
   return 0;
 }
 

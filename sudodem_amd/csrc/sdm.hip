@@ -1626,7 +1626,7 @@ int sdm_set_servo(sdm_ctx* c, const sdm_servo* sv) {
     ServoDev h;
     std::memset(&h, 0, sizeof(h));
     if (sv->keep_state) {  // the running engine's state, new attributes
-      if (!c->dServo) return fail(c, "sdm_set_servo: keep_state needs a servo that ran");
+      if (!c->dServo) return fail(c, "sdm_set_servo: keep_state needs a servo that was set before");
       HIPCHK(hipMemcpy(&h, c->dServo, sizeof(h), hipMemcpyDeviceToHost));
     } else {
       h.first = 1;

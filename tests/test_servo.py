@@ -434,3 +434,16 @@ def test_compression_engine_in_the_engine_list(oracle):
         triax.wsxx = 1.0
     with pytest.raises(AttributeError):
         S.CompressionEngine(goal_x=1.0)
+
+
+def test_keep_state_needs_a_servo_and_survives_off(oracle):
+    cfg, ctx = _scene(oracle, margin=8e-4)
+    with pytest.raises(Exception):
+        ctx.set_servo(goal=(2e4,) * 3, keep_state=1)
+    ctx.set_servo(goal=(2e4,) * 3, iterate_num_max=50, solve_num_max=50)
+    ctx.step(4)
+    ctx.set_servo(False)
+    ctx.step(2)
+    ctx.set_servo(goal=(2e4,) * 3, iterate_num_max=50, solve_num_max=50, keep_state=1)
+    ctx.step(1)
+    assert ctx.servo_state()["iterate_num"] == 5  # 4 steps on, 2 off, 1 on
