@@ -287,9 +287,9 @@ __global__ __launch_bounds__(256) void k_newton(NewtonArgs A) {
         const double angle = sqrt(angle2);
         const V3<double> ax = mk3(angVel.x / angle, angVel.y / angle, angVel.z / angle);
         const double ha = 0.5 * (angle * dt);
-        const double sh = sin(ha);
+        const double sh = xsin(ha);
         Q4 q;
-        q.w = cos(ha); q.x = sh * ax.x; q.y = sh * ax.y; q.z = sh * ax.z;
+        q.w = xcos(ha); q.x = sh * ax.x; q.y = sh * ax.y; q.z = sh * ax.z;
         ori = quat_mul(q, ori);
       }
       ori = quat_normalized(ori);

@@ -9,6 +9,8 @@
 
 #define SDM_HD __host__ __device__ __forceinline__
 
+#include "crmath.h"
+
 template <typename T>
 struct V3 {
   T x, y, z;
@@ -114,15 +116,18 @@ SDM_HD Q4 quat_mul(Q4 a, Q4 b) {
   return o;
 }
 
-// Transcendentals by precision.  Double maps to the device libm; float to its float form.
-SDM_HD double xpow(double a, double b) { return pow(a, b); }
+// Transcendentals by precision.  Double pow, sin, cos, atan2 and atan are the correctly
+// rounded versions of crmath.h (the values glibc returns to the reference for all but ~0.1 %
+// of arguments); float maps to the device's float functions.
+SDM_HD double xpow(double a, double b) { return crm::cr_pow(a, b); }
 SDM_HD float xpow(float a, float b) { return powf(a, b); }
-SDM_HD double xsin(double a) { return sin(a); }
+SDM_HD double xsin(double a) { return crm::cr_sin(a); }
 SDM_HD float xsin(float a) { return sinf(a); }
-SDM_HD double xcos(double a) { return cos(a); }
+SDM_HD double xcos(double a) { return crm::cr_cos(a); }
 SDM_HD float xcos(float a) { return cosf(a); }
-SDM_HD double xatan2(double a, double b) { return atan2(a, b); }
+SDM_HD double xatan2(double a, double b) { return crm::cr_atan2(a, b); }
 SDM_HD float xatan2(float a, float b) { return atan2f(a, b); }
+SDM_HD double xatan(double a) { return crm::cr_atan(a); }
 SDM_HD double xfabs(double a) { return fabs(a); }
 SDM_HD float xfabs(float a) { return fabsf(a); }
 SDM_HD double xsqrt(double a) { return sqrt(a); }

@@ -16,23 +16,66 @@ struct Shape {
   T rx, ry, rz, eps1, eps2;
 };
 
+// Math of the support map in double: correctly rounded (kFast = false; the FP64 form, whose
+// values must round as the reference's glibc rounds them) or the device library's (kFast; the
+// mixed form's final double evaluation, which needs double accuracy, not the same last bits).
+// Float keeps the device's float functions either way.
+template <bool kFast>
+SDM_HD double mpow(double a, double b) {
+  if constexpr (kFast) return pow(a, b);
+  else return crm::cr_pow(a, b);
+}
+template <bool kFast>
+SDM_HD float mpow(float a, float b) { return powf(a, b); }
+template <bool kFast>
+SDM_HD double matan2(double a, double b) {
+  if constexpr (kFast) return atan2(a, b);
+  else return crm::cr_atan2(a, b);
+}
+template <bool kFast>
+SDM_HD float matan2(float a, float b) { return atan2f(a, b); }
+template <bool kFast>
+SDM_HD double mcos(double a) {
+  if constexpr (kFast) return cos(a);
+  else return crm::cr_cos(a);
+}
+template <bool kFast>
+SDM_HD float mcos(float a) { return cosf(a); }
+// sin and cos of one angle (the same values as the separate functions)
+template <bool kFast>
+SDM_HD void msincos(double a, double& sn, double& cs) {
+  if constexpr (kFast) {
+    sn = sin(a);
+    cs = cos(a);
+  } else {
+    crm::cr_sincos(a, &sn, &cs);
+  }
+}
+template <bool kFast>
+SDM_HD void msincos(float a, float& sn, float& cs) {
+  sn = sinf(a);
+  cs = cosf(a);
+}
+
 // Normal2Phi, active branches (Superquadrics.cpp:284, 288).
-template <typename T>
+template <typename T, bool kFast = false>
 SDM_HD void normal2phi(const Shape<T>& s, V3<T> n, T& ph0, T& ph1) {
   const T e1 = T(1) / (T(2) - s.eps1);
   const T e2 = T(1) / (T(2) - s.eps2);
-  ph0 = xatan2(sgn(n.y) * xpow((xfabs(s.ry * n.y)), e1), sgn(n.x) * xpow((xfabs(s.rx * n.x)), e1));
-  ph1 = xatan2(sgn(n.z) * xpow(xfabs(s.rz * n.z * xpow(xfabs(xcos(ph0)), T(2) - s.eps1)), e2),
-               xpow(xfabs(s.rx * n.x), e2));
+  ph0 = matan2<kFast>(sgn(n.y) * mpow<kFast>((xfabs(s.ry * n.y)), e1), sgn(n.x) * mpow<kFast>((xfabs(s.rx * n.x)), e1));
+  ph1 = matan2<kFast>(sgn(n.z) * mpow<kFast>(xfabs(s.rz * n.z * mpow<kFast>(xfabs(mcos<kFast>(ph0)), T(2) - s.eps1)), e2),
+                      mpow<kFast>(xfabs(s.rx * n.x), e2));
 }
 
 // getSurface (Superquadrics.cpp:110-120): local coordinates of the point with parameters phi.
-template <typename T>
+template <typename T, bool kFast = false>
 SDM_HD V3<T> get_surface(const Shape<T>& s, T ph0, T ph1) {
-  const T c0 = xcos(ph0), s0 = xsin(ph0), c1 = xcos(ph1), s1 = xsin(ph1);
-  const T pc1 = xpow(xfabs(c1), s.eps2);
-  return mk3(sgn(c0) * s.rx * xpow(xfabs(c0), s.eps1) * pc1, sgn(s0) * s.ry * xpow(xfabs(s0), s.eps1) * pc1,
-             sgn(s1) * s.rz * xpow(xfabs(s1), s.eps2));
+  T c0, s0, c1, s1;
+  msincos<kFast>(ph0, s0, c0);
+  msincos<kFast>(ph1, s1, c1);
+  const T pc1 = mpow<kFast>(xfabs(c1), s.eps2);
+  return mk3(sgn(c0) * s.rx * mpow<kFast>(xfabs(c0), s.eps1) * pc1,
+             sgn(s0) * s.ry * mpow<kFast>(xfabs(s0), s.eps1) * pc1, sgn(s1) * s.rz * mpow<kFast>(xfabs(s1), s.eps2));
 }
 
 // rot_mat2local * v.  The body caches rot_mat2local and rot_mat2global from the same
@@ -45,11 +88,11 @@ SDM_HD V3<T> mulT(const M3<T>& G, V3<T> v) {
 }
 
 // Support point of a body along the global direction dir (global frame, relative to its centre).
-template <typename T>
+template <typename T, bool kFast = false>
 SDM_HD V3<T> support(const Shape<T>& s, const M3<T>& G, V3<T> dir) {
   T ph0, ph1;
-  normal2phi(s, mulT(G, dir), ph0, ph1);
-  return mul(G, get_surface(s, ph0, ph1));
+  normal2phi<T, kFast>(s, mulT(G, dir), ph0, ph1);
+  return mul(G, get_surface<T, kFast>(s, ph0, ph1));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -118,8 +161,8 @@ SDM_HD V3<float> support_f(const SupF& s, const M3<float>& G, V3<float> dir) {
 // isInside (Superquadrics.cpp:103-108); pg relative to the centre, global frame.
 SDM_HD bool is_inside(const Shape<double>& s, const M3<double>& G, V3<double> pg) {
   V3<double> p = mulT(G, pg);
-  return 1 > pow(pow(fabs(p.x / s.rx), 2.0 / s.eps1) + pow(fabs(p.y / s.ry), 2.0 / s.eps1), s.eps1 / s.eps2) +
-                 pow(fabs(p.z / s.rz), 2.0 / s.eps2);
+  return 1 > xpow(xpow(fabs(p.x / s.rx), 2.0 / s.eps1) + xpow(fabs(p.y / s.ry), 2.0 / s.eps1), s.eps1 / s.eps2) +
+                 xpow(fabs(p.z / s.rz), 2.0 / s.eps2);
 }
 
 // The pair geometry a solver needs, in evaluation type T.  pos1/pos2 are the centres in the
@@ -132,17 +175,20 @@ struct PairGeo {
 };
 
 // Disfunction (Superquadrics_Ig2.cpp:621-687).  stop_flag is sticky: set, never cleared.
-template <typename T>
+template <typename T, bool kFast = false>
 SDM_HD void lm_eval(const PairGeo<T>& g, T a0, T a1, V3<T>& p1, V3<T>& p2, V3<T>& center, bool& stop_flag,
                     int& nev) {
   nev++;
-  const V3<T> contact = mk3(xcos(a0) * xcos(a1), xsin(a0) * xcos(a1), xsin(a1));
-  p1 = support(g.s1, g.G1, contact) + g.pos1;
+  T sa0, ca0, sa1, ca1;
+  msincos<kFast>(a0, sa0, ca0);
+  msincos<kFast>(a1, sa1, ca1);
+  const V3<T> contact = mk3(ca0 * ca1, sa0 * ca1, sa1);
+  p1 = support<T, kFast>(g.s1, g.G1, contact) + g.pos1;
   {
     T ph0, ph1;
     V3<T> c2 = mulT(g.G2, contact);
-    normal2phi(g.s2, mk3(T(-1) * c2.x, T(-1) * c2.y, T(-1) * c2.z), ph0, ph1);
-    p2 = mul(g.G2, get_surface(g.s2, ph0, ph1)) + g.pos2;
+    normal2phi<T, kFast>(g.s2, mk3(T(-1) * c2.x, T(-1) * c2.y, T(-1) * c2.z), ph0, ph1);
+    p2 = mul(g.G2, get_surface<T, kFast>(g.s2, ph0, ph1)) + g.pos2;
   }
   V3<T> d = p2 - p1;
   center = mk3(T(0.5) * (p1.x + p2.x), T(0.5) * (p1.y + p2.y), T(0.5) * (p1.z + p2.z));
@@ -184,7 +230,7 @@ SDM_HD void nm_prepare(NMGeo<T>& n) {
 }
 
 // Disfun (Superquadrics_Ig2.cpp:1624-1678): distance |p2-p1| and the contact test.
-template <typename T>
+template <typename T, bool kFast = false>
 SDM_HD T nm_eval(const NMGeo<T>& n, T a0, T a1, V3<T>& p1, V3<T>& p2, bool& touching, int& nev) {
   nev++;
   if constexpr (sizeof(T) == 4) {
@@ -198,12 +244,15 @@ SDM_HD T nm_eval(const NMGeo<T>& n, T a0, T a1, V3<T>& p1, V3<T>& p2, bool& touc
     if (dot(contact, d) > T(0)) touching = false;
     return norm(d);
   } else {
-    const V3<T> contact = mul(n.gC, mk3(xcos(a0) * xcos(a1), xsin(a0) * xcos(a1), xsin(a1)));
-    p1 = support(n.g.s1, n.g.G1, contact) + n.g.pos1;
+    T sa0, ca0, sa1, ca1;
+    msincos<kFast>(a0, sa0, ca0);
+    msincos<kFast>(a1, sa1, ca1);
+    const V3<T> contact = mul(n.gC, mk3(ca0 * ca1, sa0 * ca1, sa1));
+    p1 = support<T, kFast>(n.g.s1, n.g.G1, contact) + n.g.pos1;
     T ph0, ph1;
     V3<T> c2 = mulT(n.g.G2, contact);
-    normal2phi(n.g.s2, mk3(T(-1) * c2.x, T(-1) * c2.y, T(-1) * c2.z), ph0, ph1);
-    p2 = mul(n.g.G2, get_surface(n.g.s2, ph0, ph1)) + n.g.pos2;
+    normal2phi<T, kFast>(n.g.s2, mk3(T(-1) * c2.x, T(-1) * c2.y, T(-1) * c2.z), ph0, ph1);
+    p2 = mul(n.g.G2, get_surface<T, kFast>(n.g.s2, ph0, ph1)) + n.g.pos2;
     V3<T> d = p2 - p1;
     if (dot(contact, d) > T(0)) touching = false;
     return norm(d);

@@ -528,8 +528,8 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
           double para0, para1;
           if (LAW == 0 && isNew) {
             const V3<double> d0 = x2 - x1;
-            para0 = atan2(d0.y, d0.x);
-            para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
+            para0 = xatan2(d0.y, d0.x);
+            para1 = xatan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
             para0 += 1e-5;
             para1 += 1e-5;
           } else if (isNew) {
@@ -553,12 +553,17 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
             // the rounding of the evaluation), the solver would stop after that evaluation with
             // the parameters unchanged: LM at stop 9 (lmder2 :1145-1149), NM after one function
             // call (neldermead_simplex :1731-1737).  The outcome is then written directly.
+            // (c from the device's sin/cos: a last-bit difference moves the gap by ~1e-16 of the
+            // scale, far inside the margin, and the solver route gives the same outcome)
+            double sp0, cp0, sp1, cp1;
+            sincos(para0, &sp0, &cp0);
+            sincos(para1, &sp1, &cp1);
             V3<double> c;
             if (LAW == 0) {
-              c = mk3(cos(para0) * cos(para1), sin(para0) * cos(para1), sin(para1));
+              c = mk3(cp0 * cp1, sp0 * cp1, sp1);
             } else {
               const M3<double> gC = crotation_mat(r1.pos, r2.pos);
-              c = mul(gC, mk3(cos(para0) * cos(para1), sin(para0) * cos(para1), sin(para1)));
+              c = mul(gC, mk3(cp0 * cp1, sp0 * cp1, sp1));
             }
             const V3<double> n1 = mulT(r1.G, c), n2 = mulT(r2.G, c);
             const double hb1 = r1.s.rx * fabs(n1.x) + r1.s.ry * fabs(n1.y) + r1.s.rz * fabs(n1.z);
@@ -611,8 +616,8 @@ __global__ __launch_bounds__(256) void k_queue_fill(StepArgs A, SolveIO S, const
     const double* g2 = A.B.geo + (int64_t)id2 * GEO_STRIDE;
     const V3<double> d0 = (mk3(g2[GEO_POS], g2[GEO_POS + 1], g2[GEO_POS + 2]) + sh) -
                           mk3(g1[GEO_POS], g1[GEO_POS + 1], g1[GEO_POS + 2]);
-    a0 = atan2(d0.y, d0.x) + 1e-5;
-    a1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y)) + 1e-5;
+    a0 = xatan2(d0.y, d0.x) + 1e-5;
+    a1 = xatan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y)) + 1e-5;
   } else if (isNew) {
     a0 = 0;
     a1 = 0;
@@ -700,8 +705,8 @@ __global__ __launch_bounds__(256) void k_pairs_misc(StepArgs A, const int32_t* m
       double para0, para1;
       if (LAW == 0 && isNew) {
         const V3<double> d0 = x2 - b1.pos;
-        para0 = atan2(d0.y, d0.x);
-        para1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
+        para0 = xatan2(d0.y, d0.x);
+        para1 = xatan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
         para0 += 1e-5;
         para1 += 1e-5;
       } else {
@@ -810,8 +815,8 @@ __global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int3
     if (restart) {
       const V3<double> sh = S.sh ? ld3s(S.sh, w, S.cap) : mk3(0.0, 0.0, 0.0);
       const V3<double> d0 = (ld3(A.B.pos, id2, N) + sh) - ld3(A.B.pos, id1, N);
-      double a0 = atan2(d0.y, d0.x);
-      double a1 = atan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
+      double a0 = xatan2(d0.y, d0.x);
+      double a1 = xatan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
       S.a0[w] = a0 + 1e-5;
       S.a1[w] = a1 + 1e-5;
       S.tag[w] = KIND == 1 ? 100 : 200;
@@ -831,7 +836,10 @@ __device__ __forceinline__ void contact_curvature(const BodyView& b1, const Body
   const Shape<T> s1 = cast_shape<T>(b1.s), s2 = cast_shape<T>(b2.s);
   const V3<T> cn = cast3<T>(cnd);
   const T ta0 = T(a0), ta1 = T(a1);
-  const V3<T> contact = mul(gC, mk3(xcos(ta0) * xsin(ta1), xsin(ta0) * xsin(ta1), xcos(ta1)));
+  T st0, ct0, st1, ct1;
+  msincos<false>(ta0, st0, ct0);
+  msincos<false>(ta1, st1, ct1);
+  const V3<T> contact = mul(gC, mk3(ct0 * st1, st0 * st1, ct1));
   const V3<T> c1 = mulT(G1, contact), c2 = mulT(G2, contact);
   V3<T> dmax1 = mk3(T(0.0), T(0.0), T(0.0)), dmax2 = dmax1, dmin1, dmin2;
   bool f1, f2;

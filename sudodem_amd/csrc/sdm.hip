@@ -898,9 +898,13 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
   SA.evals = c->dcnt + 4;
   if (LAW == 0) {
     zero_counter(c, DC_NEXT);
-    const unsigned grid = resident_grid(c, k_solve_lm, c->residentLM, A.np);
+    // the mixed form runs the Levenberg-Marquardt solve with the device's double functions,
+    // the FP64 form with the correctly rounded ones
+    const bool fastLM = c->prm.precision == SDM_MIXED;
+    const auto lmKernel = fastLM ? k_solve_lm<true> : k_solve_lm<false>;
+    const unsigned grid = resident_grid(c, lmKernel, c->residentLM, A.np);
     if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
-    hipLaunchKernelGGL(k_solve_lm, dim3(grid), dim3(256), 0, c->st, SA);
+    hipLaunchKernelGGL(lmKernel, dim3(grid), dim3(256), 0, c->st, SA);
     if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
     c->solverRan = true;
     // the two restarts of Ig2_..._SuperquadricsGeom::go, each a further pass over its list
@@ -915,7 +919,7 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
       RA.count = c->dcnt + DC_NR;
       RA.evals = c->dcnt + 5;
       zero_counter(c, DC_NEXT);
-      hipLaunchKernelGGL(k_solve_lm, dim3(resident_grid(c, k_solve_lm, c->residentLM, A.np)), dim3(256), 0, c->st, RA);
+      hipLaunchKernelGGL(lmKernel, dim3(resident_grid(c, lmKernel, c->residentLM, A.np)), dim3(256), 0, c->st, RA);
     }
     hipLaunchKernelGGL((k_pairs_finish<0, double>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, nullptr, A.np, nq);
   } else if (c->prm.precision == SDM_MIXED) {
@@ -1027,7 +1031,7 @@ void one_step(sdm_ctx* c) {
     A.np = c->P.cap;
     ensure_queue(c, A.np);
     ensure_scratch(c, A.np + 1);
-    resident_grid(c, k_solve_lm, c->residentLM, 1);
+    resident_grid(c, c->prm.precision == SDM_MIXED ? k_solve_lm<true> : k_solve_lm<false>, c->residentLM, 1);
     resident_grid(c, k_solve_nm<float>, c->residentNM[0], 1);
     resident_grid(c, k_solve_nm<double>, c->residentNM[1], 1);
     if (!c->graphExec || c->graphBuiltGen != c->graphGen) {

@@ -426,6 +426,8 @@ __device__ __forceinline__ bool nm_first_eval(const SolveArgs& A, int64_t w) {
 // Mixed form, final refinement in double: the depth is the double distance at the best
 // vertex and the points (hence the normal) those of the last evaluated point, as the
 // reference takes them (Superquadrics_Ig2.cpp:2439-2454).  One work item per touching pair.
+// The evaluation uses the device's double functions (double accuracy, not the FP64 form's
+// correctly rounded last bits: the iterations that chose the vertex ran in float).
 __global__ __launch_bounds__(256) void k_nm_refine(SolveArgs A) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= work_n(A)) return;
@@ -439,11 +441,11 @@ __global__ __launch_bounds__(256) void k_nm_refine(SolveArgs A) {
   bool touching = true;
   int nev = 0;
   const double a0 = A.S.a0[w], a1 = A.S.a1[w], l0 = A.S.la0[w], l1 = A.S.la1[w];
-  const double f = nm_eval(n, a0, a1, q1, q2, touching, nev);
+  const double f = nm_eval<double, true>(n, a0, a1, q1, q2, touching, nev);
   // the last point is often the best vertex: its points are those just computed
   if (__double_as_longlong(l0) != __double_as_longlong(a0) || __double_as_longlong(l1) != __double_as_longlong(a1)) {
     bool t2 = true;
-    nm_eval(n, l0, l1, q1, q2, t2, nev);
+    nm_eval<double, true>(n, l0, l1, q1, q2, t2, nev);
   }
   const int64_t S = A.S.cap;
   A.S.f[w] = f;
@@ -457,6 +459,8 @@ __global__ __launch_bounds__(256) void k_nm_refine(SolveArgs A) {
 // ---------------------------------------------------------------------------------------
 enum { LM_INIT = 0, LM_J0 = 1, LM_J1 = 2, LM_J2 = 3, LM_TRIAL = 4 };
 
+// kFast: the device's double functions (the mixed form); otherwise correctly rounded (FP64).
+template <bool kFast>
 __global__ __launch_bounds__(256) void k_solve_lm(SolveArgs A) {
   const double tau = 1.0, eps1 = 1e-15, eps2 = 1e-3, eps3 = 1e-20;
   const int itmax = 50;
@@ -499,7 +503,7 @@ __global__ __launch_bounds__(256) void k_solve_lm(SolveArgs A) {
     if (state == LM_J1) e0 = p0 + Dp0;
     if (state == LM_J2) e1 = p1v + Dp1;
     if (state == LM_TRIAL) { e0 = q0; e1 = qq1; }
-    lm_eval(g, e0, e1, lp1, lp2, lc, stop_flag, nev);
+    lm_eval<double, kFast>(g, e0, e1, lp1, lp2, lc, stop_flag, nev);
     const V3<double> fe = lp2 - lp1;
     bool finished = false;
     bool outer = false;  // go to the outer-iteration checks
@@ -552,7 +556,7 @@ __global__ __launch_bounds__(256) void k_solve_lm(SolveArgs A) {
         const double dF = p_fL2 - pDp_fL2;
         double tmp = dF / dL;
         if (tmp > 0.0) {
-          tmp = 1.0 - pow(2 * tmp - 1., 3.);
+          tmp = 1.0 - mpow<kFast>(2 * tmp - 1., 3.);
           mu = mu * ((tmp >= 0.3333333333333334) ? tmp : 0.3333333333333334);
           nu = 2;
           p0 = q0;
@@ -612,7 +616,10 @@ __global__ __launch_bounds__(256) void k_solve_lm(SolveArgs A) {
         touching = false;
         finished = true;
       } else {
-        ct = mk3(cos(p0) * cos(p1v), sin(p0) * cos(p1v), sin(p1v));
+        double sp0, cp0, sp1, cp1;
+        msincos<kFast>(p0, sp0, cp0);
+        msincos<kFast>(p1v, sp1, cp1);
+        ct = mk3(cp0 * cp1, sp0 * cp1, sp1);
         const double angle = dot(p_f, ct) / (norm(p_f) * norm(ct));
         if (fabs(angle) > 0.95) {
           stop = 8;

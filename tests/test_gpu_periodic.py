@@ -22,7 +22,7 @@ def _pairs(ctx):
     return np.stack([ctx.get("intr.id1"), ctx.get("intr.id2")], axis=1)
 
 
-def _contact_agreement(g, c, what, frac):
+def _contact_agreement(g, c, what, frac, bit_frac=0.0, sel=None):
     """Per-contact depth / normal / force agreement between the device and the oracle.
 
     The Levenberg-Marquardt solve of the Geom functor takes its forward-difference Jacobian
@@ -30,19 +30,33 @@ def _contact_agreement(g, c, what, frac):
     increments are small, that Jacobian carries the last-ulp differences of pow / atan2 /
     sin / cos between the device math library and glibc at O(1) relative weight, and the loose
     stop test |f.c| > 0.95 (:1150-1166) ends at an iterate that shows them, even when both
-    sides take the same path (stop code and evaluation count).  Bars: the same path for 99%
-    of the contacts, a fraction `frac` of them within 1e-10 relative, every one within 5%,
-    and contacts across a face (cellDist != 0) no worse than interior ones."""
-    dg, dc = g.get("geom.PenetrationDepth"), c.get("geom.PenetrationDepth")
+    sides take the same path (stop code and evaluation count).  The device evaluates pow /
+    atan2 / sin / cos correctly rounded, which is what glibc returns for all but ~0.1 % of
+    arguments, so most contacts also agree bit for bit.  Bars: the same path for 99% of the
+    contacts, a fraction `frac` of them within 1e-10 relative and `bit_frac` with the same
+    depth bits, every one within 5%, and contacts across a face (cellDist != 0) no worse than
+    interior ones.  `sel` = (device rows, oracle rows) of the interactions compared."""
+    ig, ic = sel if sel is not None else (slice(None), slice(None))
+
+    def G(name, k=None):
+        v = g.get(name)
+        return (v.reshape(-1, k) if k else v)[ig]
+
+    def C(name, k=None):
+        v = c.get(name)
+        return (v.reshape(-1, k) if k else v)[ic]
+    dg, dc = G("geom.PenetrationDepth"), C("geom.PenetrationDepth")
     act = (dc > 0) & (dg > 0)
     assert act.sum() > 300 and act.sum() >= 0.98 * (dc > 0).sum()
-    path = act & (g.get("diag.stop") == c.get("diag.stop")) & (g.get("diag.nEval") == c.get("diag.nEval"))
-    F = c.get("contact.force").reshape(-1, 3)
+    path = act & (G("diag.stop") == C("diag.stop")) & (G("diag.nEval") == C("diag.nEval"))
+    F = C("contact.force", 3)
     Fs = np.maximum(np.max(np.abs(F), axis=1), 1e-300)
     rel = np.stack([np.abs(dg - dc) / np.maximum(dc, 1e-300),
-                    np.max(np.abs(g.get("geom.normal").reshape(-1, 3) - c.get("geom.normal").reshape(-1, 3)), axis=1),
-                    np.max(np.abs(g.get("contact.force").reshape(-1, 3) - F), axis=1) / Fs], axis=1).max(axis=1)
-    cross = (c.get("intr.cellDist").reshape(-1, 3) != 0).any(axis=1)
+                    np.max(np.abs(G("geom.normal", 3) - C("geom.normal", 3)), axis=1),
+                    np.max(np.abs(G("contact.force", 3) - F), axis=1) / Fs], axis=1).max(axis=1)
+    cross = (C("intr.cellDist", 3) != 0).any(axis=1)
+    bits = np.mean(dg[act] == dc[act])
+    print("%s: depth bit-equal %.4f" % (what, bits))
     fr = {}
     for name, sel in (("interior", act & ~cross), ("across a face", act & cross)):
         fr[name] = np.mean(rel[path & sel] <= 1e-10)
@@ -50,6 +64,7 @@ def _contact_agreement(g, c, what, frac):
               % (what, name, sel.sum(), (path & sel).sum() / max(1, sel.sum()), fr[name], rel[sel].max(initial=0)))
     assert path.sum() >= 0.99 * act.sum()
     assert np.mean(rel[path] <= 1e-10) >= frac
+    assert bits >= bit_frac
     assert rel[act].max() <= 0.05
     assert (act & cross).sum() > 20
     assert fr["across a face"] >= fr["interior"] - 0.05
@@ -87,7 +102,7 @@ def test_periodic_first_run_exact(gpu, oracle, c2_small):
     assert (cd != 0).any(axis=1).sum() > 50  # pairs across the faces
     for name in ("bound.min", "bound.max"):
         assert np.array_equal(g.get(name), c.get(name)), name
-    _contact_agreement(g, c, "first step", 0.9)
+    _contact_agreement(g, c, "first step", 0.99, bit_frac=0.85)
 
 
 def test_periodic_pair_set_every_run_is_brute_force(gpu, c2_small):
@@ -169,9 +184,15 @@ def test_periodic_step_physics_from_identical_state(gpu, oracle, c2_small):
     g.step(1)
     c.step(1)
     pg, pc = _pairs(g), _pairs(c)
-    if not np.array_equal(pg, pc):  # a pair created late by the oracle's ring pass
+    sel = None
+    if not np.array_equal(pg, pc):
+        # pairs the oracle's ring pass creates late (covered by the orientation test): compare
+        # the interactions both containers hold
         assert len(pg) >= len(pc)
-        pytest.skip("container differs by late pairs; covered by the orientation test")
-    _contact_agreement(g, c, "step from restored state", 0.97)
+        kg = {(int(a), int(b)): k for k, (a, b) in enumerate(pg)}
+        common = [(kg[(int(a), int(b))], k) for k, (a, b) in enumerate(pc) if (int(a), int(b)) in kg]
+        assert len(common) >= 0.99 * len(pc)
+        sel = (np.array([x for x, _ in common]), np.array([y for _, y in common]))
+    _contact_agreement(g, c, "step from restored state", 0.995, bit_frac=0.9, sel=sel)
     xg, xc = g.get("state.pos"), c.get("state.pos")
     assert np.max(np.abs(xg - xc)) <= 1e-9 * max(1.0, np.abs(xc).max())
