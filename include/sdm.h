@@ -42,7 +42,8 @@ enum {
 };
 
 enum {
-  SDM_FP64 = 0,  /* all arithmetic in double, reference semantics */
+  SDM_FP64 = 0,  /* all arithmetic in double, reference semantics (pow, atan2, atan, sin and cos
+                    correctly rounded, the values glibc gives the reference) */
   SDM_MIXED = 1  /* narrowphase iterations in float on relative frames, final contact refinement in double */
 };
 
