@@ -1,7 +1,8 @@
-// crmath.h -- correctly rounded double pow, atan2, atan, sin, cos for the FP64 path.
+// crmath.h -- correctly rounded double pow, log, log10, atan2, atan, acos, sin, cos for the FP64 path.
 //
-// The reference runs on glibc, whose pow, atan2, atan, sin and cos return the correctly
-// rounded result for all but ~0.1 % of arguments (measured against 120-bit mpmath).  The
+// The reference runs on glibc, whose pow, log, atan2, atan, acos, sin and cos return the
+// correctly rounded result for all but ~0.1 % of arguments (measured against 120-bit mpmath);
+// its log10 is not correctly rounded and is reproduced by its own formula (cr_log10).  The
 // device library's versions are within one ulp but differ from the correctly rounded result
 // for 2-27 % of arguments, and the narrowphase solvers turn such last-bit differences into
 // different iteration paths.  These versions evaluate in double-double arithmetic to a
@@ -193,6 +194,86 @@ CRM_FN double cr_atan(double x) {
   return cr_atan2(x, 1.0);
 }
 
+// ---- log, log10 -------------------------------------------------------------------------
+// ln x as a double-double with a small relative error: near 1 (|x - 1| < 2^-7) from the
+// series of t = x - 1 (exact there), elsewhere log_dd (absolute error ~2^-100, |ln x| > 2^-8)
+CRM_FN DD log_rel_dd(double x) {
+  const double t = x - 1.0;
+  if (fabs(t) < 0.0078125) {
+    // ln(1+t) = t - t^2/2 + t^3/3 + t^4 (-1/4 + t/5 - t^2/6 + ... - t^11/15)
+    double q = -1.0 / 15;
+    q = q * t + 1.0 / 14;
+    q = q * t - 1.0 / 13;
+    q = q * t + 1.0 / 12;
+    q = q * t - 1.0 / 11;
+    q = q * t + 1.0 / 10;
+    q = q * t - 1.0 / 9;
+    q = q * t + 1.0 / 8;
+    q = q * t - 1.0 / 7;
+    q = q * t + 1.0 / 6;
+    q = q * t - 1.0 / 5;
+    q = q * t + 1.0 / 4;
+    const double t2h = t * t;
+    const double t4q = -(t2h * t2h) * q;
+    const DD t2 = two_prod(t, t);
+    const DD t3 = dd_mul_d(t2, t);
+    const DD third = {0.3333333333333333, 1.850371707708594e-17};  // 1/3 as a double-double
+    DD r = fast_two_sum(t, -0.5 * t2.hi);
+    r.lo += -0.5 * t2.lo;
+    r = dd_add(r, dd_mul(t3, third));
+    return dd_add_d(r, t4q);
+  }
+  return log_dd(x);
+}
+
+CRM_FN double cr_log(double x) {
+  if (!(x > 2.2250738585072014e-308) || !(x < INFINITY)) return log(x);
+  if (x == 1.0) return 0.0;
+  const DD r = log_rel_dd(x);
+  return r.hi + r.lo;
+}
+
+// log10 as glibc computes it (it is not correctly rounded there: 15 % of arguments differ
+// from the nearest double): the freely published fdlibm formula, x = m 2^k with m in [1, 2)
+// (in [0.5, 1) for k < 0, y = k + 1 then), log10 x = (y log10(2)_lo + log(m) / ln 10) +
+// y log10(2)_hi, on the correctly rounded log.
+CRM_FN double cr_log10(double x) {
+  if (!(x > 2.2250738585072014e-308) || !(x < INFINITY)) return log10(x);
+  const double ivln10 = 4.34294481903251816668e-01;
+  const double log10_2hi = 3.01029995663611771306e-01;
+  const double log10_2lo = 3.69423907715893078616e-13;
+  int e;
+  const double f = frexp(x, &e);  // x = f 2^e, f in [0.5, 1)
+  const int k = e - 1;
+  const int i = k < 0 ? 1 : 0;
+  const double m = ldexp(f, 1 - i);
+  const double y = (double)(k + i);
+  const double z = y * log10_2lo + ivln10 * cr_log(m);
+  return z + y * log10_2hi;
+}
+
+// ---- acos --------------------------------------------------------------------------------
+// acos x = atan2(sqrt(1 - x^2), x) with 1 - x^2 = (1 - x)(1 + x) and its square root as
+// double-doubles
+CRM_FN double cr_acos(double x) {
+  if (!(fabs(x) < 1.0)) return acos(x);
+  if (x == 0.0) return kCrPio2Hi;
+  const DD a = two_sum(1.0, -x), b = two_sum(1.0, x);
+  const DD p = dd_mul(a, b);
+  const double sh = sqrt(p.hi);
+  const DD sq = two_prod(sh, sh);
+  const double sl = ((p.hi - sq.hi) - sq.lo + p.lo) / (2.0 * sh);
+  const DD s = fast_two_sum(sh, sl);
+  const double ax = fabs(x);
+  // atan of the ratio of the smaller to the larger of (s, |x|), as in cr_atan2
+  const bool swap = s.hi > ax;
+  const DD num = swap ? DD{ax, 0.0} : s, den = swap ? s : DD{ax, 0.0};
+  DD r = atan01_dd(dd_div(num, den));
+  if (swap) r = dd_add(DD{kCrPio2Hi, kCrPio2Lo}, dd_neg(r));
+  if (x < 0.0) r = dd_add(DD{kCrPiHi, kCrPiLo}, dd_neg(r));
+  return r.hi + r.lo;
+}
+
 // ---- sin, cos ---------------------------------------------------------------------------
 // a = k pi/2 + r, |r| <= pi/4 (+ rounding), r as a double-double; sin r and cos r from the
 // j/64 tables and the series of s = r - j/64, |s| <= 1/128.
@@ -263,5 +344,34 @@ CRM_FN double cr_cos(double a) {
   const DD v = (q == 0) ? c : (q == 1) ? dd_neg(s) : (q == 2) ? dd_neg(c) : s;
   return v.hi + v.lo;
 }
+
+// ---- x87 extended precision ----------------------------------------------------------------
+// Two expressions of the reference's Hertz law are long double through M_PIl (Superquadrics.cpp:
+// 1413, 1417): on x86-64 every operation rounds to a 64-bit significand (nearest-even) and the
+// assignment rounds that to double.  An extended value is held as hi + lo with a 64-bit
+// significand; products and quotients are formed to ~2^-104 and rounded to 64 bits.
+CRM_FN DD ext_round(DD v) {
+  if (v.lo == 0.0 || v.hi == 0.0) return v;
+  int e;
+  const double f = frexp(v.hi, &e);  // ulp of a 64-bit significand at |hi|: 2^(e - 64)
+  int eu = e - 64;
+  if (fabs(f) == 0.5 && ((v.lo < 0.0) != (v.hi < 0.0))) eu -= 1;  // |v| below a power of two
+  const double u = ldexp(1.0, eu);
+  return DD{v.hi, rint(v.lo / u) * u};
+}
+CRM_FN DD ext_mul_d(DD x, double d) {
+  return ext_round(dd_add(two_prod(x.hi, d), two_prod(x.lo, d)));
+}
+CRM_FN DD ext_div_d(DD x, double d) {
+  const double q1 = x.hi / d;
+  const DD p = two_prod(q1, d);
+  DD r = two_sum(x.hi, -p.hi);
+  r.lo += x.lo - p.lo;
+  const double q2 = (r.hi + r.lo) / d;
+  return ext_round(fast_two_sum(q1, q2));
+}
+CRM_FN double ext_to_double(DD x) { return x.hi + x.lo; }
+// M_PIl: pi on a 64-bit significand
+CRM_FN DD ext_pi() { return DD{3.141592653589793116, 1.22514845490862e-16}; }
 
 }  // namespace crm

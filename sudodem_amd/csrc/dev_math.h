@@ -116,9 +116,10 @@ SDM_HD Q4 quat_mul(Q4 a, Q4 b) {
   return o;
 }
 
-// Transcendentals by precision.  Double pow, sin, cos, atan2 and atan are the correctly
-// rounded versions of crmath.h (the values glibc returns to the reference for all but ~0.1 %
-// of arguments); float maps to the device's float functions.
+// Transcendentals by precision.  Double pow, log, acos, sin, cos, atan2 and atan are the
+// correctly rounded versions of crmath.h (the values glibc returns to the reference for all
+// but ~0.1 % of arguments), log10 glibc's own formula on that log; float maps to the device's
+// float functions.
 SDM_HD double xpow(double a, double b) { return crm::cr_pow(a, b); }
 SDM_HD float xpow(float a, float b) { return powf(a, b); }
 SDM_HD double xsin(double a) { return crm::cr_sin(a); }
@@ -132,11 +133,11 @@ SDM_HD double xfabs(double a) { return fabs(a); }
 SDM_HD float xfabs(float a) { return fabsf(a); }
 SDM_HD double xsqrt(double a) { return sqrt(a); }
 SDM_HD float xsqrt(float a) { return sqrtf(a); }
-SDM_HD double xlog(double a) { return log(a); }
+SDM_HD double xlog(double a) { return crm::cr_log(a); }
 SDM_HD float xlog(float a) { return logf(a); }
-SDM_HD double xlog10(double a) { return log10(a); }
+SDM_HD double xlog10(double a) { return crm::cr_log10(a); }
 SDM_HD float xlog10(float a) { return log10f(a); }
-SDM_HD double xacos(double a) { return acos(a); }
+SDM_HD double xacos(double a) { return crm::cr_acos(a); }
 SDM_HD float xacos(float a) { return acosf(a); }
 
 template <typename T, typename S>

@@ -282,14 +282,17 @@ __device__ bool phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
         ph.ks = 8. / (2.0 * ph.mu1 - ph.mu2) * tmp;
       } else {
         // The reference forms these two in long double through M_PIl (Superquadrics.cpp:1413,
-        // 1417); the device has no extended type, so they are evaluated in double.
-        const double PI = 3.141592653589793238462643383279502884;
+        // 1417): each product and the quotient round to a 64-bit significand (emulated on
+        // double-doubles), the assignment to double.
         double a = sqrt(depth * g.E / cs / g.K);
-        ph.kn = 4. / 3. * PI * a * ph.Gab * g.alpha / g.K;
+        crm::DD t = crm::ext_mul_d(crm::ext_pi(), 4. / 3.);
+        t = crm::ext_mul_d(crm::ext_mul_d(crm::ext_mul_d(t, a), ph.Gab), g.alpha);
+        ph.kn = crm::ext_to_double(crm::ext_div_d(t, g.K));
         double e2 = 1 - 1 / (g.alpha * g.alpha);
         double ksx = ph.mu1 * g.K - ph.mu2 * (g.K - g.E) / e2;
         double ksy = ph.mu1 * g.K + ph.mu2 * ((1 - e2) * g.K - g.E) / e2;
-        ph.ks = PI * g.alpha * a * (1. / ksx + 1. / ksy);
+        const double sinv = 1. / ksx + 1. / ksy;
+        ph.ks = crm::ext_to_double(crm::ext_mul_d(crm::ext_mul_d(crm::ext_mul_d(crm::ext_pi(), g.alpha), a), sinv));
       }
     }
     V3<double> fn = mk3(g.normal.x * g.depth * ph.kn, g.normal.y * g.depth * ph.kn, g.normal.z * g.depth * ph.kn);

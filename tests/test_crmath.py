@@ -1,8 +1,9 @@
-"""Correctly rounded double pow, atan2, atan, sin, cos (sudodem_amd/csrc/crmath.h), built for
+"""Correctly rounded double pow, log, atan2, atan, acos, sin, cos (sudodem_amd/csrc/crmath.h), built for
 the host from the same source the device compiles, against 200-bit mpmath: every result on
 seeded arguments from the ranges the support map uses (and wider ones) is the double nearest
 to the exact value; special arguments take the library path; sincos equals sin and cos; and
-the results differ from glibc's only where glibc itself is not correctly rounded (<0.5 %)."""
+the results differ from glibc's only where glibc itself is not correctly rounded (<0.5 %); log10
+follows glibc's own (not correctly rounded) formula."""
 import ctypes
 import math
 import os
@@ -34,14 +35,15 @@ def crm(tmp_path_factory):
     return ev
 
 
-POW, ATAN2, SIN, COS, ATAN, SINCOS_S, SINCOS_C = range(7)
+POW, ATAN2, SIN, COS, ATAN, SINCOS_S, SINCOS_C, LOG, LOG10, ACOS = range(10)  # 10-13: extended chains
 mpmath.mp.prec = 200
 _MP = {POW: lambda x, y: mpmath.power(mpmath.mpf(x), mpmath.mpf(y)),
        ATAN2: lambda y, x: mpmath.atan2(mpmath.mpf(y), mpmath.mpf(x)),
        SIN: lambda x, y: mpmath.sin(mpmath.mpf(x)), COS: lambda x, y: mpmath.cos(mpmath.mpf(x)),
-       ATAN: lambda x, y: mpmath.atan(mpmath.mpf(x))}
+       ATAN: lambda x, y: mpmath.atan(mpmath.mpf(x)), LOG: lambda x, y: mpmath.log(mpmath.mpf(x)),
+       ACOS: lambda x, y: mpmath.acos(mpmath.mpf(x))}
 _GLIBC = {POW: math.pow, ATAN2: math.atan2, SIN: lambda x, y: math.sin(x), COS: lambda x, y: math.cos(x),
-          ATAN: lambda x, y: math.atan(x)}
+          ATAN: lambda x, y: math.atan(x), LOG: lambda x, y: math.log(x), ACOS: lambda x, y: math.acos(x)}
 
 
 def _args(fn, rng, n):
@@ -56,10 +58,16 @@ def _args(fn, rng, n):
         near = np.array([k * math.pi / 2 for k in range(-8, 9)]).repeat(n // 17 + 1)[:n] + rng.uniform(-1e-6, 1e-6, n)
         x = np.concatenate([rng.uniform(-7, 7, n), near, rng.uniform(-1e-3, 1e-3, n), rng.uniform(-1e5, 1e5, n)])
         return x, None
+    if fn == LOG:  # wide, near 1, very near 1
+        return np.concatenate([10 ** rng.uniform(-8, 8, n), 1 + rng.uniform(-0.01, 0.01, n),
+                               1 + rng.uniform(-1e-9, 1e-9, n)]), None
+    if fn == ACOS:  # the angle between curvature directions: anywhere in [-1, 1], near +-1, near 0
+        return np.concatenate([rng.uniform(-1, 1, n), 1 - 10 ** rng.uniform(-12, -1, n),
+                               -1 + 10 ** rng.uniform(-12, -1, n), rng.uniform(-1e-3, 1e-3, n)]), None
     return np.concatenate([rng.uniform(-50, 50, n), rng.normal(size=n) * 10 ** rng.uniform(-6, 6, n)]), None
 
 
-@pytest.mark.parametrize("fn", [POW, ATAN2, SIN, COS, ATAN], ids=["pow", "atan2", "sin", "cos", "atan"])
+@pytest.mark.parametrize("fn", [POW, ATAN2, SIN, COS, ATAN, LOG, ACOS], ids=["pow", "atan2", "sin", "cos", "atan", "log", "acos"])
 def test_correctly_rounded(crm, fn):
     rng = np.random.default_rng(100 + fn)
     a, b = _args(fn, rng, 1500)
@@ -69,6 +77,27 @@ def test_correctly_rounded(crm, fn):
     assert np.array_equal(got, want), int(np.sum(got != want))
     glibc = np.array([_GLIBC[fn](x, y) for x, y in zip(a, bb)])
     assert np.mean(glibc != got) < 0.005
+
+
+def test_log10_follows_glibc(crm):
+    # glibc's log10 is not correctly rounded (~15 % of these arguments): the device reproduces
+    # its formula, so results agree with glibc except where glibc's own log misrounds
+    rng = np.random.default_rng(21)
+    a = np.concatenate([10 ** rng.uniform(-8, 8, 3000), 1 + rng.uniform(-0.01, 0.01, 3000)])
+    got = crm(LOG10, a)
+    glibc = np.array([math.log10(v) for v in a])
+    assert np.mean(got != glibc) < 0.002
+    exact = np.array([float(mpmath.log10(mpmath.mpf(v))) for v in a])
+    assert np.mean(glibc != exact) > 0.05  # the formula, not the nearest double, is what is matched
+
+
+def test_extended_precision_chains_match_x87(crm):
+    # the Hertz law's long double expressions (4/3 M_PIl a Gab alpha / K, M_PIl alpha a s):
+    # the double-double emulation of 64-bit-significand rounding against the host's x87
+    rng = np.random.default_rng(31)
+    a, b = 10 ** rng.uniform(-6, 9, 200000), 10 ** rng.uniform(-6, 9, 200000)
+    for emu, x87 in ((10, 11), (12, 13)):
+        assert np.array_equal(crm(emu, a, b), crm(x87, a, b))
 
 
 def test_sincos_equals_sin_and_cos(crm):
