@@ -83,7 +83,8 @@ CRM_FN DD log_dd(double x) {
   const double m = 2.0 * f;       // [1, 2)
   const int k = e - 1;
   const int i = (int)((m - 1.0) * 128.0);
-  const double c = kCrLogC[i];
+  const CrT4 L = kCrLog[i];  // c_i, -ln c_i (hi, lo)
+  const double c = L.a;
   const DD p = two_prod(m, c);
   const DD r = two_sum(p.hi - 1.0, p.lo);  // p.hi - 1 is exact (p.hi within 1 % of 1)
   // log1p(r) = r - r^2/2 + r^3 (1/3 - r/4 + r^2/5 - ... + r^8/11)
@@ -104,9 +105,9 @@ CRM_FN DD log_dd(double x) {
   DD l1p = fast_two_sum(rh, -0.5 * sq.hi);
   l1p.lo += (r.lo - 0.5 * sq.lo) + r3q;
   // k ln2 (ln2 hi on 40 bits: k ln2_hi is exact) - ln c_i + l1p; |k ln2| > |ln c_i| when k != 0
-  const DD head = fast_two_sum((double)k * kCrLn2Hi, kCrLogHi[i]);
+  const DD head = fast_two_sum((double)k * kCrLn2Hi, L.b);
   DD t = two_sum(head.hi, l1p.hi);
-  t.lo += ((head.lo + l1p.lo) + (double)k * kCrLn2Lo) + kCrLogLo[i];
+  t.lo += ((head.lo + l1p.lo) + (double)k * kCrLn2Lo) + L.c;
   return fast_two_sum(t.hi, t.lo);
 }
 
@@ -134,7 +135,8 @@ CRM_FN DD exp_dd(DD t) {
   const DD a1 = fast_two_sum(1.0, sh);
   const DD a2 = fast_two_sum(a1.hi, 0.5 * sq.hi);
   const DD e = fast_two_sum(a2.hi, ((a1.lo + a2.lo) + (s.lo + 0.5 * sq.lo)) + s3q);
-  const DD tj = {kCrExpHi[j], kCrExpLo[j]};
+  const CrT2 T = kCrExp[j];
+  const DD tj = {T.hi, T.lo};
   DD r = dd_mul(tj, e);
   r.hi = ldexp(r.hi, m);
   r.lo = ldexp(r.lo, m);
@@ -170,7 +172,8 @@ CRM_FN DD atan01_dd(DD q) {
   p = p * u2 + 1.0 / 5;
   p = p * u2 - 1.0 / 3;
   const DD au = dd_add_d(u, uh * u2 * p);
-  const DD ai = {kCrAtanHi[i], kCrAtanLo[i]};
+  const CrT2 A = kCrAtan[i];
+  const DD ai = {A.hi, A.lo};
   return dd_add(ai, au);
 }
 
@@ -307,7 +310,8 @@ CRM_FN void sincos_dd(double a, DD& sr, DD& cr, int& quad) {
   coss.lo += sq.lo;
   coss = dd_add_d(coss, s2 * s2 * pc);
   const double sg = j < 0 ? -1.0 : 1.0;
-  const DD Sj = {sg * kCrSinHi[aj], sg * kCrSinLo[aj]}, Cj = {kCrCosHi[aj], kCrCosLo[aj]};
+  const CrT4 SC = kCrSinCos[aj];  // sin (hi, lo), cos (hi, lo) of |j|/64
+  const DD Sj = {sg * SC.a, sg * SC.b}, Cj = {SC.c, SC.d};
   sr = dd_add(dd_mul(Sj, coss), dd_mul(Cj, sins));
   cr = dd_add(dd_mul(Cj, coss), dd_neg(dd_mul(Sj, sins)));
 }

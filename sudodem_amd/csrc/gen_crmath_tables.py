@@ -20,6 +20,15 @@ def short(v, bits):
     return float(mpmath.ldexp(mpmath.nint(mpmath.ldexp(m, bits)), e - bits))
 
 
+def rec(name, kind, rows):
+    """An array of records (CrT2 / CrT4), one table line per entry: one wide load per lookup."""
+    out = ["static constexpr %s %s[%d] = {" % (kind, name, len(rows))]
+    for r in rows:
+        out.append("    {" + ", ".join(repr(float(x)) for x in r) + "},")
+    out.append("};")
+    return "\n".join(out)
+
+
 def arr(name, vals, per_line=3):
     out = ["static constexpr double %s[%d] = {" % (name, len(vals))]
     for k in range(0, len(vals), per_line):
@@ -29,24 +38,26 @@ def arr(name, vals, per_line=3):
 
 
 def main():
-    parts = ["// Generated by gen_crmath_tables.py (mpmath, 200 bits); do not edit.", "#pragma once", ""]
+    parts = ["// Generated by gen_crmath_tables.py (mpmath, 200 bits); do not edit.", "#pragma once", "",
+             "struct alignas(16) CrT2 {", "  double hi, lo;", "};",
+             "struct alignas(32) CrT4 {", "  double a, b, c, d;", "};", ""]
     # log: m in [1 + i/128, 1 + (i+1)/128) -> c_i ~ 1/(1 + (i+0.5)/128) on 12 bits, -ln(c_i)
     c, lh, ll = [], [], []
     for i in range(128):
         ci = short(1 / (1 + (mpmath.mpf(i) + 0.5) / 128), 12)
         h, l = dd(-mpmath.log(mpmath.mpf(ci)))
         c.append(ci); lh.append(h); ll.append(l)
-    parts += [arr("kCrLogC", c), arr("kCrLogHi", lh), arr("kCrLogLo", ll)]
+    parts += [rec("kCrLog", "CrT4", [(a, b, d, 0.0) for a, b, d in zip(c, lh, ll)])]
     # exp: 2^(j/128)
     eh, el = zip(*[dd(mpmath.power(2, mpmath.mpf(j) / 128)) for j in range(128)])
-    parts += [arr("kCrExpHi", eh), arr("kCrExpLo", el)]
+    parts += [rec("kCrExp", "CrT2", list(zip(eh, el)))]
     # atan(i/64), i = 0..64
     ah, al = zip(*[dd(mpmath.atan(mpmath.mpf(i) / 64)) for i in range(65)])
-    parts += [arr("kCrAtanHi", ah), arr("kCrAtanLo", al)]
+    parts += [rec("kCrAtan", "CrT2", list(zip(ah, al)))]
     # sin(j/64), cos(j/64), j = 0..51
     sh, sl = zip(*[dd(mpmath.sin(mpmath.mpf(j) / 64)) for j in range(52)])
     ch, cl = zip(*[dd(mpmath.cos(mpmath.mpf(j) / 64)) for j in range(52)])
-    parts += [arr("kCrSinHi", sh), arr("kCrSinLo", sl), arr("kCrCosHi", ch), arr("kCrCosLo", cl)]
+    parts += [rec("kCrSinCos", "CrT4", list(zip(sh, sl, ch, cl)))]
     # constants: ln 2 (hi on 40 bits), ln2/128 in three parts (36, 36, 53 bits), pi/2 in three
     # parts (33, 33, 53 bits), pi/2 and pi as double-doubles
     ln2 = mpmath.log(2)
