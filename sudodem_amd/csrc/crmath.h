@@ -154,6 +154,36 @@ CRM_FN double cr_pow(double x, double y) {
   return r.hi + r.lo;
 }
 
+// pow(x, y1) and pow(x, y2) with one logarithm (the same values as two cr_pow calls)
+CRM_FN void cr_pow_pair(double x, double y1, double y2, double* o1, double* o2) {
+  if (!(x > 0.0) || !(x < INFINITY) || !(fabs(y1) < INFINITY) || !(fabs(y2) < INFINITY) ||
+      x < 2.2250738585072014e-308 || x == 1.0) {
+    *o1 = cr_pow(x, y1);
+    *o2 = cr_pow(x, y2);
+    return;
+  }
+  const DD l = log_dd(x);
+  const double y[2] = {y1, y2};
+  double r[2];
+  for (int k = 0; k < 2; k++) {
+    if (y[k] == 0.0) {
+      r[k] = 1.0;
+    } else if (y[k] == 1.0) {
+      r[k] = x;
+    } else {
+      const DD t = dd_mul_d(l, y[k]);
+      if (!(fabs(t.hi) < 700.0)) {
+        r[k] = pow(x, y[k]);
+      } else {
+        const DD e = exp_dd(t);
+        r[k] = e.hi + e.lo;
+      }
+    }
+  }
+  *o1 = r[0];
+  *o2 = r[1];
+}
+
 // ---- atan2, atan ------------------------------------------------------------------------
 // atan(q) for q = qh + ql in [0, 1]: atan(i/64) + atan(u), u = (q - i/64) / (1 + q i/64)
 CRM_FN DD atan01_dd(DD q) {

@@ -62,9 +62,16 @@ template <typename T, bool kFast = false>
 SDM_HD void normal2phi(const Shape<T>& s, V3<T> n, T& ph0, T& ph1) {
   const T e1 = T(1) / (T(2) - s.eps1);
   const T e2 = T(1) / (T(2) - s.eps2);
-  ph0 = matan2<kFast>(sgn(n.y) * mpow<kFast>((xfabs(s.ry * n.y)), e1), sgn(n.x) * mpow<kFast>((xfabs(s.rx * n.x)), e1));
+  T px1, px2;  // |rx nx|^e1 and |rx nx|^e2
+  if constexpr (sizeof(T) == 8 && !kFast) {
+    crm::cr_pow_pair(xfabs(s.rx * n.x), e1, e2, &px1, &px2);  // one logarithm for both
+  } else {
+    px1 = mpow<kFast>(xfabs(s.rx * n.x), e1);
+    px2 = mpow<kFast>(xfabs(s.rx * n.x), e2);
+  }
+  ph0 = matan2<kFast>(sgn(n.y) * mpow<kFast>((xfabs(s.ry * n.y)), e1), sgn(n.x) * px1);
   ph1 = matan2<kFast>(sgn(n.z) * mpow<kFast>(xfabs(s.rz * n.z * mpow<kFast>(xfabs(mcos<kFast>(ph0)), T(2) - s.eps1)), e2),
-                      mpow<kFast>(xfabs(s.rx * n.x), e2));
+                      px2);
 }
 
 // getSurface (Superquadrics.cpp:110-120): local coordinates of the point with parameters phi.

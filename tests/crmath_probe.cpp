@@ -33,6 +33,8 @@ extern "C" void crm_eval(int fn, const double* a, const double* b, double* out, 
         out[i] = crm::ext_to_double(crm::ext_mul_d(crm::ext_mul_d(crm::ext_mul_d(crm::ext_pi(), a[i]), b[i]), sv));
         break;
       }
+      case 14: { double p1, p2; crm::cr_pow_pair(a[i], b[i], 0.5 * b[i] + 0.25, &p1, &p2); out[i] = p1; break; }
+      case 15: { double p1, p2; crm::cr_pow_pair(a[i], b[i], 0.5 * b[i] + 0.25, &p1, &p2); out[i] = p2; break; }
       case 13: {
         const double sv = 1. / a[i] + 1. / b[i];
         out[i] = (double)(M_PIl * a[i] * b[i] * sv);

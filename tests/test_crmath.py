@@ -100,6 +100,14 @@ def test_extended_precision_chains_match_x87(crm):
         assert np.array_equal(crm(emu, a, b), crm(x87, a, b))
 
 
+def test_pow_pair_equals_pow(crm):
+    rng = np.random.default_rng(41)
+    x = np.concatenate([10 ** rng.uniform(-8, 3, 20000), [0.0, 1.0, 2.0, np.inf]])
+    y = np.concatenate([rng.uniform(-3, 5, 20000), [2.0, 3.0, 0.0, 1.0]])
+    assert np.array_equal(crm(14, x, y), crm(POW, x, y), equal_nan=True)
+    assert np.array_equal(crm(15, x, y), crm(POW, x, 0.5 * y + 0.25), equal_nan=True)
+
+
 def test_sincos_equals_sin_and_cos(crm):
     a = np.random.default_rng(9).uniform(-10, 10, 5000)
     assert np.array_equal(crm(SINCOS_S, a), crm(SIN, a))
