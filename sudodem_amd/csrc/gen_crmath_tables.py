@@ -29,14 +29,6 @@ def rec(name, kind, rows):
     return "\n".join(out)
 
 
-def arr(name, vals, per_line=3):
-    out = ["static constexpr double %s[%d] = {" % (name, len(vals))]
-    for k in range(0, len(vals), per_line):
-        out.append("    " + ", ".join(repr(float(x)) for x in vals[k:k + per_line]) + ",")
-    out.append("};")
-    return "\n".join(out)
-
-
 def main():
     parts = ["// Generated by gen_crmath_tables.py (mpmath, 200 bits); do not edit.", "#pragma once", "",
              "struct alignas(16) CrT2 {", "  double hi, lo;", "};",
