@@ -2,8 +2,10 @@
 
 Bars: the broadphase pair list (ids, orientation, container order) is bit-exact; bounds are
 bit-exact; per-contact geometry and forces agree to 1e-10 relative for contacts whose solver
-path is the same (the device libm differs from glibc in the last ulp, which can flip a
-solver decision near a threshold; such flips are counted and bounded).
+path is the same, and most of them bit for bit: the FP64 path's transcendentals are correctly
+rounded, glibc's (the oracle's) for all but ~0.1 % of arguments, and a last-ulp difference
+where glibc misrounds can flip a solver decision near a threshold (such flips are counted
+and bounded).
 """
 import numpy as np
 import pytest

@@ -27,10 +27,10 @@ def _contact_agreement(g, c, what, frac, bit_frac=0.0, sel=None):
 
     The Levenberg-Marquardt solve of the Geom functor takes its forward-difference Jacobian
     with the step set to the last increment (lmder2, Superquadrics_Ig2.cpp:1170); once the
-    increments are small, that Jacobian carries the last-ulp differences of pow / atan2 /
-    sin / cos between the device math library and glibc at O(1) relative weight, and the loose
-    stop test |f.c| > 0.95 (:1150-1166) ends at an iterate that shows them, even when both
-    sides take the same path (stop code and evaluation count).  The device evaluates pow /
+    increments are small, that Jacobian carries any last-ulp difference of pow / atan2 /
+    sin / cos between the two sides at O(1) relative weight, and the loose stop test
+    |f.c| > 0.95 (:1150-1166) ends at an iterate that shows it, even when both sides take the
+    same path (stop code and evaluation count).  The device evaluates pow /
     atan2 / sin / cos correctly rounded, which is what glibc returns for all but ~0.1 % of
     arguments, so most contacts also agree bit for bit.  Bars: the same path for 99% of the
     contacts, a fraction `frac` of them within 1e-10 relative and `bit_frac` with the same
