@@ -33,6 +33,37 @@
 #include <omp.h>
 #endif
 
+// Math of the hot path.  Built as-is, glibc's functions, as the reference binary calls them.
+// Built with SDM_ORACLE_CRMATH (libsdm_oracle_cr.so), the correctly rounded functions of
+// crmath.h, so that a comparison with the device's FP64 form isolates everything but the
+// last bits glibc gets wrong (~0.1 % of arguments).  The body-construction arithmetic
+// (gammln) stays on glibc in both builds, as in the device library's host code.
+#ifdef SDM_ORACLE_CRMATH
+#define CRM_FN inline
+#include "crmath.h"
+namespace om {
+inline double pow(double a, double b) { return crm::cr_pow(a, b); }
+inline double atan2(double a, double b) { return crm::cr_atan2(a, b); }
+inline double atan(double a) { return crm::cr_atan(a); }
+inline double sin(double a) { return crm::cr_sin(a); }
+inline double cos(double a) { return crm::cr_cos(a); }
+inline double log(double a) { return crm::cr_log(a); }
+inline double log10(double a) { return crm::cr_log10(a); }
+inline double acos(double a) { return crm::cr_acos(a); }
+}  // namespace om
+#else
+namespace om {
+using std::acos;
+using std::atan;
+using std::atan2;
+using std::cos;
+using std::log;
+using std::log10;
+using std::pow;
+using std::sin;
+}  // namespace om
+#endif
+
 namespace {
 
 const double NaN = std::numeric_limits<double>::quiet_NaN();
@@ -154,29 +185,29 @@ void sq_initial(double rx, double ry, double rz, double eps1, double eps2, doubl
 
 // Superquadrics::Normal2Phi (Superquadrics.cpp:269-307; the active branch is :284 and :288).
 void normal2phi(const SQ& s, const double* n, double* phi) {
-  phi[0] = std::atan2(Sign(n[1]) * std::pow((std::fabs(s.ry * n[1])), 1 / (2. - s.eps1)),
-                      Sign(n[0]) * std::pow((std::fabs(s.rx * n[0])), 1 / (2. - s.eps1)));
-  phi[1] = std::atan2(Sign(n[2]) * std::pow(std::fabs(s.rz * n[2] * std::pow(std::fabs(std::cos(phi[0])), 2. - s.eps1)),
+  phi[0] = om::atan2(Sign(n[1]) * om::pow((std::fabs(s.ry * n[1])), 1 / (2. - s.eps1)),
+                      Sign(n[0]) * om::pow((std::fabs(s.rx * n[0])), 1 / (2. - s.eps1)));
+  phi[1] = om::atan2(Sign(n[2]) * om::pow(std::fabs(s.rz * n[2] * om::pow(std::fabs(om::cos(phi[0])), 2. - s.eps1)),
                                             1 / (2. - s.eps2)),
-                      std::pow(std::fabs(s.rx * n[0]), 1 / (2. - s.eps2)));
+                      om::pow(std::fabs(s.rx * n[0]), 1 / (2. - s.eps2)));
 }
 
 // Superquadrics::getSurface (Superquadrics.cpp:110-120), local coordinates.
 void get_surface(const SQ& s, const double* phi, double* p) {
-  p[0] = Sign(std::cos(phi[0])) * s.rx * std::pow(std::fabs(std::cos(phi[0])), s.eps1) *
-         std::pow(std::fabs(std::cos(phi[1])), s.eps2);
-  p[1] = Sign(std::sin(phi[0])) * s.ry * std::pow(std::fabs(std::sin(phi[0])), s.eps1) *
-         std::pow(std::fabs(std::cos(phi[1])), s.eps2);
-  p[2] = Sign(std::sin(phi[1])) * s.rz * std::pow(std::fabs(std::sin(phi[1])), s.eps2);
+  p[0] = Sign(om::cos(phi[0])) * s.rx * om::pow(std::fabs(om::cos(phi[0])), s.eps1) *
+         om::pow(std::fabs(om::cos(phi[1])), s.eps2);
+  p[1] = Sign(om::sin(phi[0])) * s.ry * om::pow(std::fabs(om::sin(phi[0])), s.eps1) *
+         om::pow(std::fabs(om::cos(phi[1])), s.eps2);
+  p[2] = Sign(om::sin(phi[1])) * s.rz * om::pow(std::fabs(om::sin(phi[1])), s.eps2);
 }
 
 // Superquadrics::isInside (Superquadrics.cpp:103-108): p relative to the centre, global frame.
 bool is_inside(const SQ& s, const double* pg) {
   double p[3];
   matvec(s.Rl, pg, p);
-  return 1 > std::pow(std::pow(std::fabs(p[0] / s.rx), 2.0 / s.eps1) + std::pow(std::fabs(p[1] / s.ry), 2.0 / s.eps1),
+  return 1 > om::pow(om::pow(std::fabs(p[0] / s.rx), 2.0 / s.eps1) + om::pow(std::fabs(p[1] / s.ry), 2.0 / s.eps1),
                       s.eps1 / s.eps2) +
-                 std::pow(std::fabs(p[2] / s.rz), 2.0 / s.eps2);
+                 om::pow(std::fabs(p[2] / s.rz), 2.0 / s.eps2);
 }
 
 // Superquadrics::Curvatures (Superquadrics.cpp:131-255).  QUIRK: K ignores the M term
@@ -187,18 +218,18 @@ void curvatures(const SQ& s, double* phi_in, double* k, double* dir_max, double*
   if (std::fabs(phi[0]) < 1E-20) phi[0] = 1E-20;
   if (std::fabs(phi[1]) < 1E-20) phi[1] = 1E-20;
   const double rx = s.rx, ry = s.ry, rz = s.rz, eps1 = s.eps1, eps2 = s.eps2;
-  double s1 = std::sin(phi[0]), s2 = std::sin(phi[1]);
-  double c1 = std::cos(phi[0]), c2 = std::cos(phi[1]);
+  double s1 = om::sin(phi[0]), s2 = om::sin(phi[1]);
+  double c1 = om::cos(phi[0]), c2 = om::cos(phi[1]);
   double s_s1 = Sign(s1), s_s2 = Sign(s2), s_c1 = Sign(c1), s_c2 = Sign(c2);
   double abs_s1 = std::fabs(s1), abs_s2 = std::fabs(s2), abs_c1 = std::fabs(c1), abs_c2 = std::fabs(c2);
-  double s10 = s_s1 * std::pow(abs_s1, eps1);
-  double s20 = s_s2 * std::pow(abs_s2, eps2);
-  double c10 = s_c1 * std::pow(abs_c1, eps1);
-  double c20 = s_c2 * std::pow(abs_c2, eps2);
-  double s12 = s_s1 * std::pow(abs_s1, eps1 - 2.0);
-  double s22 = s_s2 * std::pow(abs_s2, eps2 - 2.0);
-  double c12 = s_c1 * std::pow(abs_c1, eps1 - 2.0);
-  double c22 = s_c2 * std::pow(abs_c2, eps2 - 2.0);
+  double s10 = s_s1 * om::pow(abs_s1, eps1);
+  double s20 = s_s2 * om::pow(abs_s2, eps2);
+  double c10 = s_c1 * om::pow(abs_c1, eps1);
+  double c20 = s_c2 * om::pow(abs_c2, eps2);
+  double s12 = s_s1 * om::pow(abs_s1, eps1 - 2.0);
+  double s22 = s_s2 * om::pow(abs_s2, eps2 - 2.0);
+  double c12 = s_c1 * om::pow(abs_c1, eps1 - 2.0);
+  double c22 = s_c2 * om::pow(abs_c2, eps2 - 2.0);
   double R1[3] = {-rx * eps1 * c12 * c1 * c20 * s1, ry * eps1 * c1 * c20 * s12 * s1, 0};
   double R2[3] = {-rx * eps2 * c10 * c22 * c2 * s2, -ry * eps2 * c22 * c2 * s10 * s2, rz * eps2 * c2 * s22 * s2};
   double R11[3] = {rx * eps1 * c12 * s20 * s1 * s1 * (eps1 - 1.0) - rx * eps1 * c10 * c20,
@@ -409,8 +440,8 @@ struct LMOut {
 void disfunction(const double* para, double* center, const SQ& A, const SQ& B, const double* position1,
                  const double* position2, double* p1, double* p2, bool& stop_flag, int& nev) {
   nev++;
-  double contact[3] = {std::cos(para[0]) * std::cos(para[1]), std::sin(para[0]) * std::cos(para[1]),
-                       std::sin(para[1])};
+  double contact[3] = {om::cos(para[0]) * om::cos(para[1]), om::sin(para[0]) * om::cos(para[1]),
+                       om::sin(para[1])};
   double contact1[3], contact2[3], phi[2], t[3];
   matvec(A.Rl, contact, contact1);
   normal2phi(A, contact1, phi);
@@ -477,9 +508,9 @@ int lmder2(double* p, int itmax, const double* opts, const SQ& A, const SQ& B, c
     if (p_fL2 <= eps3) { stop = 6; break; }
     if (stop_flag) { stop = 9; touching = false; break; }
     {
-      contact[0] = std::cos(p[0]) * std::cos(p[1]);
-      contact[1] = std::sin(p[0]) * std::cos(p[1]);
-      contact[2] = std::sin(p[1]);
+      contact[0] = om::cos(p[0]) * om::cos(p[1]);
+      contact[1] = om::sin(p[0]) * om::cos(p[1]);
+      contact[2] = om::sin(p[1]);
       double angle = dot3(p_f, contact) / (norm3(p_f) * norm3(contact));
       if (std::fabs(angle) > 0.95) { stop = 8; break; }
     }
@@ -520,7 +551,7 @@ int lmder2(double* p, int itmax, const double* opts, const SQ& A, const SQ& B, c
       dF = p_fL2 - pDp_fL2;
       tmp = dF / dL;
       if (tmp > 0.0) {
-        tmp = 1.0 - std::pow(2 * tmp - 1., 3.);
+        tmp = 1.0 - om::pow(2 * tmp - 1., 3.);
         mu = mu * ((tmp >= 0.3333333333333334) ? tmp : 0.3333333333333334);
         nu = 2;
         p[0] = pDp[0]; p[1] = pDp[1];
@@ -567,7 +598,7 @@ void crotation_mat(const double* position1, const double* position2, double* R) 
 double disfun(const double* para, const SQ& A, const SQ& B, const double* position1, const double* position2,
               double* p1, double* p2, const double* gC, bool& touching, int& nev) {
   nev++;
-  double c0[3] = {std::cos(para[0]) * std::cos(para[1]), std::sin(para[0]) * std::cos(para[1]), std::sin(para[1])};
+  double c0[3] = {om::cos(para[0]) * om::cos(para[1]), om::sin(para[0]) * om::cos(para[1]), om::sin(para[1])};
   double contact[3], contact1[3], contact2[3], phi[2], t[3];
   matvec(gC, c0, contact);
   matvec(A.Rl, contact, contact1);
@@ -724,18 +755,18 @@ void hertz_mindlin(const double* curvature1, const double* curvature2, double th
   double sum_rho = rho11 + rho12 + rho21 + rho22;
   double a1 = rho11 - rho12, a2 = rho21 - rho22;
   double F_rho = -std::sqrt(a1 * a1 + a2 * a2 + 2 * (std::fabs(rho11 - rho12)) * std::fabs(rho21 - rho22) *
-                                                  std::cos(2 * theta));
+                                                  om::cos(2 * theta));
   double A = (sum_rho + F_rho) / 4.;
   double B = (sum_rho - F_rho) / 4.;
   double curvatures_sum = A + B;
-  double lg = std::log10(B / A);
+  double lg = om::log10(B / A);
   double x_squ = lg * lg;
   double gamma1 = 2 * (1 + x_squ * (0.40227436 + x_squ * (3.7491752e-2 + x_squ * (7.4855761e-4 + x_squ * 2.1667028e-6))));
   double gamma2 = 3 * (1 + x_squ * (0.42678878 + x_squ * (4.2605401e-2 + x_squ * (9.0786922e-4 + x_squ * 2.7868927e-6))));
-  double alpha = std::pow((B / A), gamma1 / gamma2);
+  double alpha = om::pow((B / A), gamma1 / gamma2);
   double m1 = 1 / (alpha * alpha);
-  double K = (1.3862944 + m1 * (0.1119723 + 0.0725296 * m1)) - std::log(m1) * (0.5 + m1 * (0.1213478 + 0.0288729 * m1));
-  double E = (1 + m1 * (0.4630151 + 0.1077812 * m1)) - std::log(m1) * (0.2452727 + 0.0412496 * m1) * m1;
+  double K = (1.3862944 + m1 * (0.1119723 + 0.0725296 * m1)) - om::log(m1) * (0.5 + m1 * (0.1213478 + 0.0288729 * m1));
+  double E = (1 + m1 * (0.4630151 + 0.1077812 * m1)) - om::log(m1) * (0.2452727 + 0.0412496 * m1) * m1;
   out[0] = curvatures_sum;
   out[1] = alpha;
   out[2] = K;
@@ -806,8 +837,8 @@ void ig2_lm(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
     para[0] = I.contactAngle[0];
     para[1] = I.contactAngle[1];
     double d0[3] = {position2[0] - position1[0], position2[1] - position1[1], position2[2] - position1[2]};
-    para[0] = std::atan2(d0[1], d0[0]);
-    para[1] = std::atan(d0[2] / std::sqrt(d0[0] * d0[0] + d0[1] * d0[1]));
+    para[0] = om::atan2(d0[1], d0[0]);
+    para[1] = om::atan(d0[2] / std::sqrt(d0[0] * d0[0] + d0[1] * d0[1]));
     para[0] += 1e-5;
     para[1] += 1e-5;
   } else {
@@ -852,8 +883,8 @@ void ig2_lm(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
     if (std::isnan(para[0]) || std::isnan(para[1]) || 7 == stop) {
       if (touching) {
         double d0[3] = {position2[0] - position1[0], position2[1] - position1[1], position2[2] - position1[2]};
-        para[0] = std::atan2(d0[1], d0[0]);
-        para[1] = std::atan(d0[2] / std::sqrt(d0[0] * d0[0] + d0[1] * d0[1]));
+        para[0] = om::atan2(d0[1], d0[0]);
+        para[1] = om::atan(d0[2] / std::sqrt(d0[0] * d0[0] + d0[1] * d0[1]));
         para[0] += 1e-5;
         para[1] += 1e-5;
         stop = lmder2(para, 50, opts, A, B, position1, position2, p1, p2, center, touching, contactNormal, nev) + 100;
@@ -868,8 +899,8 @@ void ig2_lm(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
         double q2[3] = {p1[0] - position2[0], p1[1] - position2[1], p1[2] - position2[2]};
         if (!(is_inside(A, q1)) || !(is_inside(B, q2))) {
           double d0[3] = {position2[0] - position1[0], position2[1] - position1[1], position2[2] - position1[2]};
-          para[0] = std::atan2(d0[1], d0[0]);
-          para[1] = std::atan(d0[2] / std::sqrt(d0[0] * d0[0] + d0[1] * d0[1]));
+          para[0] = om::atan2(d0[1], d0[0]);
+          para[1] = om::atan(d0[2] / std::sqrt(d0[0] * d0[0] + d0[1] * d0[1]));
           para[0] += 1e-5;
           para[1] += 1e-5;
           stop = lmder2(para, 50, opts, A, B, position1, position2, p1, p2, center, touching, contactNormal, nev) + 200;
@@ -977,7 +1008,7 @@ void ig2_nm(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
     for (int i = 0; i < 3; i++) I.contactPoint[i] = 0.5 * (p1[i] + p2[i]);
     I.depth = nm_info[1];
     // QUIRK: the curvature direction uses (cos a0 sin a1, sin a0 sin a1, cos a1) (:2463-2465).
-    double c0[3] = {std::cos(para[0]) * std::sin(para[1]), std::sin(para[0]) * std::sin(para[1]), std::cos(para[1])};
+    double c0[3] = {om::cos(para[0]) * om::sin(para[1]), om::sin(para[0]) * om::sin(para[1]), om::cos(para[1])};
     double contact[3], contact1[3], contact2[3];
     matvec(gC, c0, contact);
     matvec(A.Rl, contact, contact1);
@@ -1002,7 +1033,7 @@ void ig2_nm(sdm_ctx* c, Pair& I, const Body& b1, const Body& b2) {
     double dir2[3] = {cd2[0] * contactNormal[0], cd2[1] * contactNormal[1], cd2[2] * contactNormal[2]};
     for (int i = 0; i < 3; i++) dir1[i] -= cd1[i];
     for (int i = 0; i < 3; i++) dir2[i] -= cd2[i];
-    double angle = std::acos(dot3(dir1, dir2) / (norm3(dir1) * norm3(dir2)));
+    double angle = om::acos(dot3(dir1, dir2) / (norm3(dir1) * norm3(dir2)));
     if (std::isnan(angle)) angle = 0.0;
     I.curv1[0] = k1[0]; I.curv1[1] = k1[1];
     I.curv2[0] = k2[0]; I.curv2[1] = k2[1];
@@ -1729,8 +1760,8 @@ double newton_body(sdm_ctx* c, Body& b) {
       double angle = std::sqrt(angle2);
       double ax[3] = {b.angVel[0] / angle, b.angVel[1] / angle, b.angVel[2] / angle};
       double ha = angle * dt * 0.5;
-      double s = std::sin(ha);
-      double q[4] = {std::cos(ha), s * ax[0], s * ax[1], s * ax[2]};
+      double s = om::sin(ha);
+      double q[4] = {om::cos(ha), s * ax[0], s * ax[1], s * ax[2]};
       double o[4];
       quat_mul(q, b.ori, o);
       for (int i = 0; i < 4; i++) b.ori[i] = o[i];
@@ -2255,6 +2286,7 @@ int sdmcpu_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
   const Field* f = find_field(name);
   if (!c || !f || (!src && count > 0)) return fail(c, std::string("sdmcpu_set: bad attribute ") + (name ? name : "(null)"));
   if (!f->writable) return fail(c, std::string("sdmcpu_set: read-only attribute ") + name);
+  if (count < 0) return fail(c, std::string("sdmcpu_set: negative count for ") + name);
   const size_t es = dtype_size(f->dtype);
   const char* in = (const char*)src;
   if (f->scope == S_PAIR && std::strcmp(name, "intr.id1") == 0) {
@@ -2394,6 +2426,7 @@ int sdmcpu_mean_coordination(sdm_ctx* c, double* cn) {
     if (b.kind == SDM_BODY_WALL) continue;
     num_particles += 1;
   }
+  if (num_particles == 0) return fail(c, "sdm_mean_coordination: no particles (every body is a wall)");
   *cn = 2.0 * num_contacts / num_particles;
   return 0;
 }

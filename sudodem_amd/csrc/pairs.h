@@ -205,6 +205,12 @@ __device__ __forceinline__ V3<double> rotate_shear(const GeomState& g, V3<double
   return fs;
 }
 
+// tan(min(phi1, phi2)) of Ip2 (Superquadrics.cpp:1101-1102, 1140-1141): the tangent of the material whose
+// angle std::min picks (the first when equal), as the host's C library evaluates it.
+__device__ __forceinline__ double tan_min_friction(const MaterialDev& m1, const MaterialDev& m2) {
+  return m2.frictionAngle < m1.frictionAngle ? m2.tanF : m1.tanF;
+}
+
 // Ip2 (first time only), the law, and the write-back of one interaction.
 template <int LAW>
 __device__ bool phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1, const BodyView& b2, int id1,
@@ -231,11 +237,11 @@ __device__ bool phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
     // first six bodies -- of its specimen when several specimens share the context
     if (LAW == 0) {
       if (!b1.low6) {
-        ph.tanPhi = tan(fmin(m1.frictionAngle, m2.frictionAngle));
+        ph.tanPhi = tan_min_friction(m1, m2);
         ph.kn = 2.0 * m1.Kn * m2.Kn / (m1.Kn + m2.Kn);
         ph.ks = 2.0 * m1.Ks * m2.Ks / (m1.Ks + m2.Ks);
       } else {
-        ph.tanPhi = tan(m1.frictionAngle);
+        ph.tanPhi = m1.tanF;
         ph.kn = m1.Kn;
         ph.ks = m1.Ks;
       }
@@ -245,9 +251,9 @@ __device__ bool phys_law_store(const StepArgs& A, int64_t p, const BodyView& b1,
         ph.Gab = 1.0 / ((1 - m1.nu) / m1.G + (1 - m2.nu) / m2.G);
         ph.mu1 = 1. / m1.G + 1. / m2.G;
         ph.mu2 = m1.nu / m1.G + m2.nu / m2.G;
-        ph.tanPhi = tan(fmin(m1.frictionAngle, m2.frictionAngle));
+        ph.tanPhi = tan_min_friction(m1, m2);
       } else {
-        ph.tanPhi = tan(m1.frictionAngle);
+        ph.tanPhi = m1.tanF;
         ph.kn = m1.Kn;
         ph.ks = m1.Ks;
       }
@@ -579,7 +585,9 @@ __global__ __launch_bounds__(256) void k_pairs_prep(StepArgs A, uint32_t* qflag,
             }
           }
           if (outcome >= 0) {
-            const bool keep = early_store<LAW>(A, p, id1, id2, hadGeom, hadPhys, para0, para1, outcome, 0, stopCode);
+            // nEval: the reference's count (the one evaluation the box bound stands in for)
+            const bool keep = early_store<LAW>(A, p, id1, id2, hadGeom, hadPhys, para0, para1, outcome,
+                                               outcome == GEO_EARLY_ANG ? 1 : 0, stopCode);
             if (outcome == GEO_EARLY && keep)
               gapv = __double2float_rd((dist - (rmax1 + rmax2)) - ((double)B.disp[id1] + (double)B.disp[id2]));
           } else {

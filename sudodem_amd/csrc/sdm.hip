@@ -1281,7 +1281,7 @@ int sdm_set_materials(sdm_ctx* c, const sdm_material* mats, int32_t n) {
     std::vector<MaterialDev> md(std::max(1, n));
     for (int i = 0; i < n; i++)
       md[i] = MaterialDev{mats[i].density, mats[i].Kn, mats[i].Ks, mats[i].frictionAngle,
-                          mats[i].betan,   mats[i].betas, mats[i].G, mats[i].nu};
+                          mats[i].betan,   mats[i].betas, mats[i].G, mats[i].nu, std::tan(mats[i].frictionAngle)};
     HIPCHK(hipMemcpy(c->dmats, md.data(), md.size() * sizeof(MaterialDev), hipMemcpyHostToDevice));
   } catch (HipError& e) {
     return fail(c, e.msg);
@@ -1398,6 +1398,7 @@ int sdm_get(sdm_ctx* c, const char* name, void* dst, int64_t count) {
 int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
   const FieldInfo* f = find_field(name);
   if (!c || !f || (!src && count > 0)) return fail(c, std::string("sdm_set: bad attribute ") + (name ? name : "(null)"));
+  if (count < 0) return fail(c, std::string("sdm_set: negative count for ") + name);
   c->skipValid = false;
   c->periCover = false;
   c->graphGen++;
@@ -1569,6 +1570,7 @@ int sdm_mean_coordination(sdm_ctx* c, double* cn) {
     HIPCHK(hipMemcpyAsync(h, buf, sizeof(h), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     dfree(buf);
+    if (h[1] == 0) return fail(c, "sdm_mean_coordination: no particles (every body is a wall)");
     *cn = 2.0 * (double)h[0] / (double)h[1];
   } catch (HipError& e) {
     return fail(c, e.msg);

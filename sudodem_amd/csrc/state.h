@@ -150,4 +150,5 @@ struct PairArrays {
 
 struct MaterialDev {
   double density, Kn, Ks, frictionAngle, betan, betas, G, nu;
+  double tanF;  // tan(frictionAngle), evaluated on the host by the C library the reference calls
 };

@@ -12,16 +12,19 @@ from sudodem_amd._native import SdmLibrary
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "libsdm_oracle.so")
+# the same restatement built on the correctly rounded functions of crmath.h (glibc's values
+# for all but ~0.1 % of arguments): the reference arithmetic with glibc's misroundings removed
+ORACLE_CR_LIB = os.path.join(ORACLE_DIR, "libsdm_oracle_cr.so")
 
-_LIB = None
+_LIBS = {}
 
 
-def load_oracle() -> SdmLibrary:
-    global _LIB
-    if _LIB is None:
-        if not os.path.exists(ORACLE_LIB):
+def load_oracle(cr: bool = False) -> SdmLibrary:
+    path = ORACLE_CR_LIB if cr else ORACLE_LIB
+    if path not in _LIBS:
+        if not os.path.exists(path):
             subprocess.run(["make", "-C", ORACLE_DIR], check=True, capture_output=True)
-        lib = SdmLibrary(ORACLE_LIB, "sdmcpu_")
+        lib = SdmLibrary(path, "sdmcpu_")
         L = lib.lib
         lib.set_threads = L.sdmcpu_set_threads
         lib.set_threads.restype = ctypes.c_int
@@ -39,5 +42,5 @@ def load_oracle() -> SdmLibrary:
         lib.hertz_mindlin = L.sdmcpu_hertz_mindlin
         lib.hertz_mindlin.restype = ctypes.c_int
         lib.hertz_mindlin.argtypes = [d3, d3, ctypes.c_double, d3]
-        _LIB = lib
-    return _LIB
+        _LIBS[path] = lib
+    return _LIBS[path]

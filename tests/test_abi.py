@@ -48,3 +48,35 @@ def test_version_and_argument_errors(libname, oracle):
     p.sort_axis = 3
     assert not lib.create(ctypes.byref(p), 10, 1)
     assert b"sortAxis" in lib.last_error(None)
+
+
+def check_observable_argument_errors(lib):
+    """A scene of walls only: getMeanCN has no particle to divide by and reports an error
+    (the reference would return inf or NaN, py/_utils.cpp:522-539); a negative interaction
+    count is refused before the container is touched."""
+    from sudodem_amd import scene as S
+    mat = S.SuperquadricsMat(Kn=1e5, Ks=7e4, frictionAngle=0.5, density=2650)
+    O = S.Omega(lib=lib)
+    O.materials.append(mat)
+    O.bodies.extend([S.utils.wall(-1.0 if s > 0 else 1.0, axis=a, sense=s, material=mat) for a in range(3)
+                     for s in (1, -1)])
+    O.engines = [S.ForceResetter(), S.InsertionSortCollider([S.Bo1_Superquadrics_Aabb(), S.Bo1_Wall_Aabb()],
+                                                            verletDist=1e-4),
+                 S.InteractionLoop([S.Ig2_Wall_Superquadrics_SuperquadricsGeom(),
+                                    S.Ig2_Superquadrics_Superquadrics_SuperquadricsGeom()],
+                                   [S.Ip2_SuperquadricsMat_SuperquadricsMat_SuperquadricsPhys()], [S.SuperquadricsLaw()]),
+                 S.NewtonIntegrator(damping=0.3, gravity=(0.0, 0.0, -9.81), isSuperquadrics=1)]
+    O.dt = 1e-6
+    O.run(1)
+    ctx = O.ctx
+    with pytest.raises(nat.SdmError, match="no particles"):
+        ctx.mean_coordination()
+    n = len(ctx.get("intr.id1"))
+    assert lib.set(ctx.ptr, b"intr.id1", None, -1) != 0
+    assert b"negative" in lib.last_error(ctx.ptr)
+    assert len(ctx.get("intr.id1")) == n
+    O.run(1)  # the context still steps
+
+
+def test_observable_argument_errors_oracle(oracle):
+    check_observable_argument_errors(oracle)

@@ -86,3 +86,8 @@ def test_whole_packing_inserted_in_one_run(gpu, oracle):
         c.step(10)
         _same(g, c, k)
     assert len(g.get("intr.id1")) > 2000
+
+
+def test_observable_argument_errors_device(gpu):
+    from test_abi import check_observable_argument_errors
+    check_observable_argument_errors(gpu)
