@@ -153,6 +153,15 @@ int sdm_mean_coordination(sdm_ctx* ctx, double* cn);
 int sdm_stress(sdm_ctx* ctx, double volume, double sigma[9]);
 int sdm_body_forces(sdm_ctx* ctx, const int32_t* ids, int32_t n, double* forces);
 
+/* sdm_stress without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and result as sdm_stress (bit for bit); the
+ * block partials live in a static device array of 9 MiB per GPU, held until the process ends,
+ * which limits it to 33,554,432 interactions (larger containers: an error, use sdm_stress).
+ * Every call on one GPU shares that array: calls that overlap on one GPU (several host threads
+ * or streams) race. */
+int sdm_stress_noalloc(sdm_ctx* ctx, double volume, double sigma[9]);
+
 /* Wall stress servo: CompressionEngine (pkg/dem/Compression.cpp) on a cubic specimen as the
  * reference compiles it (NEW_SCHEME and SOLE_GAIN, Compression.hpp:26-29): action_cubic
  * (:395-604) with generalConsolidation (:324-345) or hydroConsolidation (:264-322) when z_servo,

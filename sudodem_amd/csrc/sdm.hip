@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1590,6 +1591,59 @@ int sdm_stress(sdm_ctx* c, double volume, double sigma[9]) {
     HIPCHK(hipMemcpyAsync(sigma, part + 9 * nb, 9 * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     dfree(part);
+  } catch (HipError& e) {
+    return fail(c, e.msg);
+  }
+  return 0;
+}
+
+// Scratch of sdm_stress_noalloc: the per-block partial tensors and the result, in a
+// file-scope device array instead of an allocation.  Each GPU holds its own copy (for the
+// life of the process) at its own address; host code never passes the array's name (a host
+// stand-in), only the address hipGetSymbolAddress returns on that device.
+constexpr int64_t kStressScratchBlocks = 131072;  // interactions: up to 131072 * 256 = 33,554,432
+static __device__ __align__(256) double scratch[9 * kStressScratchBlocks + 9];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on device `dev`, looked up once per device with it current.
+double* stress_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw HipError{"sdm_stress_noalloc: device index out of range"};
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      throw HipError{std::string("sdm_stress_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+    g_scratch[dev] = (double*)p;
+  }
+  return g_scratch[dev];
+}
+}  // namespace
+
+int sdm_stress_noalloc(sdm_ctx* c, double volume, double sigma[9]) {
+  if (!c || !sigma) return fail(c, "sdm_stress_noalloc: bad arguments");
+  try {
+    const int64_t nb = (int64_t)nblk(std::max<int64_t>(1, c->np));
+    if (nb > kStressScratchBlocks)
+      return fail(c, "sdm_stress_noalloc: " + std::to_string(c->np) + " interactions exceed the scratch limit of " +
+                         std::to_string(kStressScratchBlocks * 256) + "; use sdm_stress");
+    double* part = stress_scratch(c->dev);
+    if (volume == 0) volume = c->prm.periodic ? c->prm.cell[0] * c->prm.cell[1] * c->prm.cell[2] : 1.0;
+    // every block writes its partial and the final kernel its result: nothing to clear
+    hipLaunchKernelGGL(k_stress_partial, dim3((unsigned)nb), dim3(256), 0, c->st, c->P, c->np, c->B,
+                       c->prm.periodic ? 1 : 0, c->prm.cell[0], c->prm.cell[1], c->prm.cell[2], part);
+    hipLaunchKernelGGL(k_stress_final, dim3(1), dim3(64), 0, c->st, part, nb, volume,
+                       part + 9 * kStressScratchBlocks);
+    HIPCHK(hipMemcpyAsync(sigma, part + 9 * kStressScratchBlocks, 9 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
   } catch (HipError& e) {
     return fail(c, e.msg);
   }
