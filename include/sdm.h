@@ -98,6 +98,11 @@ int sdm_set_materials(sdm_ctx* ctx, const sdm_material* mats, int32_t n);
  * (pkg/dem/Superquadrics.cpp:47-93): r_max, volume and principal inertia per unit density. */
 int sdm_superquadric_props(double rx, double ry, double rz, double eps1, double eps2,
                            double* r_max, double* volume, double inertia[3]);
+/* The same for n bodies (semi-axes rxyz[3n], exponents eps[2n]; out r_max[n], volume[n],
+ * inertia[3n]): the body-construction loop of a scene script (NewSuperquadrics per body,
+ * py/_superquadrics_utils.cpp:354-379) in one call.  Bit for bit the single-body values. */
+int sdm_superquadric_props_n(int64_t n, const double* rxyz, const double* eps, double* r_max, double* volume,
+                             double* inertia);
 
 /* Attribute access.  `name` is one of the attribute names listed by sdm_field_list(); the
  * element type (SDM_F64 / SDM_I32 / SDM_I64) and the number of elements (count) are fixed by

@@ -108,6 +108,8 @@ class SdmLibrary:
         self.set_materials = fn("set_materials", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_material), ctypes.c_int32])
         self.superquadric_props = fn("superquadric_props", ctypes.c_int,
                                      [ctypes.c_double] * 5 + [ctypes.POINTER(ctypes.c_double)] * 3)
+        self.superquadric_props_n = fn("superquadric_props_n", ctypes.c_int,
+                                       [ctypes.c_int64] + [ctypes.POINTER(ctypes.c_double)] * 5)
         self.field_info = fn("field_info", ctypes.c_int,
                              [c_ctx, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)])
         self.get = fn("get", ctypes.c_int, [c_ctx, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64])
@@ -143,6 +145,18 @@ class SdmLibrary:
         rc = self.superquadric_props(rx, ry, rz, e1, e2, ctypes.byref(r), ctypes.byref(v), inert)
         self.check(rc)
         return r.value, v.value, np.array(list(inert))
+
+    def props_n(self, rxyz, eps):
+        """r_max, volume and inertia (per unit density) of n bodies in one call."""
+        rxyz = np.ascontiguousarray(rxyz, dtype=np.float64).reshape(-1, 3)
+        eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1, 2)
+        n = len(rxyz)
+        rmax, vol, inert = np.empty(n), np.empty(n), np.empty((n, 3))
+        d = ctypes.POINTER(ctypes.c_double)
+        rc = self.superquadric_props_n(n, rxyz.ctypes.data_as(d), eps.ctypes.data_as(d), rmax.ctypes.data_as(d),
+                                       vol.ctypes.data_as(d), inert.ctypes.data_as(d))
+        self.check(rc)
+        return rmax, vol, inert
 
 
 class Context:

@@ -41,14 +41,7 @@ def random_quaternions(rng, n):
 
 def superquadric_props(rxyz, eps):
     """Vectorised front end of sdm_superquadric_props (Superquadrics::Initial)."""
-    lib = nat.load_hip()
-    n = len(rxyz)
-    rmax = np.empty(n)
-    vol = np.empty(n)
-    inert = np.empty((n, 3))
-    for i in range(n):
-        rmax[i], vol[i], inert[i] = lib.props(*rxyz[i], *eps[i])
-    return rmax, vol, inert
+    return nat.load_hip().props_n(rxyz, eps)
 
 
 def support_h(rxyz, eps, R, d):

@@ -1326,6 +1326,18 @@ int sdm_superquadric_props(double rx, double ry, double rz, double eps1, double 
   return 0;
 }
 
+int sdm_superquadric_props_n(int64_t n, const double* rxyz, const double* eps, double* r_max, double* volume,
+                             double* inertia) {
+  if (n < 0 || (n > 0 && (!rxyz || !eps || !r_max || !volume || !inertia)))
+    return fail(nullptr, "sdm_superquadric_props_n: bad arguments");
+  for (int64_t i = 0; i < n; i++) {
+    const int rc = sdm_superquadric_props(rxyz[3 * i], rxyz[3 * i + 1], rxyz[3 * i + 2], eps[2 * i], eps[2 * i + 1],
+                                          &r_max[i], &volume[i], &inertia[3 * i]);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 const char* sdm_field_list(void) {
   static std::string s;
   if (s.empty())
