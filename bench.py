@@ -67,8 +67,9 @@ def mixed_error(ref, mix):
 
 
 def timed_run(ctx, steps, timers=True):
-    """Wall time of `steps` steps.  With timers the phases are bracketed by HIP events (and the
-    step graph is off); without, steady-state steps replay the captured graph."""
+    """Wall time of `steps` steps.  With timers the phases and the bandwidth-bound kernels are
+    bracketed by HIP events on the context's stream (and the step graph is off); without,
+    steady-state steps replay the captured graph."""
     ctx.reset_counters()
     ctx.set_timing(timers)
     ctx.sync()
@@ -76,7 +77,11 @@ def timed_run(ctx, steps, timers=True):
     ctx.step(steps)
     ctx.sync()
     wall = time.perf_counter() - t0
-    return wall, ctx.counters(), (ctx.timers() if timers else None)
+    if not timers:
+        return wall, ctx.counters(), None
+    tm = ctx.timers()
+    tm["kernels"] = ctx.kernel_timers()
+    return wall, ctx.counters(), tm
 
 
 def calibrate_copy(nbytes=1 << 30, reps=4):
@@ -421,6 +426,8 @@ def main():
             "solver %.3f ms/step, %d collider runs" % (name, wall / args.steps * 1e3, cnt["graph_steps"], args.steps,
                                                        wall_t / args.steps * 1e3, tm["solver_ms"] / args.steps,
                                                        cnt["collider_runs"]))
+        log("%s kernels: %s" % (name, json.dumps(tm["kernels"])))
+        log("%s timers: %s" % (name, json.dumps({k: v for k, v in tm.items() if k != "kernels"})))
         # one step from the snapshot again, kept for the per-contact error of the mixed form
         ctx.restore(snap)
         ctx.step(1)

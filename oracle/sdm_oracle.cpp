@@ -2504,6 +2504,12 @@ int sdmcpu_get_timers(sdm_ctx* c, double out[8]) {
   return 0;
 }
 
+int sdmcpu_get_kernel_timers(sdm_ctx* c, double out[10]) {
+  if (!c || !out) return -1;
+  for (int k = 0; k < 10; k++) out[k] = 0;
+  return 0;
+}
+
 int sdmcpu_set_threads(sdm_ctx* c, int32_t n) {
   if (!c || n < 1) return fail(c, "bad thread count");
 #ifndef _OPENMP

@@ -128,6 +128,7 @@ class SdmLibrary:
         self.set_tuning = fn("set_tuning", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_tuning)])
         self.get_tuning = fn("get_tuning", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_tuning)])
         self.get_timers = fn("get_timers", ctypes.c_int, [c_ctx, ctypes.POINTER(ctypes.c_double)])
+        self.get_kernel_timers = fn("get_kernel_timers", ctypes.c_int, [c_ctx, ctypes.POINTER(ctypes.c_double)])
         self.set_servo = fn("set_servo", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_servo)])
         self.get_servo = fn("get_servo", ctypes.c_int, [c_ctx, ctypes.POINTER(sdm_servo_state)])
 
@@ -291,6 +292,18 @@ class Context:
         return {"collider_ms": out[0], "interactions_ms": out[1], "newton_ms": out[2], "steps": int(out[3]),
                 "solver_ms": out[4], "solver_evals": int(out[5]), "extra_evals": int(out[6]),
                 "queued": int(out[7])}
+
+    def kernel_timers(self):
+        """Device time [ms] and launches of the integrator, the force sums, the bound update and
+        the interaction-loop dispatch pass since set_timing(True); with the mixed Hertz form, the
+        Nelder-Mead pairs listed for the solver and the solver's evaluations done in double."""
+        out = (ctypes.c_double * 10)()
+        self.lib.check(self.lib.get_kernel_timers(self.ptr, out), self.ptr)
+        r = {name: {"ms": out[2 * k], "launches": int(out[2 * k + 1])}
+             for k, name in enumerate(("k_newton", "k_force_sum", "k_bounds", "k_pairs_prep"))}
+        r["nm_listed"] = int(out[8])
+        r["nm_double_evals"] = int(out[9])
+        return r
 
     def snapshot(self):
         """Complete state (bodies, interactions, collider and integrator flags) as arrays."""

@@ -374,6 +374,25 @@ def densify(cfg, lib, iterations=150, target=2e-6, growth=0.003, max_push=5e-5, 
     return out
 
 
+def box_fraction(cfg):
+    """Solids fraction as CompressionEngine computes it (pkg/dem/Compression.cpp:425-458):
+    the particles' volume over the volume of the box the six walls enclose."""
+    A = cfg.arrays
+    part = A["kind"] == nat.SDM_BODY_SUPERQUADRIC
+    vol = float((A["mass"][part] / np.array([cfg.materials[k].density for k in A["material"][part]])).sum())
+    lo, hi = _wall_box(A)
+    return vol / float(np.prod(hi - lo))
+
+
+def _wall_box(A):
+    """Lower and upper box corners from the six walls (sense +1: the lower face of its axis)."""
+    lo, hi = np.zeros(3), np.zeros(3)
+    for k in np.nonzero(A["kind"] == nat.SDM_BODY_WALL)[0]:
+        a = A["wallAxis"][k]
+        (lo if A["wallSense"][k] > 0 else hi)[a] = A["pos"][k, a]
+    return lo, hi
+
+
 def config2(n_particles=1000000, seed_extra=0, fraction=0.62):
     """Periodic workload: n_particles superellipsoids in a cubic periodic cell, linear law.
 

@@ -1,0 +1,189 @@
+// delta.h -- the distance function of the Nelder-Mead solver as a float difference from a
+// double anchor (mixed precision form).
+//
+// Disfun (pkg/dem/Superquadrics_Ig2.cpp:1624-1678) is |p2 - p1|, two support points of
+// millimetre-size bodies whose difference is a micrometre-size gap.  Evaluated in float, each
+// support point carries rounding of a few ulp of its millimetre coordinates, ~1e-10 m, which
+// is ~1e-4 of the gap: Nelder-Mead comparisons inside that band go either way.  Here one
+// evaluation per pair -- the anchor, at the solve's first vertex -- is done in double, and
+// every other evaluation computes only its difference from the anchor in float, in relative
+// (logarithmic) form, so that its rounding scales with the difference (a few ulp of a
+// micrometre-size displacement) and not with the support point.
+//
+// For a body with local direction n (normalised direction in body axes) the support point of
+// Normal2Phi + getSurface (Superquadrics.cpp:110-120, 269-307) is, with e1 = 1/(2 - eps1),
+// e2 = 1/(2 - eps2):
+//   t  = tan phi0 = sgn(ny) |ry ny|^e1 / (sgn(nx) |rx nx|^e1)
+//   t' = tan phi1 = sgn(nz) |rz nz |cos phi0|^(2-eps1)|^e2 / |rx nx|^e2
+//   q  = (sgn(c0) rx |c0|^eps1 |c1|^eps2, sgn(s0) ry |s0|^eps1 |c1|^eps2, sgn(s1) rz |s1|^eps2)
+// with c = cos, s = sin of phi0, phi1.  When no component of n changes sign, every factor
+// changes by a relative amount:
+//   u0 = e1 (ln(1 + dny/ny) - ln(1 + dnx/nx)),  dt = t (e^u0 - 1)
+//   d ln|c0| = -1/2 ln(1 + dt (2t + dt) / (1 + t^2)),  d ln|s0| = u0 + d ln|c0|
+//   u1 = e2 (ln(1 + dnz/nz) + (2 - eps1) d ln|c0| - ln(1 + dnx/nx)),  dt' = t' (e^u1 - 1)
+//   d ln c1 = -1/2 ln(1 + dt' (2t' + dt') / (1 + t'^2)),  d ln|s1| = u1 + d ln c1
+//   dq = q (e^(d ln|q|) - 1),  d ln|qx| = eps1 d ln|c0| + eps2 d ln c1,  ...
+// evaluated with log1p / expm1 built on the hardware log2 / exp2.  The difference of the
+// contact direction itself comes from half-angle forms of the angle steps.  A factor that
+// would change sign or come within 1/4 of zero (a component of n crossing a body axis plane)
+// is not evaluated this way: the evaluation reports it, and the solver hands the pair over.
+#pragma once
+#include "narrow.h"
+
+// log(1 + x) for x > -1 and exp(x) - 1 in float, a few ulp of the result: the rounding of
+// 1 + x and of exp(x) is undone by the ratio (Goldberg's and Kahan's forms).
+SDM_HD float dlog1p(float x) {
+  const float u = 1.0f + x;
+  const float l = __builtin_amdgcn_logf(u) * 0.693147180559945309f;  // ln u
+  return u == 1.0f ? x : l * (x / (u - 1.0f));
+}
+SDM_HD float dexpm1(float x) {
+  const float u = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+  const float l = __builtin_amdgcn_logf(u) * 0.693147180559945309f;
+  return (u == 1.0f || l == 0.0f) ? x : (u - 1.0f) * (x / l);
+}
+
+// The anchor of one body: the support point and its factors at the anchor direction.
+struct DeltaBody {
+  float n[3];    // local direction at the anchor
+  float t, tp;   // tan phi0, tan phi1
+  float q[3];    // local support point
+  float e1, e2, eps1, eps2, t1;  // exponents: 1/(2 - eps1), 1/(2 - eps2), eps1, eps2, 2 - eps1
+  M3<float> G;   // rot_mat2global
+};
+
+// Double support point along local direction n with the anchor factors; returns false when a
+// factor is degenerate (a zero direction component or a zero divisor), where the relative form
+// does not apply.  kFast: the device's double functions.
+SDM_HD bool delta_anchor_local(const Shape<double>& s, V3<double> n, DeltaBody& b, V3<double>& q) {
+  const double e1 = 1.0 / (2.0 - s.eps1), e2 = 1.0 / (2.0 - s.eps2);
+  if (n.x == 0.0 || n.y == 0.0 || n.z == 0.0) return false;
+  const double ax = fabs(s.rx * n.x);
+  const double a = sgn(n.y) * pow(fabs(s.ry * n.y), e1);
+  const double bb = sgn(n.x) * pow(ax, e1);
+  const double t = a / bb;
+  const double h0 = sqrt(a * a + bb * bb);
+  const double c0 = bb / h0, s0 = a / h0;
+  const double ap = sgn(n.z) * pow(fabs(s.rz * n.z * pow(fabs(c0), 2.0 - s.eps1)), e2);
+  const double bp = pow(ax, e2);
+  const double tp = ap / bp;
+  const double h1 = sqrt(ap * ap + bp * bp);
+  const double c1 = bp / h1, s1 = ap / h1;
+  const double pc1 = pow(c1, s.eps2);
+  q = mk3(sgn(c0) * s.rx * pow(fabs(c0), s.eps1) * pc1, sgn(s0) * s.ry * pow(fabs(s0), s.eps1) * pc1,
+          sgn(s1) * s.rz * pow(fabs(s1), s.eps2));
+  b.n[0] = (float)n.x; b.n[1] = (float)n.y; b.n[2] = (float)n.z;
+  b.t = (float)t;
+  b.tp = (float)tp;
+  b.q[0] = (float)q.x; b.q[1] = (float)q.y; b.q[2] = (float)q.z;
+  b.e1 = (float)e1; b.e2 = (float)e2; b.eps1 = (float)s.eps1; b.eps2 = (float)s.eps2; b.t1 = (float)(2.0 - s.eps1);
+  const bool ok = h0 > 0.0 && h1 > 0.0 && bb != 0.0 && bp != 0.0 && c0 != 0.0 && s0 != 0.0 && c1 != 0.0 && s1 != 0.0;
+  return ok && fabs(t) < 1e30 && fabs(tp) < 1e30;
+}
+
+// Change of the global support point of a body when its local direction changes by dn.
+// Sets `ok` false when a factor would change sign or come near zero.  `cond` receives a
+// bound on the conditioning of the relative form, (|qx| + |qy| + |qz|) (1/|nx| + 1/|ny| +
+// 1/|nz|): the rounding of a direction change (~ulp |dn| in each component) enters through
+// dn_k / n_k, and every logarithmic factor passes it on with a coefficient of at most one
+// (d ln c0 / d u0 = -sin^2 phi0, d ln s0 / d u0 = cos^2 phi0, the same for phi1).  A sharper
+// first-order propagation at the new angles underestimated the error of some evaluations
+// up to 150-fold on MI355X; this one bounded it.
+SDM_HD V3<float> delta_support(const DeltaBody& b, const float dn[3], bool& ok, float& cond) {
+  const float rx = dn[0] / b.n[0], ry = dn[1] / b.n[1], rz = dn[2] / b.n[2];
+  const float lx = dlog1p(rx), ly = dlog1p(ry), lz = dlog1p(rz);
+  const float u0 = b.e1 * (ly - lx);
+  const float dt = b.t * dexpm1(u0);
+  const float w0 = dt * (2.0f * b.t + dt) / (1.0f + b.t * b.t);
+  const float lc0 = -0.5f * dlog1p(w0);
+  const float ls0 = u0 + lc0;
+  const float u1 = b.e2 * (lz + b.t1 * lc0 - lx);
+  const float dtp = b.tp * dexpm1(u1);
+  const float w1 = dtp * (2.0f * b.tp + dtp) / (1.0f + b.tp * b.tp);
+  const float lc1 = -0.5f * dlog1p(w1);
+  const float ls1 = u1 + lc1;
+  const float gx = b.eps1 * lc0 + b.eps2 * lc1, gy = b.eps1 * ls0 + b.eps2 * lc1, gz = b.eps2 * ls1;
+  // no factor may change sign or come near zero (log1p arguments above -3/4), and the
+  // exponentials stay far from overflow
+  const float lo = -0.75f, hi = 16.0f;
+  ok = ok && rx >= lo && ry >= lo && rz >= lo && w0 >= lo && w1 >= lo && fabsf(u0) <= hi && fabsf(u1) <= hi &&
+       fabsf(gx) <= hi && fabsf(gy) <= hi && fabsf(gz) <= hi;
+  const V3<float> dq = mk3(b.q[0] * dexpm1(gx), b.q[1] * dexpm1(gy), b.q[2] * dexpm1(gz));
+  cond = (fabsf(b.q[0]) + fabsf(b.q[1]) + fabsf(b.q[2])) *
+         (1.0f / fabsf(b.n[0]) + 1.0f / fabsf(b.n[1]) + 1.0f / fabsf(b.n[2]));
+  return mul(b.G, dq);
+}
+
+// A pair's anchor for the solver: both bodies, the contact direction's angles and the gap
+// vector at the anchor (global frame, CRotationMat frame gC of Geom2).
+struct DeltaPair {
+  DeltaBody b1, b2;
+  M3<float> gC;
+  float ca0, sa0, ca1, sa1;  // cos / sin of the anchor angles
+  float Fh[3], Fl[3];        // p2 - p1 at the anchor as a float pair (hi + lo: double accuracy)
+  float cu[3];               // contact direction (global) at the anchor
+};
+
+// The anchor in double at angles (a0, a1): fills `P` and returns |p2 - p1| in double, the
+// points p1, p2, the touching test (Disfun :1670-1674) and its operand cd = c.(p2 - p1).  `ok`
+// is false when the relative form does not apply to this pair.
+SDM_HD double delta_anchor(const PairGeo<double>& g, const M3<double>& gC, double a0, double a1, DeltaPair& P,
+                           V3<double>& p1, V3<double>& p2, bool& touching, double& cd, bool& ok) {
+  const double sa0 = sin(a0), ca0 = cos(a0), sa1 = sin(a1), ca1 = cos(a1);
+  const V3<double> c = mul(gC, mk3(ca0 * ca1, sa0 * ca1, sa1));
+  V3<double> q1, q2;
+  ok = delta_anchor_local(g.s1, mulT(g.G1, c), P.b1, q1);
+  ok = delta_anchor_local(g.s2, mulT(g.G2, mk3(-c.x, -c.y, -c.z)), P.b2, q2) && ok;
+  p1 = mul(g.G1, q1) + g.pos1;
+  p2 = mul(g.G2, q2) + g.pos2;
+  const V3<double> d = p2 - p1;
+  cd = dot(c, d);
+  if (cd > 0.0) touching = false;
+  P.b1.G = cast9<float>(g.G1);
+  P.b2.G = cast9<float>(g.G2);
+  P.gC = cast9<float>(gC);
+  P.ca0 = (float)ca0; P.sa0 = (float)sa0; P.ca1 = (float)ca1; P.sa1 = (float)sa1;
+  P.Fh[0] = (float)d.x; P.Fh[1] = (float)d.y; P.Fh[2] = (float)d.z;
+  P.Fl[0] = (float)(d.x - (double)P.Fh[0]);
+  P.Fl[1] = (float)(d.y - (double)P.Fh[1]);
+  P.Fl[2] = (float)(d.z - (double)P.Fh[2]);
+  P.cu[0] = (float)c.x; P.cu[1] = (float)c.y; P.cu[2] = (float)c.z;
+  return norm(d);
+}
+
+// One evaluation at the angle steps (d0, d1) from the anchor: |p2 - p1|, the touching test
+// and its operand cd, and the size `ds` of the support displacements, which bounds the
+// rounding of f (the float difference is added to the anchor's gap vector in double) and,
+// with ulp(f), of cd.  `ok` is false when the relative form left its range; the value is
+// then not usable.
+SDM_HD double delta_eval(const DeltaPair& P, float d0, float d1, bool& touching, float& cd, float& ds, float& dk,
+                         bool& ok) {
+  // cos / sin of (a + d) - cos / sin a by half-angle forms
+  float sh0, ch0, sh1, ch1;
+  sincosf(d0, &sh0, &ch0);
+  sincosf(d1, &sh1, &ch1);
+  const float h0 = sinf(0.5f * d0), h1 = sinf(0.5f * d1);
+  const float vc0 = -2.0f * h0 * h0, vc1 = -2.0f * h1 * h1;  // cos d - 1
+  const float dca0 = P.ca0 * vc0 - P.sa0 * sh0, dsa0 = P.sa0 * vc0 + P.ca0 * sh0;
+  const float dca1 = P.ca1 * vc1 - P.sa1 * sh1, dsa1 = P.sa1 * vc1 + P.ca1 * sh1;
+  const float ca1 = P.ca1 + dca1;
+  // u = (cos a0 cos a1, sin a0 cos a1, sin a1): du = d(x) y + x* d(y)
+  const V3<float> du = mk3(dca0 * ca1 + P.ca0 * dca1, dsa0 * ca1 + P.sa0 * dca1, dsa1);
+  const V3<float> dc = mul(P.gC, du);
+  const V3<float> dn1 = mulT(P.b1.G, dc);
+  const V3<float> dn2 = mulT(P.b2.G, dc);
+  const float n1[3] = {dn1.x, dn1.y, dn1.z};
+  const float n2[3] = {-dn2.x, -dn2.y, -dn2.z};
+  float k1 = 0.0f, k2 = 0.0f;
+  const V3<float> ds1 = delta_support(P.b1, n1, ok, k1);
+  const V3<float> ds2 = delta_support(P.b2, n2, ok, k2);
+  const V3<double> d = mk3(((double)P.Fh[0] + (double)P.Fl[0]) + (double)(ds2.x - ds1.x),
+                           ((double)P.Fh[1] + (double)P.Fl[1]) + (double)(ds2.y - ds1.y),
+                           ((double)P.Fh[2] + (double)P.Fl[2]) + (double)(ds2.z - ds1.z));
+  const V3<float> c = mk3(P.cu[0] + dc.x, P.cu[1] + dc.y, P.cu[2] + dc.z);
+  cd = dot(c, cast3<float>(d));
+  if (cd > 0.0f) touching = false;
+  ds = norm(ds1) + norm(ds2);
+  dk = norm(dc) * (k1 + k2);
+  return norm(d);
+}

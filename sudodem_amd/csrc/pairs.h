@@ -838,7 +838,10 @@ __global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int3
 }
 
 // Curvature directions and the Hertz terms of a Geom2 contact (Superquadrics_Ig2.cpp:2459-2503)
-// in precision T; the mixed form takes them in float (they feed only the force law).
+// in precision T.  Both forms take them in double: in float the angle between the curvature
+// directions (an acos near +-1 of nearly parallel directions, and principal directions of
+// nearly equal curvatures) was off by up to 0.1 rad for ~0.5 % of contacts on MI355X, which
+// moved their Hertz stiffness by up to 50 %.
 template <typename T>
 __device__ __forceinline__ void contact_curvature(const BodyView& b1, const BodyView& b2, double a0, double a1,
                                                   V3<double> cnd, double out[9]) {
@@ -872,7 +875,7 @@ __device__ __forceinline__ void contact_curvature(const BodyView& b1, const Body
 }
 
 // Queued pairs after the solvers: geometry outputs, curvature / Hertz terms, precompute, law.
-template <int LAW, typename TC>
+template <int LAW>
 __global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, const uint32_t* list, int64_t nmax,
                                                       const unsigned long long* count) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -919,14 +922,7 @@ __global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, con
       g.depth = S.f[w];
       // curvature direction convention of :2463-2465
       double cv[9];
-      contact_curvature<TC>(b1, b2, g.a0, g.a1, cn, cv);
-      if (sizeof(TC) == 4) {
-        // a float overflow or cancellation that the double arithmetic survives: redo in double
-        bool ok = true;
-#pragma unroll
-        for (int q = 0; q < 9; q++) ok = ok && isfinite(cv[q]);
-        if (!ok) contact_curvature<double>(b1, b2, g.a0, g.a1, cn, cv);
-      }
+      contact_curvature<double>(b1, b2, g.a0, g.a1, cn, cv);
       g.c10 = cv[0]; g.c11 = cv[1]; g.c20 = cv[2]; g.c21 = cv[3];
       g.curvAngle = cv[4];
       g.curvSum = cv[5]; g.alpha = cv[6]; g.K = cv[7]; g.E = cv[8];

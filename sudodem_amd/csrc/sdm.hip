@@ -31,6 +31,10 @@ namespace {
 
 std::string g_err;
 
+// kernels timed one by one when timing is on: the integrator, the ordered force sums, the
+// bound update of a collider run, the interaction-loop prep
+enum { KT_NEWTON = 0, KT_FORCE_SUM = 1, KT_BOUNDS = 2, KT_PREP = 3, KT_N = 4 };
+
 struct HipError {
   std::string msg;
 };
@@ -55,6 +59,18 @@ void dfree(T*& p) {
 }
 
 inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+#define KT_START(c, k)                                                      \
+  do {                                                                      \
+    if ((c)->timing) HIPCHK(hipEventRecord((c)->kev[2 * (k)], (c)->st));     \
+  } while (0)
+#define KT_STOP(c, k)                                                       \
+  do {                                                                      \
+    if ((c)->timing) {                                                      \
+      HIPCHK(hipEventRecord((c)->kev[2 * (k) + 1], (c)->st));                \
+      (c)->kran[k] = true;                                                  \
+    }                                                                       \
+  } while (0)
 
 }  // namespace
 
@@ -146,6 +162,12 @@ struct sdm_ctx {
   bool timing = false;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double tms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // per-kernel timing of the bandwidth-bound kernels (KT_*): start/stop events, whether the
+  // kernel ran in the current step, accumulated milliseconds and launches
+  hipEvent_t kev[2 * KT_N] = {};
+  bool kran[KT_N] = {};
+  double kms[KT_N + 2] = {};  // + the mixed form's touching-list and double-evaluation counts
+  double kcount[KT_N] = {};
   sdm_counters cnt;
   std::string err;
 };
@@ -291,7 +313,7 @@ void ensure_moves(sdm_ctx* c, int64_t n) {
 void free_queue(SolveIO& S) {
   dfree(S.pair); dfree(S.tag); dfree(S.id1); dfree(S.id2); dfree(S.a0); dfree(S.a1); dfree(S.del);
   dfree(S.f); dfree(S.la0); dfree(S.la1); dfree(S.p1); dfree(S.p2); dfree(S.ctr); dfree(S.ct);
-  dfree(S.nev); dfree(S.stop); dfree(S.touch); dfree(S.sh);
+  dfree(S.nev); dfree(S.stop); dfree(S.touch); dfree(S.sh); dfree(S.anc); dfree(S.fb);
 }
 
 void ensure_queue(sdm_ctx* c, int64_t n) {
@@ -313,6 +335,9 @@ void ensure_queue(sdm_ctx* c, int64_t n) {
   S.ctr = dalloc<double>(3 * cap); S.ct = dalloc<double>(3 * cap); S.nev = dalloc<int32_t>(cap);
   S.stop = dalloc<int32_t>(cap); S.touch = dalloc<int32_t>(cap);
   S.sh = c->prm.periodic ? dalloc<double>(3 * cap) : nullptr;
+  const bool mixed = c->prm.precision == SDM_MIXED;
+  S.anc = mixed ? dalloc<float>(kAnc * cap) : nullptr;
+  S.fb = mixed ? dalloc<int32_t>(cap) : nullptr;
   c->retryList = dalloc<int32_t>(cap);
   c->miscList = dalloc<int32_t>(cap);
   c->qflag = dalloc<uint32_t>(cap + 1);
@@ -383,8 +408,9 @@ void zero_counter(sdm_ctx* c, int k) { hipLaunchKernelGGL(k_zero_u64, dim3(1), d
 // Device counter slots: [0] contact evals, [1] distance evals, [2] collider candidates,
 // [3] solver batch counter, [4] solver evaluations, [5] other evaluations, [6] wall/sphere
 // list, [7]-[9] periodic collider, [10] solver queue, [11] touching list, [12] LM restarts,
-// [13] the container size and [14] Scene::iter of the step.
-enum { DC_NEXT = 3, DC_MISC = 6, DC_NQ = 10, DC_NT = 11, DC_NR = 12, DC_NP = 13 };
+// [13] the container size and [14] Scene::iter of the step, [15] the mixed form's solver
+// evaluations done in double.
+enum { DC_NEXT = 3, DC_MISC = 6, DC_NQ = 10, DC_NT = 11, DC_NR = 12, DC_NP = 13, DC_NDBL = 15 };
 
 // Exclusive scan of in[0..n) into out[0..n) with the total left on the device in dcnt[slot].
 void scan_to_counter(sdm_ctx* c, const uint32_t* in, uint32_t* out, int64_t n, int slot) {
@@ -476,8 +502,10 @@ void collider_run_peri(sdm_ctx* c) {
   }
   HIPCHK(hipMemcpyAsync(c->prevMin, c->B.bmin, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->prevMax, c->B.bmax, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  KT_START(c, KT_BOUNDS);
   hipLaunchKernelGGL(k_bounds, dim3(nblk(N)), dim3(256), 0, c->st, c->B, c->iter, c->prm.verlet_dist,
                      c->prm.min_sweep_dist_factor, c->prm.target_interv);
+  KT_STOP(c, KT_BOUNDS);
   if (!c->strideActive && c->prm.verlet_dist > 0 && c->maxVelocitySq >= 0) c->strideActive = 1;
   ensure_scratch(c, std::max<int64_t>({n2 + 1, c->np + 1, 2 * c->np + 1}));
   ensure_moves(c, c->np + 1);
@@ -734,8 +762,10 @@ void collider_run(sdm_ctx* c) {
     doInitSort = true;
   }
   // BoundDispatcher::action with sweepDist = verletDist
+  KT_START(c, KT_BOUNDS);
   hipLaunchKernelGGL(k_bounds, dim3(nblk(N)), dim3(256), 0, c->st, c->B, c->iter, c->prm.verlet_dist,
                      c->prm.min_sweep_dist_factor, c->prm.target_interv);
+  KT_STOP(c, KT_BOUNDS);
   if (!c->strideActive && c->prm.verlet_dist > 0 && c->maxVelocitySq >= 0) c->strideActive = 1;
   ensure_scratch(c, std::max<int64_t>({n2 + 1, c->np + 1, 2 * c->np + 1}));
   ensure_moves(c, c->np + 1);
@@ -850,7 +880,9 @@ void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA) {
   S2.count = nt;
   S2.evals = c->dcnt + 4;
   S2.next = c->dcnt + DC_NEXT;
+  S2.dbl = c->dcnt + DC_NDBL;
   zero_counter(c, DC_NEXT);
+  zero_counter(c, DC_NDBL);
   const int ti = sizeof(T) == 4 ? 0 : 1;
   const unsigned grid = resident_grid(c, k_solve_nm<T>, c->residentNM[ti], nmax);
   if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
@@ -863,11 +895,7 @@ void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA) {
     RF.evals = c->dcnt + 5;
     hipLaunchKernelGGL(k_nm_refine, dim3(nblk(nmax)), dim3(256), 0, c->st, RF);
   }
-  // the mixed form evaluates the curvature and Hertz terms of the force law in float
-  if (sizeof(T) == 4)
-    hipLaunchKernelGGL((k_pairs_finish<1, float>), dim3(nblk(nmax)), dim3(256), 0, c->st, A, c->S, byPair, nmax, nt);
-  else
-    hipLaunchKernelGGL((k_pairs_finish<1, double>), dim3(nblk(nmax)), dim3(256), 0, c->st, A, c->S, byPair, nmax, nt);
+  hipLaunchKernelGGL(k_pairs_finish<1>, dim3(nblk(nmax)), dim3(256), 0, c->st, A, c->S, byPair, nmax, nt);
 }
 
 // InteractionLoop::action: prep (every interaction), wall / sphere pairs, solvers (queued
@@ -879,8 +907,10 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
   ensure_queue(c, A.np);
   ensure_scratch(c, A.np + 1);
   zero_counter(c, DC_MISC);
+  KT_START(c, KT_PREP);
   hipLaunchKernelGGL((k_pairs_prep<LAW>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->qflag, c->miscList,
                      c->dcnt + DC_MISC);
+  KT_STOP(c, KT_PREP);
   // the solver queue: scan of the flags (no shared counter), then the slots in interaction order
   scan_to_counter(c, c->qflag, c->qpos, A.np, DC_NQ);
   const unsigned long long* nq = c->dcnt + DC_NQ;
@@ -922,7 +952,7 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
       zero_counter(c, DC_NEXT);
       hipLaunchKernelGGL(lmKernel, dim3(resident_grid(c, lmKernel, c->residentLM, A.np)), dim3(256), 0, c->st, RA);
     }
-    hipLaunchKernelGGL((k_pairs_finish<0, double>), dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, nullptr, A.np, nq);
+    hipLaunchKernelGGL(k_pairs_finish<0>, dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, nullptr, A.np, nq);
   } else if (c->prm.precision == SDM_MIXED) {
     nm_pass<float>(c, A, SA);
   } else {
@@ -965,8 +995,10 @@ void step_body(sdm_ctx* c, const StepArgs& A) {
   NA.maxDisp = c->dMaxDisp;
   NA.quiet = c->servoOn ? &c->dServo->quiet : nullptr;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, (uint32_t*)c->dMaxDisp, 1);
+  KT_START(c, KT_FORCE_SUM);
   hipLaunchKernelGGL(k_force_sum, dim3((unsigned)((c->B.n + 15) / 16)), dim3(256), 0, c->st, c->B, c->P, c->csrOff,
                      c->csrPair);
+  KT_STOP(c, KT_FORCE_SUM);
   if (c->servoOn) {
     // CompressionEngine between InteractionLoop and NewtonIntegrator; every decision on the device
     ServoDev* S = c->dServo;
@@ -981,7 +1013,9 @@ void step_body(sdm_ctx* c, const StepArgs& A) {
                        c->prm.gravity[2], c->servoPart);
     hipLaunchKernelGGL(k_servo_check_final, dim3(1), dim3(64), 0, c->st, S, c->B, c->servoPart, nb, iterDev);
   }
+  KT_START(c, KT_NEWTON);
   hipLaunchKernelGGL(k_newton, dim3(nblk(c->B.n)), dim3(256), 0, c->st, NA);
+  KT_STOP(c, KT_NEWTON);
   if (c->servoOn) hipLaunchKernelGGL(k_servo_end, dim3(1), dim3(64), 0, c->st, c->dServo);
 }
 
@@ -1060,9 +1094,20 @@ void one_step(sdm_ctx* c) {
   HIPCHK(hipMemcpyAsync(&code, c->dMaxDisp, sizeof(int), hipMemcpyDeviceToHost, c->st));
   if (c->servoOn)
     HIPCHK(hipMemcpyAsync(&servoDone, &c->dServo->completedNow, sizeof(int), hipMemcpyDeviceToHost, c->st));
-  if (c->timing) HIPCHK(hipMemcpyAsync(&nqueued, c->dcnt + DC_NQ, sizeof(nqueued), hipMemcpyDeviceToHost, c->st));
+  unsigned long long nlisted[2] = {0, 0};  // touching list, evaluations done in double
+  if (c->timing) {
+    HIPCHK(hipMemcpyAsync(&nqueued, c->dcnt + DC_NQ, sizeof(nqueued), hipMemcpyDeviceToHost, c->st));
+    if (c->solverRan && c->prm.law == SDM_LAW_HERTZ && c->prm.precision == SDM_MIXED) {
+      HIPCHK(hipMemcpyAsync(&nlisted[0], c->dcnt + DC_NT, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(&nlisted[1], c->dcnt + DC_NDBL, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+    }
+  }
   HIPCHK(hipStreamSynchronize(c->st));
-  if (c->timing) c->tms[7] += (double)nqueued;
+  if (c->timing) {
+    c->tms[7] += (double)nqueued;
+    c->kms[KT_N] += (double)nlisted[0];
+    c->kms[KT_N + 1] += (double)nlisted[1];
+  }
   if (c->timing) {
     float a = 0, b = 0, d = 0;
     HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
@@ -1076,6 +1121,14 @@ void one_step(sdm_ctx* c) {
       float e = 0;
       HIPCHK(hipEventElapsedTime(&e, c->ev[4], c->ev[5]));
       c->tms[4] += e;
+    }
+    for (int k = 0; k < KT_N; k++) {
+      if (!c->kran[k]) continue;
+      float e = 0;
+      HIPCHK(hipEventElapsedTime(&e, c->kev[2 * k], c->kev[2 * k + 1]));
+      c->kms[k] += e;
+      c->kcount[k] += 1;
+      c->kran[k] = false;
     }
   }
   c->maxVelocitySq = code == 2 ? 2.0 : (code == 1 ? 0.5 : 0.0);
@@ -1262,6 +1315,8 @@ void sdm_destroy(sdm_ctx* c) {
   dfree(c->dmats); dfree(c->dcnt); dfree(c->dMaxDisp); dfree(c->dServo); dfree(c->servoPart);
   for (int k = 0; k < 6; k++)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+  for (int k = 0; k < 2 * KT_N; k++)
+    if (c->kev[k]) (void)hipEventDestroy(c->kev[k]);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -1539,6 +1594,8 @@ int sdm_set_timing(sdm_ctx* c, int32_t on) {
   try {
     for (int k = 0; k < 6; k++)
       if (!c->ev[k]) HIPCHK(hipEventCreate(&c->ev[k]));
+    for (int k = 0; k < 2 * KT_N; k++)
+      if (!c->kev[k]) HIPCHK(hipEventCreate(&c->kev[k]));
     HIPCHK(hipMemsetAsync(c->dcnt + 4, 0, 2 * sizeof(unsigned long long), c->st));
     HIPCHK(hipStreamSynchronize(c->st));
   } catch (HipError& e) {
@@ -1546,6 +1603,12 @@ int sdm_set_timing(sdm_ctx* c, int32_t on) {
   }
   c->timing = on != 0;
   for (int k = 0; k < 8; k++) c->tms[k] = 0;
+  for (int k = 0; k < KT_N; k++) {
+    c->kms[k] = 0;
+    c->kcount[k] = 0;
+    c->kran[k] = false;
+  }
+  c->kms[KT_N] = c->kms[KT_N + 1] = 0;
   return 0;
 }
 
@@ -1787,6 +1850,17 @@ int sdm_get_timers(sdm_ctx* c, double out[8]) {
   out[5] = (double)ev[0];
   out[6] = (double)ev[1];
   out[7] = c->tms[7];
+  return 0;
+}
+
+int sdm_get_kernel_timers(sdm_ctx* c, double out[10]) {
+  if (!c || !out) return -1;
+  for (int k = 0; k < KT_N; k++) {
+    out[2 * k] = c->kms[k];
+    out[2 * k + 1] = c->kcount[k];
+  }
+  out[8] = c->kms[KT_N];
+  out[9] = c->kms[KT_N + 1];
   return 0;
 }
 

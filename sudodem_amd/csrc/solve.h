@@ -10,13 +10,18 @@
 //   neldermead_simplex + amotry (pkg/dem/Superquadrics_Ig2.cpp:1681-1875)
 //   lmder2 + JacfunctionApp     (Superquadrics_Ig2.cpp:1032-1354)
 //
-// Mixed precision (Nelder-Mead only): the distance is evaluated in float on a frame centred
-// at body 1, while the simplex vertices, sums and moves stay in double.  When every
-// comparison of float distances agrees with the double one, the vertices are bitwise those
-// of the double solver, and the final double evaluations (k_nm_refine) reproduce its depth
-// and points exactly.
+// Mixed precision (Nelder-Mead only): the first vertex of a pair is evaluated in double and
+// every other vertex as a float difference from it (delta.h), whose rounding bound scales
+// with the support displacement and not with the millimetre-size support point.  The
+// touching test of an evaluation is taken in float only when |c.(p2 - p1)| exceeds that
+// bound; otherwise, and where the difference form leaves its range (a direction component
+// crossing a body axis plane), the point is evaluated in double and becomes the new anchor.
+// The simplex vertices, sums and moves stay in double; comparisons and the convergence test
+// use the float values, so a comparison closer than the rounding bound can take the other
+// branch than the double solver.  The final double evaluations (k_nm_refine) give the depth
+// at the best vertex and the points of the last evaluation.
 #pragma once
-#include "narrow.h"
+#include "delta.h"
 #include "state.h"
 
 struct SolveIO {  // indexed by work slot
@@ -38,8 +43,15 @@ struct SolveIO {  // indexed by work slot
   int32_t* stop;   // LM stop code / NM nfunc
   int32_t* touch;  // 1 = touching
   double* sh;      // LM: [3][cap] periodic shift of body 2 (shift2); nullptr outside a periodic cell
+  float* anc;      // mixed NM: [kAnc][cap] the double anchor of the first vertex, as floats
+  int32_t* fb;     // mixed NM: 1 = no anchor at the first vertex
   int64_t cap;
 };
+
+// Rows of SolveIO::anc: per body (offset 0 and 8) the local direction (3), tan phi0, tan
+// phi1, the local support point (3); then cos / sin of the anchor angles (16-19), the gap
+// vector p2 - p1 as high (20-22) and low (23-25) float parts, the contact direction (26-28).
+constexpr int kAnc = 29;
 
 struct SolveArgs {
   const double* geo;  // packed body records (state.h): pos, rot_mat2global, semi-axes, exponents, r_max
@@ -50,6 +62,7 @@ struct SolveArgs {
   const unsigned long long* count;  // device-side number of work items (nullptr: nwork)
   unsigned long long* next;   // work counter (64-entry batches), zero at launch
   unsigned long long* evals;  // distance evaluations done by this launch (added)
+  unsigned long long* dbl = nullptr;  // mixed NM: evaluations done in double (added)
 };
 
 // Work items of a launch: the device-side count (written by an earlier kernel of the same
@@ -213,21 +226,111 @@ __device__ __forceinline__ int64_t take_work(bool want, WaveBatch& b, unsigned l
 // ---------------------------------------------------------------------------------------
 enum { NM_V = 0, NM_REFL = 1, NM_EXP = 2, NM_CON = 3, NM_SHR = 4 };
 
+// Rounding bound of a difference evaluation (delta_eval): |f - f_exact| <= kDeltaErr 2^-24
+// (ds + 4 dk) with ds the support displacements and dk the conditioning term; measured on
+// MI355X against double evaluations over 10^6 random pairs and steps up to 0.03 rad, the
+// largest |f - f_exact| / (2^-24 (ds + 4 dk)) was 1.29.  The touching operand cd adds
+// 4 2^-24 f.
+constexpr float kDeltaErr = 4.0f;
+
+__device__ __forceinline__ void store_anchor(const SolveIO& S, int64_t w, const DeltaPair& P) {
+  const int64_t c = S.cap;
+  float* a = S.anc;
+  const DeltaBody* b[2] = {&P.b1, &P.b2};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int o = 8 * k;
+    a[(o + 0) * c + w] = b[k]->n[0]; a[(o + 1) * c + w] = b[k]->n[1]; a[(o + 2) * c + w] = b[k]->n[2];
+    a[(o + 3) * c + w] = b[k]->t; a[(o + 4) * c + w] = b[k]->tp;
+    a[(o + 5) * c + w] = b[k]->q[0]; a[(o + 6) * c + w] = b[k]->q[1]; a[(o + 7) * c + w] = b[k]->q[2];
+  }
+  a[16 * c + w] = P.ca0; a[17 * c + w] = P.sa0; a[18 * c + w] = P.ca1; a[19 * c + w] = P.sa1;
+  a[20 * c + w] = P.Fh[0]; a[21 * c + w] = P.Fh[1]; a[22 * c + w] = P.Fh[2];
+  a[23 * c + w] = P.Fl[0]; a[24 * c + w] = P.Fl[1]; a[25 * c + w] = P.Fl[2];
+  a[26 * c + w] = P.cu[0]; a[27 * c + w] = P.cu[1]; a[28 * c + w] = P.cu[2];
+}
+
+// The anchor of a queued pair from its stored rows and the two body records.
+__device__ __forceinline__ void load_anchor(const SolveIO& S, int64_t w, const GeoRec& r1, const GeoRec& r2,
+                                            DeltaPair& P) {
+  const int64_t c = S.cap;
+  const float* a = S.anc;
+  DeltaBody* b[2] = {&P.b1, &P.b2};
+  const GeoRec* r[2] = {&r1, &r2};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int o = 8 * k;
+    b[k]->n[0] = a[(o + 0) * c + w]; b[k]->n[1] = a[(o + 1) * c + w]; b[k]->n[2] = a[(o + 2) * c + w];
+    b[k]->t = a[(o + 3) * c + w]; b[k]->tp = a[(o + 4) * c + w];
+    b[k]->q[0] = a[(o + 5) * c + w]; b[k]->q[1] = a[(o + 6) * c + w]; b[k]->q[2] = a[(o + 7) * c + w];
+    const Shape<double>& s = r[k]->s;
+    b[k]->e1 = (float)(1.0 / (2.0 - s.eps1)); b[k]->e2 = (float)(1.0 / (2.0 - s.eps2));
+    b[k]->eps1 = (float)s.eps1; b[k]->eps2 = (float)s.eps2; b[k]->t1 = (float)(2.0 - s.eps1);
+    b[k]->G = cast9<float>(r[k]->G);
+  }
+  P.ca0 = a[16 * c + w]; P.sa0 = a[17 * c + w]; P.ca1 = a[18 * c + w]; P.sa1 = a[19 * c + w];
+  P.Fh[0] = a[20 * c + w]; P.Fh[1] = a[21 * c + w]; P.Fh[2] = a[22 * c + w];
+  P.Fl[0] = a[23 * c + w]; P.Fl[1] = a[24 * c + w]; P.Fl[2] = a[25 * c + w];
+  P.cu[0] = a[26 * c + w]; P.cu[1] = a[27 * c + w]; P.cu[2] = a[28 * c + w];
+  P.gC = cast9<float>(crotation_mat(r1.pos, r2.pos));
+}
+
+// Mixed form, the evaluations the difference form does not take: a new double anchor at
+// (b0, b1), whose value and touching test are double; where the relative form does not apply
+// there either (a direction component exactly zero), the double distance without an anchor.
+// `anchored` tells which; P is the new anchor when it is set.
+__device__ __forceinline__ double nm_mixed_point(const SolveArgs& A, int64_t w, double b0, double b1, DeltaPair& P,
+                                           bool& anchored, bool& touching) {
+  const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
+  PairGeo<double> g;
+  pair_geo<double>(r1, r2, true, g);
+  const M3<double> gC = crotation_mat(r1.pos, r2.pos);
+  V3<double> p1, p2;
+  double cd = 0.0;
+  bool ok = true, t = touching;
+  const double f = delta_anchor(g, gC, b0, b1, P, p1, p2, t, cd, ok);
+  anchored = ok && isfinite(f);
+  if (anchored) {
+    touching = t;
+    return f;
+  }
+  NMGeo<double> n;
+  n.g = g;
+  n.gC = gC;
+  int nev = 0;
+  return nm_eval<double, true>(n, b0, b1, p1, p2, touching, nev);
+}
+
+// T = double: the reference arithmetic (correctly rounded functions), every evaluation in
+// double.  T = float: the mixed form -- each evaluation is a float difference from the pair's
+// double anchor (delta.h), taken only when it is in the difference form's range and its
+// touching test is certified (|c.(p2 - p1)| above the evaluation's rounding bound); any other
+// evaluation is done in double at that point, which becomes the new anchor (nm_mixed_point).
+// The simplex, its sums and moves stay in double; the vertex comparisons and the convergence
+// test are taken on the float values.
+//
+// Two waves per SIMD: the register file then holds two of these state machines, whose
+// evaluations hide each other's latency; the few values that no longer fit are spilled
+// (measured: the FP64 form's step 17 % faster than at the one wave per SIMD that the
+// unconstrained allocation gives).
 template <typename T>
-__global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
-  constexpr bool kRel = sizeof(T) == 4;
-  const T ftol = T(1e-3);
-  const T TINY = T(1.0e-10);
+__global__ __launch_bounds__(256, 2) void k_solve_nm(SolveArgs A) {
+  constexpr bool kMixed = sizeof(T) == 4;
+  const double ftol = 1e-3;
+  const double TINY = 1.0e-10;
   const int NMAX = 5000;
-  NMGeo<T> n;
-  double v[3][2], psum[2], pt[2];
-  T y[3], ysave = T(0);
+  const double U = kDeltaErr * 5.9604644775390625e-08;  // kDeltaErr 2^-24
+  NMGeo<double> n;
+  DeltaPair P;
+  bool anchored = false;
+  double v[3][2], psum[2], pt[2], anc0 = 0, anc1 = 0;
+  double y[3], ysave = 0.0;
   int ihi = 0, ilo = 0, inhi = 0, state = NM_V, vi = 0, nfunc = 0, nev = 0;
   bool touching = true;
   int64_t w = -1;
-  V3<T> q1 = mk3(T(0), T(0), T(0)), q2 = q1;
+  V3<double> q1 = mk3(0.0, 0.0, 0.0), q2 = q1;
   bool done = false;
-  unsigned long long tot = 0;
+  unsigned long long tot = 0, dbl = 0;
   WaveBatch batch;
   const int64_t nwork = work_n(A);
   for (;;) {
@@ -239,30 +342,56 @@ __global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
       } else {
         w = A.list ? A.list[got] : got;
         const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
-        pair_geo<T>(r1, r2, kRel, n.g);
-        n.gC = cast9<T>(crotation_mat(r1.pos, r2.pos));
-        nm_prepare(n);
+        if constexpr (kMixed) {
+          anchored = A.S.fb[w] == 0;
+          if (anchored) load_anchor(A.S, w, r1, r2, P);
+        } else {
+          pair_geo<double>(r1, r2, false, n.g);
+          n.gC = crotation_mat(r1.pos, r2.pos);
+        }
         const double a0 = A.S.a0[w], a1 = A.S.a1[w], del = A.S.del[w];
+        anc0 = a0;
+        anc1 = a1;
         v[0][0] = a0; v[0][1] = a1;
         v[1][0] = a0 + del; v[1][1] = a1;
         v[2][0] = a0; v[2][1] = a1 + del;
-        // the first vertex was evaluated by k_nm_first (touching there): continue at the second
-        y[0] = T(A.S.f[w]);
-        pt[0] = v[1][0]; pt[1] = v[1][1];
         state = NM_V;
-        vi = 1;
-        nfunc = 1;
         nev = 0;
         touching = true;
+        // the first vertex was evaluated by k_nm_first (touching there): continue at the second
+        y[0] = A.S.f[w];
+        pt[0] = v[1][0]; pt[1] = v[1][1];
+        vi = 1;
+        nfunc = 1;
       }
     }
     if (__ballot(!done) == 0) break;
-    if (done) continue;
+    if (done || w < 0) continue;
     // ---- the one distance evaluation of this iteration ----
-    const T f = nm_eval(n, T(pt[0]), T(pt[1]), q1, q2, touching, nev);
+    double f;
+    if constexpr (kMixed) {
+      nev++;
+      bool ok = anchored, t = touching;
+      if (anchored) {
+        float cd = 0.0f, ds = 0.0f, dk = 0.0f;
+        f = delta_eval(P, (float)(pt[0] - anc0), (float)(pt[1] - anc1), t, cd, ds, dk, ok);
+        const double ef = U * ((double)ds + 4.0 * (double)dk) + 4e-16 * f;
+        ok = ok && fabs((double)cd) > ef + 4.0 * 5.9604644775390625e-08 * f;
+      }
+      if (ok) {
+        touching = t;
+      } else {
+        dbl++;
+        f = nm_mixed_point(A, w, pt[0], pt[1], P, anchored, touching);
+        anc0 = pt[0];
+        anc1 = pt[1];
+      }
+    } else {
+      f = nm_eval<double>(n, pt[0], pt[1], q1, q2, touching, nev);
+    }
     nfunc++;
     bool finished = false;
-    T fmin = T(0);
+    double fmin = 0.0;
     bool decide = false;
     if (state == NM_V) {
       y[vi] = f;
@@ -302,7 +431,7 @@ __global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
       }
     } else {
       // amotry: accept the trial point if it beats the highest vertex
-      const T ytry = f;
+      const double ytry = f;
       if (ytry < y[ihi]) {
         y[ihi] = ytry;
         psum[0] += pt[0] - v[ihi][0];
@@ -357,9 +486,10 @@ __global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
         } else if (y[i] > y[inhi] && i != ihi)
           inhi = i;
       }
-      const T rtol = T(2.0) * xfabs(y[ihi] - y[ilo]) / (xfabs(y[ihi]) + xfabs(y[ilo]) + TINY);
+      const double yh = y[ihi], yl = y[ilo];
+      const double rtol = 2.0 * fabs(yh - yl) / (fabs(yh) + fabs(yl) + TINY);
       if (rtol < ftol) {
-        const T t = y[0]; y[0] = y[ilo]; y[ilo] = t;
+        const double t = y[0]; y[0] = y[ilo]; y[ilo] = t;
         const double t0 = v[ilo][0], t1 = v[ilo][1];
         v[ilo][0] = v[0][0]; v[ilo][1] = v[0][1];
         v[0][0] = t0; v[0][1] = t1;
@@ -379,19 +509,24 @@ __global__ __launch_bounds__(256) void k_solve_nm(SolveArgs A) {
       const int64_t S = A.S.cap;
       A.S.a0[w] = v[0][0];
       A.S.a1[w] = v[0][1];
-      A.S.f[w] = double(fmin);
+      A.S.f[w] = fmin;
       A.S.la0[w] = pt[0];
       A.S.la1[w] = pt[1];
-      st3s(A.S.p1, w, S, cast3<double>(q1));
-      st3s(A.S.p2, w, S, cast3<double>(q2));
-      A.S.nev[w] += nev;
+      if (!kMixed) {  // the mixed form's points come from the double refinement
+        st3s(A.S.p1, w, S, q1);
+        st3s(A.S.p2, w, S, q2);
+      }
       A.S.stop[w] = nfunc;
       A.S.touch[w] = touching ? 1 : 0;
+      A.S.nev[w] += nev;
       tot += nev;
       w = -1;
     }
   }
   add_evals(A.evals, tot);
+  if constexpr (kMixed) {
+    if (A.dbl) add_evals(A.dbl, dbl);
+  }
 }
 
 // Expected evaluation count of a queued pair, used to start the long solves first.
@@ -403,31 +538,62 @@ __device__ __forceinline__ uint32_t nm_cost_key(int prevEval, bool prevContact, 
 // The first distance evaluation of every queued pair (the first simplex vertex,
 // neldermead_simplex :1731-1737), one pair per lane.  Pairs that are not touching there are
 // complete after it (the caller stores their outcome); the others are listed for the solver
-// with their first value in S.f.  Returns whether the pair is touching.
+// with their first value in S.f.  Returns whether the pair is listed.
+// T = double: the reference arithmetic.  T = float (mixed form): the double anchor of the
+// difference evaluations, with the device's double functions (S.fb = 1: the difference form
+// does not apply at the first vertex, whose distance is then evaluated in double without an
+// anchor).  Only touching pairs store their anchor.
 template <typename T>
 __device__ __forceinline__ bool nm_first_eval(const SolveArgs& A, int64_t w) {
-  constexpr bool kRel = sizeof(T) == 4;
-  NMGeo<T> n;
   const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
-  pair_geo<T>(r1, r2, kRel, n.g);
-  n.gC = cast9<T>(crotation_mat(r1.pos, r2.pos));
-  nm_prepare(n);
-  V3<T> q1, q2;
   bool touching = true;
-  int nev = 0;
-  const T f = nm_eval(n, T(A.S.a0[w]), T(A.S.a1[w]), q1, q2, touching, nev);
-  A.S.f[w] = double(f);
-  A.S.nev[w] = 1;
-  A.S.stop[w] = 1;
-  A.S.touch[w] = touching ? 1 : 0;
-  return touching;
+  if constexpr (sizeof(T) == 4) {
+    PairGeo<double> g;
+    pair_geo<double>(r1, r2, true, g);
+    const M3<double> gC = crotation_mat(r1.pos, r2.pos);
+    const double a0 = A.S.a0[w], a1 = A.S.a1[w];
+    DeltaPair P;
+    V3<double> p1, p2;
+    double cd = 0.0;
+    bool ok = true;
+    double f = delta_anchor(g, gC, a0, a1, P, p1, p2, touching, cd, ok);
+    ok = ok && isfinite(f);
+    if (!ok) {
+      NMGeo<double> n;
+      n.g = g;
+      n.gC = gC;
+      int nev = 0;
+      touching = true;
+      f = nm_eval<double, true>(n, a0, a1, p1, p2, touching, nev);
+    } else if (touching) {
+      store_anchor(A.S, w, P);
+    }
+    A.S.f[w] = f;
+    A.S.fb[w] = ok ? 0 : 1;
+    A.S.nev[w] = 1;
+    A.S.stop[w] = 1;
+    A.S.touch[w] = touching ? 1 : 0;
+    return touching;
+  } else {
+    NMGeo<double> n;
+    pair_geo<double>(r1, r2, false, n.g);
+    n.gC = crotation_mat(r1.pos, r2.pos);
+    V3<double> q1, q2;
+    int nev = 0;
+    const double f = nm_eval<double>(n, A.S.a0[w], A.S.a1[w], q1, q2, touching, nev);
+    A.S.f[w] = f;
+    A.S.nev[w] = 1;
+    A.S.stop[w] = 1;
+    A.S.touch[w] = touching ? 1 : 0;
+    return touching;
+  }
 }
 
 // Mixed form, final refinement in double: the depth is the double distance at the best
 // vertex and the points (hence the normal) those of the last evaluated point, as the
-// reference takes them (Superquadrics_Ig2.cpp:2439-2454).  One work item per touching pair.
+// reference takes them (Superquadrics_Ig2.cpp:2439-2454).  One work item per listed pair.
 // The evaluation uses the device's double functions (double accuracy, not the FP64 form's
-// correctly rounded last bits: the iterations that chose the vertex ran in float).
+// correctly rounded last bits).
 __global__ __launch_bounds__(256) void k_nm_refine(SolveArgs A) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= work_n(A)) return;
