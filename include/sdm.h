@@ -141,10 +141,10 @@ int sdm_get_timers(sdm_ctx* ctx, double out[8]);
  * sdm_set_timing(ctx, 1), for k = 0 the integrator (NewtonIntegrator), 1 the ordered force
  * sums (ForceContainer), 2 the bound update of a collider run (BoundDispatcher), 3 the
  * interaction loop's dispatch pass; out[8] the Nelder-Mead pairs touching at their first
- * vertex and out[9] the mixed form's solver evaluations done in double (summed over the
- * timed steps; 0 outside the mixed Hertz form).  The reference's TimingInfo has no per-kernel
+ * vertex, out[9] the mixed form's solver evaluations done in double and out[10] its pairs
+ * handed to the resume launch (summed over the timed steps; 0 outside the mixed Hertz form).  The reference's TimingInfo has no per-kernel
  * split. */
-int sdm_get_kernel_timers(sdm_ctx* ctx, double out[10]);
+int sdm_get_kernel_timers(sdm_ctx* ctx, double out[11]);
 
 /* Steady-state steps as a replayed graph.  No counterpart in the reference (its engines are
  * called one by one).  When on (the default), a step without a collider run, host write or

@@ -296,13 +296,15 @@ class Context:
     def kernel_timers(self):
         """Device time [ms] and launches of the integrator, the force sums, the bound update and
         the interaction-loop dispatch pass since set_timing(True); with the mixed Hertz form, the
-        Nelder-Mead pairs listed for the solver and the solver's evaluations done in double."""
-        out = (ctypes.c_double * 10)()
+        Nelder-Mead pairs listed for the solver, the solver's evaluations done in double and the
+        pairs handed to its resume launch."""
+        out = (ctypes.c_double * 11)()
         self.lib.check(self.lib.get_kernel_timers(self.ptr, out), self.ptr)
         r = {name: {"ms": out[2 * k], "launches": int(out[2 * k + 1])}
              for k, name in enumerate(("k_newton", "k_force_sum", "k_bounds", "k_pairs_prep"))}
         r["nm_listed"] = int(out[8])
         r["nm_double_evals"] = int(out[9])
+        r["nm_handed_over"] = int(out[10])
         return r
 
     def snapshot(self):

@@ -18,15 +18,18 @@
 //   q  = (sgn(c0) rx |c0|^eps1 |c1|^eps2, sgn(s0) ry |s0|^eps1 |c1|^eps2, sgn(s1) rz |s1|^eps2)
 // with c = cos, s = sin of phi0, phi1.  When no component of n changes sign, every factor
 // changes by a relative amount:
-//   u0 = e1 (ln(1 + dny/ny) - ln(1 + dnx/nx)),  dt = t (e^u0 - 1)
-//   d ln|c0| = -1/2 ln(1 + dt (2t + dt) / (1 + t^2)),  d ln|s0| = u0 + d ln|c0|
-//   u1 = e2 (ln(1 + dnz/nz) + (2 - eps1) d ln|c0| - ln(1 + dnx/nx)),  dt' = t' (e^u1 - 1)
-//   d ln c1 = -1/2 ln(1 + dt' (2t' + dt') / (1 + t'^2)),  d ln|s1| = u1 + d ln c1
-//   dq = q (e^(d ln|q|) - 1),  d ln|qx| = eps1 d ln|c0| + eps2 d ln c1,  ...
-// evaluated with log1p / expm1 built on the hardware log2 / exp2.  The difference of the
-// contact direction itself comes from half-angle forms of the angle steps.  A factor that
-// would change sign or come within 1/4 of zero (a component of n crossing a body axis plane)
-// is not evaluated this way: the evaluation reports it, and the solver hands the pair over.
+// Every factor changes by a ratio: with lx = ln|1 + dnx/nx| (the same for y, z),
+//   u0 = e1 (ly - lx) = d ln|t|,  t'^2 / t^2 = e^(2 u0)
+//   d ln|c0| = -1/2 ln(1 + sin^2 phi0 (e^(2 u0) - 1)),  d ln|s0| = u0 + d ln|c0|
+//   u1 = e2 (lz + (2 - eps1) d ln|c0| - lx),  d ln c1 = -1/2 ln(1 + sin^2 phi1 (e^(2 u1) - 1))
+//   d ln|s1| = u1 + d ln c1,  d ln|qx| = eps1 d ln|c0| + eps2 d ln c1,  ...
+//   dq = q (e^(d ln|q|) - 1), or -q (e^(d ln|q|) + 1) where the component's sign changes
+// (sgn qx = sgn nx, sgn qy = sgn ny, sgn qz = sgn nz), evaluated with log1p / expm1 built on
+// the hardware log2 / exp2: the rounding of each term scales with the change.  The
+// difference of the contact direction itself comes from half-angle forms of the angle steps.
+// A component of n that comes within kNear of its anchor value of zero (|1 + dn/n| < kNear),
+// or a factor outside the exponentials' range, is not evaluated this way: the evaluation
+// reports it, and the solver evaluates that point in double.
 #pragma once
 #include "narrow.h"
 
@@ -42,15 +45,25 @@ SDM_HD float dexpm1(float x) {
   const float l = __builtin_amdgcn_logf(u) * 0.693147180559945309f;
   return (u == 1.0f || l == 0.0f) ? x : (u - 1.0f) * (x / l);
 }
+// log|1 + x|: log1p above -1; below, the log of -(1 + x) (a sign change, not near -1 here)
+SDM_HD float dlogabs1p(float x) {
+  return x > -1.0f ? dlog1p(x) : __builtin_amdgcn_logf(-1.0f - x) * 0.693147180559945309f;
+}
 
 // The anchor of one body: the support point and its factors at the anchor direction.
 struct DeltaBody {
   float n[3];    // local direction at the anchor
-  float t, tp;   // tan phi0, tan phi1
+  float sq0, sq1;  // sin^2 phi0, sin^2 phi1
   float q[3];    // local support point
   float e1, e2, eps1, eps2, t1;  // exponents: 1/(2 - eps1), 1/(2 - eps2), eps1, eps2, 2 - eps1
   M3<float> G;   // rot_mat2global
 };
+
+// a^b for a >= 0 as exp2(b log2 a) with the device's double log2 / exp2: a relative error of
+// a few 1e-15 (|b log2 a| < 60 here) against the ~1e-16 of pow, far below the float rounding
+// of the differences taken from the anchor, and a fraction of pow's registers, which the
+// solver's inline re-anchoring has to leave to its other waves.
+SDM_HD double apow(double a, double b) { return exp2(b * log2(a)); }
 
 // Double support point along local direction n with the anchor factors; returns false when a
 // factor is degenerate (a zero direction component or a zero divisor), where the relative form
@@ -59,58 +72,65 @@ SDM_HD bool delta_anchor_local(const Shape<double>& s, V3<double> n, DeltaBody& 
   const double e1 = 1.0 / (2.0 - s.eps1), e2 = 1.0 / (2.0 - s.eps2);
   if (n.x == 0.0 || n.y == 0.0 || n.z == 0.0) return false;
   const double ax = fabs(s.rx * n.x);
-  const double a = sgn(n.y) * pow(fabs(s.ry * n.y), e1);
-  const double bb = sgn(n.x) * pow(ax, e1);
+  const double a = sgn(n.y) * apow(fabs(s.ry * n.y), e1);
+  const double bb = sgn(n.x) * apow(ax, e1);
   const double t = a / bb;
   const double h0 = sqrt(a * a + bb * bb);
   const double c0 = bb / h0, s0 = a / h0;
-  const double ap = sgn(n.z) * pow(fabs(s.rz * n.z * pow(fabs(c0), 2.0 - s.eps1)), e2);
-  const double bp = pow(ax, e2);
+  const double ap = sgn(n.z) * apow(fabs(s.rz * n.z * apow(fabs(c0), 2.0 - s.eps1)), e2);
+  const double bp = apow(ax, e2);
   const double tp = ap / bp;
   const double h1 = sqrt(ap * ap + bp * bp);
   const double c1 = bp / h1, s1 = ap / h1;
-  const double pc1 = pow(c1, s.eps2);
-  q = mk3(sgn(c0) * s.rx * pow(fabs(c0), s.eps1) * pc1, sgn(s0) * s.ry * pow(fabs(s0), s.eps1) * pc1,
-          sgn(s1) * s.rz * pow(fabs(s1), s.eps2));
+  const double pc1 = apow(c1, s.eps2);
+  q = mk3(sgn(c0) * s.rx * apow(fabs(c0), s.eps1) * pc1, sgn(s0) * s.ry * apow(fabs(s0), s.eps1) * pc1,
+          sgn(s1) * s.rz * apow(fabs(s1), s.eps2));
   b.n[0] = (float)n.x; b.n[1] = (float)n.y; b.n[2] = (float)n.z;
-  b.t = (float)t;
-  b.tp = (float)tp;
+  b.sq0 = (float)(s0 * s0);
+  b.sq1 = (float)(s1 * s1);
   b.q[0] = (float)q.x; b.q[1] = (float)q.y; b.q[2] = (float)q.z;
   b.e1 = (float)e1; b.e2 = (float)e2; b.eps1 = (float)s.eps1; b.eps2 = (float)s.eps2; b.t1 = (float)(2.0 - s.eps1);
   const bool ok = h0 > 0.0 && h1 > 0.0 && bb != 0.0 && bp != 0.0 && c0 != 0.0 && s0 != 0.0 && c1 != 0.0 && s1 != 0.0;
   return ok && fabs(t) < 1e30 && fabs(tp) < 1e30;
 }
 
+// Smallest ratio of a new direction component to its anchor value that the relative form
+// takes (|1 + dn/n|).
+constexpr float kNear = 0.015625f;
+
 // Change of the global support point of a body when its local direction changes by dn.
-// Sets `ok` false when a factor would change sign or come near zero.  `cond` receives a
-// bound on the conditioning of the relative form, (|qx| + |qy| + |qz|) (1/|nx| + 1/|ny| +
-// 1/|nz|): the rounding of a direction change (~ulp |dn| in each component) enters through
-// dn_k / n_k, and every logarithmic factor passes it on with a coefficient of at most one
-// (d ln c0 / d u0 = -sin^2 phi0, d ln s0 / d u0 = cos^2 phi0, the same for phi1).  A sharper
-// first-order propagation at the new angles underestimated the error of some evaluations
-// up to 150-fold on MI355X; this one bounded it.
+// Sets `ok` false when a component comes near zero or a factor leaves its range.  `cond`
+// receives a bound on the conditioning of the relative form, (|qx| + |qy| + |qz|) sum_k
+// 1/min(|n_k|, |n_k + dn_k|): the rounding of a direction change (~ulp |dn| in each
+// component) enters through dn_k / n_k and the ratio 1 + dn_k / n_k, and every logarithmic
+// factor passes it on with a coefficient of at most one (d ln c0 / d u0 = -sin^2 phi0,
+// d ln s0 / d u0 = cos^2 phi0, the same for phi1).  A sharper first-order propagation at the
+// new angles underestimated the error of some evaluations up to 150-fold on MI355X; this
+// one bounded it.
 SDM_HD V3<float> delta_support(const DeltaBody& b, const float dn[3], bool& ok, float& cond) {
   const float rx = dn[0] / b.n[0], ry = dn[1] / b.n[1], rz = dn[2] / b.n[2];
-  const float lx = dlog1p(rx), ly = dlog1p(ry), lz = dlog1p(rz);
+  const float lx = dlogabs1p(rx), ly = dlogabs1p(ry), lz = dlogabs1p(rz);
   const float u0 = b.e1 * (ly - lx);
-  const float dt = b.t * dexpm1(u0);
-  const float w0 = dt * (2.0f * b.t + dt) / (1.0f + b.t * b.t);
+  const float w0 = b.sq0 * dexpm1(2.0f * u0);
   const float lc0 = -0.5f * dlog1p(w0);
   const float ls0 = u0 + lc0;
   const float u1 = b.e2 * (lz + b.t1 * lc0 - lx);
-  const float dtp = b.tp * dexpm1(u1);
-  const float w1 = dtp * (2.0f * b.tp + dtp) / (1.0f + b.tp * b.tp);
+  const float w1 = b.sq1 * dexpm1(2.0f * u1);
   const float lc1 = -0.5f * dlog1p(w1);
   const float ls1 = u1 + lc1;
   const float gx = b.eps1 * lc0 + b.eps2 * lc1, gy = b.eps1 * ls0 + b.eps2 * lc1, gz = b.eps2 * ls1;
-  // no factor may change sign or come near zero (log1p arguments above -3/4), and the
-  // exponentials stay far from overflow
+  // every component at least kNear of its anchor value away from zero, cos^2 of the angles
+  // at least 1/4 of the anchor's, the exponentials far from overflow
   const float lo = -0.75f, hi = 16.0f;
-  ok = ok && rx >= lo && ry >= lo && rz >= lo && w0 >= lo && w1 >= lo && fabsf(u0) <= hi && fabsf(u1) <= hi &&
-       fabsf(gx) <= hi && fabsf(gy) <= hi && fabsf(gz) <= hi;
-  const V3<float> dq = mk3(b.q[0] * dexpm1(gx), b.q[1] * dexpm1(gy), b.q[2] * dexpm1(gz));
+  const float ax = fabsf(1.0f + rx), ay = fabsf(1.0f + ry), az = fabsf(1.0f + rz);
+  ok = ok && ax >= kNear && ay >= kNear && az >= kNear && w0 >= lo && w1 >= lo && fabsf(u0) <= hi &&
+       fabsf(u1) <= hi && fabsf(gx) <= hi && fabsf(gy) <= hi && fabsf(gz) <= hi;
+  const float ex = dexpm1(gx), ey = dexpm1(gy), ez = dexpm1(gz);
+  const V3<float> dq = mk3(b.q[0] * (rx < -1.0f ? -(ex + 2.0f) : ex), b.q[1] * (ry < -1.0f ? -(ey + 2.0f) : ey),
+                           b.q[2] * (rz < -1.0f ? -(ez + 2.0f) : ez));
   cond = (fabsf(b.q[0]) + fabsf(b.q[1]) + fabsf(b.q[2])) *
-         (1.0f / fabsf(b.n[0]) + 1.0f / fabsf(b.n[1]) + 1.0f / fabsf(b.n[2]));
+         (1.0f / (fabsf(b.n[0]) * fminf(ax, 1.0f)) + 1.0f / (fabsf(b.n[1]) * fminf(ay, 1.0f)) +
+          1.0f / (fabsf(b.n[2]) * fminf(az, 1.0f)));
   return mul(b.G, dq);
 }
 

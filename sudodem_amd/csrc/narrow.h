@@ -56,6 +56,26 @@ SDM_HD void msincos(float a, float& sn, float& cs) {
   sn = sinf(a);
   cs = cosf(a);
 }
+template <bool kFast>
+SDM_HD double msin(double a) {
+  if constexpr (kFast) return sin(a);
+  else return crm::cr_sin(a);
+}
+template <bool kFast>
+SDM_HD double mlog(double a) {
+  if constexpr (kFast) return log(a);
+  else return xlog(a);
+}
+template <bool kFast>
+SDM_HD double mlog10(double a) {
+  if constexpr (kFast) return log10(a);
+  else return xlog10(a);
+}
+template <bool kFast>
+SDM_HD double macos(double a) {
+  if constexpr (kFast) return acos(a);
+  else return xacos(a);
+}
 
 // Normal2Phi, active branches (Superquadrics.cpp:284, 288).
 template <typename T, bool kFast = false>
@@ -267,26 +287,27 @@ SDM_HD T nm_eval(const NMGeo<T>& n, T a0, T a1, V3<T>& p1, V3<T>& p2, bool& touc
 }
 
 // Curvatures (Superquadrics.cpp:131-255).  Keeps the reference's expressions: K ignores the
-// M term, and R11/R22 use the coefficient choices of :172-178.  T = double is the reference
-// arithmetic; the mixed form evaluates it in float (the force law's inputs).
-template <typename T>
-SDM_HD void sq_curvatures(const Shape<T>& s, const M3<T>& Rg, T ph0, T ph1, T& k1o, T& k2o, V3<T>& dir_max,
-                          V3<T>& dir_min, bool& max_flag) {
+// M term, and R11/R22 use the coefficient choices of :172-178.  kFast: the device's double
+// functions instead of the correctly rounded ones.
+template <bool kFast = false>
+SDM_HD void sq_curvatures(const Shape<double>& s, const M3<double>& Rg, double ph0, double ph1, double& k1o, double& k2o,
+                          V3<double>& dir_max, V3<double>& dir_min, bool& max_flag) {
+  using T = double;
   max_flag = true;
   if (xfabs(ph0) < T(1E-20)) ph0 = T(1E-20);
   if (xfabs(ph1) < T(1E-20)) ph1 = T(1E-20);
   const T rx = s.rx, ry = s.ry, rz = s.rz, eps1 = s.eps1, eps2 = s.eps2;
-  T s1 = xsin(ph0), s2 = xsin(ph1), c1 = xcos(ph0), c2 = xcos(ph1);
+  T s1 = msin<kFast>(ph0), s2 = msin<kFast>(ph1), c1 = mcos<kFast>(ph0), c2 = mcos<kFast>(ph1);
   T s_s1 = sgn(s1), s_s2 = sgn(s2), s_c1 = sgn(c1), s_c2 = sgn(c2);
   T abs_s1 = xfabs(s1), abs_s2 = xfabs(s2), abs_c1 = xfabs(c1), abs_c2 = xfabs(c2);
-  T s10 = s_s1 * xpow(abs_s1, eps1);
-  T s20 = s_s2 * xpow(abs_s2, eps2);
-  T c10 = s_c1 * xpow(abs_c1, eps1);
-  T c20 = s_c2 * xpow(abs_c2, eps2);
-  T s12 = s_s1 * xpow(abs_s1, eps1 - T(2.0));
-  T s22 = s_s2 * xpow(abs_s2, eps2 - T(2.0));
-  T c12 = s_c1 * xpow(abs_c1, eps1 - T(2.0));
-  T c22 = s_c2 * xpow(abs_c2, eps2 - T(2.0));
+  T s10 = s_s1 * mpow<kFast>(abs_s1, eps1);
+  T s20 = s_s2 * mpow<kFast>(abs_s2, eps2);
+  T c10 = s_c1 * mpow<kFast>(abs_c1, eps1);
+  T c20 = s_c2 * mpow<kFast>(abs_c2, eps2);
+  T s12 = s_s1 * mpow<kFast>(abs_s1, eps1 - T(2.0));
+  T s22 = s_s2 * mpow<kFast>(abs_s2, eps2 - T(2.0));
+  T c12 = s_c1 * mpow<kFast>(abs_c1, eps1 - T(2.0));
+  T c22 = s_c2 * mpow<kFast>(abs_c2, eps2 - T(2.0));
   V3<T> R1 = mk3(-rx * eps1 * c12 * c1 * c20 * s1, ry * eps1 * c1 * c20 * s12 * s1, T(0.0));
   V3<T> R2 = mk3(-rx * eps2 * c10 * c22 * c2 * s2, -ry * eps2 * c22 * c2 * s10 * s2, rz * eps2 * c2 * s22 * s2);
   V3<T> R11 = mk3(rx * eps1 * c12 * s20 * s1 * s1 * (eps1 - T(1.0)) - rx * eps1 * c10 * c20,
@@ -323,26 +344,28 @@ SDM_HD void sq_curvatures(const Shape<T>& s, const M3<T>& Rg, T ph0, T ph1, T& k
 }
 
 // SuperquadricsGeom2::HertzMindlin (Superquadrics.cpp:974-1026).
-template <typename T>
-SDM_HD void hertz_mindlin(T c10, T c11, T c20, T c21, T theta, T& curvSum, T& alpha, T& Ko, T& Eo) {
+template <bool kFast = false>
+SDM_HD void hertz_mindlin(double c10, double c11, double c20, double c21, double theta, double& curvSum, double& alpha,
+                          double& Ko, double& Eo) {
+  using T = double;
   T rho11 = xfabs(c10), rho12 = xfabs(c11), rho21 = xfabs(c20), rho22 = xfabs(c21);
   if (rho11 * rho12 < T(1E-18)) rho11 = rho12 = T(0.0);
   if (rho21 * rho22 < T(1E-18)) rho21 = rho22 = T(0.0);
   T sum_rho = rho11 + rho12 + rho21 + rho22;
   T a1 = rho11 - rho12, a2 = rho21 - rho22;
-  T F_rho = -xsqrt(a1 * a1 + a2 * a2 + T(2) * (xfabs(rho11 - rho12)) * xfabs(rho21 - rho22) * xcos(T(2) * theta));
+  T F_rho = -xsqrt(a1 * a1 + a2 * a2 + T(2) * (xfabs(rho11 - rho12)) * xfabs(rho21 - rho22) * mcos<kFast>(T(2) * theta));
   T A = (sum_rho + F_rho) / T(4.);
   T B = (sum_rho - F_rho) / T(4.);
   curvSum = A + B;
-  T lg = xlog10(B / A);
+  T lg = mlog10<kFast>(B / A);
   T x_squ = lg * lg;
   T gamma1 = T(2) * (T(1) + x_squ * (T(0.40227436) + x_squ * (T(3.7491752e-2) + x_squ * (T(7.4855761e-4) +
                                                                                        x_squ * T(2.1667028e-6)))));
   T gamma2 = T(3) * (T(1) + x_squ * (T(0.42678878) + x_squ * (T(4.2605401e-2) + x_squ * (T(9.0786922e-4) +
                                                                                        x_squ * T(2.7868927e-6)))));
-  alpha = xpow((B / A), gamma1 / gamma2);
+  alpha = mpow<kFast>((B / A), gamma1 / gamma2);
   T m1 = T(1) / (alpha * alpha);
   Ko = (T(1.3862944) + m1 * (T(0.1119723) + T(0.0725296) * m1)) -
-       xlog(m1) * (T(0.5) + m1 * (T(0.1213478) + T(0.0288729) * m1));
-  Eo = (T(1) + m1 * (T(0.4630151) + T(0.1077812) * m1)) - xlog(m1) * (T(0.2452727) + T(0.0412496) * m1) * m1;
+       mlog<kFast>(m1) * (T(0.5) + m1 * (T(0.1213478) + T(0.0288729) * m1));
+  Eo = (T(1) + m1 * (T(0.4630151) + T(0.1077812) * m1)) - mlog<kFast>(m1) * (T(0.2452727) + T(0.0412496) * m1) * m1;
 }

@@ -838,44 +838,47 @@ __global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int3
 }
 
 // Curvature directions and the Hertz terms of a Geom2 contact (Superquadrics_Ig2.cpp:2459-2503)
-// in precision T.  Both forms take them in double: in float the angle between the curvature
-// directions (an acos near +-1 of nearly parallel directions, and principal directions of
-// nearly equal curvatures) was off by up to 0.1 rad for ~0.5 % of contacts on MI355X, which
-// moved their Hertz stiffness by up to 50 %.
-template <typename T>
+// in double, with the correctly rounded functions (the FP64 form) or the device's (kFast, the
+// mixed form).  A float form of these was off by up to 0.1 rad in the angle between the
+// curvature directions (an acos near +-1 of nearly parallel directions, principal directions
+// of nearly equal curvatures) for ~0.5 % of contacts on MI355X, which moved their Hertz
+// stiffness by up to 50 %.
+template <bool kFast>
 __device__ __forceinline__ void contact_curvature(const BodyView& b1, const BodyView& b2, double a0, double a1,
-                                                  V3<double> cnd, double out[9]) {
-  const M3<T> gC = cast9<T>(crotation_mat(b1.pos, b2.pos));
-  const M3<T> G1 = cast9<T>(b1.G), G2 = cast9<T>(b2.G);
-  const Shape<T> s1 = cast_shape<T>(b1.s), s2 = cast_shape<T>(b2.s);
-  const V3<T> cn = cast3<T>(cnd);
-  const T ta0 = T(a0), ta1 = T(a1);
+                                                  V3<double> cn, double out[9]) {
+  using T = double;
+  const M3<T> gC = crotation_mat(b1.pos, b2.pos);
+  const M3<T>& G1 = b1.G;
+  const M3<T>& G2 = b2.G;
+  const Shape<T>& s1 = b1.s;
+  const Shape<T>& s2 = b2.s;
   T st0, ct0, st1, ct1;
-  msincos<false>(ta0, st0, ct0);
-  msincos<false>(ta1, st1, ct1);
+  msincos<kFast>(a0, st0, ct0);
+  msincos<kFast>(a1, st1, ct1);
   const V3<T> contact = mul(gC, mk3(ct0 * st1, st0 * st1, ct1));
   const V3<T> c1 = mulT(G1, contact), c2 = mulT(G2, contact);
   V3<T> dmax1 = mk3(T(0.0), T(0.0), T(0.0)), dmax2 = dmax1, dmin1, dmin2;
   bool f1, f2;
   T ph0, ph1, k10, k11, k20, k21;
-  normal2phi(s1, c1, ph0, ph1);
-  sq_curvatures(s1, G1, ph0, ph1, k10, k11, dmax1, dmin1, f1);
-  normal2phi(s2, mk3(T(-1.0) * c2.x, T(-1.0) * c2.y, T(-1.0) * c2.z), ph0, ph1);
-  sq_curvatures(s2, G2, ph0, ph1, k20, k21, dmax2, dmin2, f2);
+  normal2phi<T, kFast>(s1, c1, ph0, ph1);
+  sq_curvatures<kFast>(s1, G1, ph0, ph1, k10, k11, dmax1, dmin1, f1);
+  normal2phi<T, kFast>(s2, mk3(T(-1.0) * c2.x, T(-1.0) * c2.y, T(-1.0) * c2.z), ph0, ph1);
+  sq_curvatures<kFast>(s2, G2, ph0, ph1, k20, k21, dmax2, dmin2, f2);
   const V3<T> cd1 = (f1 && f2) ? dmax1 : dmin1;
   const V3<T> cd2 = (f1 && f2) ? dmax2 : dmin2;
   const V3<T> dir1 = mk3(cd1.x * cn.x, cd1.y * cn.y, cd1.z * cn.z) - cd1;
   const V3<T> dir2 = mk3(cd2.x * cn.x, cd2.y * cn.y, cd2.z * cn.z) - cd2;
-  T angle = xacos(dot(dir1, dir2) / (norm(dir1) * norm(dir2)));
+  T angle = macos<kFast>(dot(dir1, dir2) / (norm(dir1) * norm(dir2)));
   if (isnan(angle)) angle = T(0.0);
   T cs, al, K, E;
-  hertz_mindlin(k10, k11, k20, k21, angle, cs, al, K, E);
+  hertz_mindlin<kFast>(k10, k11, k20, k21, angle, cs, al, K, E);
   out[0] = k10; out[1] = k11; out[2] = k20; out[3] = k21; out[4] = angle;
   out[5] = cs; out[6] = al; out[7] = K; out[8] = E;
 }
 
 // Queued pairs after the solvers: geometry outputs, curvature / Hertz terms, precompute, law.
-template <int LAW>
+// kFast: the curvature with the device's double functions (the mixed form).
+template <int LAW, bool kFast = false>
 __global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, const uint32_t* list, int64_t nmax,
                                                       const unsigned long long* count) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -922,7 +925,7 @@ __global__ __launch_bounds__(256) void k_pairs_finish(StepArgs A, SolveIO S, con
       g.depth = S.f[w];
       // curvature direction convention of :2463-2465
       double cv[9];
-      contact_curvature<double>(b1, b2, g.a0, g.a1, cn, cv);
+      contact_curvature<kFast>(b1, b2, g.a0, g.a1, cn, cv);
       g.c10 = cv[0]; g.c11 = cv[1]; g.c20 = cv[2]; g.c21 = cv[3];
       g.curvAngle = cv[4];
       g.curvSum = cv[5]; g.alpha = cv[6]; g.K = cv[7]; g.E = cv[8];

@@ -132,7 +132,7 @@ struct sdm_ctx {
   uint32_t* qflag = nullptr;  // per interaction / per slot flags and their exclusive scan
   uint32_t* qpos = nullptr;
   int numCU = 0;
-  int residentNM[2] = {0, 0};  // resident blocks of k_solve_nm<float>, <double> (0: not queried)
+  int residentNM[3] = {0, 0, 0};  // resident blocks of k_solve_nm<float>, <double>, <float, true> (0: not queried)
   int residentLM = 0;
   bool matChecked = false;
   bool solverRan = false;
@@ -166,7 +166,7 @@ struct sdm_ctx {
   // kernel ran in the current step, accumulated milliseconds and launches
   hipEvent_t kev[2 * KT_N] = {};
   bool kran[KT_N] = {};
-  double kms[KT_N + 2] = {};  // + the mixed form's touching-list and double-evaluation counts
+  double kms[KT_N + 3] = {};  // + the mixed form's touching-list, double-evaluation and hand-over counts
   double kcount[KT_N] = {};
   sdm_counters cnt;
   std::string err;
@@ -409,8 +409,9 @@ void zero_counter(sdm_ctx* c, int k) { hipLaunchKernelGGL(k_zero_u64, dim3(1), d
 // [3] solver batch counter, [4] solver evaluations, [5] other evaluations, [6] wall/sphere
 // list, [7]-[9] periodic collider, [10] solver queue, [11] touching list, [12] LM restarts,
 // [13] the container size and [14] Scene::iter of the step, [15] the mixed form's solver
-// evaluations done in double.
-enum { DC_NEXT = 3, DC_MISC = 6, DC_NQ = 10, DC_NT = 11, DC_NR = 12, DC_NP = 13, DC_NDBL = 15 };
+// evaluations done in double, [16] its pairs handed to the resume launch.
+constexpr int kCounters = 24;
+enum { DC_NEXT = 3, DC_MISC = 6, DC_NQ = 10, DC_NT = 11, DC_NR = 12, DC_NP = 13, DC_NDBL = 15, DC_NHO = 16 };
 
 // Exclusive scan of in[0..n) into out[0..n) with the total left on the device in dcnt[slot].
 void scan_to_counter(sdm_ctx* c, const uint32_t* in, uint32_t* out, int64_t n, int slot) {
@@ -881,8 +882,11 @@ void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA) {
   S2.evals = c->dcnt + 4;
   S2.next = c->dcnt + DC_NEXT;
   S2.dbl = c->dcnt + DC_NDBL;
+  S2.hoList = c->retryList;
+  S2.hoCount = c->dcnt + DC_NHO;
   zero_counter(c, DC_NEXT);
   zero_counter(c, DC_NDBL);
+  zero_counter(c, DC_NHO);
   const int ti = sizeof(T) == 4 ? 0 : 1;
   const unsigned grid = resident_grid(c, k_solve_nm<T>, c->residentNM[ti], nmax);
   if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
@@ -890,12 +894,23 @@ void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA) {
   if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
   c->solverRan = true;
   if (sizeof(T) == 4) {
+    // the pairs the main launch handed over, with their double evaluations
+    SolveArgs R = S2;
+    R.list = c->retryList;
+    R.count = c->dcnt + DC_NHO;
+    R.hoList = nullptr;
+    R.hoCount = nullptr;
+    zero_counter(c, DC_NEXT);
+    hipLaunchKernelGGL((k_solve_nm<float, true>), dim3(resident_grid(c, k_solve_nm<float, true>, c->residentNM[2], nmax)),
+                       dim3(256), 0, c->st, R);
     SolveArgs RF = S2;
     RF.list = reinterpret_cast<const int32_t*>(byPair);
     RF.evals = c->dcnt + 5;
     hipLaunchKernelGGL(k_nm_refine, dim3(nblk(nmax)), dim3(256), 0, c->st, RF);
   }
-  hipLaunchKernelGGL(k_pairs_finish<1>, dim3(nblk(nmax)), dim3(256), 0, c->st, A, c->S, byPair, nmax, nt);
+  // the mixed form takes the curvature with the device's double functions
+  hipLaunchKernelGGL((k_pairs_finish<1, sizeof(T) == 4>), dim3(nblk(nmax)), dim3(256), 0, c->st, A, c->S, byPair, nmax,
+                     nt);
 }
 
 // InteractionLoop::action: prep (every interaction), wall / sphere pairs, solvers (queued
@@ -1069,6 +1084,7 @@ void one_step(sdm_ctx* c) {
     resident_grid(c, c->prm.precision == SDM_MIXED ? k_solve_lm<true> : k_solve_lm<false>, c->residentLM, 1);
     resident_grid(c, k_solve_nm<float>, c->residentNM[0], 1);
     resident_grid(c, k_solve_nm<double>, c->residentNM[1], 1);
+    resident_grid(c, k_solve_nm<float, true>, c->residentNM[2], 1);
     if (!c->graphExec || c->graphBuiltGen != c->graphGen) {
       if (c->graphExec) {
         HIPCHK(hipGraphExecDestroy(c->graphExec));
@@ -1094,12 +1110,13 @@ void one_step(sdm_ctx* c) {
   HIPCHK(hipMemcpyAsync(&code, c->dMaxDisp, sizeof(int), hipMemcpyDeviceToHost, c->st));
   if (c->servoOn)
     HIPCHK(hipMemcpyAsync(&servoDone, &c->dServo->completedNow, sizeof(int), hipMemcpyDeviceToHost, c->st));
-  unsigned long long nlisted[2] = {0, 0};  // touching list, evaluations done in double
+  unsigned long long nlisted[3] = {0, 0, 0};  // touching list, evaluations done in double, handed over
   if (c->timing) {
     HIPCHK(hipMemcpyAsync(&nqueued, c->dcnt + DC_NQ, sizeof(nqueued), hipMemcpyDeviceToHost, c->st));
     if (c->solverRan && c->prm.law == SDM_LAW_HERTZ && c->prm.precision == SDM_MIXED) {
       HIPCHK(hipMemcpyAsync(&nlisted[0], c->dcnt + DC_NT, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipMemcpyAsync(&nlisted[1], c->dcnt + DC_NDBL, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(&nlisted[2], c->dcnt + DC_NHO, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
     }
   }
   HIPCHK(hipStreamSynchronize(c->st));
@@ -1107,6 +1124,7 @@ void one_step(sdm_ctx* c) {
     c->tms[7] += (double)nqueued;
     c->kms[KT_N] += (double)nlisted[0];
     c->kms[KT_N + 1] += (double)nlisted[1];
+    c->kms[KT_N + 2] += (double)nlisted[2];
   }
   if (c->timing) {
     float a = 0, b = 0, d = 0;
@@ -1271,8 +1289,8 @@ sdm_ctx* sdm_create(const sdm_params* params, int32_t n_bodies, int32_t n_materi
     }
     c->mats.resize(n_materials);
     c->dmats = dalloc<MaterialDev>(std::max(1, n_materials));
-    c->dcnt = dalloc<unsigned long long>(16);
-    HIPCHK(hipMemset(c->dcnt, 0, 16 * sizeof(unsigned long long)));
+    c->dcnt = dalloc<unsigned long long>(kCounters);
+    HIPCHK(hipMemset(c->dcnt, 0, kCounters * sizeof(unsigned long long)));
     HIPCHK(hipDeviceGetAttribute(&c->numCU, hipDeviceAttributeMultiprocessorCount, c->dev));
     c->dMaxDisp = dalloc<int>(1);
     ensure_pair_cap(c, 1024);
@@ -1608,7 +1626,7 @@ int sdm_set_timing(sdm_ctx* c, int32_t on) {
     c->kcount[k] = 0;
     c->kran[k] = false;
   }
-  c->kms[KT_N] = c->kms[KT_N + 1] = 0;
+  c->kms[KT_N] = c->kms[KT_N + 1] = c->kms[KT_N + 2] = 0;
   return 0;
 }
 
@@ -1853,7 +1871,7 @@ int sdm_get_timers(sdm_ctx* c, double out[8]) {
   return 0;
 }
 
-int sdm_get_kernel_timers(sdm_ctx* c, double out[10]) {
+int sdm_get_kernel_timers(sdm_ctx* c, double out[11]) {
   if (!c || !out) return -1;
   for (int k = 0; k < KT_N; k++) {
     out[2 * k] = c->kms[k];
@@ -1861,6 +1879,7 @@ int sdm_get_kernel_timers(sdm_ctx* c, double out[10]) {
   }
   out[8] = c->kms[KT_N];
   out[9] = c->kms[KT_N + 1];
+  out[10] = c->kms[KT_N + 2];
   return 0;
 }
 

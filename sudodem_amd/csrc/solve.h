@@ -15,7 +15,8 @@
 // with the support displacement and not with the millimetre-size support point.  The
 // touching test of an evaluation is taken in float only when |c.(p2 - p1)| exceeds that
 // bound; otherwise, and where the difference form leaves its range (a direction component
-// crossing a body axis plane), the point is evaluated in double and becomes the new anchor.
+// coming within 1/64 of its anchor value of zero), the point is evaluated in double and
+// becomes the new anchor.
 // The simplex vertices, sums and moves stay in double; comparisons and the convergence test
 // use the float values, so a comparison closer than the rounding bound can take the other
 // branch than the double solver.  The final double evaluations (k_nm_refine) give the depth
@@ -44,11 +45,11 @@ struct SolveIO {  // indexed by work slot
   int32_t* touch;  // 1 = touching
   double* sh;      // LM: [3][cap] periodic shift of body 2 (shift2); nullptr outside a periodic cell
   float* anc;      // mixed NM: [kAnc][cap] the double anchor of the first vertex, as floats
-  int32_t* fb;     // mixed NM: 1 = no anchor at the first vertex
+  int32_t* fb;     // mixed NM: 1 = no anchor at the first vertex; handed-over stage and indices
   int64_t cap;
 };
 
-// Rows of SolveIO::anc: per body (offset 0 and 8) the local direction (3), tan phi0, tan
+// Rows of SolveIO::anc: per body (offset 0 and 8) the local direction (3), sin^2 phi0, sin^2
 // phi1, the local support point (3); then cos / sin of the anchor angles (16-19), the gap
 // vector p2 - p1 as high (20-22) and low (23-25) float parts, the contact direction (26-28).
 constexpr int kAnc = 29;
@@ -63,6 +64,8 @@ struct SolveArgs {
   unsigned long long* next;   // work counter (64-entry batches), zero at launch
   unsigned long long* evals;  // distance evaluations done by this launch (added)
   unsigned long long* dbl = nullptr;  // mixed NM: evaluations done in double (added)
+  int32_t* hoList = nullptr;          // mixed NM: slots handed to the resume launch
+  unsigned long long* hoCount = nullptr;  // their number (zero before the main launch)
 };
 
 // Work items of a launch: the device-side count (written by an earlier kernel of the same
@@ -232,6 +235,8 @@ enum { NM_V = 0, NM_REFL = 1, NM_EXP = 2, NM_CON = 3, NM_SHR = 4 };
 // largest |f - f_exact| / (2^-24 (ds + 4 dk)) was 1.29.  The touching operand cd adds
 // 4 2^-24 f.
 constexpr float kDeltaErr = 4.0f;
+// Lanes per wave that the mixed form's resume launch uses.
+constexpr int kResumeLanes = 4;
 
 __device__ __forceinline__ void store_anchor(const SolveIO& S, int64_t w, const DeltaPair& P) {
   const int64_t c = S.cap;
@@ -241,7 +246,7 @@ __device__ __forceinline__ void store_anchor(const SolveIO& S, int64_t w, const 
   for (int k = 0; k < 2; k++) {
     const int o = 8 * k;
     a[(o + 0) * c + w] = b[k]->n[0]; a[(o + 1) * c + w] = b[k]->n[1]; a[(o + 2) * c + w] = b[k]->n[2];
-    a[(o + 3) * c + w] = b[k]->t; a[(o + 4) * c + w] = b[k]->tp;
+    a[(o + 3) * c + w] = b[k]->sq0; a[(o + 4) * c + w] = b[k]->sq1;
     a[(o + 5) * c + w] = b[k]->q[0]; a[(o + 6) * c + w] = b[k]->q[1]; a[(o + 7) * c + w] = b[k]->q[2];
   }
   a[16 * c + w] = P.ca0; a[17 * c + w] = P.sa0; a[18 * c + w] = P.ca1; a[19 * c + w] = P.sa1;
@@ -261,7 +266,7 @@ __device__ __forceinline__ void load_anchor(const SolveIO& S, int64_t w, const G
   for (int k = 0; k < 2; k++) {
     const int o = 8 * k;
     b[k]->n[0] = a[(o + 0) * c + w]; b[k]->n[1] = a[(o + 1) * c + w]; b[k]->n[2] = a[(o + 2) * c + w];
-    b[k]->t = a[(o + 3) * c + w]; b[k]->tp = a[(o + 4) * c + w];
+    b[k]->sq0 = a[(o + 3) * c + w]; b[k]->sq1 = a[(o + 4) * c + w];
     b[k]->q[0] = a[(o + 5) * c + w]; b[k]->q[1] = a[(o + 6) * c + w]; b[k]->q[2] = a[(o + 7) * c + w];
     const Shape<double>& s = r[k]->s;
     b[k]->e1 = (float)(1.0 / (2.0 - s.eps1)); b[k]->e2 = (float)(1.0 / (2.0 - s.eps2));
@@ -275,28 +280,37 @@ __device__ __forceinline__ void load_anchor(const SolveIO& S, int64_t w, const G
   P.gC = cast9<float>(crotation_mat(r1.pos, r2.pos));
 }
 
-// Mixed form, the evaluations the difference form does not take: a new double anchor at
-// (b0, b1), whose value and touching test are double; where the relative form does not apply
-// there either (a direction component exactly zero), the double distance without an anchor.
-// `anchored` tells which; P is the new anchor when it is set.
-__device__ __forceinline__ double nm_mixed_point(const SolveArgs& A, int64_t w, double b0, double b1, DeltaPair& P,
-                                           bool& anchored, bool& touching) {
+// Mixed form, an evaluation the difference form does not take: a new double anchor at
+// (b0, b1), whose value and touching test are double.  `anchored` is false where the relative
+// form does not apply at that point either (a direction component exactly zero); P is then
+// not an anchor and the value not usable.
+__device__ __forceinline__ double nm_reanchor(const SolveArgs& A, int64_t w, double b0, double b1, DeltaPair& P,
+                                              bool& anchored, bool& touching) {
   const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
   PairGeo<double> g;
   pair_geo<double>(r1, r2, true, g);
-  const M3<double> gC = crotation_mat(r1.pos, r2.pos);
   V3<double> p1, p2;
   double cd = 0.0;
   bool ok = true, t = touching;
-  const double f = delta_anchor(g, gC, b0, b1, P, p1, p2, t, cd, ok);
+  DeltaPair Q;
+  const double f = delta_anchor(g, crotation_mat(r1.pos, r2.pos), b0, b1, Q, p1, p2, t, cd, ok);
+  // the old anchor is dead from here on (the new one or none): no register keeps it
+  P = Q;
   anchored = ok && isfinite(f);
-  if (anchored) {
-    touching = t;
-    return f;
-  }
+  if (anchored) touching = t;
+  return f;
+}
+
+// The same, and where no anchor applies, the double distance itself.
+__device__ __forceinline__ double nm_mixed_point(const SolveArgs& A, int64_t w, double b0, double b1, DeltaPair& P,
+                                                 bool& anchored, bool& touching) {
+  const double f = nm_reanchor(A, w, b0, b1, P, anchored, touching);
+  if (anchored) return f;
+  const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
   NMGeo<double> n;
-  n.g = g;
-  n.gC = gC;
+  pair_geo<double>(r1, r2, true, n.g);
+  n.gC = crotation_mat(r1.pos, r2.pos);
+  V3<double> p1, p2;
   int nev = 0;
   return nm_eval<double, true>(n, b0, b1, p1, p2, touching, nev);
 }
@@ -309,12 +323,20 @@ __device__ __forceinline__ double nm_mixed_point(const SolveArgs& A, int64_t w, 
 // The simplex, its sums and moves stay in double; the vertex comparisons and the convergence
 // test are taken on the float values.
 //
-// Two waves per SIMD: the register file then holds two of these state machines, whose
-// evaluations hide each other's latency; the few values that no longer fit are spilled
-// (measured: the FP64 form's step 17 % faster than at the one wave per SIMD that the
-// unconstrained allocation gives).
-template <typename T>
-__global__ __launch_bounds__(256, 2) void k_solve_nm(SolveArgs A) {
+// The mixed form runs in two launches.  The main one (kResume false) re-anchors inline; a
+// pair whose due point admits no anchor (a direction component exactly zero) is handed over
+// with its state (the vertices and their values, the sums, the point, the stage and vertex
+// indices, the counts), and the lane takes the next pair.  The resume launch (kResume true)
+// takes the handed-over pairs -- normally none -- and finishes each solve with the general
+// double evaluation inline; its register demand would cut the residency of every wave of
+// the main launch.
+//
+// Residency: two waves per SIMD for the main launches (measured on MI355X: the FP64 form's
+// step 17 % faster than at the one wave per SIMD that the unconstrained allocation gives; the
+// mixed main launch 19 % faster than at three, where it spills), one for the resume launch,
+// whose few pairs want the shortest iteration (no spills).
+template <typename T, bool kResume = false>
+__global__ __launch_bounds__(256, kResume ? 1 : 2) void k_solve_nm(SolveArgs A) {
   constexpr bool kMixed = sizeof(T) == 4;
   const double ftol = 1e-3;
   const double TINY = 1.0e-10;
@@ -322,7 +344,7 @@ __global__ __launch_bounds__(256, 2) void k_solve_nm(SolveArgs A) {
   const double U = kDeltaErr * 5.9604644775390625e-08;  // kDeltaErr 2^-24
   NMGeo<double> n;
   DeltaPair P;
-  bool anchored = false;
+  bool anchored = false, due = false;  // mixed form: the pair's anchor is set; a double evaluation is due
   double v[3][2], psum[2], pt[2], anc0 = 0, anc1 = 0;
   double y[3], ysave = 0.0;
   int ihi = 0, ilo = 0, inhi = 0, state = NM_V, vi = 0, nfunc = 0, nev = 0;
@@ -333,59 +355,110 @@ __global__ __launch_bounds__(256, 2) void k_solve_nm(SolveArgs A) {
   unsigned long long tot = 0, dbl = 0;
   WaveBatch batch;
   const int64_t nwork = work_n(A);
+  const int64_t S = A.S.cap;
   for (;;) {
     const bool idle = (w < 0) && !done;
-    const int64_t got = take_work(idle, batch, A.next, nwork);
+    int64_t got;
+    if constexpr (kResume) {
+      // few pairs, each a long serial chain: kResumeLanes lanes per wave, one pair each, so
+      // that a wave seldom goes through the double code for one of its lanes
+      got = fetch_work(idle && (threadIdx.x & 63) < kResumeLanes, A.next);
+      if (got >= nwork) got = -1;
+    } else {
+      got = take_work(idle, batch, A.next, nwork);
+    }
     if (idle) {
       if (got < 0) {
         done = true;
       } else {
         w = A.list ? A.list[got] : got;
-        const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
-        if constexpr (kMixed) {
-          anchored = A.S.fb[w] == 0;
-          if (anchored) load_anchor(A.S, w, r1, r2, P);
-        } else {
-          pair_geo<double>(r1, r2, false, n.g);
-          n.gC = crotation_mat(r1.pos, r2.pos);
-        }
-        const double a0 = A.S.a0[w], a1 = A.S.a1[w], del = A.S.del[w];
-        anc0 = a0;
-        anc1 = a1;
-        v[0][0] = a0; v[0][1] = a1;
-        v[1][0] = a0 + del; v[1][1] = a1;
-        v[2][0] = a0; v[2][1] = a1 + del;
-        state = NM_V;
-        nev = 0;
         touching = true;
-        // the first vertex was evaluated by k_nm_first (touching there): continue at the second
-        y[0] = A.S.f[w];
-        pt[0] = v[1][0]; pt[1] = v[1][1];
-        vi = 1;
-        nfunc = 1;
+        nev = 0;
+        if constexpr (kResume) {
+          // the state the main launch handed over, and the double evaluation it left due
+          const int32_t code = A.S.fb[w];
+          state = code & 7; vi = (code >> 3) & 3; ilo = (code >> 5) & 3; ihi = (code >> 7) & 3; inhi = (code >> 9) & 3;
+          v[0][0] = A.S.p1[w]; v[0][1] = A.S.p1[S + w]; v[1][0] = A.S.p1[2 * S + w];
+          v[1][1] = A.S.p2[w]; v[2][0] = A.S.p2[S + w]; v[2][1] = A.S.p2[2 * S + w];
+          psum[0] = A.S.ctr[w]; psum[1] = A.S.ctr[S + w];
+          pt[0] = A.S.la0[w]; pt[1] = A.S.la1[w];
+          y[0] = A.S.a0[w]; y[1] = A.S.a1[w]; y[2] = A.S.del[w]; ysave = A.S.f[w];
+          nfunc = A.S.stop[w];
+          anchored = false;
+          due = true;
+        } else {
+          const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
+          if constexpr (kMixed) {
+            anchored = A.S.fb[w] == 0;
+            if (anchored) load_anchor(A.S, w, r1, r2, P);
+          } else {
+            pair_geo<double>(r1, r2, false, n.g);
+            n.gC = crotation_mat(r1.pos, r2.pos);
+          }
+          const double a0 = A.S.a0[w], a1 = A.S.a1[w], del = A.S.del[w];
+          anc0 = a0;
+          anc1 = a1;
+          v[0][0] = a0; v[0][1] = a1;
+          v[1][0] = a0 + del; v[1][1] = a1;
+          v[2][0] = a0; v[2][1] = a1 + del;
+          state = NM_V;
+          // the first vertex was evaluated by k_nm_first (touching there): continue at the second
+          y[0] = A.S.f[w];
+          pt[0] = v[1][0]; pt[1] = v[1][1];
+          vi = 1;
+          nfunc = 1;
+        }
       }
     }
     if (__ballot(!done) == 0) break;
     if (done || w < 0) continue;
     // ---- the one distance evaluation of this iteration ----
-    double f;
+    double f = 0.0;
     if constexpr (kMixed) {
-      nev++;
-      bool ok = anchored, t = touching;
-      if (anchored) {
-        float cd = 0.0f, ds = 0.0f, dk = 0.0f;
-        f = delta_eval(P, (float)(pt[0] - anc0), (float)(pt[1] - anc1), t, cd, ds, dk, ok);
-        const double ef = U * ((double)ds + 4.0 * (double)dk) + 4e-16 * f;
-        ok = ok && fabs((double)cd) > ef + 4.0 * 5.9604644775390625e-08 * f;
+      if (!due) {
+        nev++;
+        bool ok = anchored, t = touching;
+        if (anchored) {
+          float cd = 0.0f, ds = 0.0f, dk = 0.0f;
+          f = delta_eval(P, (float)(pt[0] - anc0), (float)(pt[1] - anc1), t, cd, ds, dk, ok);
+          const double ef = U * ((double)ds + 4.0 * (double)dk) + 4e-16 * f;
+          ok = ok && fabs((double)cd) > ef + 4.0 * 5.9604644775390625e-08 * f;
+        }
+        if (ok)
+          touching = t;
+        else
+          due = true;
       }
-      if (ok) {
-        touching = t;
-      } else {
+      if (due) {
         dbl++;
-        f = nm_mixed_point(A, w, pt[0], pt[1], P, anchored, touching);
+        if constexpr (kResume) {
+          f = nm_mixed_point(A, w, pt[0], pt[1], P, anchored, touching);
+        } else {
+          f = nm_reanchor(A, w, pt[0], pt[1], P, anchored, touching);
+          due = !anchored;
+        }
         anc0 = pt[0];
         anc1 = pt[1];
       }
+      if constexpr (!kResume) {
+        if (due) {
+          // hand over: the state before this iteration's evaluation, which is due in double
+          A.S.fb[w] = state | (vi << 3) | (ilo << 5) | (ihi << 7) | (inhi << 9);
+          A.S.p1[w] = v[0][0]; A.S.p1[S + w] = v[0][1]; A.S.p1[2 * S + w] = v[1][0];
+          A.S.p2[w] = v[1][1]; A.S.p2[S + w] = v[2][0]; A.S.p2[2 * S + w] = v[2][1];
+          A.S.ctr[w] = psum[0]; A.S.ctr[S + w] = psum[1];
+          A.S.la0[w] = pt[0]; A.S.la1[w] = pt[1];
+          A.S.a0[w] = y[0]; A.S.a1[w] = y[1]; A.S.del[w] = y[2]; A.S.f[w] = ysave;
+          A.S.stop[w] = nfunc;
+          A.S.nev[w] += nev;
+          tot += nev;
+          A.hoList[atomicAdd(A.hoCount, 1ull)] = (int32_t)w;
+          due = false;
+          w = -1;
+          continue;
+        }
+      }
+      due = false;
     } else {
       f = nm_eval<double>(n, pt[0], pt[1], q1, q2, touching, nev);
     }
@@ -506,7 +579,6 @@ __global__ __launch_bounds__(256, 2) void k_solve_nm(SolveArgs A) {
       }
     }
     if (finished) {
-      const int64_t S = A.S.cap;
       A.S.a0[w] = v[0][0];
       A.S.a1[w] = v[0][1];
       A.S.f[w] = fmin;
