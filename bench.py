@@ -489,12 +489,15 @@ def main():
         "mixed_over_fp64": pssmx / pss64,
         "graph_steps": {"fp64": c64["graph_steps"], "mixed": cmx["graph_steps"]},
         "contact_evals_per_s": cmx["contact_evals"] / wmx,
+        "evaluated_pairs_per_s": tmx["queued"] / K / (wmx / K),
         "dist_evals_per_s": cmx["dist_evals"] / wmx,
         "interactions": n_pairs,
         "contacts": n_contacts,
         "solids_fraction": cfg.solids_fraction,
         "queued_per_step": {"fp64": t64["queued"] / K, "mixed": tmx["queued"] / K},
         "phases_ms_per_step": {k: tmx[k] / K for k in ("collider_ms", "interactions_ms", "newton_ms", "solver_ms")},
+        "kernels_ms_per_step": {name: {k: v["ms"] / K for k, v in tm["kernels"].items() if isinstance(v, dict)}
+                                for name, tm in (("fp64", t64), ("mixed", tmx))},
         "mixed_error_vs_fp64": mixed_error(one_step["fp64"], one_step["mixed"]),
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -136,15 +136,19 @@ int sdm_reset_counters(sdm_ctx* ctx);
  *   out[7] interactions handed to the solver kernel (summed over the timed steps) */
 int sdm_set_timing(sdm_ctx* ctx, int32_t on);
 int sdm_get_timers(sdm_ctx* ctx, double out[8]);
-/* Per-kernel device times of the bandwidth-bound kernels under the same timing switch (events
- * around each launch): out[2k] milliseconds and out[2k+1] launches summed since
- * sdm_set_timing(ctx, 1), for k = 0 the integrator (NewtonIntegrator), 1 the ordered force
- * sums (ForceContainer), 2 the bound update of a collider run (BoundDispatcher), 3 the
- * interaction loop's dispatch pass; out[8] the Nelder-Mead pairs touching at their first
- * vertex, out[9] the mixed form's solver evaluations done in double and out[10] its pairs
- * handed to the resume launch (summed over the timed steps; 0 outside the mixed Hertz form).  The reference's TimingInfo has no per-kernel
- * split. */
-int sdm_get_kernel_timers(sdm_ctx* ctx, double out[11]);
+/* Per-kernel device times under the same timing switch (events around each launch or group
+ * of launches): out[2k] milliseconds and out[2k+1] launches summed since sdm_set_timing(ctx, 1),
+ * for k = 0 the integrator (NewtonIntegrator), 1 the ordered force sums (ForceContainer), 2 the
+ * bound update of a collider run (BoundDispatcher), 3 the interaction loop's dispatch pass,
+ * 4 the solver queue (flag scan and fill), 5 the wall and sphere pairs, 6 the Nelder-Mead first
+ * vertex, 7 the Nelder-Mead solver list (scan, fill, cost sort), 8 the main solver launch,
+ * 9 the mixed form's resume launch or the Levenberg-Marquardt restarts, 10 the mixed form's
+ * double refinement, 11 the finish (curvature, stiffness, force law, stores); out[24] the
+ * Nelder-Mead pairs touching at their first vertex, out[25] the mixed form's solver
+ * evaluations done in double and out[26] its pairs handed to the resume launch (summed over
+ * the timed steps; 0 outside the mixed Hertz form).  The reference's TimingInfo has no
+ * per-kernel split. */
+int sdm_get_kernel_timers(sdm_ctx* ctx, double out[27]);
 
 /* Steady-state steps as a replayed graph.  No counterpart in the reference (its engines are
  * called one by one).  When on (the default), a step without a collider run, host write or
@@ -251,7 +255,8 @@ int sdm_get_servo(sdm_ctx* ctx, sdm_servo_state* out);
  * them by timing a workload's own steps. */
 typedef struct sdm_tuning {
   int32_t solver_blocks_per_cu; /* persistent solvers: blocks per CU (0 = the occupancy maximum) */
-  int32_t sort_by_cost;         /* Nelder-Mead list sorted by expected cost, longest first (1) or not (0) */
+  int32_t sort_by_cost;         /* Nelder-Mead list sorted by expected cost, longest first (1) or not (0);
+                                   default 1 for SDM_FP64, 0 for SDM_MIXED */
   int32_t misc_blocks_per_cu;   /* wall / sphere kernel: blocks per CU (>= 1) */
   int32_t graph;                /* steady-state steps replayed from a captured graph (sdm_set_graph) */
 } sdm_tuning;

@@ -293,18 +293,24 @@ class Context:
                 "solver_ms": out[4], "solver_evals": int(out[5]), "extra_evals": int(out[6]),
                 "queued": int(out[7])}
 
+    KERNEL_TIMERS = ("k_newton", "k_force_sum", "k_bounds", "k_pairs_prep", "queue", "k_pairs_misc", "k_nm_first",
+                     "nm_list", "solver", "resume_restarts", "k_nm_refine", "k_pairs_finish")
+
     def kernel_timers(self):
-        """Device time [ms] and launches of the integrator, the force sums, the bound update and
-        the interaction-loop dispatch pass since set_timing(True); with the mixed Hertz form, the
-        Nelder-Mead pairs listed for the solver, the solver's evaluations done in double and the
-        pairs handed to its resume launch."""
-        out = (ctypes.c_double * 11)()
+        """Device time [ms] and launches of each kernel (or group of launches) of the step since
+        set_timing(True): the integrator, the force sums, the bound update, the interaction-loop
+        dispatch pass, the solver queue, the wall and sphere pairs, the Nelder-Mead first vertex
+        and solver list, the main solver launch, the resume launch or LM restarts, the mixed
+        form's double refinement and the finish; with the mixed Hertz form, the Nelder-Mead
+        pairs listed for the solver, the solver's evaluations done in double and the pairs
+        handed to its resume launch."""
+        n = len(self.KERNEL_TIMERS)
+        out = (ctypes.c_double * (2 * n + 3))()
         self.lib.check(self.lib.get_kernel_timers(self.ptr, out), self.ptr)
-        r = {name: {"ms": out[2 * k], "launches": int(out[2 * k + 1])}
-             for k, name in enumerate(("k_newton", "k_force_sum", "k_bounds", "k_pairs_prep"))}
-        r["nm_listed"] = int(out[8])
-        r["nm_double_evals"] = int(out[9])
-        r["nm_handed_over"] = int(out[10])
+        r = {name: {"ms": out[2 * k], "launches": int(out[2 * k + 1])} for k, name in enumerate(self.KERNEL_TIMERS)}
+        r["nm_listed"] = int(out[2 * n])
+        r["nm_double_evals"] = int(out[2 * n + 1])
+        r["nm_handed_over"] = int(out[2 * n + 2])
         return r
 
     def snapshot(self):

@@ -34,16 +34,19 @@
 #include "narrow.h"
 
 // log(1 + x) for x > -1 and exp(x) - 1 in float, a few ulp of the result: the rounding of
-// 1 + x and of exp(x) is undone by the ratio (Goldberg's and Kahan's forms).
+// 1 + x and of exp(x) is undone by the ratio (Goldberg's and Kahan's forms).  The ratios use
+// the hardware reciprocal (1 ulp) instead of a correctly rounded division.
 SDM_HD float dlog1p(float x) {
+#pragma clang fp contract(fast)
   const float u = 1.0f + x;
   const float l = __builtin_amdgcn_logf(u) * 0.693147180559945309f;  // ln u
-  return u == 1.0f ? x : l * (x / (u - 1.0f));
+  return u == 1.0f ? x : l * (x * __builtin_amdgcn_rcpf(u - 1.0f));
 }
 SDM_HD float dexpm1(float x) {
+#pragma clang fp contract(fast)
   const float u = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
   const float l = __builtin_amdgcn_logf(u) * 0.693147180559945309f;
-  return (u == 1.0f || l == 0.0f) ? x : (u - 1.0f) * (x / l);
+  return (u == 1.0f || l == 0.0f) ? x : (u - 1.0f) * (x * __builtin_amdgcn_rcpf(l));
 }
 // log|1 + x|: log1p above -1; below, the log of -(1 + x) (a sign change, not near -1 here)
 SDM_HD float dlogabs1p(float x) {
@@ -52,7 +55,7 @@ SDM_HD float dlogabs1p(float x) {
 
 // The anchor of one body: the support point and its factors at the anchor direction.
 struct DeltaBody {
-  float n[3];    // local direction at the anchor
+  float in[3];   // reciprocals of the local direction's components at the anchor
   float sq0, sq1;  // sin^2 phi0, sin^2 phi1
   float q[3];    // local support point
   float e1, e2, eps1, eps2, t1;  // exponents: 1/(2 - eps1), 1/(2 - eps2), eps1, eps2, 2 - eps1
@@ -85,7 +88,7 @@ SDM_HD bool delta_anchor_local(const Shape<double>& s, V3<double> n, DeltaBody& 
   const double pc1 = apow(c1, s.eps2);
   q = mk3(sgn(c0) * s.rx * apow(fabs(c0), s.eps1) * pc1, sgn(s0) * s.ry * apow(fabs(s0), s.eps1) * pc1,
           sgn(s1) * s.rz * apow(fabs(s1), s.eps2));
-  b.n[0] = (float)n.x; b.n[1] = (float)n.y; b.n[2] = (float)n.z;
+  b.in[0] = (float)(1.0 / n.x); b.in[1] = (float)(1.0 / n.y); b.in[2] = (float)(1.0 / n.z);
   b.sq0 = (float)(s0 * s0);
   b.sq1 = (float)(s1 * s1);
   b.q[0] = (float)q.x; b.q[1] = (float)q.y; b.q[2] = (float)q.z;
@@ -108,7 +111,8 @@ constexpr float kNear = 0.015625f;
 // new angles underestimated the error of some evaluations up to 150-fold on MI355X; this
 // one bounded it.
 SDM_HD V3<float> delta_support(const DeltaBody& b, const float dn[3], bool& ok, float& cond) {
-  const float rx = dn[0] / b.n[0], ry = dn[1] / b.n[1], rz = dn[2] / b.n[2];
+#pragma clang fp contract(fast)
+  const float rx = dn[0] * b.in[0], ry = dn[1] * b.in[1], rz = dn[2] * b.in[2];
   const float lx = dlogabs1p(rx), ly = dlogabs1p(ry), lz = dlogabs1p(rz);
   const float u0 = b.e1 * (ly - lx);
   const float w0 = b.sq0 * dexpm1(2.0f * u0);
@@ -129,8 +133,9 @@ SDM_HD V3<float> delta_support(const DeltaBody& b, const float dn[3], bool& ok, 
   const V3<float> dq = mk3(b.q[0] * (rx < -1.0f ? -(ex + 2.0f) : ex), b.q[1] * (ry < -1.0f ? -(ey + 2.0f) : ey),
                            b.q[2] * (rz < -1.0f ? -(ez + 2.0f) : ez));
   cond = (fabsf(b.q[0]) + fabsf(b.q[1]) + fabsf(b.q[2])) *
-         (1.0f / (fabsf(b.n[0]) * fminf(ax, 1.0f)) + 1.0f / (fabsf(b.n[1]) * fminf(ay, 1.0f)) +
-          1.0f / (fabsf(b.n[2]) * fminf(az, 1.0f)));
+         (fabsf(b.in[0]) * __builtin_amdgcn_rcpf(fminf(ax, 1.0f)) +
+          fabsf(b.in[1]) * __builtin_amdgcn_rcpf(fminf(ay, 1.0f)) +
+          fabsf(b.in[2]) * __builtin_amdgcn_rcpf(fminf(az, 1.0f)));
   return mul(b.G, dq);
 }
 
@@ -178,11 +183,13 @@ SDM_HD double delta_anchor(const PairGeo<double>& g, const M3<double>& gC, doubl
 // then not usable.
 SDM_HD double delta_eval(const DeltaPair& P, float d0, float d1, bool& touching, float& cd, float& ds, float& dk,
                          bool& ok) {
-  // cos / sin of (a + d) - cos / sin a by half-angle forms
-  float sh0, ch0, sh1, ch1;
-  sincosf(d0, &sh0, &ch0);
-  sincosf(d1, &sh1, &ch1);
-  const float h0 = sinf(0.5f * d0), h1 = sinf(0.5f * d1);
+#pragma clang fp contract(fast)
+  // cos / sin of (a + d) - cos / sin a by half-angle forms: sin d = 2 sin(d/2) cos(d/2),
+  // cos d - 1 = -2 sin^2(d/2)
+  float h0, g0, h1, g1;
+  sincosf(0.5f * d0, &h0, &g0);
+  sincosf(0.5f * d1, &h1, &g1);
+  const float sh0 = 2.0f * h0 * g0, sh1 = 2.0f * h1 * g1;
   const float vc0 = -2.0f * h0 * h0, vc1 = -2.0f * h1 * h1;  // cos d - 1
   const float dca0 = P.ca0 * vc0 - P.sa0 * sh0, dsa0 = P.sa0 * vc0 + P.ca0 * sh0;
   const float dca1 = P.ca1 * vc1 - P.sa1 * sh1, dsa1 = P.sa1 * vc1 + P.ca1 * sh1;

@@ -49,7 +49,7 @@ struct SolveIO {  // indexed by work slot
   int64_t cap;
 };
 
-// Rows of SolveIO::anc: per body (offset 0 and 8) the local direction (3), sin^2 phi0, sin^2
+// Rows of SolveIO::anc: per body (offset 0 and 8) the reciprocals of the local direction (3), sin^2 phi0, sin^2
 // phi1, the local support point (3); then cos / sin of the anchor angles (16-19), the gap
 // vector p2 - p1 as high (20-22) and low (23-25) float parts, the contact direction (26-28).
 constexpr int kAnc = 29;
@@ -237,6 +237,21 @@ enum { NM_V = 0, NM_REFL = 1, NM_EXP = 2, NM_CON = 3, NM_SHR = 4 };
 constexpr float kDeltaErr = 4.0f;
 // Lanes per wave that the mixed form's resume launch uses.
 constexpr int kResumeLanes = 4;
+// Mixed form: the main launch re-anchors inline (true) or hands every pair whose evaluation is
+// due in double over to the resume launch (false), which keeps the double support map out of
+// the main launch and its registers; blocks of 256 per CU of the main launch.  Measured on the
+// 200k bed (MI355X): handing over, the main launch takes 0.55 ms at two or three waves per SIMD
+// but the resume launch 0.8 ms -- the handed-over solves are chains of evaluations that run
+// after the main launch, their length set by the pairs that re-anchor most often -- against
+// 0.90 ms for the inline form at two waves per SIMD (1.2 ms at three, where it spills).
+#ifndef SDM_NM_INLINE
+#define SDM_NM_INLINE 1
+#endif
+#ifndef SDM_NM_BLOCKS
+#define SDM_NM_BLOCKS 2
+#endif
+constexpr bool kInlineAnchor = SDM_NM_INLINE != 0;
+constexpr int kMixedMainBlocks = SDM_NM_BLOCKS;
 
 __device__ __forceinline__ void store_anchor(const SolveIO& S, int64_t w, const DeltaPair& P) {
   const int64_t c = S.cap;
@@ -245,7 +260,7 @@ __device__ __forceinline__ void store_anchor(const SolveIO& S, int64_t w, const 
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     const int o = 8 * k;
-    a[(o + 0) * c + w] = b[k]->n[0]; a[(o + 1) * c + w] = b[k]->n[1]; a[(o + 2) * c + w] = b[k]->n[2];
+    a[(o + 0) * c + w] = b[k]->in[0]; a[(o + 1) * c + w] = b[k]->in[1]; a[(o + 2) * c + w] = b[k]->in[2];
     a[(o + 3) * c + w] = b[k]->sq0; a[(o + 4) * c + w] = b[k]->sq1;
     a[(o + 5) * c + w] = b[k]->q[0]; a[(o + 6) * c + w] = b[k]->q[1]; a[(o + 7) * c + w] = b[k]->q[2];
   }
@@ -265,7 +280,7 @@ __device__ __forceinline__ void load_anchor(const SolveIO& S, int64_t w, const G
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     const int o = 8 * k;
-    b[k]->n[0] = a[(o + 0) * c + w]; b[k]->n[1] = a[(o + 1) * c + w]; b[k]->n[2] = a[(o + 2) * c + w];
+    b[k]->in[0] = a[(o + 0) * c + w]; b[k]->in[1] = a[(o + 1) * c + w]; b[k]->in[2] = a[(o + 2) * c + w];
     b[k]->sq0 = a[(o + 3) * c + w]; b[k]->sq1 = a[(o + 4) * c + w];
     b[k]->q[0] = a[(o + 5) * c + w]; b[k]->q[1] = a[(o + 6) * c + w]; b[k]->q[2] = a[(o + 7) * c + w];
     const Shape<double>& s = r[k]->s;
@@ -336,7 +351,7 @@ __device__ __forceinline__ double nm_mixed_point(const SolveArgs& A, int64_t w, 
 // mixed main launch 19 % faster than at three, where it spills), one for the resume launch,
 // whose few pairs want the shortest iteration (no spills).
 template <typename T, bool kResume = false>
-__global__ __launch_bounds__(256, kResume ? 1 : 2) void k_solve_nm(SolveArgs A) {
+__global__ __launch_bounds__(256, kResume ? 1 : (sizeof(T) == 4 ? kMixedMainBlocks : 2)) void k_solve_nm(SolveArgs A) {
   constexpr bool kMixed = sizeof(T) == 4;
   const double ftol = 1e-3;
   const double TINY = 1.0e-10;
@@ -430,15 +445,18 @@ __global__ __launch_bounds__(256, kResume ? 1 : 2) void k_solve_nm(SolveArgs A) 
           due = true;
       }
       if (due) {
-        dbl++;
         if constexpr (kResume) {
+          dbl++;
           f = nm_mixed_point(A, w, pt[0], pt[1], P, anchored, touching);
-        } else {
+          anc0 = pt[0];
+          anc1 = pt[1];
+        } else if constexpr (kInlineAnchor) {
+          dbl++;
           f = nm_reanchor(A, w, pt[0], pt[1], P, anchored, touching);
           due = !anchored;
+          anc0 = pt[0];
+          anc1 = pt[1];
         }
-        anc0 = pt[0];
-        anc1 = pt[1];
       }
       if constexpr (!kResume) {
         if (due) {
@@ -615,10 +633,47 @@ __device__ __forceinline__ uint32_t nm_cost_key(int prevEval, bool prevContact, 
 // difference evaluations, with the device's double functions (S.fb = 1: the difference form
 // does not apply at the first vertex, whose distance is then evaluated in double without an
 // anchor).  Only touching pairs store their anchor.
+// Certified "not touching" at the first vertex from a float evaluation.  The reference stores
+// nothing of a pair that is not touching at its first vertex but that outcome (depth 0, new
+// shear, the start angles), so a float evaluation whose touching operand cd = c.(p2 - p1)
+// exceeds its error bound decides it exactly.  cd = c.x21 - h1(c) - h2(-c) with h the support
+// functions: its gradient in c is bounded by |x21| + |q1| + |q2|, so a direction rounded to
+// float (angles of magnitude |a| carry 2^-24 (|a| + 1) each) moves it by a few 2^-24
+// (|x21| + r1 + r2) (|a0| + |a1| + 4); the float support points (a few ulp of r each, fpow and
+// the rotation), the centre difference and the product add a few 2^-24 (|x21| + r1 + r2).
+// The bound takes 256 2^-24 (|x21|_1 + 2 (r1 + r2)) (|a0| + |a1| + 4), over 10x these.  A NaN
+// anywhere fails the test.
+__device__ __forceinline__ bool nm_first_apart(const GeoRec& r1, const GeoRec& r2, const M3<double>& gC, double a0,
+                                               double a1) {
+  NMGeo<float> n;
+  pair_geo<float>(r1, r2, true, n.g);
+  n.gC = cast9<float>(gC);
+  nm_prepare(n);
+  float s0, c0, s1, c1;
+  sincosf((float)a0, &s0, &c0);
+  sincosf((float)a1, &s1, &c1);
+  const V3<float> c = mul(n.gC, mk3(c0 * c1, s0 * c1, s1));
+  const V3<float> p1 = support_f(n.f1, n.g.G1, c);
+  const V3<float> p2 = support_f(n.f2, n.g.G2, mk3(-c.x, -c.y, -c.z)) + n.g.pos2;
+  const float cd = dot(c, p2 - p1);
+  const V3<double> x21 = r2.pos - r1.pos;
+  const double scale = (fabs(x21.x) + fabs(x21.y) + fabs(x21.z) + 2.0 * (r1.rmax + r2.rmax)) *
+                       (fabs(a0) + fabs(a1) + 4.0);
+  return (double)cd > 256.0 * 5.9604644775390625e-08 * scale;
+}
+
 template <typename T>
 __device__ __forceinline__ bool nm_first_eval(const SolveArgs& A, int64_t w) {
   const GeoRec r1 = load_rec(A.geo, A.S.id1[w]), r2 = load_rec(A.geo, A.S.id2[w]);
   bool touching = true;
+  if (nm_first_apart(r1, r2, crotation_mat(r1.pos, r2.pos), A.S.a0[w], A.S.a1[w])) {
+    A.S.f[w] = 0.0;  // not read for a pair that is not touching
+    if constexpr (sizeof(T) == 4) A.S.fb[w] = 0;
+    A.S.nev[w] = 1;
+    A.S.stop[w] = 1;
+    A.S.touch[w] = 0;
+    return false;
+  }
   if constexpr (sizeof(T) == 4) {
     PairGeo<double> g;
     pair_geo<double>(r1, r2, true, g);
