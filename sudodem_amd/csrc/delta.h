@@ -57,6 +57,7 @@ SDM_HD float dlogabs1p(float x) {
 struct DeltaBody {
   float in[3];   // reciprocals of the local direction's components at the anchor
   float sq0, sq1;  // sin^2 phi0, sin^2 phi1
+  float cq0, cq1;  // cos^2 phi0, cos^2 phi1 (each to its own relative accuracy)
   float q[3];    // local support point
   float e1, e2, eps1, eps2, t1;  // exponents: 1/(2 - eps1), 1/(2 - eps2), eps1, eps2, 2 - eps1
   M3<float> G;   // rot_mat2global
@@ -91,51 +92,80 @@ SDM_HD bool delta_anchor_local(const Shape<double>& s, V3<double> n, DeltaBody& 
   b.in[0] = (float)(1.0 / n.x); b.in[1] = (float)(1.0 / n.y); b.in[2] = (float)(1.0 / n.z);
   b.sq0 = (float)(s0 * s0);
   b.sq1 = (float)(s1 * s1);
+  b.cq0 = (float)(c0 * c0);
+  b.cq1 = (float)(c1 * c1);
   b.q[0] = (float)q.x; b.q[1] = (float)q.y; b.q[2] = (float)q.z;
   b.e1 = (float)e1; b.e2 = (float)e2; b.eps1 = (float)s.eps1; b.eps2 = (float)s.eps2; b.t1 = (float)(2.0 - s.eps1);
   const bool ok = h0 > 0.0 && h1 > 0.0 && bb != 0.0 && bp != 0.0 && c0 != 0.0 && s0 != 0.0 && c1 != 0.0 && s1 != 0.0;
-  return ok && fabs(t) < 1e30 && fabs(tp) < 1e30;
+  // exponents 1/(2 - eps) above 2 (eps > 1.5, towards the octahedral limit) amplify the float
+  // rounding of the logarithms beyond the stated bound: such bodies are evaluated in double
+  return ok && fabs(t) < 1e30 && fabs(tp) < 1e30 && s.eps1 <= 1.5 && s.eps2 <= 1.5;
 }
 
-// Smallest ratio of a new direction component to its anchor value that the relative form
-// takes (|1 + dn/n|).
-constexpr float kNear = 0.015625f;
+// Smallest magnitude of a new direction component relative to its anchor value that the
+// relative form takes (|1 + dn/n|): below it the logarithm of the ratio leaves float range.
+constexpr float kNear = 9.094947017729282e-13f;  // 2^-40
+// Largest magnitude of the logarithmic factors (e^(2 u) stays far from float overflow).
+constexpr float kLogMax = 40.0f;
+
+// ln(a + b e^(2u)) for a, b >= 0 with a + b = 1: the log1p of w = b (e^(2u) - 1) while w stays
+// above -1/2, accurate to its own relative rounding however small the change; below, the
+// logarithm of the sum of two positive terms, which has no cancellation and whose magnitude
+// is then above ln 2.
+SDM_HD float lnmix(float a, float b, float u) {
+#pragma clang fp contract(fast)
+  const float w = b * dexpm1(2.0f * u);
+  float r = dlog1p(w);
+  if (w < -0.5f)  // rare: a branch, not a select, so that waves without such a lane skip it
+    r = __builtin_amdgcn_logf(a + b * __builtin_amdgcn_exp2f(2.0f * u * 1.44269504088896341f)) *
+        0.693147180559945309f;
+  return r;
+}
 
 // Change of the global support point of a body when its local direction changes by dn.
-// Sets `ok` false when a component comes near zero or a factor leaves its range.  `cond`
-// receives a bound on the conditioning of the relative form, (|qx| + |qy| + |qz|) sum_k
-// 1/min(|n_k|, |n_k + dn_k|): the rounding of a direction change (~ulp |dn| in each
-// component) enters through dn_k / n_k and the ratio 1 + dn_k / n_k, and every logarithmic
-// factor passes it on with a coefficient of at most one (d ln c0 / d u0 = -sin^2 phi0,
-// d ln s0 / d u0 = cos^2 phi0, the same for phi1).  A sharper first-order propagation at the
-// new angles underestimated the error of some evaluations up to 150-fold on MI355X; this
-// one bounded it.
-SDM_HD V3<float> delta_support(const DeltaBody& b, const float dn[3], bool& ok, float& cond) {
+// With l_k = ln|1 + dn_k/n_k| the changes of the logarithms of the factors are
+//   u0 = e1 (ly - lx)                      tan phi0 ratio
+//   lc0 = -1/2 ln(cq0 + sq0 e^(2 u0))      cos phi0 ratio
+//   ls0 = -1/2 ln(sq0 + cq0 e^(-2 u0))     sin phi0 ratio
+//   u1 = e2 (lz + t1 lc0 - lx) = e2 (lz + t1 ls0 - ly)   (t1 e1 = 1)
+// and the same for phi1; each is taken in the form without cancellation for small changes:
+// of lc0 and ls0 the one of the larger of cos^2 / sin^2 through lnmix, the other from ls0 - lc0
+// = u0; u1 through the form whose middle term (t1 lc0 or t1 ls0) is the smaller.  A change
+// of any size is then carried with rounding that scales with it, except where two large
+// logarithms cancel in u0, u1 or the exponents g (the direction leaving a principal plane on
+// two axes at once); `lgk` bounds that absolute rounding, (|qx| + |qy| + |qz|) (|lx| + |ly| +
+// |lz| + |u0| + |u1|).  Sets `ok` false when a component comes within kNear of zero relative
+// to its anchor value or a factor leaves kLogMax.  `cond` bounds the conditioning of the ratio
+// form, (|qx| + |qy| + |qz|) sum_k 1/min(|n_k|, |n_k + dn_k|): the rounding of a direction
+// change (~ulp |dn| in each component) enters through dn_k / n_k and every logarithmic factor
+// passes it on with a coefficient of at most one.
+SDM_HD V3<float> delta_support(const DeltaBody& b, const float dn[3], bool& ok, float& cond, float& lgk) {
 #pragma clang fp contract(fast)
   const float rx = dn[0] * b.in[0], ry = dn[1] * b.in[1], rz = dn[2] * b.in[2];
-  const float lx = dlogabs1p(rx), ly = dlogabs1p(ry), lz = dlogabs1p(rz);
-  const float u0 = b.e1 * (ly - lx);
-  const float w0 = b.sq0 * dexpm1(2.0f * u0);
-  const float lc0 = -0.5f * dlog1p(w0);
-  const float ls0 = u0 + lc0;
-  const float u1 = b.e2 * (lz + b.t1 * lc0 - lx);
-  const float w1 = b.sq1 * dexpm1(2.0f * u1);
-  const float lc1 = -0.5f * dlog1p(w1);
-  const float ls1 = u1 + lc1;
-  const float gx = b.eps1 * lc0 + b.eps2 * lc1, gy = b.eps1 * ls0 + b.eps2 * lc1, gz = b.eps2 * ls1;
-  // every component at least kNear of its anchor value away from zero, cos^2 of the angles
-  // at least 1/4 of the anchor's, the exponentials far from overflow
-  const float lo = -0.75f, hi = 16.0f;
   const float ax = fabsf(1.0f + rx), ay = fabsf(1.0f + ry), az = fabsf(1.0f + rz);
-  ok = ok && ax >= kNear && ay >= kNear && az >= kNear && w0 >= lo && w1 >= lo && fabsf(u0) <= hi &&
-       fabsf(u1) <= hi && fabsf(gx) <= hi && fabsf(gy) <= hi && fabsf(gz) <= hi;
+  const float lx = dlogabs1p(rx), ly = dlogabs1p(ry), lz = dlogabs1p(rz);
+  // of the two logarithms of an angle, the one on the side of the larger of cos^2 / sin^2
+  // through lnmix, the other as its difference with u (ls - lc = u exactly), which then does
+  // not cancel for small changes
+  const float u0 = b.e1 * (ly - lx);
+  const bool c0big = b.cq0 >= 0.5f;
+  const float m0 = -0.5f * lnmix(c0big ? b.cq0 : b.sq0, c0big ? b.sq0 : b.cq0, c0big ? u0 : -u0);
+  const float lc0 = c0big ? m0 : m0 - u0, ls0 = c0big ? m0 + u0 : m0;
+  const float u1 = b.e2 * (lz + (fabsf(lc0) <= fabsf(ls0) ? b.t1 * lc0 - lx : b.t1 * ls0 - ly));
+  const bool c1big = b.cq1 >= 0.5f;
+  const float m1 = -0.5f * lnmix(c1big ? b.cq1 : b.sq1, c1big ? b.sq1 : b.cq1, c1big ? u1 : -u1);
+  const float lc1 = c1big ? m1 : m1 - u1, ls1 = c1big ? m1 + u1 : m1;
+  const float gx = b.eps1 * lc0 + b.eps2 * lc1, gy = b.eps1 * ls0 + b.eps2 * lc1, gz = b.eps2 * ls1;
+  ok = ok && ax >= kNear && ay >= kNear && az >= kNear && fabsf(u0) <= kLogMax && fabsf(u1) <= kLogMax &&
+       fabsf(gx) <= kLogMax && fabsf(gy) <= kLogMax && fabsf(gz) <= kLogMax;
   const float ex = dexpm1(gx), ey = dexpm1(gy), ez = dexpm1(gz);
   const V3<float> dq = mk3(b.q[0] * (rx < -1.0f ? -(ex + 2.0f) : ex), b.q[1] * (ry < -1.0f ? -(ey + 2.0f) : ey),
                            b.q[2] * (rz < -1.0f ? -(ez + 2.0f) : ez));
-  cond = (fabsf(b.q[0]) + fabsf(b.q[1]) + fabsf(b.q[2])) *
-         (fabsf(b.in[0]) * __builtin_amdgcn_rcpf(fminf(ax, 1.0f)) +
-          fabsf(b.in[1]) * __builtin_amdgcn_rcpf(fminf(ay, 1.0f)) +
-          fabsf(b.in[2]) * __builtin_amdgcn_rcpf(fminf(az, 1.0f)));
+  const float qn = fabsf(b.q[0]) + fabsf(b.q[1]) + fabsf(b.q[2]);
+  cond = qn * (fabsf(b.in[0]) * __builtin_amdgcn_rcpf(fminf(ax, 1.0f)) +
+               fabsf(b.in[1]) * __builtin_amdgcn_rcpf(fminf(ay, 1.0f)) +
+               fabsf(b.in[2]) * __builtin_amdgcn_rcpf(fminf(az, 1.0f)));
+  lgk = qn * (fabsf(lx) + fabsf(ly) + fabsf(lz) + fabsf(u0) + fabsf(u1));
   return mul(b.G, dq);
 }
 
@@ -182,7 +212,7 @@ SDM_HD double delta_anchor(const PairGeo<double>& g, const M3<double>& gC, doubl
 // with ulp(f), of cd.  `ok` is false when the relative form left its range; the value is
 // then not usable.
 SDM_HD double delta_eval(const DeltaPair& P, float d0, float d1, bool& touching, float& cd, float& ds, float& dk,
-                         bool& ok) {
+                         float& dl, bool& ok) {
 #pragma clang fp contract(fast)
   // cos / sin of (a + d) - cos / sin a by half-angle forms: sin d = 2 sin(d/2) cos(d/2),
   // cos d - 1 = -2 sin^2(d/2)
@@ -201,9 +231,9 @@ SDM_HD double delta_eval(const DeltaPair& P, float d0, float d1, bool& touching,
   const V3<float> dn2 = mulT(P.b2.G, dc);
   const float n1[3] = {dn1.x, dn1.y, dn1.z};
   const float n2[3] = {-dn2.x, -dn2.y, -dn2.z};
-  float k1 = 0.0f, k2 = 0.0f;
-  const V3<float> ds1 = delta_support(P.b1, n1, ok, k1);
-  const V3<float> ds2 = delta_support(P.b2, n2, ok, k2);
+  float k1 = 0.0f, k2 = 0.0f, l1 = 0.0f, l2 = 0.0f;
+  const V3<float> ds1 = delta_support(P.b1, n1, ok, k1, l1);
+  const V3<float> ds2 = delta_support(P.b2, n2, ok, k2, l2);
   const V3<double> d = mk3(((double)P.Fh[0] + (double)P.Fl[0]) + (double)(ds2.x - ds1.x),
                            ((double)P.Fh[1] + (double)P.Fl[1]) + (double)(ds2.y - ds1.y),
                            ((double)P.Fh[2] + (double)P.Fl[2]) + (double)(ds2.z - ds1.z));
@@ -212,5 +242,6 @@ SDM_HD double delta_eval(const DeltaPair& P, float d0, float d1, bool& touching,
   if (cd > 0.0f) touching = false;
   ds = norm(ds1) + norm(ds2);
   dk = norm(dc) * (k1 + k2);
+  dl = l1 + l2;
   return norm(d);
 }

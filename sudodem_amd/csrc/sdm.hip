@@ -885,13 +885,17 @@ void nm_pass(sdm_ctx* c, const StepArgs& A, SolveArgs SA) {
   scan_to_counter(c, c->qflag, c->qpos, nmax, DC_NT);
   const unsigned long long* nt = c->dcnt + DC_NT;
   hipLaunchKernelGGL(k_list_fill, dim3(nblk(nmax)), dim3(256), 0, c->st, c->qflag, c->qpos, c->k1, nmax, c->k0, c->v0);
-  hipLaunchKernelGGL(k_pad_keys, dim3(nblk(nmax)), dim3(256), 0, c->st, c->k0, c->v0, nt, nmax);
   // the list in queue order (increasing interaction index) for the finish kernel, whose
-  // record reads and writes then follow memory order; a copy sorted by expected cost for
-  // the solver
-  uint32_t* byPair = c->u0;
-  hipLaunchKernelGGL(k_copy_u32, dim3(nblk(nmax)), dim3(256), 0, c->st, byPair, c->v0, nt, nmax);
-  if (c->tune.sort_by_cost) sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, nmax, 8, c->rsScratch, c->st);
+  // record reads and writes then follow memory order; with sort_by_cost, a copy sorted by
+  // expected cost for the solver (the solver reads only the first nt entries: no padding
+  // without the sort)
+  const uint32_t* byPair = c->v0;
+  if (c->tune.sort_by_cost) {
+    hipLaunchKernelGGL(k_pad_keys, dim3(nblk(nmax)), dim3(256), 0, c->st, c->k0, c->v0, nt, nmax);
+    hipLaunchKernelGGL(k_copy_u32, dim3(nblk(nmax)), dim3(256), 0, c->st, c->u0, c->v0, nt, nmax);
+    byPair = c->u0;
+    sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, nmax, 8, c->rsScratch, c->st);
+  }
   KT_STOP(c, KT_LIST);
   const uint32_t* list = c->v0;
   SolveArgs S2 = SA;

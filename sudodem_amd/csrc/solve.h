@@ -49,10 +49,11 @@ struct SolveIO {  // indexed by work slot
   int64_t cap;
 };
 
-// Rows of SolveIO::anc: per body (offset 0 and 8) the reciprocals of the local direction (3), sin^2 phi0, sin^2
-// phi1, the local support point (3); then cos / sin of the anchor angles (16-19), the gap
-// vector p2 - p1 as high (20-22) and low (23-25) float parts, the contact direction (26-28).
-constexpr int kAnc = 29;
+// Rows of SolveIO::anc: per body (offset 0 and 10) the reciprocals of the local direction (3),
+// sin^2 and cos^2 of phi0 and phi1 (4), the local support point (3); then cos / sin of the
+// anchor angles (20-23), the gap vector p2 - p1 as high (24-26) and low (27-29) float parts,
+// the contact direction (30-32).
+constexpr int kAnc = 33;
 
 struct SolveArgs {
   const double* geo;  // packed body records (state.h): pos, rot_mat2global, semi-axes, exponents, r_max
@@ -230,11 +231,15 @@ __device__ __forceinline__ int64_t take_work(bool want, WaveBatch& b, unsigned l
 enum { NM_V = 0, NM_REFL = 1, NM_EXP = 2, NM_CON = 3, NM_SHR = 4 };
 
 // Rounding bound of a difference evaluation (delta_eval): |f - f_exact| <= kDeltaErr 2^-24
-// (ds + 4 dk) with ds the support displacements and dk the conditioning term; measured on
-// MI355X against double evaluations over 10^6 random pairs and steps up to 0.03 rad, the
-// largest |f - f_exact| / (2^-24 (ds + 4 dk)) was 1.29.  The touching operand cd adds
-// 4 2^-24 f.
-constexpr float kDeltaErr = 4.0f;
+// (ds + 4 dk + dl) with ds the support displacements, dk the conditioning term and dl the
+// bound of cancelling logarithms; measured on MI355X against double evaluations at the same
+// angles over 4.2M random pairs (exponents U[0.6,1.4], steps 1e-7 to 0.5 rad), the largest
+// |f - f_exact| / (2^-24 (ds + 4 dk + dl)) was 0.23; with the contact direction forced within
+// 1e-7..0.1 of a principal plane of one body, 0.23 for steps above 1e-5 rad, and up to 16 below,
+// where the double evaluation itself (cos of an angle within 1e-11 of pi/2) is off by more than
+// the difference.  kDeltaErr = 1 keeps a factor 4 over the largest measured ratio.  The
+// touching operand cd adds 4 2^-24 f, and the anchor's own double rounding 2^-48 (|q1| + |q2|).
+constexpr float kDeltaErr = 1.0f;
 // Lanes per wave that the mixed form's resume launch uses.
 constexpr int kResumeLanes = 4;
 // Mixed form: the main launch re-anchors inline (true) or hands every pair whose evaluation is
@@ -259,15 +264,16 @@ __device__ __forceinline__ void store_anchor(const SolveIO& S, int64_t w, const 
   const DeltaBody* b[2] = {&P.b1, &P.b2};
 #pragma unroll
   for (int k = 0; k < 2; k++) {
-    const int o = 8 * k;
+    const int o = 10 * k;
     a[(o + 0) * c + w] = b[k]->in[0]; a[(o + 1) * c + w] = b[k]->in[1]; a[(o + 2) * c + w] = b[k]->in[2];
     a[(o + 3) * c + w] = b[k]->sq0; a[(o + 4) * c + w] = b[k]->sq1;
-    a[(o + 5) * c + w] = b[k]->q[0]; a[(o + 6) * c + w] = b[k]->q[1]; a[(o + 7) * c + w] = b[k]->q[2];
+    a[(o + 5) * c + w] = b[k]->cq0; a[(o + 6) * c + w] = b[k]->cq1;
+    a[(o + 7) * c + w] = b[k]->q[0]; a[(o + 8) * c + w] = b[k]->q[1]; a[(o + 9) * c + w] = b[k]->q[2];
   }
-  a[16 * c + w] = P.ca0; a[17 * c + w] = P.sa0; a[18 * c + w] = P.ca1; a[19 * c + w] = P.sa1;
-  a[20 * c + w] = P.Fh[0]; a[21 * c + w] = P.Fh[1]; a[22 * c + w] = P.Fh[2];
-  a[23 * c + w] = P.Fl[0]; a[24 * c + w] = P.Fl[1]; a[25 * c + w] = P.Fl[2];
-  a[26 * c + w] = P.cu[0]; a[27 * c + w] = P.cu[1]; a[28 * c + w] = P.cu[2];
+  a[20 * c + w] = P.ca0; a[21 * c + w] = P.sa0; a[22 * c + w] = P.ca1; a[23 * c + w] = P.sa1;
+  a[24 * c + w] = P.Fh[0]; a[25 * c + w] = P.Fh[1]; a[26 * c + w] = P.Fh[2];
+  a[27 * c + w] = P.Fl[0]; a[28 * c + w] = P.Fl[1]; a[29 * c + w] = P.Fl[2];
+  a[30 * c + w] = P.cu[0]; a[31 * c + w] = P.cu[1]; a[32 * c + w] = P.cu[2];
 }
 
 // The anchor of a queued pair from its stored rows and the two body records.
@@ -279,19 +285,20 @@ __device__ __forceinline__ void load_anchor(const SolveIO& S, int64_t w, const G
   const GeoRec* r[2] = {&r1, &r2};
 #pragma unroll
   for (int k = 0; k < 2; k++) {
-    const int o = 8 * k;
+    const int o = 10 * k;
     b[k]->in[0] = a[(o + 0) * c + w]; b[k]->in[1] = a[(o + 1) * c + w]; b[k]->in[2] = a[(o + 2) * c + w];
     b[k]->sq0 = a[(o + 3) * c + w]; b[k]->sq1 = a[(o + 4) * c + w];
-    b[k]->q[0] = a[(o + 5) * c + w]; b[k]->q[1] = a[(o + 6) * c + w]; b[k]->q[2] = a[(o + 7) * c + w];
+    b[k]->cq0 = a[(o + 5) * c + w]; b[k]->cq1 = a[(o + 6) * c + w];
+    b[k]->q[0] = a[(o + 7) * c + w]; b[k]->q[1] = a[(o + 8) * c + w]; b[k]->q[2] = a[(o + 9) * c + w];
     const Shape<double>& s = r[k]->s;
     b[k]->e1 = (float)(1.0 / (2.0 - s.eps1)); b[k]->e2 = (float)(1.0 / (2.0 - s.eps2));
     b[k]->eps1 = (float)s.eps1; b[k]->eps2 = (float)s.eps2; b[k]->t1 = (float)(2.0 - s.eps1);
     b[k]->G = cast9<float>(r[k]->G);
   }
-  P.ca0 = a[16 * c + w]; P.sa0 = a[17 * c + w]; P.ca1 = a[18 * c + w]; P.sa1 = a[19 * c + w];
-  P.Fh[0] = a[20 * c + w]; P.Fh[1] = a[21 * c + w]; P.Fh[2] = a[22 * c + w];
-  P.Fl[0] = a[23 * c + w]; P.Fl[1] = a[24 * c + w]; P.Fl[2] = a[25 * c + w];
-  P.cu[0] = a[26 * c + w]; P.cu[1] = a[27 * c + w]; P.cu[2] = a[28 * c + w];
+  P.ca0 = a[20 * c + w]; P.sa0 = a[21 * c + w]; P.ca1 = a[22 * c + w]; P.sa1 = a[23 * c + w];
+  P.Fh[0] = a[24 * c + w]; P.Fh[1] = a[25 * c + w]; P.Fh[2] = a[26 * c + w];
+  P.Fl[0] = a[27 * c + w]; P.Fl[1] = a[28 * c + w]; P.Fl[2] = a[29 * c + w];
+  P.cu[0] = a[30 * c + w]; P.cu[1] = a[31 * c + w]; P.cu[2] = a[32 * c + w];
   P.gC = cast9<float>(crotation_mat(r1.pos, r2.pos));
 }
 
@@ -434,9 +441,12 @@ __global__ __launch_bounds__(256, kResume ? 1 : (sizeof(T) == 4 ? kMixedMainBloc
         nev++;
         bool ok = anchored, t = touching;
         if (anchored) {
-          float cd = 0.0f, ds = 0.0f, dk = 0.0f;
-          f = delta_eval(P, (float)(pt[0] - anc0), (float)(pt[1] - anc1), t, cd, ds, dk, ok);
-          const double ef = U * ((double)ds + 4.0 * (double)dk) + 4e-16 * f;
+          float cd = 0.0f, ds = 0.0f, dk = 0.0f, dl = 0.0f;
+          f = delta_eval(P, (float)(pt[0] - anc0), (float)(pt[1] - anc1), t, cd, ds, dk, dl, ok);
+          const float qa = fabsf(P.b1.q[0]) + fabsf(P.b1.q[1]) + fabsf(P.b1.q[2]) + fabsf(P.b2.q[0]) +
+                           fabsf(P.b2.q[1]) + fabsf(P.b2.q[2]);
+          const double ef =
+              U * ((double)ds + 4.0 * (double)dk + (double)dl) + 4e-16 * f + 3.552713678800501e-15 * (double)qa;
           ok = ok && fabs((double)cd) > ef + 4.0 * 5.9604644775390625e-08 * f;
         }
         if (ok)
@@ -721,6 +731,21 @@ __device__ __forceinline__ bool nm_first_eval(const SolveArgs& A, int64_t w) {
 // reference takes them (Superquadrics_Ig2.cpp:2439-2454).  One work item per listed pair.
 // The evaluation uses the device's double functions (double accuracy, not the FP64 form's
 // correctly rounded last bits).
+// A double evaluation at (b0, b1) for the refinement: the anchor's formula (delta_anchor:
+// powers as exp2(b log2 a), the angles' cos / sin as ratios, no atan2 or sin / cos of them),
+// the general evaluation where that formula does not apply (a direction component zero).
+__device__ __forceinline__ double nm_double_point(const NMGeo<double>& n, double b0, double b1, V3<double>& p1,
+                                                  V3<double>& p2, int& nev) {
+  DeltaPair P;
+  bool touching = true, ok = true;
+  double cd = 0.0;
+  nev++;
+  const double f = delta_anchor(n.g, n.gC, b0, b1, P, p1, p2, touching, cd, ok);
+  if (ok && isfinite(f)) return f;
+  nev--;
+  return nm_eval<double, true>(n, b0, b1, p1, p2, touching, nev);
+}
+
 __global__ __launch_bounds__(256) void k_nm_refine(SolveArgs A) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= work_n(A)) return;
@@ -731,15 +756,12 @@ __global__ __launch_bounds__(256) void k_nm_refine(SolveArgs A) {
   pair_geo<double>(r1, r2, false, n.g);
   n.gC = crotation_mat(r1.pos, r2.pos);
   V3<double> q1, q2;
-  bool touching = true;
   int nev = 0;
   const double a0 = A.S.a0[w], a1 = A.S.a1[w], l0 = A.S.la0[w], l1 = A.S.la1[w];
-  const double f = nm_eval<double, true>(n, a0, a1, q1, q2, touching, nev);
+  const double f = nm_double_point(n, a0, a1, q1, q2, nev);
   // the last point is often the best vertex: its points are those just computed
-  if (__double_as_longlong(l0) != __double_as_longlong(a0) || __double_as_longlong(l1) != __double_as_longlong(a1)) {
-    bool t2 = true;
-    nm_eval<double, true>(n, l0, l1, q1, q2, t2, nev);
-  }
+  if (__double_as_longlong(l0) != __double_as_longlong(a0) || __double_as_longlong(l1) != __double_as_longlong(a1))
+    nm_double_point(n, l0, l1, q1, q2, nev);
   const int64_t S = A.S.cap;
   A.S.f[w] = f;
   st3s(A.S.p1, w, S, q1);
