@@ -9,10 +9,13 @@ fires) + InteractionLoop + NewtonIntegrator.
 
 Procedure: generate the seeded lattice (configs.config1), densify it with the grow-and-relax
 packing algorithm run by the engine itself (configs.densify: contacts with ~2 um overlaps,
-coordination ~3.5), run W warm-up steps in FP64, snapshot the state, then time K steps of the
+coordination ~3.3, solids fraction 0.573), run W warm-up steps in FP64, snapshot the state, then time K steps of the
 all-FP64 form and K steps of the mixed FP32/FP64 form, each from that same snapshot.  The CPU
 oracle (the restatement of the reference's OpenMP loop) runs a bounded sample from the same
-snapshot on the host cores.  Prints one JSON line.
+snapshot on the host cores (median of 5 short repeats).  Prints one JSON line, with the
+device time of every kernel of the step (HIP events on the step's stream) and roofline
+fractions of the dominant VALU-bound kernel and of the HBM-bound integrator, bound update
+and collider.
 """
 from __future__ import annotations
 
@@ -34,6 +37,40 @@ SOLVER_TRAFFIC_BYTES = (221226 * 2.0 + 166754 * 1.0) * 1024
 FLOP_EQ_PER_EVAL = 392  # pow=8, sin/cos=4, atan2=16, div=4, add/mul=1 over one Disfun evaluation
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+# compulsory HBM bytes: NewtonIntegrator reads F, T, v, L, q, x, m, I and flags (188 B) and writes
+# v, L, omega, q, x (128 B) per body, and refreshes the 112 B of the packed body record the
+# narrowphase reads (position, velocities, rotation); the bound update reads x and r_max and
+# writes min / max (80 B) and the record's bound line (48 B); the collider reads the bounds
+# (48 B per body) and writes 8 B per pair
+NEWTON_BYTES_PER_BODY = 316 + 112
+BOUNDS_BYTES_PER_BODY = 80 + 48
+
+
+def host_cpu():
+    """CPU model name of the host (first 'model name' of /proc/cpuinfo) and its logical CPUs."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: OMP_NUM_THREADS when set (the host share given to this
+    process), else every CPU this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def log(*a):
@@ -99,38 +136,37 @@ def calibrate_copy(nbytes=1 << 30, reps=4):
 
 
 def adjoint_bench(args, n=20000, steps=500, loss_steps=100, checkpoint_every=50, wall_speed=0.05):
-    """Differentiation workload: n superellipsoids (the config-0 size and shape distributions)
-    densified into a bed, the top wall lowered onto the tallest particles and moving down at
-    wall_speed; a `steps`-step frozen-contact rollout whose loss is the mean top-wall normal
-    force over the last `loss_steps` steps; gradients w.r.t. (kn, ks, mu) and the 3n start
-    velocities, checkpoints every `checkpoint_every` steps.  The kn gradient is checked against
-    a central difference of the device forward."""
+    """Differentiation workload: n superellipsoids (the plain workload's size and shape
+    distributions, linear law Kn 1e5 / Ks 7e4 / mu 0.5) settled in a box by the wall servo
+    (configs.config3: 20 kPa on every wall, 20,000 steps), then the top wall driven down at
+    wall_speed: a `steps`-step frozen-contact rollout (local damping 0.3, gravity) whose loss is
+    the mean top-wall normal force over the last `loss_steps` steps; gradients w.r.t. (kn, ks,
+    mu) and the 3n start velocities, all float32 at the interface (the kernels work in double),
+    checkpoints every `checkpoint_every` steps.  The three parameter gradients and a random
+    directional derivative of the velocity gradient are checked against the FP64 forward-mode
+    restatement (tests/frozen_ref.py: the exact derivative of the same discrete rollout) and
+    against central differences of the device forward (the best of several step sizes: the loss
+    is piecewise smooth, every contact leaving the window and every Cundall sign switch is a
+    jump)."""
     import torch
 
     from sudodem_amd import configs, load_hip
     from sudodem_amd.adjoint import FrozenProblem, rollout
-    from sudodem_amd.scene import rot_matrices
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     lib = load_hip()
     t0 = time.perf_counter()
-    cfg = configs.densify(configs.config1(n_particles=n, box_xy=40e-3), lib, iterations=100)
-    A = cfg.arrays
-    part = A["kind"] == 0
-    ez = np.zeros((int(part.sum()), 3))
-    ez[:, 2] = 1.0
-    tops = A["pos"][part, 2] + configs.support_h(A["rxyz"][part], A["eps"][part], rot_matrices(A["ori"][part]), ez)
-    A["pos"][5, 2] = float(np.sort(tops)[-40]) - 1e-6  # the 40 tallest particles touch the top wall
-    ctx = cfg.open(lib)
-    ctx.step(20)
-    arr = FrozenProblem.arrays_from_context(ctx, loss_body=5, wall_velocity={5: (0.0, 0.0, -wall_speed)})
+    ctx, cfg = configs.config3(lib, n_particles=n, box_xy=40e-3, wall_speed=wall_speed, log=log)
+    cn = ctx.mean_coordination()
+    arr = FrozenProblem.arrays_from_context(ctx, loss_body=configs.TOP_WALL,
+                                            wall_velocity={configs.TOP_WALL: (0.0, 0.0, -wall_speed)})
     ctx.close()
     P = FrozenProblem(arr, cfg.dt, 0.3, (0.0, 0.0, -9.81), steps, loss_steps, checkpoint_every)
-    log("window: %d bodies, %d contacts (%d on the top wall), prepared in %.1fs" % (
-        P.n, P.m, int(arr["in_loss"].sum()), time.perf_counter() - t0))
+    log("window: %d bodies, %d contacts (%d on the top wall), coordination %.2f, prepared in %.1fs" % (
+        P.n, P.m, int(arr["in_loss"].sum()), cn, time.perf_counter() - t0))
     params0 = (1e5, 7e4, 0.5)
-    p = torch.tensor(params0, dtype=torch.float64, device="cuda", requires_grad=True)
-    v = torch.tensor(arr["v0"], dtype=torch.float64, device="cuda", requires_grad=True)
+    p = torch.tensor(params0, dtype=torch.float32, device="cuda", requires_grad=True)
+    v = torch.tensor(arr["v0"], dtype=torch.float32, device="cuda", requires_grad=True)
     rollout(P, p.detach(), v.detach())  # first call: allocation and code load
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -141,23 +177,99 @@ def adjoint_bench(args, n=20000, steps=500, loss_steps=100, checkpoint_every=50,
     loss.backward()
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
-    h = 1e-6 * params0[0]
-    lp = float(rollout(P, torch.tensor((params0[0] + h,) + params0[1:], dtype=torch.float64, device="cuda"), v.detach()))
-    lm = float(rollout(P, torch.tensor((params0[0] - h,) + params0[1:], dtype=torch.float64, device="cuda"), v.detach()))
-    fd = (lp - lm) / (2 * h)
-    g = p.grad.cpu().numpy()
-    nb = int(arr["dynamic"].sum())
+    g = p.grad.double().cpu().numpy()
+    gv = v.grad.double().cpu().numpy()
+    pd = p.detach().double()
+    vd = v.detach().double()
+    v32 = vd.cpu().numpy()  # the float32 start velocities, as the kernels took them
+    dyn = arr["dynamic"] == 1
+    rng = np.random.default_rng(11)
+    dv = np.zeros_like(arr["v0"])
+    dv[dyn] = rng.normal(size=(int(dyn.sum()), 3))
+    dad = float(np.sum(gv * dv))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from frozen_ref import rollout_jvp
+    # exact derivatives of the same discrete rollout (Coulomb cap and Cundall branches held at
+    # their values) by the FP64 forward-mode restatement on the host, one direction per pass
+    t0 = time.perf_counter()
+    jvp = []
+    for k in range(3):
+        d = [0.0, 0.0, 0.0]
+        d[k] = 1.0
+        jvp.append(rollout_jvp(arr, params0, v32, steps, loss_steps, cfg.dt, 0.3, (0.0, 0.0, -9.81), dparams=d)[1])
+    jv = rollout_jvp(arr, params0, v32, steps, loss_steps, cfg.dt, 0.3, (0.0, 0.0, -9.81), dv0=dv)[1]
+    tjvp = time.perf_counter() - t0
+    fm_rel = [abs(g[k] - jvp[k]) / max(abs(jvp[k]), 1e-300) for k in range(3)] + [abs(dad - jv) / max(abs(jv), 1e-300)]
+
+    def L(pp, vv):
+        return float(rollout(P, pp, vv))
+
+    def fd_best(f, x0, tangent, rel_steps):
+        # the loss is piecewise smooth (contacts leaving the window, Cundall sign switches): the
+        # smallest difference over the step sizes, of the steps that straddle no such event
+        errs = []
+        for r in rel_steps:
+            h = r * x0
+            errs.append(abs((f(h) - f(-h)) / (2 * h) - tangent) / max(abs(tangent), 1e-300))
+        return min(errs)
+    fd_rel = []
+    for k in range(3):
+        def fk(h, k=k):
+            e = torch.zeros(3, dtype=torch.float64, device="cuda")
+            e[k] = h
+            return L(pd + e, vd)
+        fd_rel.append(fd_best(fk, params0[k], g[k], (1e-3, 1e-4, 1e-5, 1e-6)))
+    dvt = torch.tensor(dv, dtype=torch.float64, device="cuda")
+    fd_rel.append(fd_best(lambda h: L(pd, vd + h * dvt), 1.0, dad, (1e-4, 1e-5, 1e-6, 1e-7)))
+    # the same window without local damping (no Cundall switches): gradients against central
+    # differences where the loss is smooth but for contacts leaving the window
+    P0 = FrozenProblem(arr, cfg.dt, 0.0, (0.0, 0.0, -9.81), steps, loss_steps, checkpoint_every)
+    p0 = pd.clone().requires_grad_(True)
+    v0t = vd.clone().requires_grad_(True)
+    l0 = rollout(P0, p0, v0t)
+    l0.backward()
+    g0, gv0 = p0.grad.cpu().numpy(), v0t.grad.cpu().numpy()
+    fd0_rel = []
+    for k in range(3):
+        def fk0(h, k=k):
+            e = torch.zeros(3, dtype=torch.float64, device="cuda")
+            e[k] = h
+            return float(rollout(P0, pd + e, vd))
+        fd0_rel.append(fd_best(fk0, params0[k], g0[k], (1e-3, 1e-4, 1e-5, 1e-6)))
+    fd0_rel.append(fd_best(lambda h: float(rollout(P0, pd, vd + h * dvt)), 1.0, float(np.sum(gv0 * dv)),
+                           (1e-4, 1e-5, 1e-6, 1e-7)))
+    log("loss %.6g; gradients (kn, ks, mu) %s, velocity direction %.6g; forward-mode rel. differences %s (%.1fs); "
+        "central differences, best step: %s; without damping: %s" % (float(loss.detach()), g.tolist(), dad, fm_rel,
+                                                                      tjvp, fd_rel, fd0_rel))
+    cpu = None
+    if not args.no_cpu:
+        cs = 20
+        t0 = time.perf_counter()
+        rollout_jvp(arr, params0, v32, cs, 1, cfg.dt, 0.3, (0.0, 0.0, -9.81), dparams=(1.0, 0.0, 0.0))
+        cw = time.perf_counter() - t0
+        nb0 = int(dyn.sum())
+        cpu = {"value": nb0 * cs / cw, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": "%d steps of the same window, forward + one tangent direction, FP64 forward-mode "
+                         "restatement of the frozen rollout (numpy, one thread), %.1fs wall; a full velocity "
+                         "gradient by forward mode would take 3n such passes" % (cs, cw)}
+    nb = int(dyn.sum())
     print(json.dumps({
         "metric": "frozen-contact rollout, forward + adjoint, particle-steps/s",
         "value": nb * steps / (tf + tb), "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": 1,
         "ms_per_step": (tf + tb) / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "%d superellipsoids, %d-step window, top wall at %.2f m/s, loss over the last %d steps, "
-                               "checkpoints every %d steps" % (nb, steps, wall_speed, loss_steps, checkpoint_every),
-                   "contacts": P.m},
+        "dtype": "f64 (f32 parameters and velocities at the interface)", "data": "synthetic",
+        "config": {"workload": "%d superellipsoids settled by the wall servo (20 kPa), %d-step window, top wall at "
+                               "%.2f m/s, local damping 0.3, loss over the last %d steps, checkpoints every %d steps"
+                               % (nb, steps, wall_speed, loss_steps, checkpoint_every),
+                   "contacts": P.m, "top_wall_contacts": int(arr["in_loss"].sum()), "coordination": cn},
         "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "loss": float(loss.detach()),
-        "grad_kn_ks_mu": g.tolist(), "grad_v0_norm": float(v.grad.norm()),
-        "grad_kn_central_difference": fd, "grad_kn_rel_err": abs(g[0] - fd) / max(abs(fd), 1e-300),
+        "grad_kn_ks_mu": g.tolist(), "grad_v0_norm": float(np.linalg.norm(gv)), "grad_v0_direction": dad,
+        "forward_mode_rel_err": {"kn": fm_rel[0], "ks": fm_rel[1], "mu": fm_rel[2], "v0_direction": fm_rel[3]},
+        "central_difference_rel_err_best_step": {"kn": fd_rel[0], "ks": fd_rel[1], "mu": fd_rel[2],
+                                                 "v0_direction": fd_rel[3]},
+        "central_difference_rel_err_best_step_no_damping": {"kn": fd0_rel[0], "ks": fd0_rel[1], "mu": fd0_rel[2],
+                                                            "v0_direction": fd0_rel[3], "loss": float(l0)},
+        "cpu_baseline": cpu,
     }))
 
 
@@ -191,7 +303,16 @@ def batch_bench(args):
     tb = time.perf_counter() - t0
     graph_steps = bc.counters()["graph_steps"]
     log("batched: %.3f ms/step (%d of %d steps replayed from the step graph)" % (tb / K * 1e3, graph_steps, K))
+    # the same number of steps again with kernel timers: the dominant kernel's roofline
+    _, _, tm = timed_run(bc, K, timers=True)
     bc.close()
+    solver_s = tm["solver_ms"] * 1e-3
+    ach = tm["solver_evals"] * FLOP_EQ_PER_EVAL / solver_s if solver_s > 0 else 0.0
+    roof = {"bound": "valu", "kernel": "k_solve_lm (FP64 Levenberg-Marquardt, main launch)", "achieved": ach / 1e12,
+            "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None,
+            "flop_eq_per_eval": FLOP_EQ_PER_EVAL, "evals_per_step": tm["solver_evals"] / K,
+            "kernel_ms_per_step": tm["solver_ms"] / K}
+    kernels = {k: v["ms"] / K for k, v in tm["kernels"].items() if isinstance(v, dict)}
     ctxs = [c.open(lib) for c in specs]
     for c in ctxs:
         c.step(W)
@@ -205,6 +326,34 @@ def batch_bench(args):
         c.sync()
     ts = time.perf_counter() - t0
     log("one by one: %.3f ms/step" % (ts / K * 1e3))
+    for c in ctxs:
+        c.close()
+    cpu = None
+    if not args.no_cpu:
+        # bounded sample: the first specimens, each alone on the oracle, from their warm state
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import load_oracle
+        orc = load_oracle()
+        threads = cpu_threads()
+        model, ncpu = host_cpu()
+        done_ps, cw, used = 0, 0.0, 0
+        for c in specs:
+            cctx = c.open(orc)
+            orc.check(orc.set_threads(cctx.ptr, threads), cctx.ptr)
+            cctx.step(2)  # initial sort and sweep outside the sample
+            t0 = time.perf_counter()
+            cctx.step(20)
+            cw += time.perf_counter() - t0
+            cctx.close()
+            done_ps += (c.n - 6) * 20
+            used += 1
+            if cw > 10.0:
+                break
+        cpu = {"value": done_ps / cw, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "host": {"model": model, "logical_cpus": ncpu},
+               "sample": "20 steps of each of the first %d specimens alone (%d particles), FP64 oracle (restatement "
+                         "of the reference loop), %d threads, %.1fs wall" % (used, done_ps // 20, threads, cw)}
+        log("cpu baseline: %.0f particle-steps/s" % cpu["value"])
     print(json.dumps({
         "metric": "aggregate particle-steps/s, %d independent specimens in one context" % len(specs),
         "value": nparts * K / tb, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
@@ -216,6 +365,9 @@ def batch_bench(args):
         "batched_over_one_by_one": ts / tb,
         "tuning": tuning, "graph_steps": graph_steps,
         "autotune_trials": [dict(t, ms_per_step=ms * 1e3) for t, ms in trials],
+        "kernels_ms_per_step": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }))
 
 
@@ -310,7 +462,7 @@ def periodic_bench(args):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_lib import load_oracle
         orc = load_oracle()
-        threads = max(1, min(os.cpu_count() or 1, 16))
+        threads = cpu_threads()
         small = cfg.base
         cctx = small.open(orc)
         orc.check(orc.set_threads(cctx.ptr, threads), cctx.ptr)
@@ -341,6 +493,13 @@ def periodic_bench(args):
         "collider_runs": cnt["collider_runs"], "interactions": int(len(depth)), "contacts": int((depth > 0).sum()),
         "solids_fraction": cfg.solids_fraction,
         "phases_ms_per_step": {k: tm[k] / K for k in ("collider_ms", "interactions_ms", "newton_ms", "solver_ms")},
+        "kernels_ms_per_step": {k: v["ms"] / K for k, v in tm["kernels"].items() if isinstance(v, dict)},
+        "roofline": {"bound": "valu", "kernel": "k_solve_lm (FP64 Levenberg-Marquardt, main launch)",
+                     "achieved": tm["solver_evals"] * FLOP_EQ_PER_EVAL / (tm["solver_ms"] * 1e-3) / 1e12,
+                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": tm["solver_evals"] * FLOP_EQ_PER_EVAL / (tm["solver_ms"] * 1e-3) / PEAK_FP64,
+                     "traffic": None, "flop_eq_per_eval": FLOP_EQ_PER_EVAL, "evals_per_step": tm["solver_evals"] / K,
+                     "kernel_ms_per_step": tm["solver_ms"] / K},
         "pair_set_check": check,
         "cpu_baseline": cpu,
     }))
@@ -349,10 +508,11 @@ def periodic_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=200000)
-    ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="steps of each CPU baseline repeat")
+    ap.add_argument("--cpu-repeats", type=int, default=5, help="repeats of the CPU baseline sample (median)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--densify", type=int, default=150, help="iterations of the packing algorithm")
     ap.add_argument("--batch", type=int, default=0, metavar="SPECIMENS",
@@ -445,8 +605,27 @@ def main():
     achieved = tmx["solver_evals"] * FLOP_EQ_PER_EVAL / solver_s if solver_s > 0 else 0.0
     roof = {"bound": "valu", "kernel": "k_solve_nm<float>", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": SOLVER_TRAFFIC_BYTES if args.n == 200000 else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE / WRITE_SIZE of this kernel on this workload, scaled by a copy "
+                              "of known size (profiles/c1_200k_hbm_traffic.md); not measured by this run",
             "flop_eq_per_eval": FLOP_EQ_PER_EVAL, "evals_per_step": tmx["solver_evals"] / K,
             "kernel_ms_per_step": tmx["solver_ms"] / K}
+    km = tmx["kernels"]
+    nb = cfg.n
+    runs = max(1, cmx["collider_runs"])
+    def hbm(name, bytes_per, ms, launches, note):
+        ach = bytes_per / (ms * 1e-3 / max(1, launches)) if ms > 0 else 0.0
+        return {"kernel": name, "bound": "hbm", "algorithmic_bytes_per_launch": bytes_per,
+                "ms_per_launch": ms / max(1, launches), "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": ach / PEAK_HBM, "bytes": note}
+    rooflines = [
+        hbm("k_newton", NEWTON_BYTES_PER_BODY * nb, km["k_newton"]["ms"], km["k_newton"]["launches"],
+            "%d B per body (316 B state + 112 B record)" % NEWTON_BYTES_PER_BODY),
+        hbm("k_bounds", BOUNDS_BYTES_PER_BODY * nb, km["k_bounds"]["ms"], km["k_bounds"]["launches"],
+            "%d B per body per collider run" % BOUNDS_BYTES_PER_BODY),
+        hbm("collider run", 48 * nb + 8 * n_pairs, tmx["collider_ms"], runs,
+            "48 B per body + 8 B per pair per run; time = device events around the whole run, host "
+            "synchronisations included"),
+    ]
     solver64_s = t64["solver_ms"] * 1e-3
     ach64 = t64["solver_evals"] * FLOP_EQ_PER_EVAL / solver64_s if solver64_s > 0 else 0.0
 
@@ -455,18 +634,25 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_lib import load_oracle
         orc = load_oracle()
-        threads = max(1, min(os.cpu_count() or 1, 16))
+        threads = cpu_threads()
+        model, ncpu = host_cpu()
         cctx = Context(orc, cfg.params(SDM_FP64), cfg.n, [m._native() for m in cfg.materials])
-        cctx.restore(snap)
         orc.check(orc.set_threads(cctx.ptr, threads), cctx.ptr)
-        t0 = time.perf_counter()
-        cctx.step(args.cpu_steps)
-        cw = time.perf_counter() - t0
+        walls = []
+        for _ in range(args.cpu_repeats):  # each repeat from the same snapshot
+            cctx.restore(snap)
+            t0 = time.perf_counter()
+            cctx.step(args.cpu_steps)
+            walls.append(time.perf_counter() - t0)
         cctx.close()
+        cw = float(np.median(walls))
         cpu = {"value": N * args.cpu_steps / cw, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": "%d steps of the same 200k-particle snapshot, FP64 oracle (restatement of the "
-                         "reference loop), OpenMP %d threads, %.1fs wall" % (args.cpu_steps, threads, cw)}
-        log("cpu baseline: %.0f particle-steps/s" % cpu["value"])
+               "host": {"model": model, "logical_cpus": ncpu},
+               "sample": "median of %d repeats of %d steps from the same 200k-particle snapshot (%s s wall each), "
+                         "FP64 oracle (restatement of the reference loop, OpenMP per-thread force buffers as the "
+                         "reference's), %d threads" % (args.cpu_repeats, args.cpu_steps,
+                                                       ", ".join("%.2f" % w for w in walls), threads)}
+        log("cpu baseline: %.0f particle-steps/s (%s, %d threads)" % (cpu["value"], model, threads))
 
     out = {
         "metric": "particle-steps/s at 200k superellipsoids: FP32/FP64 mix vs all-FP64 GPU",
@@ -490,6 +676,10 @@ def main():
         "graph_steps": {"fp64": c64["graph_steps"], "mixed": cmx["graph_steps"]},
         "contact_evals_per_s": cmx["contact_evals"] / wmx,
         "evaluated_pairs_per_s": tmx["queued"] / K / (wmx / K),
+        "evals_note": "contact_evals_per_s counts the Ig2 invocations the reference makes (every interaction "
+                      "with a bounding-sphere overlap or a stored geometry, including those the step skips "
+                      "because their outcome is provably unchanged); evaluated_pairs_per_s counts the pairs "
+                      "whose first vertex the step actually evaluates; dist_evals_per_s the distance evaluations",
         "dist_evals_per_s": cmx["dist_evals"] / wmx,
         "interactions": n_pairs,
         "contacts": n_contacts,
@@ -500,6 +690,7 @@ def main():
                                 for name, tm in (("fp64", t64), ("mixed", tmx))},
         "mixed_error_vs_fp64": mixed_error(one_step["fp64"], one_step["mixed"]),
         "roofline": roof,
+        "rooflines_hbm": rooflines,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

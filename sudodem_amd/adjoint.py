@@ -118,11 +118,18 @@ class FrozenProblem:
         }
 
 
+def ctx_dtype(t: torch.Tensor):
+    return t.dtype if t.is_floating_point() else torch.float64
+
+
 class FrozenRollout(torch.autograd.Function):
-    """loss = FrozenRollout.apply(params[3], v0[n, 3], problem) with params = (kn, ks, mu)."""
+    """loss = FrozenRollout.apply(params[3], v0[n, 3], problem) with params = (kn, ks, mu).
+    params and v0 may be float32 or float64: the kernels work in double, the loss is double and
+    each gradient comes back in the dtype of its input."""
 
     @staticmethod
     def forward(ctx, params, v0, problem):
+        params_in, v0_in = params, v0
         lib = nat.load_hip()
         L = _bind(lib)
         sp = ctypes.byref(problem.struct)
@@ -135,6 +142,7 @@ class FrozenRollout(torch.autograd.Function):
         lib.check(L.sdm_frozen_forward(sp, params.data_ptr(), v0.data_ptr(), problem.w0.data_ptr(), ck.data_ptr(),
                                        work.data_ptr(), ctypes.byref(loss)))
         ctx.problem, ctx.ck, ctx.work, ctx.params = problem, ck, work, params
+        ctx.dtypes = (ctx_dtype(params_in), ctx_dtype(v0_in))
         return torch.tensor(loss.value, dtype=torch.float64, device=v0.device)
 
     @staticmethod
@@ -148,7 +156,9 @@ class FrozenRollout(torch.autograd.Function):
         lib.check(L.sdm_frozen_backward(ctypes.byref(problem.struct), ctx.params.data_ptr(), ctx.ck.data_ptr(),
                                         ctx.work.data_ptr(), gp, gv.data_ptr()))
         g = torch.tensor(list(gp), dtype=torch.float64, device=gv.device)
-        return grad_loss * g, grad_loss * gv, None
+        # gradients in the dtypes of the inputs (float32 parameters and velocities are taken in
+        # double by the kernels)
+        return (grad_loss * g).to(ctx.dtypes[0]), (grad_loss * gv).to(ctx.dtypes[1]), None
 
 
 def rollout(problem: FrozenProblem, params: torch.Tensor, v0: torch.Tensor) -> torch.Tensor:

@@ -629,3 +629,48 @@ def config4(n_specimens=256, seed_extra=0, n_min=2000, n_max=50000):
     for sp in specs:  # one verletDist for all: 0.1 x the mean r_max of the whole batch
         sp.engines[1].verletDist = 0.1 * float(np.mean(np.concatenate([c.arrays["r_max"][6:] for c in specs])))
     return specs
+
+
+TOP_WALL = 5  # the z-max wall of _box_walls (sense -1)
+
+
+def config3(lib, n_particles=20000, box_xy=40e-3, wall_speed=0.05, settle_steps=20000, goal=2e4, log=None):
+    """Differentiation workload: n_particles superellipsoids of the plain workload's size and
+    shape distributions and linear law (Kn 1e5, Ks 7e4, mu 0.5, the walls too) on the
+    lattice of config1 (box_xy wide), settled in the box by the wall servo (CompressionEngine,
+    `goal` Pa on every wall, max_vel 1 m/s; gravity, local damping 0.3, dt 1e-6 s) for
+    `settle_steps` steps, so that the packing is jammed and every wall carries load; the servo
+    is then switched off and the top wall given the downward velocity `wall_speed` (Newton
+    translates a blocked body with its velocity).  Returns the open engine context at that
+    state (the caller freezes its contacts and closes it) and the Config."""
+    cfg = config1(n_particles=n_particles, box_xy=box_xy)
+    p_mat = SuperquadricsMat(label="mat1", Kn=1e5, Ks=7e4, frictionAngle=math.atan(0.5), density=2650.0)
+    # the walls have the particles' friction, so that one (kn, ks, mu) describes every contact
+    # of the frozen window, as the rollout's parameters do
+    w_mat = SuperquadricsMat(label="wallmat", Kn=1e5, Ks=7e4, frictionAngle=math.atan(0.5), density=2650.0)
+    cfg.materials = [p_mat, w_mat]
+    cfg.engines = linear_engines(0.1 * float(np.mean(cfg.arrays["r_max"][6:])))
+    A = cfg.arrays
+    part = A["kind"] == nat.SDM_BODY_SUPERQUADRIC
+    R = rot_matrices(A["ori"][part])
+    for axis in range(3):  # walls on the lattice's outermost surfaces
+        e = np.zeros((int(part.sum()), 3))
+        e[:, axis] = 1.0
+        A["pos"][2 * axis, axis] = float((A["pos"][part, axis] - support_h(A["rxyz"][part], A["eps"][part], R, -e)).min())
+        A["pos"][2 * axis + 1, axis] = float((A["pos"][part, axis] + support_h(A["rxyz"][part], A["eps"][part], R, e)).max())
+    ctx = cfg.open(lib)
+    ctx.set_servo(goal=(goal, goal, goal), max_vel=1.0, gain_alpha=0.5, iterate_num_max=100, solve_num_max=2000,
+                  echo_interval=1000)
+    done = 0
+    while done < settle_steps:
+        k = min(5000, settle_steps - done)
+        ctx.step(k)
+        done += k
+        if log:
+            st = ctx.servo_state()
+            log("settle %d: wall stresses %s" % (done, np.round(st["wall_stress"]).tolist()))
+    ctx.set_servo(False)
+    vel = ctx.get("state.vel").reshape(-1, 3)
+    vel[TOP_WALL] = (0.0, 0.0, -wall_speed)
+    ctx.set("state.vel", vel)
+    return ctx, cfg

@@ -15,8 +15,12 @@ def _sgn(a):
     return np.sign(a)
 
 
-def rollout_jvp(A, params, v0, steps, loss_steps, dt, damping, gravity, dparams=(0.0, 0.0, 0.0), dv0=None):
-    """Returns (loss, dloss) along the direction (dparams, dv0)."""
+def rollout_jvp(A, params, v0, steps, loss_steps, dt, damping, gravity, dparams=(0.0, 0.0, 0.0), dv0=None,
+                stats=None):
+    """Returns (loss, dloss) along the direction (dparams, dv0).  `stats` (a dict), when given,
+    receives the per-contact normal and shear forces of the first step ("fn1", "fs1"), the
+    number of contact-steps on the Coulomb cap ("capped") and of Cundall factors other than 1
+    ("damped")."""
     kn, ks, mu = params
     dkn, dks, dmu = dparams
     n, m = len(A["mass"]), len(A["depth0"])
@@ -35,6 +39,10 @@ def rollout_jvp(A, params, v0, steps, loss_steps, dt, damping, gravity, dparams=
     g = np.asarray(gravity, float)
     loss = dloss = 0.0
     inl = A["in_loss"].astype(bool)
+    ent = A["csr_ent"].astype(np.int64)
+    ce, side = ent >> 1, ent & 1
+    owner = np.repeat(np.arange(n), np.diff(A["csr_off"]))
+    arm = np.where(side[:, None] == 0, ra[ce], rb[ce])
     for t in range(steps):
         ua, ub = x[a_] - A["pos0"][a_], x[b_] - A["pos0"][b_]
         dua, dub = dx[a_], dx[b_]
@@ -61,6 +69,11 @@ def rollout_jvp(A, params, v0, steps, loss_steps, dt, damping, gravity, dparams=
             drho = (dmu * fnm[cap] + mu * dfnm[cap]) / ft[cap] - mu * fnm[cap] * dft / ft[cap] ** 2
             fsn[cap] = rho[:, None] * ftr[cap]
             dfsn[cap] = drho[:, None] * ftr[cap] + rho[:, None] * dftr[cap]
+        if stats is not None:
+            if t == 0:
+                stats["fn1"] = np.where(act[:, None], fnm[:, None] * nrm, 0.0)
+                stats["fs1"] = np.where(act[:, None], fsn, 0.0)
+            stats["capped"] = stats.get("capped", 0) + int(cap.sum())
         F = -(fnm[:, None] * nrm) - fsn
         dF = -(dfnm[:, None] * nrm) - dfsn
         F[~act] = 0
@@ -70,30 +83,24 @@ def rollout_jvp(A, params, v0, steps, loss_steps, dt, damping, gravity, dparams=
         if t >= steps - loss_steps:
             loss += fnm[act & inl].sum() / loss_steps
             dloss += dfnm[act & inl].sum() / loss_steps
-        # forces and torques per body, in contact order per body
+        # forces and torques per body, in contact order per body: np.add.at adds the entries in
+        # array order, and the entries of one body are contiguous in list order
         Fb = np.zeros((n, 3))
         dFb = np.zeros((n, 3))
         Tb = np.zeros((n, 3))
         dTb = np.zeros((n, 3))
-        for i in range(n):
-            for e in range(A["csr_off"][i], A["csr_off"][i + 1]):
-                ent = int(A["csr_ent"][e])
-                c = ent >> 1
-                if ent & 1 == 0:
-                    Fb[i] += F[c]
-                    dFb[i] += dF[c]
-                    Tb[i] += np.cross(ra[c], F[c])
-                    dTb[i] += np.cross(ra[c], dF[c])
-                else:
-                    Fb[i] -= F[c]
-                    dFb[i] -= dF[c]
-                    Tb[i] -= np.cross(rb[c], F[c])
-                    dTb[i] -= np.cross(rb[c], dF[c])
+        sgn_e = np.where(side == 0, 1.0, -1.0)[:, None]
+        np.add.at(Fb, owner, sgn_e * F[ce])
+        np.add.at(dFb, owner, sgn_e * dF[ce])
+        np.add.at(Tb, owner, sgn_e * np.cross(arm, F[ce]))
+        np.add.at(dTb, owner, sgn_e * np.cross(arm, dF[ce]))
         m_ = A["mass"][:, None]
         acc = Fb / np.where(m_ > 0, m_, 1.0) + g
         dacc = dFb / np.where(m_ > 0, m_, 1.0)
         ca = 1 - damping * _sgn(acc * (v + 0.5 * dt * acc))
         cT = 1 - damping * _sgn(Tb * w)
+        if stats is not None:
+            stats["damped"] = stats.get("damped", 0) + int(((ca != 1) & dyn[:, None]).sum() + ((cT != 1) & dyn[:, None]).sum())
         vn = np.where(dyn[:, None], v + dt * ca * acc, v)
         dvn = np.where(dyn[:, None], dv + dt * ca * dacc, 0.0)
         wn = np.where(dyn[:, None], w + dt * np.einsum("nij,nj->ni", invJ, cT * Tb), w)
