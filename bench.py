@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 # HBM bytes per launch of k_solve_nm<float> on this workload: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (median over the timed steps), scaled by the same counters on a 1 GiB copy
 # (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); profiles/c1_200k_hbm_traffic.md.
-SOLVER_TRAFFIC_BYTES = (221226 * 2.0 + 166754 * 1.0) * 1024
+SOLVER_TRAFFIC_BYTES = (229236 * 2.0 + 99257 * 1.0) * 1024
 FLOP_EQ_PER_EVAL = 392  # pow=8, sin/cos=4, atan2=16, div=4, add/mul=1 over one Disfun evaluation
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
