@@ -2131,7 +2131,8 @@ const Field kFields[] = {
     {"collider.strideActive", S_SCALAR, SDM_I32, 1, 2, true},
     {"collider.fastestBodyMaxDist", S_SCALAR, SDM_F64, 1, 3, true},
     {"newton.maxVelocitySq", S_SCALAR, SDM_F64, 1, 4, true}, {"collider.numReinit", S_SCALAR, SDM_I32, 1, 5, true},
-    {"collider.bounds", S_BOUNDS, SDM_I32, 1, 0, true},
+    {"collider.bounds", S_BOUNDS, SDM_I32, 1, 0, true}, {"collider.periods", S_BOUNDS, SDM_I32, 1, 1, true},
+    {"collider.loIdx", S_SCALAR, SDM_I64, 3, 9, true},
     {"scene.dt", S_SCALAR, SDM_F64, 1, 6, true}, {"newton.damping", S_SCALAR, SDM_F64, 1, 7, true},
     {"newton.gravity", S_SCALAR, SDM_F64, 3, 8, true},
 };
@@ -2282,12 +2283,16 @@ int sdmcpu_get(sdm_ctx* c, const char* name, void* dst, int64_t count) {
       case 6: *(double*)dst = c->prm.dt; break;
       case 7: *(double*)dst = c->prm.damping; break;
       case 8: std::memcpy(dst, c->prm.gravity, 3 * sizeof(double)); break;
+      case 9:
+        for (int a = 0; a < 3; a++) ((int64_t*)dst)[a] = c->loIdx[a];
+        break;
     }
   } else {
     int32_t* o = (int32_t*)dst;
     size_t m = c->BB[0].size();
     for (int a = 0; a < 3; a++)
-      for (size_t i = 0; i < m; i++) o[a * m + i] = c->BB[a][i].id * 2 + (c->BB[a][i].isMin ? 0 : 1);
+      for (size_t i = 0; i < m; i++)
+        o[a * m + i] = f->off == 0 ? c->BB[a][i].id * 2 + (c->BB[a][i].isMin ? 0 : 1) : c->BB[a][i].period;
   }
   return 0;
 }
@@ -2305,6 +2310,15 @@ int sdmcpu_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
     c->found_dirty = true;
   }
   int64_t n = field_count(c, f);
+  if (f->scope == S_BOUNDS && f->off == 1) {
+    // the periods of the current bound vectors
+    if (count != (int64_t)(3 * c->BB[0].size())) return fail(c, "collider.periods: count must be 3 x the bound vector");
+    const int32_t* h = (const int32_t*)src;
+    const size_t m = c->BB[0].size();
+    for (int a = 0; a < 3; a++)
+      for (size_t i = 0; i < m; i++) c->BB[a][i].period = h[a * m + i];
+    return 0;
+  }
   if (f->scope == S_BOUNDS) {
     if (count % 3) return fail(c, "collider.bounds: count must be 3*2N");
     size_t m = (size_t)(count / 3);
@@ -2337,6 +2351,13 @@ int sdmcpu_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
       case 6: c->prm.dt = *(const double*)src; break;
       case 7: c->prm.damping = *(const double*)src; break;
       case 8: std::memcpy(c->prm.gravity, src, 3 * sizeof(double)); break;
+      case 9:
+        for (int a = 0; a < 3; a++) {
+          const int64_t v = ((const int64_t*)src)[a];
+          if (v < 0 || (v > 0 && v >= (int64_t)c->BB[a].size())) return fail(c, "collider.loIdx: outside the bound vector");
+          c->loIdx[a] = v;
+        }
+        break;
     }
   }
   return 0;

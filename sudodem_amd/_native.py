@@ -316,7 +316,7 @@ class Context:
     def snapshot(self):
         """Complete state (bodies, interactions, collider and integrator flags) as arrays."""
         return {name: self.get(name) for name in self.lib.fields()
-                if not (self.info(name)[1] == 0 and name == "collider.bounds")}
+                if not (self.info(name)[1] == 0 and name in ("collider.bounds", "collider.periods"))}
 
     def restore(self, snap):
         """Load a snapshot taken from any implementation of the ABI."""

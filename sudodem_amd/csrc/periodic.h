@@ -1,26 +1,18 @@
-// periodic.h -- broadphase of a periodic cell.
+// periodic.h -- broadphase helpers of a periodic cell.
 //
 // In a periodic scene InsertionSortCollider keeps each axis as a ring of bounds with a
 // period per bound and insertion-sorts the ring (insertionSortPeri,
-// pkg/common/InsertionSortCollider.cpp:416-463); a pair is created when the boxes overlap
-// in the sense of spatialOverlapPeri (:491-543), which also fixes the pair's cellDist.
-// Between collider runs the container only loses pairs at the collider's own erase step,
-// so after every run it holds exactly the real interactions plus every pair whose boxes
-// overlap now.  This file finds that set without the ring: a cell list over the periodic
-// cell (each body listed in every cell its box touches, sorted by cell), where every pair
-// sharing cells is tested once, in the first cell of the cyclic intersection of the two
-// cell ranges, with spatialOverlapPeri's own arithmetic.
-//
-// Orientation and order of the new pairs:
-//  * first run (std::sort + sweep, :336-397): body A scans, along the sort axis, the lower
-//    bounds that follow its own lower bound in the wrapped ring until its upper bound; the
-//    pair is (A, B) and is inserted in the order (ring rank of A's lower bound, distance
-//    to B's lower bound).  Both come from the ranks of the sorted wrapped bounds here.
-//  * later runs: a pair is created by a lower bound crossing an upper bound, oriented
-//    (owner of the lower bound, other); the crossing axis is the first axis on which the
-//    boxes were apart at the previous run, and the crossed lower bound is the one that now
-//    lies inside the other box.  These pairs are appended in (axis, id1, id2) order, which
-//    is not the ring traversal order of insertionSortPeri.
+// pkg/common/InsertionSortCollider.cpp:416-463; ring.h and ring_dev.h walk it on the
+// device); a pair is created when the boxes overlap in the sense of spatialOverlapPeri
+// (:491-543), which also fixes the pair's cellDist.  This file holds that test, cellWrap,
+// the erase flags, and the first run's sweep (:336-397), found here without the sweep: a
+// cell list over the periodic cell (each body listed in every cell its box touches, sorted
+// by cell), where every pair sharing cells is tested once, in the first cell of the cyclic
+// intersection of the two cell ranges, with spatialOverlapPeri's own arithmetic.  Body A
+// scans, along the sort axis, the lower bounds that follow its own lower bound in the wrapped
+// ring until its upper bound; the pair is (A, B) and is inserted in the order (ring rank of
+// A's lower bound, distance to B's lower bound), both from the ranks of the sorted wrapped
+// bounds.
 #pragma once
 #include "collider.h"
 
@@ -140,26 +132,18 @@ __device__ __forceinline__ void screen_box(double m, double M, double L, float& 
 
 // Cell range of every body: start cell s[k] and count c[k] per axis (c = n: the whole
 // axis), and the number of cells the box touches.  Also the boxes of the pair screen,
-// box[4i .. 4i+3] = centre and half widths now, then at the previous collider run (`prev`
-// set; a copy of the current ones otherwise); box[4i].w is 1 for a wide box.
-__global__ void k_peri_ranges(BodyArrays B, PeriCell pc, const double* pmin, const double* pmax, int prev,
-                              int32_t* rng, uint32_t* cnt, float4* box) {
+// box[2i], box[2i+1] = centre and half widths; box[2i].w is 1 for a wide box.
+__global__ void k_peri_ranges(BodyArrays B, PeriCell pc, int32_t* rng, uint32_t* cnt, float4* box) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t N = B.n;
   if (i >= N) return;
   uint32_t total = 1;
-  float cen[3], hw[3], pcen[3], phw[3];
+  float cen[3], hw[3];
   bool wide = false;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     const int n = pc.n[k];
     screen_box(B.bmin[k * N + i], B.bmax[k * N + i], pc.L[k], cen[k], hw[k], wide);
-    if (prev) {
-      screen_box(pmin[k * N + i], pmax[k * N + i], pc.L[k], pcen[k], phw[k], wide);
-    } else {
-      pcen[k] = cen[k];
-      phw[k] = hw[k];
-    }
     const double lo = B.bmin[k * N + i] - pc.eps;
     const double w = (B.bmax[k * N + i] + pc.eps) - lo;
     int per;
@@ -178,10 +162,8 @@ __global__ void k_peri_ranges(BodyArrays B, PeriCell pc, const double* pmin, con
     total *= (uint32_t)c;
   }
   cnt[i] = total;
-  box[4 * i] = make_float4(cen[0], cen[1], cen[2], wide ? 1.0f : 0.0f);
-  box[4 * i + 1] = make_float4(hw[0], hw[1], hw[2], 0.0f);
-  box[4 * i + 2] = make_float4(pcen[0], pcen[1], pcen[2], 0.0f);
-  box[4 * i + 3] = make_float4(phw[0], phw[1], phw[2], 0.0f);
+  box[2 * i] = make_float4(cen[0], cen[1], cen[2], wide ? 1.0f : 0.0f);
+  box[2 * i + 1] = make_float4(hw[0], hw[1], hw[2], 0.0f);
 }
 
 // Entries (cell, body) of every body at its offset.
@@ -231,88 +213,33 @@ __device__ __forceinline__ int first_shared(int sa, int ca, int sb, int cb, int 
 }
 
 struct PeriOrder {
-  int init;                  // first run: orientation and order of the sorted-ring sweep
-  const uint32_t* rankMin;   // init: ring rank of each body's lower / upper bound on the sort axis
+  const uint32_t* rankMin;  // ring rank of each body's lower / upper bound on the sort axis
   const uint32_t* rankMax;
   uint32_t n2;
-  const double* pmin;        // later runs: bounds of the previous run, [3][N]
-  const double* pmax;
 };
 
-// Membership in the container as it was before this run's erase step: with `cover` through
-// the per-body lists of the last run (a scan of the shorter of the two bodies' lists),
-// otherwise through a hash of all its pairs.
-struct PeriFound {
-  int cover;
-  const int32_t* off;   // per-body list offsets [N + 1]
-  const int32_t* csr;   // list entries (interaction << 2 | ...)
-  const int32_t* cid;   // endpoints of every interaction
-  const uint64_t* tab;  // hash of upair keys (without cover)
-  uint64_t mask;
-};
-
-__device__ __forceinline__ bool peri_in_container(const PeriFound& F, int a, int b) {
-  if (!F.cover) return hash_contains(F.tab, F.mask, upair(a, b));
-  int x = a, y = b;
-  if (F.off[b + 1] - F.off[b] < F.off[a + 1] - F.off[a]) {
-    x = b;
-    y = a;
-  }
-  const int e1 = F.off[x + 1];
-  for (int e = F.off[x]; e < e1; e++) {
-    const int2 ab = reinterpret_cast<const int2*>(F.cid)[F.csr[e] >> 2];
-    if (ab.x == y || ab.y == y) return true;
-  }
-  return false;
-}
-
-// A pair that survived the screens: orientation and order key, then spatialOverlapPeri's
-// exact test with (id1, id2) as the reference evaluates it, then (for an unscreened pair) the
-// container lookup; a new overlapping pair becomes a candidate.
+// A pair that survived the screen: the sweep's orientation and order key, then
+// spatialOverlapPeri's exact test with (id1, id2) as the reference evaluates it, then the
+// container lookup (hash of its unordered pairs); a new overlapping pair becomes a candidate.
 __device__ __forceinline__ void peri_candidate(const BodyArrays& B, const PeriCell& pc, const PeriOrder& po, int a, int b,
-                                               bool screened, const PeriFound& F, const Cands& C,
+                                               const uint64_t* tab, uint64_t mask, const Cands& C,
                                                unsigned long long* err) {
-  const int64_t N = B.n;
+  const uint32_t n2 = po.n2;
+  const uint32_t dAB = (po.rankMin[b] + n2 - po.rankMin[a]) % n2, dAA = (po.rankMax[a] + n2 - po.rankMin[a]) % n2;
+  const uint32_t dBA = (po.rankMin[a] + n2 - po.rankMin[b]) % n2, dBB = (po.rankMax[b] + n2 - po.rankMin[b]) % n2;
+  const bool aScans = dAB < dAA, bScans = dBA < dBB;
+  if (!aScans && !bScans) return;  // never met in the sweep
+  const uint64_t kA = ((uint64_t)po.rankMin[a] << 32) | dAB, kB = ((uint64_t)po.rankMin[b] << 32) | dBA;
   int id1, id2;
   uint64_t okey;
-  if (po.init) {
-    const uint32_t n2 = po.n2;
-    const uint32_t dAB = (po.rankMin[b] + n2 - po.rankMin[a]) % n2, dAA = (po.rankMax[a] + n2 - po.rankMin[a]) % n2;
-    const uint32_t dBA = (po.rankMin[a] + n2 - po.rankMin[b]) % n2, dBB = (po.rankMax[b] + n2 - po.rankMin[b]) % n2;
-    const bool aScans = dAB < dAA, bScans = dBA < dBB;
-    if (!aScans && !bScans) return;  // never met in the sweep
-    const uint64_t kA = ((uint64_t)po.rankMin[a] << 32) | dAB, kB = ((uint64_t)po.rankMin[b] << 32) | dBA;
-    if (aScans && (!bScans || kA < kB)) {
-      id1 = a; id2 = b; okey = kA;
-    } else {
-      id1 = b; id2 = a; okey = kB;
-    }
+  if (aScans && (!bScans || kA < kB)) {
+    id1 = a; id2 = b; okey = kA;
   } else {
-    int axis = 3;
-    id1 = a < b ? a : b;
-    id2 = a < b ? b : a;
-    for (int k = 0; k < 3 && axis == 3; k++) {
-      int p;
-      const int was = axis_overlap_peri(po.pmin[k * N + a], po.pmax[k * N + a], po.pmin[k * N + b],
-                                        po.pmax[k * N + b], pc.L[k], p);
-      if (was != 0) continue;
-      const int now = axis_overlap_peri(B.bmin[k * N + a], B.bmax[k * N + a], B.bmin[k * N + b],
-                                        B.bmax[k * N + b], pc.L[k], p);
-      if (now != 1) continue;
-      axis = k;
-      const double mA = B.bmin[k * N + a], MA = B.bmax[k * N + a], mB = B.bmin[k * N + b], MB = B.bmax[k * N + b];
-      const bool bInA = cell_wrap_rel(mB, mA, mA + pc.L[k]) <= MA - mA;
-      const bool aInB = cell_wrap_rel(mA, mB, mB + pc.L[k]) <= MB - mB;
-      if (bInA != aInB) {
-        id1 = bInA ? b : a;
-        id2 = bInA ? a : b;
-      }
-    }
-    okey = ((uint64_t)axis << 62) | ((uint64_t)(uint32_t)id1 << 31) | (uint64_t)(uint32_t)id2;
+    id1 = b; id2 = a; okey = kB;
   }
   int32_t per[3];
   if (!spatial_overlap_peri(B, pc, id1, id2, per, err)) return;
-  if (!screened && peri_in_container(F, id1, id2)) return;
+  if (hash_contains(tab, mask, upair(id1, id2))) return;
   const unsigned long long slot = atomicAdd(C.count, 1ull);
   if ((int64_t)slot < C.cap) {
     C.a[slot] = id1;
@@ -335,18 +262,14 @@ __device__ __forceinline__ void peri_candidate(const BodyArrays& B, const PeriCe
 // pair is met once and all lanes loop equally long.  A pair of narrow boxes is screened in
 // single precision (minimum-image distance of the centres against the half widths plus a
 // margin far above the screen's rounding, so no pair the exact test accepts is dropped).
-// With F.cover set, the container holds every pair whose boxes overlapped at the previous
-// collider run (that run appended all of them, and pairs leave the container only at the
-// collider's erase), so a pair that certainly overlapped then (screen with the margin taken
-// off) is present.  The remaining narrow pairs are looked up in the container; only new
-// ones reach the exact test.
 __global__ __launch_bounds__(64) void k_peri_cell_pairs(BodyArrays B, PeriCell pc, PeriOrder po, const int32_t* rng,
                                                         const float4* box, const uint32_t* val,
-                                                        const uint32_t* runStart, const uint32_t* runEnd, PeriFound F,
-                                                        Cands C, unsigned long long* err) {
+                                                        const uint32_t* runStart, const uint32_t* runEnd,
+                                                        const uint64_t* tab, uint64_t mask, Cands C,
+                                                        unsigned long long* err) {
   __shared__ int32_t sId[SDM_PERI_TILE];
   __shared__ uint32_t sRg[3][SDM_PERI_TILE];
-  __shared__ float4 sBox[4][SDM_PERI_TILE];
+  __shared__ float4 sBox[2][SDM_PERI_TILE];
   const uint32_t cell = blockIdx.x;
   const uint32_t s0 = runStart[cell];
   const int kc = (int)(runEnd[cell] - s0);
@@ -360,7 +283,7 @@ __global__ __launch_bounds__(64) void k_peri_cell_pairs(BodyArrays B, PeriCell p
 #pragma unroll
     for (int k = 0; k < 3; k++) sRg[k][j] = (uint32_t)rng[6 * a + k] | ((uint32_t)rng[6 * a + 3 + k] << 16);
 #pragma unroll
-    for (int q = 0; q < 4; q++) sBox[q][j] = box[4 * a + q];
+    for (int q = 0; q < 2; q++) sBox[q][j] = box[2 * a + q];
   }
   __syncthreads();
   auto ent_id = [&](int j) -> int { return j < SDM_PERI_TILE ? sId[j] : (int)val[s0 + j]; };
@@ -369,7 +292,7 @@ __global__ __launch_bounds__(64) void k_peri_cell_pairs(BodyArrays B, PeriCell p
     const int a = (int)val[s0 + j];
     return (uint32_t)rng[6 * a + k] | ((uint32_t)rng[6 * a + 3 + k] << 16);
   };
-  auto ent_box = [&](int j, int q) -> float4 { return j < SDM_PERI_TILE ? sBox[q][j] : box[4 * (int)val[s0 + j] + q]; };
+  auto ent_box = [&](int j, int q) -> float4 { return j < SDM_PERI_TILE ? sBox[q][j] : box[2 * (int)val[s0 + j] + q]; };
   for (int i = threadIdx.x; i < kc; i += blockDim.x) {
     const int a = ent_id(i);
     uint32_t ra[3];
@@ -404,31 +327,10 @@ __global__ __launch_bounds__(64) void k_peri_cell_pairs(BodyArrays B, PeriCell p
           apart = apart || fabsf(dk) > h[k] + pc.mf[k];
         }
         if (apart) continue;
-        if (F.cover) {
-          const float4 pa = ent_box(i, 2), pha = ent_box(i, 3), pb = ent_box(j, 2), phb = ent_box(j, 3);
-          const float e[3] = {pa.x - pb.x, pa.y - pb.y, pa.z - pb.z};
-          const float g[3] = {pha.x + phb.x, pha.y + phb.y, pha.z + phb.z};
-          bool was = true;
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            const float dk = e[k] - pc.Lf[k] * rintf(e[k] * pc.iLf[k]);
-            was = was && fabsf(dk) <= g[k] - pc.mf[k];
-          }
-          if (was) continue;
-        }
-        if (peri_in_container(F, a, b)) continue;
       }
-      peri_candidate(B, pc, po, a, b, screened, F, C, err);
+      peri_candidate(B, pc, po, a, b, tab, mask, C, err);
     }
   }
-}
-
-// The periodic candidates are distinct: all of them, with their order keys and indices.
-__global__ void k_cand_all(const Cands C, int64_t nc, uint64_t* okey, uint32_t* oidx) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  okey[c] = C.key[c];
-  oidx[c] = (uint32_t)c;
 }
 
 // Sort keys of the wrapped bounds on the sort axis (first run: coord = cellWrap(bound, 0, L)).

@@ -23,6 +23,7 @@
 #include "observe.h"
 #include "periodic.h"
 #include "radix.h"
+#include "ring_dev.h"
 #include "servo.h"
 
 #define SDM_VERSION "sdm-hip 1.0 (gfx950)"
@@ -165,14 +166,22 @@ struct sdm_ctx {
   uint32_t* stage = nullptr;  // attribute staging buffer (32-bit words)
   int64_t stageCap = 0;
   // periodic cell broadphase (periodic.h)
-  double *prevMin = nullptr, *prevMax = nullptr;  // bounds of the previous collider run
   int32_t* cellRng = nullptr;                     // per body: start cell and count per axis
   uint32_t *rankMin = nullptr, *rankMax = nullptr;
   uint32_t *peCnt = nullptr, *peOff = nullptr;    // cells per body and their exclusive scan
   uint32_t* runEnd = nullptr;                     // [runCap] end, then [runCap] start of each cell's entries
   int64_t runCap = 0;
-  float4* pbox = nullptr;                          // [4N] single-precision boxes of the pair screen
-  bool periCover = false;  // the container holds every pair overlapping at the last collider run's bounds
+  float4* pbox = nullptr;                          // [2N] single-precision boxes of the pair screen
+  // the rings of a periodic cell (ring.h): bb[a] holds the handles, rper[a] the periods,
+  // loIdx[a] the ring head (host copy; the run keeps it in rdev while it walks)
+  int32_t* rper[3] = {nullptr, nullptr, nullptr};
+  int64_t loIdx[3] = {0, 0, 0};
+  double* rx[3] = {nullptr, nullptr, nullptr};  // the run's coordinates
+  int32_t *rbh[3] = {nullptr, nullptr, nullptr}, *rbp[3] = {nullptr, nullptr, nullptr};  // the rings before the run
+  int32_t *rth = nullptr, *rtp = nullptr;  // window permutation temporaries
+  double* rtx = nullptr;
+  int64_t* rdev = nullptr;  // [0,3) loIdx, [3,6) error bits (u32 in 8-byte slots), window state
+  int64_t ringK[3][4] = {};  // per axis: start / end sequential lengths, half width around loIdx, prefix window
   // per-phase timing
   bool timing = false;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -201,6 +210,10 @@ void alloc_bodies(sdm_ctx* c, int32_t n) {
   SDM_BODY_FIELDS(SDM_ALLOC)
 #undef SDM_ALLOC
   for (int a = 0; a < 3; a++) c->bb[a] = dalloc<int32_t>(2 * (int64_t)n);
+  for (int a = 0; a < 3; a++) {
+    c->rper[a] = dalloc<int32_t>(std::max<int64_t>(1, 2 * (int64_t)n));
+    HIPCHK(hipMemset(c->rper[a], 0, std::max<int64_t>(1, 2 * (int64_t)n) * sizeof(int32_t)));
+  }
   c->bbTmp = dalloc<int32_t>(2 * (int64_t)n);
   for (int a = 0; a < 3; a++) c->qs[a] = dalloc<uint32_t>(2 * (int64_t)n);
   c->smin = dalloc<uint32_t>(2 * (int64_t)n + 1);
@@ -487,10 +500,153 @@ void erase_flagged(sdm_ctx* c) {
   c->cnt.pairs_erased += Etot;
 }
 
+// The ring walk of a later collider run (insertionSortPeri on the three axes, ring.h): the
+// candidates (overlapping pairs of the inversions, in the reference's order) land in C; the
+// rings and loIdx are updated.  A run whose parallel phases fail their conditions is undone
+// and repeated with longer sequential phases.  Returns the number of candidates.
+int64_t ring_run(sdm_ctx* c, const PeriCell& pc) {
+  const int64_t N = c->B.n;
+  const int64_t n2 = 2 * N;
+  if (!c->rx[0]) {
+    for (int a = 0; a < 3; a++) {
+      c->rx[a] = dalloc<double>(n2);
+      c->rbh[a] = dalloc<int32_t>(n2);
+      c->rbp[a] = dalloc<int32_t>(n2);
+    }
+    c->rth = dalloc<int32_t>(n2);
+    c->rtp = dalloc<int32_t>(n2);
+    c->rtx = dalloc<double>(n2);
+    c->rdev = dalloc<int64_t>(32);
+  }
+  for (int a = 0; a < 3; a++) {
+    if (c->ringK[a][0] <= 0) {
+      c->ringK[a][0] = c->ringK[a][1] = 64;
+      c->ringK[a][2] = 32;
+      c->ringK[a][3] = 256;
+    }
+    HIPCHK(hipMemcpyAsync(c->rbh[a], c->bb[a], n2 * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->rbp[a], c->rper[a], n2 * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+  }
+  RingSet R;
+  for (int a = 0; a < 3; a++) {
+    R.h[a] = c->bb[a];
+    R.per[a] = c->rper[a];
+    R.x[a] = c->rx[a];
+    R.L[a] = pc.L[a];
+  }
+  R.n2 = n2;
+  R.lo = c->rdev;
+  R.err = (uint32_t*)(c->rdev + 3);  // three u32 in rdev[3..4]
+  int64_t* winState = c->rdev + 8;   // r0[3], r0base[3], minKey[3] per window
+  ensure_cands(c, std::max<int64_t>(c->C.cap, n2));
+  ensure_scratch(c, n2 + 1);
+  bool grew[3] = {false, false, false};
+  for (int tries = 0;; tries++) {
+    if (tries > 0) {
+      for (int a = 0; a < 3; a++) {
+        HIPCHK(hipMemcpyAsync(c->bb[a], c->rbh[a], n2 * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->rper[a], c->rbp[a], n2 * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+      }
+    }
+    int64_t hdr[5] = {c->loIdx[0], c->loIdx[1], c->loIdx[2], 0, 0};
+    HIPCHK(hipMemcpyAsync(c->rdev, hdr, sizeof(hdr), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemsetAsync(c->dcnt + 2, 0, sizeof(unsigned long long), c->st));
+    c->C.count = c->dcnt + 2;
+    // the five phases per axis
+    RingSpan s0, s1, s2;
+    RingWin w1, w2;
+    for (int a = 0; a < 3; a++) {
+      const int64_t* K = c->ringK[a];
+      const int64_t lo = c->loIdx[a];
+      const int64_t P = std::min(K[0], n2);
+      const int64_t Q = std::max(n2 - K[1], P);
+      const int64_t A = std::min(std::max(lo - K[2], P), Q);
+      const int64_t Bq = std::min(std::max(lo + K[2], A), Q);
+      s0.s[a] = 0; s0.e[a] = P;
+      w1.i0[a] = P; w1.i1[a] = A; w1.ws[a] = std::max<int64_t>(1, P - K[3]);
+      s1.s[a] = A; s1.e[a] = Bq;
+      w2.i0[a] = Bq; w2.i1[a] = Q; w2.ws[a] = std::max<int64_t>(1, Bq - K[3]);
+      s2.s[a] = Q; s2.e[a] = n2;
+    }
+    w1.r0 = winState; w1.r0base = winState + 3; w1.minKey = (uint64_t*)(winState + 6); w1.bit = RING_ERR_WIN1;
+    w2.r0 = winState + 12; w2.r0base = winState + 15; w2.minKey = (uint64_t*)(winState + 18); w2.bit = RING_ERR_WIN2;
+    unsigned long long* perr = c->dcnt + 7;
+    hipLaunchKernelGGL(k_ring_coords, dim3(nblk(3 * n2)), dim3(256), 0, c->st, c->B, R);
+    hipLaunchKernelGGL(k_ring_walk, dim3(3), dim3(64), 0, c->st, c->B, pc, R, s0, c->C, perr);
+    for (int wi = 0; wi < 2; wi++) {
+      const RingWin& w = wi == 0 ? w1 : w2;
+      int64_t tot = 0;
+      for (int a = 0; a < 3; a++) tot += w.i1[a] > w.i0[a] ? w.i1[a] - w.ws[a] : 0;
+      if (tot > 0) {
+        hipLaunchKernelGGL(k_ring_win_init, dim3(1), dim3(64), 0, c->st, R, w);
+        hipLaunchKernelGGL(k_ring_win_scan, dim3(nblk(tot)), dim3(256), 0, c->st, R, w);
+        hipLaunchKernelGGL(k_ring_win_check, dim3(1), dim3(64), 0, c->st, R, w);
+        for (int a = 0; a < 3; a++) {
+          if (w.i1[a] <= w.i0[a]) continue;
+          const int64_t ws = w.ws[a], m = w.i1[a] - ws;
+          hipLaunchKernelGGL(k_ring_keys, dim3(nblk(m)), dim3(256), 0, c->st, R, a, ws, m, w.r0, c->k0, c->v0);
+          sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, m, 64, c->rsScratch, c->st);
+          sdmrs::scan<sdmrs::OpMin, true, true>(c->v0, c->u0, m, c->rsScratch, c->st);
+          hipLaunchKernelGGL(k_ring_cross, dim3(nblk(m)), dim3(256), 0, c->st, c->B, pc, R, a, ws, w.i0[a] - ws, m,
+                             c->v0, c->u0, c->C, perr);
+          hipLaunchKernelGGL(k_ring_gather, dim3(nblk(m)), dim3(256), 0, c->st, R, a, ws, m, c->v0, c->rth, c->rtp,
+                             c->rtx);
+          HIPCHK(hipMemcpyAsync(c->bb[a] + ws, c->rth, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+          HIPCHK(hipMemcpyAsync(c->rper[a] + ws, c->rtp, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+          HIPCHK(hipMemcpyAsync(c->rx[a] + ws, c->rtx, m * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+        }
+      }
+      hipLaunchKernelGGL(k_ring_walk, dim3(3), dim3(64), 0, c->st, c->B, pc, R, wi == 0 ? s1 : s2, c->C, perr);
+    }
+    int64_t back[5];
+    unsigned long long cnt = 0;
+    HIPCHK(hipMemcpyAsync(back, c->rdev, sizeof(back), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&cnt, c->dcnt + 2, sizeof(cnt), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    uint32_t errs[3];
+    std::memcpy(errs, back + 3, sizeof(errs));
+    bool redo = false;
+    for (int a = 0; a < 3; a++) {
+      if (errs[a] & RING_ERR_NONFINITE) throw HipError{"periodic collider: a bound coordinate is not finite"};
+      if (errs[a] & RING_ERR_RUN) throw HipError{"periodic collider: the ring walk did not end"};
+      int64_t* K = c->ringK[a];
+      if (errs[a] & RING_ERR_WIN1) K[0] *= 2;
+      if (errs[a] & RING_ERR_WIN2) {
+        K[2] *= 2;
+        if (w2.i0[a] <= c->loIdx[a] || s1.e[a] <= s1.s[a]) K[1] *= 2;  // no separate zone around loIdx
+      }
+      if (errs[a] & (RING_ERR_WIN1 | RING_ERR_WIN2)) {
+        K[3] *= 2;
+        grew[a] = true;
+        redo = true;
+      }
+    }
+    if ((int64_t)cnt > c->C.cap) {
+      HIPCHK(hipStreamSynchronize(c->st));
+      ensure_cands(c, (int64_t)cnt);
+      redo = true;
+    }
+    if (redo) {
+      if (tries > 60) throw HipError{"periodic collider: the ring walk's phases did not converge"};
+      continue;
+    }
+    for (int a = 0; a < 3; a++) {
+      c->loIdx[a] = back[a];
+      if (!grew[a])  // the phase lengths relax towards their defaults
+        for (int k = 0; k < 4; k++) {
+          const int64_t d = k == 2 ? 32 : (k == 3 ? 256 : 64);
+          c->ringK[a][k] = std::max(d, c->ringK[a][k] - c->ringK[a][k] / 8);
+        }
+    }
+    return (int64_t)cnt;
+  }
+}
+
 // InsertionSortCollider::action in a periodic cell (InsertionSortCollider.cpp:191-400 with
-// its periodic branches): bounds; the cell-list search of periodic.h for every pair that
-// overlaps now and is not in the container; the erase by spatialOverlapPeri (which removes
-// only pairs that do not overlap now, so it may follow the search); the new pairs appended.
+// its periodic branches).  First run: the rings sorted by the wrapped coordinates, the
+// sweep's pairs from the cell list of periodic.h.  Later runs: the erase step by
+// spatialOverlapPeri, then insertionSortPeri on the three rings (ring_run), whose inversions
+// create the new pairs in the reference's order.
 void collider_run_peri(sdm_ctx* c) {
   c->cnt.collider_runs++;
   c->skipValid = false;
@@ -498,26 +654,6 @@ void collider_run_peri(sdm_ctx* c) {
   const int64_t N = c->B.n;
   const int64_t n2 = 2 * N;
   const bool doInitSort = !c->bbValid;
-  if (doInitSort) {
-    // no bound ring is kept in a periodic cell: collider.bounds reads as the body order
-    std::vector<int32_t> h(n2);
-    for (int64_t i = 0; i < n2; i++) h[i] = (int32_t)i;
-    for (int a = 0; a < 3; a++) HIPCHK(hipMemcpy(c->bb[a], h.data(), n2 * sizeof(int32_t), hipMemcpyHostToDevice));
-    c->bbValid = true;
-    c->numReinit++;
-  }
-  if (!c->prevMin) {
-    c->prevMin = dalloc<double>(3 * N);
-    c->prevMax = dalloc<double>(3 * N);
-    c->cellRng = dalloc<int32_t>(6 * N);
-    c->rankMin = dalloc<uint32_t>(N);
-    c->rankMax = dalloc<uint32_t>(N);
-    c->peCnt = dalloc<uint32_t>(N + 1);
-    c->peOff = dalloc<uint32_t>(N + 1);
-    c->pbox = dalloc<float4>(4 * N);
-  }
-  HIPCHK(hipMemcpyAsync(c->prevMin, c->B.bmin, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(c->prevMax, c->B.bmax, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
   KT_START(c, KT_BOUNDS);
   hipLaunchKernelGGL(k_bounds, dim3(nblk(N)), dim3(256), 0, c->st, c->B, c->iter, c->prm.verlet_dist,
                      c->prm.min_sweep_dist_factor, c->prm.target_interv);
@@ -526,7 +662,8 @@ void collider_run_peri(sdm_ctx* c) {
   ensure_scratch(c, std::max<int64_t>({n2 + 1, c->np + 1, 2 * c->np + 1}));
   ensure_moves(c, c->np + 1);
 
-  // box widths (grid size) and the wall check: dcnt[7] error bits, [8] width sum, [9] max
+  // box widths (grid size of the first run) and the wall check: dcnt[7] error bits, [8]
+  // width sum, [9] max
   unsigned long long* err = c->dcnt + 7;
   HIPCHK(hipMemsetAsync(c->dcnt + 7, 0, 3 * sizeof(unsigned long long), c->st));
   hipLaunchKernelGGL(k_width_stats, dim3(nblk(N)), dim3(256), 0, c->st, c->B, (double*)(c->dcnt + 8), c->dcnt + 9,
@@ -560,75 +697,6 @@ void collider_run_peri(sdm_ctx* c) {
   ncell = (int64_t)pc.n[0] * pc.n[1] * pc.n[2];
   pc.eps = 1e-9 * std::max(pc.L[0], std::max(pc.L[1], pc.L[2]));
 
-  // ---- first run: ring ranks of the wrapped bounds on the sort axis ----
-  PeriOrder po;
-  po.init = doInitSort ? 1 : 0;
-  po.rankMin = c->rankMin;
-  po.rankMax = c->rankMax;
-  po.n2 = (uint32_t)n2;
-  po.pmin = c->prevMin;
-  po.pmax = c->prevMax;
-  if (doInitSort) {
-    hipLaunchKernelGGL(k_peri_bound_keys, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, pc, c->prm.sort_axis, n2, c->k0,
-                       c->v0);
-    sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, n2, 64, c->rsScratch, c->st);
-    hipLaunchKernelGGL(k_peri_ranks, dim3(nblk(n2)), dim3(256), 0, c->st, c->v0, n2, c->rankMin, c->rankMax);
-  }
-
-  // ---- cell lists ----
-  // cover: the container and its per-body lists are those the last run left (no host write
-  // since), so they hold every pair that overlapped at that run's bounds
-  const int cover = (c->periCover && !doInitSort && c->csrValid) ? 1 : 0;
-  c->periCover = false;
-  hipLaunchKernelGGL(k_peri_ranges, dim3(nblk(N)), dim3(256), 0, c->st, c->B, pc, c->prevMin, c->prevMax, cover,
-                     c->cellRng, c->peCnt, c->pbox);
-  const int64_t m = exclusive_total(c, c->peCnt, c->peOff, N);
-  ensure_scratch(c, m + 1);
-  uint32_t* ckey = (uint32_t*)c->k0;
-  uint32_t* cval = c->v0;
-  hipLaunchKernelGGL(k_peri_entries, dim3(nblk(N)), dim3(256), 0, c->st, N, pc, c->cellRng, c->peOff, ckey, cval);
-  int bits = 1;
-  while ((1ll << bits) < ncell) bits++;
-  sdmrs::sort_pairs<uint32_t, uint32_t>(ckey, cval, (uint32_t*)c->k1, c->v1, m, bits, c->rsScratch, c->st);
-  if (ncell > c->runCap) {
-    HIPCHK(hipStreamSynchronize(c->st));
-    dfree(c->runEnd);
-    c->runCap = ncell;
-    c->runEnd = dalloc<uint32_t>(2 * ncell);
-  }
-  uint32_t* runStart = c->runEnd + c->runCap;
-  HIPCHK(hipMemsetAsync(c->runEnd, 0, ncell * sizeof(uint32_t), c->st));
-  HIPCHK(hipMemsetAsync(runStart, 0, ncell * sizeof(uint32_t), c->st));
-  hipLaunchKernelGGL(k_cell_ends, dim3(nblk(m)), dim3(256), 0, c->st, ckey, m, runStart, c->runEnd);
-
-  // ---- candidates: overlapping pairs not in the container (before its erase step) ----
-  PeriFound F;
-  F.cover = cover;
-  F.off = c->csrOff;
-  F.csr = c->csrPair;
-  F.cid = c->P.cid;
-  F.tab = nullptr;
-  F.mask = 0;
-  if (!cover) {
-    ensure_hash(c, c->np + 1024);
-    const uint64_t mask = (uint64_t)c->hcap - 1;
-    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->htab, c->hcap, SDM_HEMPTY);
-    if (c->np > 0)
-      hipLaunchKernelGGL(k_hash_insert_pairs, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.cid, c->np, c->htab, mask);
-    F.tab = c->htab;
-    F.mask = mask;
-  }
-  int64_t nc = 0;
-  for (int pass = 0; pass < 2; pass++) {
-    HIPCHK(hipMemsetAsync(c->dcnt + 2, 0, sizeof(unsigned long long), c->st));
-    c->C.count = c->dcnt + 2;
-    hipLaunchKernelGGL(k_peri_cell_pairs, dim3((unsigned)ncell), dim3(64), 0, c->st, c->B, pc, po, c->cellRng,
-                       c->pbox, cval, runStart, c->runEnd, F, c->C, err);
-    nc = (int64_t)read_counter(c, 2);
-    if (nc <= c->C.cap) break;
-    ensure_cands(c, nc);  // rerun with room for all
-  }
-
   // ---- conditionalyEraseNonReal ----
   if (c->np > 0) {
     hipLaunchKernelGGL(k_erase_flags_peri, dim3(nblk(c->np)), dim3(256), 0, c->st, c->B, c->P, c->np, pc, c->u0, err);
@@ -636,20 +704,83 @@ void collider_run_peri(sdm_ctx* c) {
   }
   if ((read_counter(c, 7) & 1) != 0) throw HipError{"body larger than half of the cell size encountered"};
 
+  int64_t nc = 0;
+  if (!doInitSort) {
+    // ---- insertionSortPeri on the three rings ----
+    nc = ring_run(c, pc);
+  } else {
+    // ---- first run: the rings sorted by the wrapped coordinates, loIdx = 0 ----
+    for (int a = 0; a < 3; a++) {
+      hipLaunchKernelGGL(k_peri_bound_keys, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, pc, a, n2, c->k0, c->v0);
+      sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, n2, 64, c->rsScratch, c->st);
+      hipLaunchKernelGGL(k_ring_init, dim3(nblk(n2)), dim3(256), 0, c->st, c->B, pc, a, c->v0, n2, c->bb[a],
+                         c->rper[a]);
+      c->loIdx[a] = 0;
+    }
+    c->bbValid = true;
+    c->numReinit++;
+    if (!c->rankMin) {
+      c->rankMin = dalloc<uint32_t>(N);
+      c->rankMax = dalloc<uint32_t>(N);
+      c->cellRng = dalloc<int32_t>(6 * N);
+      c->peCnt = dalloc<uint32_t>(N + 1);
+      c->peOff = dalloc<uint32_t>(N + 1);
+    }
+    if (!c->pbox) c->pbox = dalloc<float4>(2 * N);
+    PeriOrder po;
+    po.rankMin = c->rankMin;
+    po.rankMax = c->rankMax;
+    po.n2 = (uint32_t)n2;
+    hipLaunchKernelGGL(k_peri_ranks, dim3(nblk(n2)), dim3(256), 0, c->st, (const uint32_t*)c->bb[c->prm.sort_axis],
+                       n2, c->rankMin, c->rankMax);
+
+    // ---- the sweep's pairs from a cell list ----
+    hipLaunchKernelGGL(k_peri_ranges, dim3(nblk(N)), dim3(256), 0, c->st, c->B, pc, c->cellRng, c->peCnt, c->pbox);
+    const int64_t m = exclusive_total(c, c->peCnt, c->peOff, N);
+    ensure_scratch(c, m + 1);
+    uint32_t* ckey = (uint32_t*)c->k0;
+    uint32_t* cval = c->v0;
+    hipLaunchKernelGGL(k_peri_entries, dim3(nblk(N)), dim3(256), 0, c->st, N, pc, c->cellRng, c->peOff, ckey, cval);
+    int bits = 1;
+    while ((1ll << bits) < ncell) bits++;
+    sdmrs::sort_pairs<uint32_t, uint32_t>(ckey, cval, (uint32_t*)c->k1, c->v1, m, bits, c->rsScratch, c->st);
+    if (ncell > c->runCap) {
+      HIPCHK(hipStreamSynchronize(c->st));
+      dfree(c->runEnd);
+      c->runCap = ncell;
+      c->runEnd = dalloc<uint32_t>(2 * ncell);
+    }
+    uint32_t* runStart = c->runEnd + c->runCap;
+    HIPCHK(hipMemsetAsync(c->runEnd, 0, ncell * sizeof(uint32_t), c->st));
+    HIPCHK(hipMemsetAsync(runStart, 0, ncell * sizeof(uint32_t), c->st));
+    hipLaunchKernelGGL(k_cell_ends, dim3(nblk(m)), dim3(256), 0, c->st, ckey, m, runStart, c->runEnd);
+    ensure_hash(c, c->np + 1024);
+    const uint64_t mask = (uint64_t)c->hcap - 1;
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(c->hcap)), dim3(256), 0, c->st, c->htab, c->hcap, SDM_HEMPTY);
+    if (c->np > 0)
+      hipLaunchKernelGGL(k_hash_insert_pairs, dim3(nblk(c->np)), dim3(256), 0, c->st, c->P.cid, c->np, c->htab, mask);
+    for (int pass = 0; pass < 2; pass++) {
+      HIPCHK(hipMemsetAsync(c->dcnt + 2, 0, sizeof(unsigned long long), c->st));
+      c->C.count = c->dcnt + 2;
+      hipLaunchKernelGGL(k_peri_cell_pairs, dim3((unsigned)ncell), dim3(64), 0, c->st, c->B, pc, po, c->cellRng,
+                         c->pbox, cval, runStart, c->runEnd, c->htab, mask, c->C, err);
+      nc = (int64_t)read_counter(c, 2);
+      if (nc <= c->C.cap) break;
+      ensure_cands(c, nc);  // rerun with room for all
+    }
+  }
+  if ((read_counter(c, 7) & 1) != 0) throw HipError{"body larger than half of the cell size encountered"};
+
   // ---- append the new pairs in order-key order ----
   if (nc > 0) {
     const int64_t start = c->np;
-    ensure_scratch(c, nc + 1);
-    hipLaunchKernelGGL(k_cand_all, dim3(nblk(nc)), dim3(256), 0, c->st, c->C, nc, c->k0, c->v0);
-    sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, nc, 64, c->rsScratch, c->st);
-    ensure_pair_cap(c, c->np + nc);
-    hipLaunchKernelGGL(k_new_pairs, dim3(nblk(nc)), dim3(256), 0, c->st, c->P, c->np, nc, c->C, c->v0, c->iter);
-    c->np += nc;
-    c->cnt.pairs_inserted += nc;
-    hipLaunchKernelGGL(k_new_celldist, dim3(nblk(nc)), dim3(256), 0, c->st, c->P, start, nc, c->B, pc, err);
+    ensure_hash(c, c->np + nc);
+    fill_found(c);
+    const int64_t ns = append_candidates(c, nc);
+    if (ns > 0)
+      hipLaunchKernelGGL(k_new_celldist, dim3(nblk(ns)), dim3(256), 0, c->st, c->P, start, ns, c->B, pc, err);
   }
   build_csr(c);
-  c->periCover = true;
 }
 
 // The first run's pairs of a one-specimen aperiodic scene without the sweep (periodic.h,
@@ -1226,6 +1357,8 @@ const FieldInfo kFields[] = {
     {"newton.maxVelocitySq", S_SCALAR, SDM_F64, 1, 8, 4},
     {"collider.numReinit", S_SCALAR, SDM_I32, 1, 4, 5},
     {"collider.bounds", S_BOUNDS, SDM_I32, 1, 4, 0},
+    {"collider.periods", S_BOUNDS, SDM_I32, 1, 4, 1},
+    {"collider.loIdx", S_SCALAR, SDM_I64, 3, 8, 9},
     {"scene.dt", S_SCALAR, SDM_F64, 1, 8, 6},
     {"newton.damping", S_SCALAR, SDM_F64, 1, 8, 7},
     {"newton.gravity", S_SCALAR, SDM_F64, 3, 8, 8},
@@ -1367,13 +1500,15 @@ void sdm_destroy(sdm_ctx* c) {
   for (int a = 0; a < 3; a++) {
     dfree(c->bb[a]);
     dfree(c->qs[a]);
+    dfree(c->rper[a]); dfree(c->rx[a]); dfree(c->rbh[a]); dfree(c->rbp[a]);
   }
+  dfree(c->rth); dfree(c->rtp); dfree(c->rtx); dfree(c->rdev);
   dfree(c->bbTmp); dfree(c->smin); dfree(c->csrOff); dfree(c->csrPair);
   dfree(c->k0); dfree(c->k1); dfree(c->v0); dfree(c->v1); dfree(c->u0); dfree(c->u1); dfree(c->rsScratch);
   dfree(c->htab); dfree(c->dkeys); dfree(c->dvals);
   dfree(c->C.a); dfree(c->C.b); dfree(c->C.key);
   dfree(c->msrc); dfree(c->mdst); dfree(c->holePos); dfree(c->stage);
-  dfree(c->prevMin); dfree(c->prevMax); dfree(c->cellRng); dfree(c->rankMin); dfree(c->rankMax);
+  dfree(c->cellRng); dfree(c->rankMin); dfree(c->rankMax);
   dfree(c->peCnt); dfree(c->peOff); dfree(c->runEnd); dfree(c->pbox);
   dfree(c->dmats); dfree(c->dcnt); dfree(c->dMaxDisp); dfree(c->dServo); dfree(c->servoPart);
   for (int k = 0; k < 6; k++)
@@ -1514,11 +1649,12 @@ int sdm_get(sdm_ctx* c, const char* name, void* dst, int64_t count) {
         case 6: *(double*)dst = c->prm.dt; break;
         case 7: *(double*)dst = c->prm.damping; break;
         case 8: std::memcpy(dst, c->prm.gravity, 3 * sizeof(double)); break;
+        case 9: std::memcpy(dst, c->loIdx, 3 * sizeof(int64_t)); break;
       }
     } else {
       const int64_t n2 = 2 * (int64_t)c->B.n;
       for (int a = 0; a < 3 && c->bbValid; a++)
-        HIPCHK(hipMemcpy((int32_t*)dst + a * n2, c->bb[a], n2 * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy((int32_t*)dst + a * n2, f->off == 0 ? c->bb[a] : c->rper[a], n2 * 4, hipMemcpyDeviceToHost));
     }
   } catch (HipError& e) {
     return fail(c, e.msg);
@@ -1531,7 +1667,6 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
   if (!c || !f || (!src && count > 0)) return fail(c, std::string("sdm_set: bad attribute ") + (name ? name : "(null)"));
   if (count < 0) return fail(c, std::string("sdm_set: negative count for ") + name);
   c->skipValid = false;
-  c->periCover = false;
   c->graphGen++;
   try {
     HIPCHK(hipStreamSynchronize(c->st));
@@ -1544,11 +1679,16 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
       c->cidValid = false;
     }
     if (f->scope == S_BOUNDS) {
+      // collider.bounds: the handles of the three bound vectors, with every period reset to 0
+      // as the reference's fresh Bounds have; collider.periods: the periods of those handles
       const int64_t n2 = 2 * (int64_t)c->B.n;
-      if (count != 3 * n2) return fail(c, "collider.bounds: count must be 6*N");
-      for (int a = 0; a < 3; a++)
-        HIPCHK(hipMemcpy(c->bb[a], (const int32_t*)src + a * n2, n2 * 4, hipMemcpyHostToDevice));
-      c->bbValid = true;
+      if (count != 3 * n2) return fail(c, std::string(name) + ": count must be 6*N");
+      for (int a = 0; a < 3; a++) {
+        HIPCHK(hipMemcpy(f->off == 0 ? c->bb[a] : c->rper[a], (const int32_t*)src + a * n2, n2 * 4,
+                         hipMemcpyHostToDevice));
+        if (f->off == 0 && n2 > 0) HIPCHK(hipMemset(c->rper[a], 0, n2 * 4));
+      }
+      if (f->off == 0) c->bbValid = true;
       return 0;
     }
     const int64_t n = field_count(c, f);
@@ -1591,6 +1731,13 @@ int sdm_set(sdm_ctx* c, const char* name, const void* src, int64_t count) {
         case 6: c->prm.dt = *(const double*)src; break;
         case 7: c->prm.damping = *(const double*)src; break;
         case 8: std::memcpy(c->prm.gravity, src, 3 * sizeof(double)); break;
+        case 9:
+          for (int a = 0; a < 3; a++) {
+            const int64_t v = ((const int64_t*)src)[a];
+            if (v < 0 || (v > 0 && v >= 2 * (int64_t)c->B.n)) return fail(c, "collider.loIdx: outside the bound vector");
+            c->loIdx[a] = v;
+          }
+          break;
       }
     }
   } catch (HipError& e) {
