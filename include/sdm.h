@@ -143,12 +143,14 @@ int sdm_get_timers(sdm_ctx* ctx, double out[8]);
  * 4 the solver queue (flag scan and fill), 5 the wall and sphere pairs, 6 the Nelder-Mead first
  * vertex, 7 the Nelder-Mead solver list (scan, fill, cost sort), 8 the main solver launch,
  * 9 the mixed form's resume launch or the Levenberg-Marquardt restarts, 10 the mixed form's
- * double refinement, 11 the finish (curvature, stiffness, force law, stores); out[24] the
- * Nelder-Mead pairs touching at their first vertex, out[25] the mixed form's solver
- * evaluations done in double and out[26] its pairs handed to the resume launch (summed over
- * the timed steps; 0 outside the mixed Hertz form).  The reference's TimingInfo has no
- * per-kernel split. */
-int sdm_get_kernel_timers(sdm_ctx* ctx, double out[27]);
+ * double refinement, 11 the finish (curvature, stiffness, force law, stores), 12 the periodic
+ * collider's ring walk in its sequential phases (with the run's coordinates), 13 its parallel
+ * phases (window checks, sorts, crossings); out[28] the Nelder-Mead pairs touching at their
+ * first vertex, out[29] the mixed form's solver evaluations done in double, out[30] its pairs
+ * handed to the resume launch (summed over the timed steps; 0 outside the mixed Hertz form),
+ * out[31] the periodic collider runs whose ring walk was repeated with longer sequential
+ * phases.  The reference's TimingInfo has no per-kernel split. */
+int sdm_get_kernel_timers(sdm_ctx* ctx, double out[32]);
 
 /* Steady-state steps as a replayed graph.  No counterpart in the reference (its engines are
  * called one by one).  When on (the default), a step without a collider run, host write or

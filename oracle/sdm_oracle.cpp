@@ -2525,9 +2525,9 @@ int sdmcpu_get_timers(sdm_ctx* c, double out[8]) {
   return 0;
 }
 
-int sdmcpu_get_kernel_timers(sdm_ctx* c, double out[27]) {
+int sdmcpu_get_kernel_timers(sdm_ctx* c, double out[32]) {
   if (!c || !out) return -1;
-  for (int k = 0; k < 27; k++) out[k] = 0;
+  for (int k = 0; k < 32; k++) out[k] = 0;
   return 0;
 }
 

@@ -294,7 +294,8 @@ class Context:
                 "queued": int(out[7])}
 
     KERNEL_TIMERS = ("k_newton", "k_force_sum", "k_bounds", "k_pairs_prep", "queue", "k_pairs_misc", "k_nm_first",
-                     "nm_list", "solver", "resume_restarts", "k_nm_refine", "k_pairs_finish")
+                     "nm_list", "solver", "resume_restarts", "k_nm_refine", "k_pairs_finish", "ring_walk",
+                     "ring_windows")
 
     def kernel_timers(self):
         """Device time [ms] and launches of each kernel (or group of launches) of the step since
@@ -305,12 +306,13 @@ class Context:
         pairs listed for the solver, the solver's evaluations done in double and the pairs
         handed to its resume launch."""
         n = len(self.KERNEL_TIMERS)
-        out = (ctypes.c_double * (2 * n + 3))()
+        out = (ctypes.c_double * (2 * n + 4))()
         self.lib.check(self.lib.get_kernel_timers(self.ptr, out), self.ptr)
         r = {name: {"ms": out[2 * k], "launches": int(out[2 * k + 1])} for k, name in enumerate(self.KERNEL_TIMERS)}
         r["nm_listed"] = int(out[2 * n])
         r["nm_double_evals"] = int(out[2 * n + 1])
         r["nm_handed_over"] = int(out[2 * n + 2])
+        r["ring_repeats"] = int(out[2 * n + 3])
         return r
 
     def snapshot(self):

@@ -20,14 +20,15 @@
 //   [B, Q)   parallel;
 //   [Q, n2)  sequential: the end of the vector, which the first phase may have reshuffled.
 // A parallel phase inserts the elements of [I0, I1) into the prefix before I0.  Insertion
-// sort inserts each element behind the last element of the prefix that is not greater, so if
-// the window [W0, I0) of the prefix is sorted, no inserted element is smaller than the
-// element at W0 - 1 (exact comparison), and loIdx lies outside [W0 - 1, I1], then the phase
-// is the stable sort of [W0, I1) by coordinate (the ring's +L and period moves happen only at
-// loIdx), and the inversions are its crossings: an inserted lower bound passes the upper
-// bounds that precede it in position and follow it after the sort, the nearest first.  The
-// device checks these conditions when it runs the phase; a run that fails them is undone and
-// repeated with longer sequential phases.
+// sort inserts each element behind the last element before it that is not greater, so if
+// the window [W0, I0) of the prefix is sorted, no inserted element is smaller than both the
+// element at W0 and the one at W0 - 1 (exact comparisons; it then stops inside the window or
+// on top of W0 - 1), and loIdx lies outside [W0 - 1, I1], then the phase is the stable sort of
+// [W0, I1) by coordinate (the ring's +L and period moves happen only at loIdx), and the
+// inversions are its crossings: an inserted lower bound passes the upper bounds that precede
+// it in position and follow it after the sort, the nearest first.  The device checks these
+// conditions when it runs the phase; a run that fails them is undone and repeated with
+// longer sequential phases.
 #pragma once
 #include <stdint.h>
 

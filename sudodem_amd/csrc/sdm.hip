@@ -48,7 +48,9 @@ enum {
   KT_RESUME = 9,     // mixed NM resume launch; LM restart checks and passes
   KT_REFINE = 10,    // k_nm_refine
   KT_FINISH = 11,    // k_pairs_finish
-  KT_N = 12
+  KT_RING_WALK = 12, // periodic ring walk: coordinates and the sequential phases
+  KT_RING_WIN = 13,  // periodic ring walk: the parallel phases (checks, sorts, crossings)
+  KT_N = 14
 };
 
 struct HipError {
@@ -181,7 +183,7 @@ struct sdm_ctx {
   int32_t *rth = nullptr, *rtp = nullptr;  // window permutation temporaries
   double* rtx = nullptr;
   int64_t* rdev = nullptr;  // [0,3) loIdx, [3,6) error bits (u32 in 8-byte slots), window state
-  int64_t ringK[3][4] = {};  // per axis: start / end sequential lengths, half width around loIdx, prefix window
+  hipEvent_t rev[6] = {};    // the ring walk's phase boundaries (timing)
   // per-phase timing
   bool timing = false;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -190,7 +192,8 @@ struct sdm_ctx {
   // kernel ran in the current step, accumulated milliseconds and launches
   hipEvent_t kev[2 * KT_N] = {};
   bool kran[KT_N] = {};
-  double kms[KT_N + 3] = {};  // + the mixed form's touching-list, double-evaluation and hand-over counts
+  double kms[KT_N + 4] = {};  // + the mixed form's touching-list, double-evaluation and hand-over counts,
+                              // the ring walk's repeated runs
   double kcount[KT_N] = {};
   sdm_counters cnt;
   std::string err;
@@ -502,8 +505,15 @@ void erase_flagged(sdm_ctx* c) {
 
 // The ring walk of a later collider run (insertionSortPeri on the three axes, ring.h): the
 // candidates (overlapping pairs of the inversions, in the reference's order) land in C; the
-// rings and loIdx are updated.  A run whose parallel phases fail their conditions is undone
-// and repeated with longer sequential phases.  Returns the number of candidates.
+// rings and loIdx are updated.  Per axis the walk runs as five phases in pass order:
+// sequential [0, P), parallel up to the zone around loIdx, sequential through that zone,
+// parallel up to Q, sequential [Q, n2).  Before a parallel phase runs, the device checks its
+// conditions; when they fail, the sequential phase before it goes on over the positions the
+// check names, and the check is repeated.  Returns the number of candidates.
+constexpr int64_t kRingStart = 128;   // sequential phase at the vector's start and at its end
+constexpr int64_t kRingLoHalf = 64;   // half width of the sequential zone around loIdx
+constexpr int64_t kRingWindow = 1024; // prefix window of a parallel phase
+
 int64_t ring_run(sdm_ctx* c, const PeriCell& pc) {
   const int64_t N = c->B.n;
   const int64_t n2 = 2 * N;
@@ -516,16 +526,7 @@ int64_t ring_run(sdm_ctx* c, const PeriCell& pc) {
     c->rth = dalloc<int32_t>(n2);
     c->rtp = dalloc<int32_t>(n2);
     c->rtx = dalloc<double>(n2);
-    c->rdev = dalloc<int64_t>(32);
-  }
-  for (int a = 0; a < 3; a++) {
-    if (c->ringK[a][0] <= 0) {
-      c->ringK[a][0] = c->ringK[a][1] = 64;
-      c->ringK[a][2] = 32;
-      c->ringK[a][3] = 256;
-    }
-    HIPCHK(hipMemcpyAsync(c->rbh[a], c->bb[a], n2 * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->rbp[a], c->rper[a], n2 * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+    c->rdev = dalloc<int64_t>(64);
   }
   RingSet R;
   for (int a = 0; a < 3; a++) {
@@ -537,66 +538,117 @@ int64_t ring_run(sdm_ctx* c, const PeriCell& pc) {
   R.n2 = n2;
   R.lo = c->rdev;
   R.err = (uint32_t*)(c->rdev + 3);  // three u32 in rdev[3..4]
-  int64_t* winState = c->rdev + 8;   // r0[3], r0base[3], minKey[3] per window
+  RingWin w;
+  w.r0 = c->rdev + 8;
+  w.r0base = c->rdev + 12;
+  w.minKey = (uint64_t*)(c->rdev + 16);
+  w.bottom = (uint64_t*)(c->rdev + 20);
+  w.ext = c->rdev + 24;
   ensure_cands(c, std::max<int64_t>(c->C.cap, n2));
   ensure_scratch(c, n2 + 1);
-  bool grew[3] = {false, false, false};
+  unsigned long long* perr = c->dcnt + 7;
   for (int tries = 0;; tries++) {
-    if (tries > 0) {
-      for (int a = 0; a < 3; a++) {
-        HIPCHK(hipMemcpyAsync(c->bb[a], c->rbh[a], n2 * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(hipMemcpyAsync(c->rper[a], c->rbp[a], n2 * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
-      }
+    // the rings before the run (kept for a repeat with more room for candidates)
+    for (int a = 0; a < 3; a++) {
+      const bool restore = tries > 0;
+      HIPCHK(hipMemcpyAsync(restore ? c->bb[a] : c->rbh[a], restore ? c->rbh[a] : c->bb[a], n2 * sizeof(int32_t),
+                            hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(restore ? c->rper[a] : c->rbp[a], restore ? c->rbp[a] : c->rper[a],
+                            n2 * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
     }
     int64_t hdr[5] = {c->loIdx[0], c->loIdx[1], c->loIdx[2], 0, 0};
     HIPCHK(hipMemcpyAsync(c->rdev, hdr, sizeof(hdr), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemsetAsync(c->dcnt + 2, 0, sizeof(unsigned long long), c->st));
     c->C.count = c->dcnt + 2;
-    // the five phases per axis
-    RingSpan s0, s1, s2;
-    RingWin w1, w2;
+    // phase boundaries per axis: 0 <= P <= A <= B <= Q <= n2
+    int64_t bnd[3][4];
     for (int a = 0; a < 3; a++) {
-      const int64_t* K = c->ringK[a];
       const int64_t lo = c->loIdx[a];
-      const int64_t P = std::min(K[0], n2);
-      const int64_t Q = std::max(n2 - K[1], P);
-      const int64_t A = std::min(std::max(lo - K[2], P), Q);
-      const int64_t Bq = std::min(std::max(lo + K[2], A), Q);
-      s0.s[a] = 0; s0.e[a] = P;
-      w1.i0[a] = P; w1.i1[a] = A; w1.ws[a] = std::max<int64_t>(1, P - K[3]);
-      s1.s[a] = A; s1.e[a] = Bq;
-      w2.i0[a] = Bq; w2.i1[a] = Q; w2.ws[a] = std::max<int64_t>(1, Bq - K[3]);
-      s2.s[a] = Q; s2.e[a] = n2;
+      const int64_t P = std::min(kRingStart, n2);
+      const int64_t Q = std::max(n2 - kRingStart, P);
+      const int64_t A = std::min(std::max(lo - kRingLoHalf, P), Q);
+      const int64_t Bq = std::min(std::max(lo + kRingLoHalf, A), Q);
+      bnd[a][0] = P; bnd[a][1] = A; bnd[a][2] = Bq; bnd[a][3] = Q;
     }
-    w1.r0 = winState; w1.r0base = winState + 3; w1.minKey = (uint64_t*)(winState + 6); w1.bit = RING_ERR_WIN1;
-    w2.r0 = winState + 12; w2.r0base = winState + 15; w2.minKey = (uint64_t*)(winState + 18); w2.bit = RING_ERR_WIN2;
-    unsigned long long* perr = c->dcnt + 7;
+    if (c->timing) HIPCHK(hipEventRecord(c->rev[0], c->st));
     hipLaunchKernelGGL(k_ring_coords, dim3(nblk(3 * n2)), dim3(256), 0, c->st, c->B, R);
-    hipLaunchKernelGGL(k_ring_walk, dim3(3), dim3(64), 0, c->st, c->B, pc, R, s0, c->C, perr);
+    RingSpan sp;
+    for (int a = 0; a < 3; a++) { sp.s[a] = 0; sp.e[a] = bnd[a][0]; }
+    hipLaunchKernelGGL(k_ring_walk, dim3(3), dim3(64), 0, c->st, c->B, pc, R, sp, c->C, perr);
+    if (c->timing) HIPCHK(hipEventRecord(c->rev[1], c->st));
     for (int wi = 0; wi < 2; wi++) {
-      const RingWin& w = wi == 0 ? w1 : w2;
-      int64_t tot = 0;
-      for (int a = 0; a < 3; a++) tot += w.i1[a] > w.i0[a] ? w.i1[a] - w.ws[a] : 0;
-      if (tot > 0) {
-        hipLaunchKernelGGL(k_ring_win_init, dim3(1), dim3(64), 0, c->st, R, w);
-        hipLaunchKernelGGL(k_ring_win_scan, dim3(nblk(tot)), dim3(256), 0, c->st, R, w);
-        hipLaunchKernelGGL(k_ring_win_check, dim3(1), dim3(64), 0, c->st, R, w);
-        for (int a = 0; a < 3; a++) {
-          if (w.i1[a] <= w.i0[a]) continue;
-          const int64_t ws = w.ws[a], m = w.i1[a] - ws;
-          hipLaunchKernelGGL(k_ring_keys, dim3(nblk(m)), dim3(256), 0, c->st, R, a, ws, m, w.r0, c->k0, c->v0);
-          sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, m, 64, c->rsScratch, c->st);
-          sdmrs::scan<sdmrs::OpMin, true, true>(c->v0, c->u0, m, c->rsScratch, c->st);
-          hipLaunchKernelGGL(k_ring_cross, dim3(nblk(m)), dim3(256), 0, c->st, c->B, pc, R, a, ws, w.i0[a] - ws, m,
-                             c->v0, c->u0, c->C, perr);
-          hipLaunchKernelGGL(k_ring_gather, dim3(nblk(m)), dim3(256), 0, c->st, R, a, ws, m, c->v0, c->rth, c->rtp,
-                             c->rtx);
-          HIPCHK(hipMemcpyAsync(c->bb[a] + ws, c->rth, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
-          HIPCHK(hipMemcpyAsync(c->rper[a] + ws, c->rtp, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
-          HIPCHK(hipMemcpyAsync(c->rx[a] + ws, c->rtx, m * sizeof(double), hipMemcpyDeviceToDevice, c->st));
-        }
+      // the parallel phase [cur, end) after the sequential one that ends at cur
+      int64_t cur[3], end[3];
+      for (int a = 0; a < 3; a++) {
+        cur[a] = bnd[a][2 * wi];
+        end[a] = bnd[a][2 * wi + 1];
       }
-      hipLaunchKernelGGL(k_ring_walk, dim3(3), dim3(64), 0, c->st, c->B, pc, R, wi == 0 ? s1 : s2, c->C, perr);
+      for (int it = 0;; it++) {
+        int64_t tot = 0;
+        for (int a = 0; a < 3; a++) {
+          w.i0[a] = cur[a];
+          w.i1[a] = end[a];
+          w.ws[a] = std::max<int64_t>(1, cur[a] - kRingWindow);
+          tot = std::max(tot, end[a] > cur[a] ? end[a] - w.ws[a] : 0);
+        }
+        if (tot == 0) break;
+        hipLaunchKernelGGL(k_ring_win_init, dim3(1), dim3(64), 0, c->st, R, w);
+        hipLaunchKernelGGL(k_ring_win_scan, dim3(nblk(tot), 3), dim3(256), 0, c->st, R, w);
+        hipLaunchKernelGGL(k_ring_win_check, dim3(1), dim3(64), 0, c->st, R, w);
+        hipLaunchKernelGGL(k_ring_win_viol, dim3(nblk(tot), 3), dim3(256), 0, c->st, R, w);
+        int64_t ext[3];
+        HIPCHK(hipMemcpyAsync(ext, w.ext, sizeof(ext), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        bool any = false;
+        RingSpan xs;
+        for (int a = 0; a < 3; a++) {
+          xs.s[a] = xs.e[a] = 0;
+          if (ext[a] > 0 && end[a] > cur[a]) {
+            xs.s[a] = cur[a];
+            xs.e[a] = std::min(end[a], std::max(ext[a], cur[a] + 64));
+            cur[a] = xs.e[a];
+            any = true;
+          }
+        }
+        if (!any) break;
+        if (it > 4 * n2 / 64 + 8) throw HipError{"periodic collider: the ring walk's phases did not settle"};
+        hipLaunchKernelGGL(k_ring_walk, dim3(3), dim3(64), 0, c->st, c->B, pc, R, xs, c->C, perr);
+        c->kms[KT_N + 3] += c->timing ? 1 : 0;
+      }
+      for (int a = 0; a < 3; a++) {
+        if (end[a] <= cur[a]) continue;
+        const int64_t ws = std::max<int64_t>(1, cur[a] - kRingWindow), m = end[a] - ws;
+        hipLaunchKernelGGL(k_ring_keys, dim3(nblk(m)), dim3(256), 0, c->st, R, a, ws, m, w.r0, c->k0, c->v0);
+        sdmrs::sort_pairs<uint64_t, uint32_t>(c->k0, c->v0, c->k1, c->v1, m, 64, c->rsScratch, c->st);
+        sdmrs::scan<sdmrs::OpMin, true, true>(c->v0, c->u0, m, c->rsScratch, c->st);
+        hipLaunchKernelGGL(k_ring_gather, dim3(nblk(m)), dim3(256), 0, c->st, R, a, ws, m, c->v0, c->rth, c->rtp,
+                           c->rtx);
+        RingCross X;
+        X.vals = c->v0;
+        X.smin = c->u0;
+        X.hn = c->rth;
+        X.ws = ws;
+        X.ins = cur[a] - ws;
+        X.m = m;
+        X.a = a;
+        X.longList = c->u1;
+        X.longCount = (unsigned long long*)(c->rdev + 30);
+        X.longCap = m;
+        HIPCHK(hipMemsetAsync(X.longCount, 0, sizeof(unsigned long long), c->st));
+        hipLaunchKernelGGL(k_ring_cross, dim3(nblk(m)), dim3(256), 0, c->st, c->B, pc, X, c->C, perr);
+        hipLaunchKernelGGL(k_ring_cross_long, dim3(std::max(1, c->numCU) * 2), dim3(256), 0, c->st, c->B, pc, X, c->C,
+                           perr);
+        HIPCHK(hipMemcpyAsync(c->bb[a] + ws, c->rth, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->rper[a] + ws, c->rtp, m * sizeof(int32_t), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->rx[a] + ws, c->rtx, m * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+      }
+      if (c->timing) HIPCHK(hipEventRecord(c->rev[2 + 2 * wi], c->st));
+      for (int a = 0; a < 3; a++) {
+        sp.s[a] = wi == 0 ? bnd[a][1] : bnd[a][3];
+        sp.e[a] = wi == 0 ? bnd[a][2] : n2;
+      }
+      hipLaunchKernelGGL(k_ring_walk, dim3(3), dim3(64), 0, c->st, c->B, pc, R, sp, c->C, perr);
+      if (c->timing) HIPCHK(hipEventRecord(c->rev[3 + 2 * wi], c->st));
     }
     int64_t back[5];
     unsigned long long cnt = 0;
@@ -605,39 +657,24 @@ int64_t ring_run(sdm_ctx* c, const PeriCell& pc) {
     HIPCHK(hipStreamSynchronize(c->st));
     uint32_t errs[3];
     std::memcpy(errs, back + 3, sizeof(errs));
-    bool redo = false;
+    if (c->timing) {
+      float t[5];
+      for (int k = 0; k < 5; k++) HIPCHK(hipEventElapsedTime(&t[k], c->rev[k], c->rev[k + 1]));
+      c->kms[KT_RING_WALK] += t[0] + t[2] + t[4];
+      c->kms[KT_RING_WIN] += t[1] + t[3];
+      c->kcount[KT_RING_WALK] += 3;
+      c->kcount[KT_RING_WIN] += 2;
+    }
     for (int a = 0; a < 3; a++) {
       if (errs[a] & RING_ERR_NONFINITE) throw HipError{"periodic collider: a bound coordinate is not finite"};
       if (errs[a] & RING_ERR_RUN) throw HipError{"periodic collider: the ring walk did not end"};
-      int64_t* K = c->ringK[a];
-      if (errs[a] & RING_ERR_WIN1) K[0] *= 2;
-      if (errs[a] & RING_ERR_WIN2) {
-        K[2] *= 2;
-        if (w2.i0[a] <= c->loIdx[a] || s1.e[a] <= s1.s[a]) K[1] *= 2;  // no separate zone around loIdx
-      }
-      if (errs[a] & (RING_ERR_WIN1 | RING_ERR_WIN2)) {
-        K[3] *= 2;
-        grew[a] = true;
-        redo = true;
-      }
     }
     if ((int64_t)cnt > c->C.cap) {
-      HIPCHK(hipStreamSynchronize(c->st));
+      if (tries > 2) throw HipError{"periodic collider: candidate buffer"};
       ensure_cands(c, (int64_t)cnt);
-      redo = true;
-    }
-    if (redo) {
-      if (tries > 60) throw HipError{"periodic collider: the ring walk's phases did not converge"};
       continue;
     }
-    for (int a = 0; a < 3; a++) {
-      c->loIdx[a] = back[a];
-      if (!grew[a])  // the phase lengths relax towards their defaults
-        for (int k = 0; k < 4; k++) {
-          const int64_t d = k == 2 ? 32 : (k == 3 ? 256 : 64);
-          c->ringK[a][k] = std::max(d, c->ringK[a][k] - c->ringK[a][k] / 8);
-        }
-    }
+    for (int a = 0; a < 3; a++) c->loIdx[a] = back[a];
     return (int64_t)cnt;
   }
 }
@@ -1515,6 +1552,8 @@ void sdm_destroy(sdm_ctx* c) {
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
   for (int k = 0; k < 2 * KT_N; k++)
     if (c->kev[k]) (void)hipEventDestroy(c->kev[k]);
+  for (int k = 0; k < 6; k++)
+    if (c->rev[k]) (void)hipEventDestroy(c->rev[k]);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -1806,6 +1845,8 @@ int sdm_set_timing(sdm_ctx* c, int32_t on) {
       if (!c->ev[k]) HIPCHK(hipEventCreate(&c->ev[k]));
     for (int k = 0; k < 2 * KT_N; k++)
       if (!c->kev[k]) HIPCHK(hipEventCreate(&c->kev[k]));
+    for (int k = 0; k < 6; k++)
+      if (!c->rev[k]) HIPCHK(hipEventCreate(&c->rev[k]));
     HIPCHK(hipMemsetAsync(c->dcnt + 4, 0, 2 * sizeof(unsigned long long), c->st));
     HIPCHK(hipStreamSynchronize(c->st));
   } catch (HipError& e) {
@@ -1818,7 +1859,7 @@ int sdm_set_timing(sdm_ctx* c, int32_t on) {
     c->kcount[k] = 0;
     c->kran[k] = false;
   }
-  c->kms[KT_N] = c->kms[KT_N + 1] = c->kms[KT_N + 2] = 0;
+  c->kms[KT_N] = c->kms[KT_N + 1] = c->kms[KT_N + 2] = c->kms[KT_N + 3] = 0;
   return 0;
 }
 
@@ -2063,15 +2104,13 @@ int sdm_get_timers(sdm_ctx* c, double out[8]) {
   return 0;
 }
 
-int sdm_get_kernel_timers(sdm_ctx* c, double out[27]) {
+int sdm_get_kernel_timers(sdm_ctx* c, double out[32]) {
   if (!c || !out) return -1;
   for (int k = 0; k < KT_N; k++) {
     out[2 * k] = c->kms[k];
     out[2 * k + 1] = c->kcount[k];
   }
-  out[2 * KT_N] = c->kms[KT_N];
-  out[2 * KT_N + 1] = c->kms[KT_N + 1];
-  out[2 * KT_N + 2] = c->kms[KT_N + 2];
+  for (int k = 0; k < 4; k++) out[2 * KT_N + k] = c->kms[KT_N + k];
   return 0;
 }
 
