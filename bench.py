@@ -372,11 +372,12 @@ def batch_bench(args):
 
 
 def _sample_pair_set_check(ctx, cfg, real_before, n_sample=2000, seed=0):
-    """Exactness of the periodic pair set on a sample of bodies, right after a collider run:
-    for each sampled body, the brute-force set of partners whose boxes overlap it in the
-    periodic cell (closed intervals; candidates from a periodic k-d tree in the max norm),
-    against the interactions of that body in the container.  Returns the number of missing
-    pairs and of extra pairs that were not real contacts when the collider ran."""
+    """The periodic pair set on a sample of bodies, right after a collider run: for each sampled
+    body, the brute-force set of partners whose boxes overlap it in the periodic cell (closed
+    intervals; candidates from a periodic k-d tree in the max norm), against the interactions
+    of that body in the container.  Returns the number of missing pairs (the reference's walk
+    from vector index 0 can leave a crossing for the next run: the caller checks that run) and
+    of extra pairs that were not real contacts when the collider ran, and the missing pairs."""
     from scipy.spatial import cKDTree
     L = np.array(cfg.cell)
     bmin, bmax = ctx.get("bound.min").reshape(-1, 3), ctx.get("bound.max").reshape(-1, 3)
@@ -398,6 +399,7 @@ def _sample_pair_set_check(ctx, cfg, real_before, n_sample=2000, seed=0):
                 if r:
                     have_real.setdefault(x, set()).add(y)
     missing = extra = checked = 0
+    missing_pairs = []
     for i in sample:
         cand = tree.query_ball_point(np.mod(c[i], L), r=0.5 * (w[i].max() + wmax) * (1 + 1e-9), p=np.inf)
         cand = np.array([j for j in cand if j != i], dtype=np.int64)
@@ -407,10 +409,26 @@ def _sample_pair_set_check(ctx, cfg, real_before, n_sample=2000, seed=0):
         expect = set(int(j) for j in cand[ok])
         got = have.get(int(i), set())
         missing += len(expect - got)
+        missing_pairs += [(int(i), j) for j in sorted(expect - got)]
         extra += len((got - expect) - have_real.get(int(i), set()))
         checked += len(expect)
     return {"bodies_sampled": int(len(sample)), "overlapping_pairs_checked": checked, "missing": missing,
-            "extra_non_real": extra}
+            "extra_non_real": extra}, missing_pairs
+
+
+def _late_pairs_resolved(ctx, cfg, pairs):
+    """After the next collider run: how many of `pairs` still overlap and are still absent."""
+    L = np.array(cfg.cell)
+    bmin, bmax = ctx.get("bound.min").reshape(-1, 3), ctx.get("bound.max").reshape(-1, 3)
+    have = {(min(a, b), max(a, b)) for a, b in zip(ctx.get("intr.id1").tolist(), ctx.get("intr.id2").tolist())}
+    bad = 0
+    for a, b in pairs:
+        c = 0.5 * (bmin[[a, b]] + bmax[[a, b]])
+        w = bmax[[a, b]] - bmin[[a, b]]
+        d = c[1] - c[0]
+        d -= L * np.round(d / L)
+        bad += bool(np.all(np.abs(d) <= 0.5 * (w[0] + w[1]))) and (min(a, b), max(a, b)) not in have
+    return bad
 
 
 def periodic_bench(args):
@@ -454,7 +472,13 @@ def periodic_bench(args):
             break
     else:
         raise RuntimeError("no collider run within 100 steps")
-    check = _sample_pair_set_check(ctx, cfg, real_before=real_before)
+    check, late = _sample_pair_set_check(ctx, cfg, real_before=real_before)
+    runs = ctx.counters()["collider_runs"]
+    for _ in range(100):
+        ctx.step(1)
+        if ctx.counters()["collider_runs"] > runs:
+            break
+    check["late_pairs_missing_after_next_run"] = _late_pairs_resolved(ctx, cfg, late)
     log("pair-set check: %s" % check)
     ctx.close()
     cpu = None
