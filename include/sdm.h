@@ -146,10 +146,11 @@ int sdm_get_timers(sdm_ctx* ctx, double out[8]);
  * double refinement, 11 the finish (curvature, stiffness, force law, stores), 12 the periodic
  * collider's ring walk in its sequential phases (with the run's coordinates), 13 its parallel
  * phases (window checks, sorts, crossings); out[28] the Nelder-Mead pairs touching at their
- * first vertex, out[29] the mixed form's solver evaluations done in double, out[30] its pairs
- * handed to the resume launch (summed over the timed steps; 0 outside the mixed Hertz form),
- * out[31] the periodic collider runs whose ring walk was repeated with longer sequential
- * phases.  The reference's TimingInfo has no per-kernel split. */
+ * first vertex, out[29] the mixed form's solver evaluations done in double (Nelder-Mead or
+ * Levenberg-Marquardt, anchors included), out[30] the pairs handed to the resume launch
+ * (summed over the timed steps; out[28] and out[30] are 0 outside the mixed Hertz form),
+ * out[31] the launches that extended a sequential phase of the periodic ring walk after a
+ * parallel phase's check failed.  The reference's TimingInfo has no per-kernel split. */
 int sdm_get_kernel_timers(sdm_ctx* ctx, double out[32]);
 
 /* Steady-state steps as a replayed graph.  No counterpart in the reference (its engines are

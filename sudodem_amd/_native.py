@@ -303,8 +303,9 @@ class Context:
         dispatch pass, the solver queue, the wall and sphere pairs, the Nelder-Mead first vertex
         and solver list, the main solver launch, the resume launch or LM restarts, the mixed
         form's double refinement and the finish; with the mixed Hertz form, the Nelder-Mead
-        pairs listed for the solver, the solver's evaluations done in double and the pairs
-        handed to its resume launch."""
+        pairs listed for the solver and the pairs handed to its resume launch; with either mixed
+        form, the solver's evaluations done in double (anchors included); and the launches
+        that extended a sequential phase of the periodic collider's ring walk."""
         n = len(self.KERNEL_TIMERS)
         out = (ctypes.c_double * (2 * n + 4))()
         self.lib.check(self.lib.get_kernel_timers(self.ptr, out), self.ptr)
@@ -312,7 +313,7 @@ class Context:
         r["nm_listed"] = int(out[2 * n])
         r["nm_double_evals"] = int(out[2 * n + 1])
         r["nm_handed_over"] = int(out[2 * n + 2])
-        r["ring_repeats"] = int(out[2 * n + 3])
+        r["ring_extensions"] = int(out[2 * n + 3])
         return r
 
     def snapshot(self):

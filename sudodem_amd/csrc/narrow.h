@@ -173,12 +173,15 @@ SDM_HD SupF sup_f(const Shape<float>& s) {
   return f;
 }
 
-SDM_HD V3<float> support_f(const SupF& s, const M3<float>& G, V3<float> dir) {
+// minTrig: the smallest of the four cos / sin factors of the support-map angles is written
+// there when given.
+SDM_HD V3<float> support_f(const SupF& s, const M3<float>& G, V3<float> dir, float* minTrig = nullptr) {
   const V3<float> n = mulT(G, dir);
   const float ax = fabsf(s.rx * n.x);
   float c0, s0, c1, s1;
   unit_angle(copysignf(fpow(fabsf(s.ry * n.y), s.e1), n.y), copysignf(fpow(ax, s.e1), n.x), c0, s0);
   unit_angle(copysignf(fpow(fabsf(s.rz * n.z * fpow(fabsf(c0), s.t1)), s.e2), n.z), fpow(ax, s.e2), c1, s1);
+  if (minTrig) *minTrig = fminf(fminf(fabsf(c0), fabsf(s0)), fminf(fabsf(c1), fabsf(s1)));
   const float pc1 = fpow(fabsf(c1), s.eps2);
   const V3<float> q = mk3(copysignf(s.rx, c0) * fpow(fabsf(c0), s.eps1) * pc1,
                           copysignf(s.ry, s0) * fpow(fabsf(s0), s.eps1) * pc1, copysignf(s.rz, s1) * fpow(fabsf(s1), s.eps2));

@@ -787,6 +787,41 @@ __global__ __launch_bounds__(256) void k_nm_first(StepArgs A, SolveArgs SA, uint
   add_counters(A.counters, 0, (w < nw && !go) ? 1ull : 0ull);
 }
 
+// The first distance evaluation of every queued pair of the Levenberg-Marquardt pass, as a
+// certified float screen (nm_first_apart, with the global contact direction: no contact
+// frame).  A pair whose float touching operand c.(p2 - p1) exceeds its rounding bound is not
+// touching at its start angles: lmder2 sets its sticky stop flag at that evaluation and its
+// first outer check ends it with stop 9 (Superquadrics_Ig2.cpp:679-683, 1127-1147), angles and
+// contact direction unchanged, after one evaluation.  That outcome is stored here; the points
+// of a pair that is not touching are never read.  Flags: 1 for the pairs left to the solver.
+__global__ __launch_bounds__(256) void k_lm_first(SolveArgs A, uint32_t* tflag) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nw = work_n(A);
+  bool apart = false;
+  if (w < nw) {
+    GeoRec r2 = load_rec(A.geo, A.S.id2[w]);
+    if (A.S.sh) r2.pos = r2.pos + ld3s(A.S.sh, w, A.S.cap);
+    const GeoRec r1 = load_rec(A.geo, A.S.id1[w]);
+    const M3<double> I = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
+    apart = nm_first_apart(r1, r2, I, A.S.a0[w], A.S.a1[w], (float)kLmMinTrig);
+    if (apart) {
+      A.S.nev[w] += 1;
+      A.S.stop[w] = 9;
+      A.S.touch[w] = 0;
+    }
+  }
+  if (w < A.nwork) tflag[w] = (w < nw && !apart) ? 1u : 0u;  // zero past the count, for the scan
+  add_evals(A.evals, apart ? 1ull : 0ull);
+}
+
+// Compaction of the flagged slots, in slot order.
+__global__ __launch_bounds__(256) void k_list_compact(const uint32_t* flag, const uint32_t* pos, int64_t n,
+                                                      uint32_t* listOut) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n || !flag[w]) return;
+  listOut[pos[w]] = (uint32_t)w;
+}
+
 // Compaction of the touching slots (in slot order) with their sort keys.
 __global__ __launch_bounds__(256) void k_list_fill(const uint32_t* flag, const uint32_t* pos, const uint64_t* key,
                                                    int64_t n, uint64_t* keysOut, uint32_t* listOut) {

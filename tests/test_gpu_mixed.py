@@ -8,7 +8,8 @@ and a comparison within float rounding is the only way to a different solver pat
 step from identical states, every contact): the same contacts, overlap within 1e-5 relative
 for at least 99.9 % of them and within 1e-8 for 99 %, at least 99 % on the oracle's solver
 path, normals within 1e-6 and normal forces within 1e-6 relative for 99.9 % (the curvature
-and Hertz terms are double in both forms).
+and Hertz terms are double in both forms).  The Levenberg-Marquardt solve's mixed form (linear
+law, periodic cell and box) is checked at the end of this file.
 """
 import numpy as np
 import pytest
@@ -141,3 +142,56 @@ def test_mixed_handover_against_oracle(gpu, oracle):
     # these mirror-symmetric pairs vertex values tie to within that rounding: a tie the
     # other way changes the last evaluated point, hence the normal, not the best vertex.
     assert np.mean(r["normal"] <= 1e-6) >= 0.3 and np.mean(r["normal_force"] <= 1e-6) >= 0.3
+
+
+def _linear_bed(gpu):
+    import math
+    from sudodem_amd.scene import SuperquadricsMat
+    cfg = configs.config1(n_particles=1000, box_xy=20e-3)
+    cfg.engines = configs.linear_engines(cfg.engines[1].verletDist)
+    cfg.materials = [SuperquadricsMat(label="mat1", Kn=1e5, Ks=7e4, frictionAngle=math.atan(0.5), density=2650.0),
+                     SuperquadricsMat(label="wallmat", Kn=1e5, Ks=7e4, frictionAngle=0.0, density=2650.0)]
+    return configs.densify(cfg, gpu, iterations=40)
+
+
+@pytest.mark.parametrize("case", ["periodic", "periodic_shaken", "linear_bed"])
+def test_mixed_levenberg_marquardt_against_oracle(gpu, oracle, case):
+    """The mixed form of the Levenberg-Marquardt solve (Superquadrics_Ig2.cpp:1056-1354): the
+    state machine, its decisions and each outer iteration's base point stay in double (a new
+    anchor, device double functions); the two forward-difference Jacobian columns and the trial
+    points are float differences from it, certified on the touching test and on the acceptance
+    test dF > 0 and done in double inside their rounding bound.  Exponents up to 1.95 and
+    support factors down to 1e-6 take the float form.  Bars from identical states, one step,
+    every contact: no contact classified differently, the oracle's solver path for 99 %, depth
+    within 1e-5 relative for 99 % (measured 99.8 % periodic, 100 % bed)."""
+    if case == "linear_bed":
+        cfg, warm, vel = _linear_bed(gpu), 2, None
+    else:
+        cfg = configs.densify(configs.config2(n_particles=2000), gpu, iterations=80, growth=0.01, relax=20)
+        warm = 3
+        vel = np.random.default_rng(1).normal(scale=1.0, size=(cfg.n, 3)) if case == "periodic_shaken" else None
+    c = cfg.open(oracle)  # the state (rings included) comes from the oracle
+    if vel is not None:
+        c.set("state.vel", vel)
+    c.step(warm)
+    g = cfg.open(gpu, precision=SDM_MIXED)
+    g.restore(c.snapshot())
+    g.set_timing(True)
+    g.step(1)
+    c.step(1)
+    assert np.array_equal(g.get("intr.id1"), c.get("intr.id1")) and np.array_equal(g.get("intr.id2"), c.get("intr.id2"))
+    ndbl, nev = g.kernel_timers()["nm_double_evals"], g.timers()["solver_evals"]
+    r = _per_contact(g, c)
+    d = r["depth"]
+    print("%s: %d contacts, %d classified differently, same path %.4f, depth rel err p99 %.3g max %.3g, "
+          "frac <= 1e-5 %.5f; %d of %d evaluations in double"
+          % (case, r["contacts"], r["classified_differently"], r["same_path"], np.percentile(d, 99), d.max(),
+             np.mean(d <= 1e-5), ndbl, nev))
+    assert r["contacts"] > 500
+    assert r["classified_differently"] == 0
+    assert r["same_path"] >= 0.99
+    assert np.mean(d <= 1e-5) >= 0.99
+    assert np.mean(r["normal_force"] <= 1e-5) >= 0.99
+    assert 0 < ndbl <= nev
+    if case != "linear_bed":  # long solves: the Jacobian columns and trials run in float
+        assert ndbl < 0.8 * nev
