@@ -306,12 +306,18 @@ def batch_bench(args):
     # the same number of steps again with kernel timers: the dominant kernel's roofline
     _, _, tm = timed_run(bc, K, timers=True)
     bc.close()
-    solver_s = tm["solver_ms"] * 1e-3
-    ach = tm["solver_evals"] * FLOP_EQ_PER_EVAL / solver_s if solver_s > 0 else 0.0
-    roof = {"bound": "valu", "kernel": "k_solve_lm (FP64 Levenberg-Marquardt, main launch)", "achieved": ach / 1e12,
-            "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None,
-            "flop_eq_per_eval": FLOP_EQ_PER_EVAL, "evals_per_step": tm["solver_evals"] / K,
-            "kernel_ms_per_step": tm["solver_ms"] / K}
+    # Specimens at rest: the certified screen of the first evaluation leaves the LM solver a few
+    # hundred evaluations per step, and the step is the interaction pass and the integrator;
+    # the integrator's HBM roofline (compulsory bytes per body) is the one with a fixed count
+    km = tm["kernels"]
+    nb = nparts + 6 * len(specs)
+    nms = km["k_newton"]["ms"] / max(1, km["k_newton"]["launches"])
+    ach = NEWTON_BYTES_PER_BODY * nb / (nms * 1e-3) if nms > 0 else 0.0
+    roof = {"bound": "hbm", "kernel": "k_newton", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": ach / PEAK_HBM, "traffic": None, "algorithmic_bytes_per_launch": NEWTON_BYTES_PER_BODY * nb,
+            "bytes": "%d B per body (316 B state + 112 B record)" % NEWTON_BYTES_PER_BODY, "ms_per_launch": nms,
+            "solver": {"kernel": "k_solve_lm (FP64 Levenberg-Marquardt, main launch)",
+                       "evals_per_step": tm["solver_evals"] / K, "kernel_ms_per_step": tm["solver_ms"] / K}}
     kernels = {k: v["ms"] / K for k, v in tm["kernels"].items() if isinstance(v, dict)}
     ctxs = [c.open(lib) for c in specs]
     for c in ctxs:
@@ -434,14 +440,15 @@ def _late_pairs_resolved(ctx, cfg, pairs):
 def periodic_bench(args):
     """Periodic workload: args.periodic superellipsoids (r lognormal mean 1 mm / log-sigma 0.25,
     rz = U[0.5,1.5] r, eps U[0.3,1.9]) in a cubic periodic cell, linear law (Levenberg-Marquardt
-    Geom with the periodic shift), FP64, no gravity.  Seeded lattice + grow-and-relax packing,
+    Geom with the periodic shift), FP64 (--mixed: the mixed form), no gravity.  Seeded lattice +
+    grow-and-relax packing,
     W warm-up steps, K timed steps; pair-set exactness checked on a sample of bodies after the
     timed run; the CPU oracle runs a bounded sample on a 20k-particle cell of the same
     distributions (its initial sweep over a million bodies alone would take hours).  The dense
     packing is an 8000-particle cell compressed under dynamics and tiled (configs.config2_dense)."""
     import torch
 
-    from sudodem_amd import SDM_FP64, configs, load_hip
+    from sudodem_amd import SDM_FP64, SDM_MIXED, configs, load_hip
     from sudodem_amd._native import Context
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -452,7 +459,7 @@ def periodic_bench(args):
     log("dense periodic packing of %d bodies in %.1fs (cell %.1f mm, solids fraction %.3f)" % (
         cfg.n, time.perf_counter() - t0, cfg.cell[0] * 1e3, cfg.solids_fraction))
     rng = np.random.default_rng(7)
-    ctx = cfg.open(lib, precision=SDM_FP64)
+    ctx = cfg.open(lib, precision=SDM_MIXED if args.mixed else SDM_FP64)
     ctx.set("state.vel", rng.normal(scale=0.05, size=(cfg.n, 3)))  # agitation: the collider keeps running
     t0 = time.perf_counter()
     ctx.step(args.warmup)
@@ -508,7 +515,7 @@ def periodic_bench(args):
         "metric": "particle-steps/s and contact-evals/s, periodic cell, %d superellipsoids" % N,
         "value": N * K / wall, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
         "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "f32+f64" if args.mixed else "f64", "data": "synthetic",
         "config": {"workload": "%d superellipsoids (eps U[0.3,1.9], rz/rx U[0.5,1.5], lognormal sizes) in a "
                                "%.1f mm periodic cell (a %d-particle compressed cell tiled %dx%dx%d), linear law "
                                "(LM Geom), dt=1e-6 s" % (N, cfg.cell[0] * 1e3, cfg.base.n, k, k, k),
@@ -518,7 +525,8 @@ def periodic_bench(args):
         "solids_fraction": cfg.solids_fraction,
         "phases_ms_per_step": {k: tm[k] / K for k in ("collider_ms", "interactions_ms", "newton_ms", "solver_ms")},
         "kernels_ms_per_step": {k: v["ms"] / K for k, v in tm["kernels"].items() if isinstance(v, dict)},
-        "roofline": {"bound": "valu", "kernel": "k_solve_lm (FP64 Levenberg-Marquardt, main launch)",
+        "roofline": {"bound": "valu", "kernel": "k_solve_lm (%s Levenberg-Marquardt, main launch)" % (
+                         "mixed" if args.mixed else "FP64"),
                      "achieved": tm["solver_evals"] * FLOP_EQ_PER_EVAL / (tm["solver_ms"] * 1e-3) / 1e12,
                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": tm["solver_evals"] * FLOP_EQ_PER_EVAL / (tm["solver_ms"] * 1e-3) / PEAK_FP64,
@@ -543,6 +551,7 @@ def main():
                     help="batching workload instead: this many box specimens in one context vs one by one")
     ap.add_argument("--adjoint", action="store_true",
                     help="differentiation workload instead: 20k-body frozen-contact rollout, forward + adjoint")
+    ap.add_argument("--mixed", action="store_true", help="with --periodic: the mixed precision form")
     ap.add_argument("--periodic", type=int, default=0, metavar="PARTICLES",
                     help="periodic workload instead: this many superellipsoids in a periodic cell (1000000)")
     ap.add_argument("--calibrate-copy", action="store_true",
