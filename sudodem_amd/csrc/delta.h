@@ -71,9 +71,8 @@ SDM_HD double apow(double a, double b) { return exp2(b * log2(a)); }
 
 // Double support point along local direction n with the anchor factors; returns false when a
 // factor is degenerate (a zero direction component or a zero divisor), where the relative form
-// does not apply, or an exponent is above epsMax.
-SDM_HD bool delta_anchor_local(const Shape<double>& s, V3<double> n, DeltaBody& b, V3<double>& q,
-                               double epsMax = 1.5, double minTrig = 0.0) {
+// does not apply.  kFast: the device's double functions.
+SDM_HD bool delta_anchor_local(const Shape<double>& s, V3<double> n, DeltaBody& b, V3<double>& q) {
   const double e1 = 1.0 / (2.0 - s.eps1), e2 = 1.0 / (2.0 - s.eps2);
   if (n.x == 0.0 || n.y == 0.0 || n.z == 0.0) return false;
   const double ax = fabs(s.rx * n.x);
@@ -99,10 +98,8 @@ SDM_HD bool delta_anchor_local(const Shape<double>& s, V3<double> n, DeltaBody& 
   b.e1 = (float)e1; b.e2 = (float)e2; b.eps1 = (float)s.eps1; b.eps2 = (float)s.eps2; b.t1 = (float)(2.0 - s.eps1);
   const bool ok = h0 > 0.0 && h1 > 0.0 && bb != 0.0 && bp != 0.0 && c0 != 0.0 && s0 != 0.0 && c1 != 0.0 && s1 != 0.0;
   // exponents 1/(2 - eps) above 2 (eps > 1.5, towards the octahedral limit) amplify the float
-  // rounding of the logarithms beyond the Nelder-Mead form's stated bound: there such bodies
-  // are evaluated in double (epsMax 1.5)
-  return ok && fabs(t) < 1e30 && fabs(tp) < 1e30 && s.eps1 <= epsMax && s.eps2 <= epsMax && !(fabs(c0) < minTrig) &&
-         !(fabs(s0) < minTrig) && !(fabs(c1) < minTrig) && !(fabs(s1) < minTrig);
+  // rounding of the logarithms beyond the stated bound: such bodies are evaluated in double
+  return ok && fabs(t) < 1e30 && fabs(tp) < 1e30 && s.eps1 <= 1.5 && s.eps2 <= 1.5;
 }
 
 // Smallest magnitude of a new direction component relative to its anchor value that the
@@ -142,8 +139,7 @@ SDM_HD float lnmix(float a, float b, float u) {
 // form, (|qx| + |qy| + |qz|) sum_k 1/min(|n_k|, |n_k + dn_k|): the rounding of a direction
 // change (~ulp |dn| in each component) enters through dn_k / n_k and every logarithmic factor
 // passes it on with a coefficient of at most one.
-SDM_HD V3<float> delta_support(const DeltaBody& b, const float dn[3], bool& ok, float& cond, float& lgk,
-                               float lnMin = -3.0e38f) {
+SDM_HD V3<float> delta_support(const DeltaBody& b, const float dn[3], bool& ok, float& cond, float& lgk) {
 #pragma clang fp contract(fast)
   const float rx = dn[0] * b.in[0], ry = dn[1] * b.in[1], rz = dn[2] * b.in[2];
   const float ax = fabsf(1.0f + rx), ay = fabsf(1.0f + ry), az = fabsf(1.0f + rz);
@@ -160,9 +156,6 @@ SDM_HD V3<float> delta_support(const DeltaBody& b, const float dn[3], bool& ok, 
   const float m1 = -0.5f * lnmix(c1big ? b.cq1 : b.sq1, c1big ? b.sq1 : b.cq1, c1big ? u1 : -u1);
   const float lc1 = c1big ? m1 : m1 - u1, ls1 = c1big ? m1 + u1 : m1;
   const float gx = b.eps1 * lc0 + b.eps2 * lc1, gy = b.eps1 * ls0 + b.eps2 * lc1, gz = b.eps2 * ls1;
-  if (lnMin > -1.0e38f)  // the new cos / sin factors (logarithms) stay above the given floor
-    ok = ok && 0.5f * __logf(b.cq0) + lc0 >= lnMin && 0.5f * __logf(b.sq0) + ls0 >= lnMin &&
-         0.5f * __logf(b.cq1) + lc1 >= lnMin && 0.5f * __logf(b.sq1) + ls1 >= lnMin;
   ok = ok && ax >= kNear && ay >= kNear && az >= kNear && fabsf(u0) <= kLogMax && fabsf(u1) <= kLogMax &&
        fabsf(gx) <= kLogMax && fabsf(gy) <= kLogMax && fabsf(gz) <= kLogMax;
   const float ex = dexpm1(gx), ey = dexpm1(gy), ez = dexpm1(gz);
@@ -190,13 +183,12 @@ struct DeltaPair {
 // points p1, p2, the touching test (Disfun :1670-1674) and its operand cd = c.(p2 - p1).  `ok`
 // is false when the relative form does not apply to this pair.
 SDM_HD double delta_anchor(const PairGeo<double>& g, const M3<double>& gC, double a0, double a1, DeltaPair& P,
-                           V3<double>& p1, V3<double>& p2, bool& touching, double& cd, bool& ok,
-                           double epsMax = 1.5, double minTrig = 0.0) {
+                           V3<double>& p1, V3<double>& p2, bool& touching, double& cd, bool& ok) {
   const double sa0 = sin(a0), ca0 = cos(a0), sa1 = sin(a1), ca1 = cos(a1);
   const V3<double> c = mul(gC, mk3(ca0 * ca1, sa0 * ca1, sa1));
   V3<double> q1, q2;
-  ok = delta_anchor_local(g.s1, mulT(g.G1, c), P.b1, q1, epsMax, minTrig);
-  ok = delta_anchor_local(g.s2, mulT(g.G2, mk3(-c.x, -c.y, -c.z)), P.b2, q2, epsMax, minTrig) && ok;
+  ok = delta_anchor_local(g.s1, mulT(g.G1, c), P.b1, q1);
+  ok = delta_anchor_local(g.s2, mulT(g.G2, mk3(-c.x, -c.y, -c.z)), P.b2, q2) && ok;
   p1 = mul(g.G1, q1) + g.pos1;
   p2 = mul(g.G2, q2) + g.pos2;
   const V3<double> d = p2 - p1;
@@ -252,38 +244,4 @@ SDM_HD double delta_eval(const DeltaPair& P, float d0, float d1, bool& touching,
   dk = norm(dc) * (k1 + k2);
   dl = l1 + l2;
   return norm(d);
-}
-
-// The same evaluation for the Levenberg-Marquardt solve, which needs the gap vector p2 - p1
-// (its residual and forward-difference Jacobian) and the points: returns p2 - p1 (double: the
-// anchor's gap plus the float difference), the two support displacements from the anchor's
-// points, the touching operand cd = c.(p2 - p1), and the terms of the rounding bound (err:
-// ds + 4 dk + dl as in delta_eval).
-SDM_HD V3<double> delta_eval_vec(const DeltaPair& P, float d0, float d1, V3<float>& ds1, V3<float>& ds2, float& cd,
-                                 float& err, bool& ok, float lnMin) {
-#pragma clang fp contract(fast)
-  float h0, g0, h1, g1;
-  sincosf(0.5f * d0, &h0, &g0);
-  sincosf(0.5f * d1, &h1, &g1);
-  const float sh0 = 2.0f * h0 * g0, sh1 = 2.0f * h1 * g1;
-  const float vc0 = -2.0f * h0 * h0, vc1 = -2.0f * h1 * h1;  // cos d - 1
-  const float dca0 = P.ca0 * vc0 - P.sa0 * sh0, dsa0 = P.sa0 * vc0 + P.ca0 * sh0;
-  const float dca1 = P.ca1 * vc1 - P.sa1 * sh1, dsa1 = P.sa1 * vc1 + P.ca1 * sh1;
-  const float ca1 = P.ca1 + dca1;
-  const V3<float> du = mk3(dca0 * ca1 + P.ca0 * dca1, dsa0 * ca1 + P.sa0 * dca1, dsa1);
-  const V3<float> dc = mul(P.gC, du);
-  const V3<float> dn1 = mulT(P.b1.G, dc);
-  const V3<float> dn2 = mulT(P.b2.G, dc);
-  const float n1[3] = {dn1.x, dn1.y, dn1.z};
-  const float n2[3] = {-dn2.x, -dn2.y, -dn2.z};
-  float k1 = 0.0f, k2 = 0.0f, l1 = 0.0f, l2 = 0.0f;
-  ds1 = delta_support(P.b1, n1, ok, k1, l1, lnMin);
-  ds2 = delta_support(P.b2, n2, ok, k2, l2, lnMin);
-  const V3<double> d = mk3(((double)P.Fh[0] + (double)P.Fl[0]) + (double)(ds2.x - ds1.x),
-                           ((double)P.Fh[1] + (double)P.Fl[1]) + (double)(ds2.y - ds1.y),
-                           ((double)P.Fh[2] + (double)P.Fl[2]) + (double)(ds2.z - ds1.z));
-  const V3<float> c = mk3(P.cu[0] + dc.x, P.cu[1] + dc.y, P.cu[2] + dc.z);
-  cd = dot(c, cast3<float>(d));
-  err = (norm(ds1) + norm(ds2)) + 4.0f * norm(dc) * (k1 + k2) + (l1 + l2);
-  return d;
 }

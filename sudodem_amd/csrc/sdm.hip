@@ -151,7 +151,7 @@ struct sdm_ctx {
   uint32_t* qpos = nullptr;
   int numCU = 0;
   int residentNM[3] = {0, 0, 0};  // resident blocks of k_solve_nm<float>, <double>, <float, true> (0: not queried)
-  int residentLM[3] = {0, 0, 0};  // resident blocks of the main, restart and mixed difference-form LM launches
+  int residentLM[2] = {0, 0};  // resident blocks of the main and restart k_solve_lm launches
   bool matChecked = false;
   bool solverRan = false;
   bool geoDirty = true;
@@ -1147,26 +1147,22 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
   SA.evals = c->dcnt + 4;
   if (LAW == 0) {
     zero_counter(c, DC_NEXT);
-    // The FP64 form solves in double with the correctly rounded functions.  The mixed form has
-    // two: float differences from a double anchor per outer iteration (the shorter chain of
-    // dependent instructions per evaluation, one wave per SIMD) where every listed pair is
-    // resident at once and the launch is latency-bound, and double with the device's
-    // functions at two waves per SIMD where the list is longer (throughput-bound); the count
-    // is known only on the device, so both are launched and each returns at once unless it is
-    // the one for that count.
+    // The FP64 form solves in double with the correctly rounded functions, the mixed form in
+    // double with the device's own; both on two lanes per pair (each lane one body's support
+    // point), the main launch at two waves per SIMD, the restarts' fewer and longer solves at one
     const bool mixedLM = c->prm.precision == SDM_MIXED;
-    const auto lmKernel = mixedLM ? k_solve_lm<true, false, 2> : k_solve_lm<false, false, 2>;
-    // the restarts' long solves, fewer than the chip has lanes: two lanes per pair, in double
-    // (the mixed form with the device's functions)
-    const auto lmRestart = mixedLM ? k_solve_lm<true, false, 1, true> : k_solve_lm<false, false, 1, true>;
+    const auto lmKernel = mixedLM ? k_solve_lm<true, 2, true> : k_solve_lm<false, 2, true>;
+    const auto lmRestart = mixedLM ? k_solve_lm<true, 1, true> : k_solve_lm<false, 1, true>;
     if (mixedLM) {
       SA.dbl = c->dcnt + DC_NDBL;
       zero_counter(c, DC_NDBL);
     }
     // the certified float screen of every pair's first evaluation, then the solver over the
     // pairs it leaves (in queue order)
+    SolveArgs SF = SA;
+    SF.evals = c->dcnt + 5;  // counted with the other evaluations outside the solver launch
     KT_START(c, KT_FIRST);
-    hipLaunchKernelGGL(k_lm_first, dim3(nblk(A.np)), dim3(256), 0, c->st, SA, c->qflag);
+    hipLaunchKernelGGL(k_lm_first, dim3(nblk(A.np)), dim3(256), 0, c->st, SF, c->qflag);
     KT_STOP(c, KT_FIRST);
     KT_START(c, KT_LIST);
     scan_to_counter(c, c->qflag, c->qpos, A.np, DC_NT);
@@ -1175,14 +1171,9 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
     SolveArgs SL = SA;
     SL.list = reinterpret_cast<const int32_t*>(c->v0);
     SL.count = c->dcnt + DC_NT;
-    const unsigned grid = resident_grid(c, lmKernel, c->residentLM[0], A.np);
+    const unsigned grid = resident_grid(c, lmKernel, c->residentLM[0], 2 * A.np);
     if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
     KT_START(c, KT_SOLVE);
-    if (mixedLM) {
-      SL.lmSplit = (int64_t)std::max(1, c->numCU) * 4 * 64;  // lanes at one wave per SIMD
-      hipLaunchKernelGGL((k_solve_lm<true, true>), dim3(resident_grid(c, k_solve_lm<true, true>, c->residentLM[2], A.np)),
-                         dim3(256), 0, c->st, SL);
-    }
     hipLaunchKernelGGL(lmKernel, dim3(grid), dim3(256), 0, c->st, SL);
     KT_STOP(c, KT_SOLVE);
     if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
@@ -1320,10 +1311,9 @@ void one_step(sdm_ctx* c) {
     A.np = c->P.cap;
     ensure_queue(c, A.np);
     ensure_scratch(c, A.np + 1);
-    resident_grid(c, k_solve_lm<true, true>, c->residentLM[2], 1);
-    resident_grid(c, c->prm.precision == SDM_MIXED ? k_solve_lm<true, false, 2> : k_solve_lm<false, false, 2>,
+    resident_grid(c, c->prm.precision == SDM_MIXED ? k_solve_lm<true, 2, true> : k_solve_lm<false, 2, true>,
                   c->residentLM[0], 1);
-    resident_grid(c, c->prm.precision == SDM_MIXED ? k_solve_lm<true, false, 1, true> : k_solve_lm<false, false, 1, true>,
+    resident_grid(c, c->prm.precision == SDM_MIXED ? k_solve_lm<true, 1, true> : k_solve_lm<false, 1, true>,
                   c->residentLM[1], 1);
     resident_grid(c, k_solve_nm<float>, c->residentNM[0], 1);
     resident_grid(c, k_solve_nm<double>, c->residentNM[1], 1);

@@ -21,11 +21,10 @@
 // use the float values, so a comparison closer than the rounding bound can take the other
 // branch than the double solver.  The final double evaluations (k_nm_refine) give the depth
 // at the best vertex and the points of the last evaluation.
-// Mixed precision, Levenberg-Marquardt: each outer iteration's base point is a double anchor,
-// the Jacobian's two forward-difference points and the trial steps are float differences
-// from it, with the touching and acceptance decisions certified (k_solve_lm<true, true>).
-// Both solvers start with a certified float screen of the first evaluation: a pair that is
-// not touching there is complete after it, in either precision form.
+// Mixed precision, Levenberg-Marquardt: double with the device's own functions (a float
+// difference form was slower on MI355X, see k_solve_lm).  Both solvers start with a certified
+// float screen of the first evaluation: a pair that is not touching there is complete after
+// it, in either precision form.
 #pragma once
 #include "delta.h"
 #include "state.h"
@@ -72,8 +71,6 @@ struct SolveArgs {
   unsigned long long* dbl = nullptr;  // mixed NM: evaluations done in double (added)
   int32_t* hoList = nullptr;          // mixed NM: slots handed to the resume launch
   unsigned long long* hoCount = nullptr;  // their number (zero before the main launch)
-  int64_t lmSplit = -1;  // mixed LM: the difference form runs up to this many work items, the
-                         // double form above it (-1: the launched form always runs)
 };
 
 // Work items of a launch: the device-side count (written by an earlier kernel of the same
@@ -784,87 +781,36 @@ __global__ __launch_bounds__(256) void k_nm_refine(SolveArgs A) {
 // ---------------------------------------------------------------------------------------
 // Levenberg-Marquardt (lmder2 with opts {1, 1e-15, 1e-3, 1e-20}, itmax 50)
 // ---------------------------------------------------------------------------------------
-enum { LM_INIT = 0, LM_J0 = 1, LM_J1 = 2, LM_J2 = 3, LM_TRIAL = 4 };
+enum { LM_INIT = 0, LM_J1 = 2, LM_J2 = 3, LM_TRIAL = 4 };  // f(p) of the Jacobian (1) is never evaluated: see k_solve_lm
 
-// Largest exponent the mixed Levenberg-Marquardt form evaluates as float differences: the
-// exponents 1/(2 - eps) of the support map reach 20 there, which scales the relative rounding
-// of the differences by as much -- ~1e-6, still far below what its forward-difference
-// Jacobian resolves (the solve has no certified bound to keep, unlike the Nelder-Mead form).
-constexpr double kLmEpsMax = 1.95;
-// Smallest cos / sin factor of the support map the mixed Levenberg-Marquardt form evaluates as
-// float differences.  The reference takes cos and sin of the rounded atan2 angles (Normal2Phi,
-// getSurface), which carry an absolute error of ~1e-16 that the difference form does not have
-// (it keeps the factors' ratios exact): near a principal plane -- an exponent close to 2 makes
-// |rx nx|^(1/(2 - eps)) vanish against |ry ny|^(1/(2 - eps)) -- the two support maps part by up
-// to tens of percent.  Above 1e-6 that error is below 1e-10 relative in a factor, ~2e-9 after
-// the largest exponent 20, and the solve follows the reference's decisions; below, the point is
-// evaluated in double.
+// Smallest cos / sin factor of a support-map angle at which the certified float screen of a
+// pair's first Levenberg-Marquardt evaluation decides (k_lm_first).  The reference takes cos and
+// sin of the rounded atan2 angles (Normal2Phi, getSurface), which carry an absolute error of
+// ~1e-16 that the float support map does not have (it forms the factors as ratios): near a
+// principal plane -- an exponent close to 2 makes |rx nx|^(1/(2 - eps)) vanish against
+// |ry ny|^(1/(2 - eps)) -- the reference's support point parts from the exact one by up to tens
+// of percent.  Above 1e-6 that error is below 1e-10 relative in a factor, ~2e-9 after the
+// largest exponent 20.
 constexpr double kLmMinTrig = 1e-6;
-constexpr float kLmLnMinTrig = -13.815511f;  // ln 1e-6
-// Rounding bound of a difference evaluation's gap vector and touching operand: 2^-24 x 64 times
-// the Nelder-Mead form's terms (support displacements, conditioning, cancelling logarithms),
-// a margin for the exponents up to 20 (kLmEpsMax), plus 4 ulp of the gap; decisions inside it
-// are taken in double.
-constexpr float kLmDeltaErr = 64.0f * 5.9604645e-8f;
 
-// The mixed form's anchor rows of a Levenberg-Marquardt solve (the contact direction is global:
-// no contact frame), read back with the pair's shapes and rotations.
-__device__ __forceinline__ void lm_load_anchor(const SolveIO& S, int64_t w, const PairGeo<double>& g, DeltaPair& P) {
-  const int64_t c = S.cap;
-  const float* a = S.anc;
-  DeltaBody* b[2] = {&P.b1, &P.b2};
-  const Shape<double>* sh[2] = {&g.s1, &g.s2};
-  const M3<double>* G[2] = {&g.G1, &g.G2};
-#pragma unroll
-  for (int k = 0; k < 2; k++) {
-    const int o = 10 * k;
-    b[k]->in[0] = a[(o + 0) * c + w]; b[k]->in[1] = a[(o + 1) * c + w]; b[k]->in[2] = a[(o + 2) * c + w];
-    b[k]->sq0 = a[(o + 3) * c + w]; b[k]->sq1 = a[(o + 4) * c + w];
-    b[k]->cq0 = a[(o + 5) * c + w]; b[k]->cq1 = a[(o + 6) * c + w];
-    b[k]->q[0] = a[(o + 7) * c + w]; b[k]->q[1] = a[(o + 8) * c + w]; b[k]->q[2] = a[(o + 9) * c + w];
-    const Shape<double>& s = *sh[k];
-    b[k]->e1 = (float)(1.0 / (2.0 - s.eps1)); b[k]->e2 = (float)(1.0 / (2.0 - s.eps2));
-    b[k]->eps1 = (float)s.eps1; b[k]->eps2 = (float)s.eps2; b[k]->t1 = (float)(2.0 - s.eps1);
-    b[k]->G = cast9<float>(*G[k]);
-  }
-  P.ca0 = a[20 * c + w]; P.sa0 = a[21 * c + w]; P.ca1 = a[22 * c + w]; P.sa1 = a[23 * c + w];
-  P.Fh[0] = a[24 * c + w]; P.Fh[1] = a[25 * c + w]; P.Fh[2] = a[26 * c + w];
-  P.Fl[0] = a[27 * c + w]; P.Fl[1] = a[28 * c + w]; P.Fl[2] = a[29 * c + w];
-  P.cu[0] = a[30 * c + w]; P.cu[1] = a[31 * c + w]; P.cu[2] = a[32 * c + w];
-  P.gC = M3<float>{1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
-}
-
-// kFast: the device's double functions; otherwise correctly rounded (FP64).  kDelta (with
-// kFast, the mixed form): each outer iteration evaluates the distance function once in double
-// -- at the accepted point, the Jacobian's first evaluation (JacfunctionApp's f(p)), which
-// becomes the anchor -- and its two forward-difference evaluations and every trial step as
-// float differences from that anchor (delta.h); a point the difference form does not take
-// (a direction component at zero, an exponent above kLmEpsMax, a support factor below
-// kLmMinTrig, a decision inside the rounding bound) is evaluated in double.  The state
-// machine, its sums and the Jacobian stay in double.
-// kWaves: resident waves per SIMD the register allocation is held to.  Measured on MI355X
-// (1M-body periodic cell): the FP64 form's main launch 12 % faster at two than at the one the
-// unconstrained allocation gives (256 VGPRs + 66 AGPRs; two spill 204 B per lane), its restart
-// launches (a few ten thousand long solves: latency-bound) slightly slower; the mixed form
-// spills 524 B per lane at two and runs 40 % slower.
+// kFast: the device's double functions (the mixed form); otherwise correctly rounded (FP64).
 // kTwoLane: two adjacent lanes take each pair, each holding one body: both run the pair's
 // state machine (the same arithmetic on the same values, so they stay in step) and each
 // evaluates its own body's support point, which they exchange -- p2 is body 2's support along
-// -c (Disfunction's mulT(G2, c) negated is mulT(G2, -c) exactly).  Half the latency per
-// evaluation, for launches with fewer pairs than the chip has lanes (the restarts' long solves).
-template <bool kFast, bool kDelta = false, int kWaves = 1, bool kTwoLane = false>
+// -c (Disfunction's mulT(G2, c) negated is mulT(G2, -c) exactly).  An evaluation then takes
+// half the instructions per lane, so a wave's iteration half the time: the launch's length is
+// set by its longest chains of dependent evaluations (~250 for the solves that reach itmax),
+// which start when the queue has drained.  Measured on MI355X (1M-body periodic cell, FP64):
+// main launch 12.0 -> 9.7 ms/step against one lane per pair; restarts 7.5 -> 5.5 ms/step.
+// kWaves: resident waves per SIMD the register allocation is held to: two for the main launch
+// (12 % faster than one, at 144 B of spill per lane), one for the restarts (fewer pairs than
+// lanes: latency-bound).
+template <bool kFast, int kWaves = 1, bool kTwoLane = false>
 __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
-  static_assert(!(kDelta && kTwoLane), "the two-lane form evaluates in double");
-  if (A.lmSplit >= 0 && (kDelta ? work_n(A) > A.lmSplit : work_n(A) <= A.lmSplit)) return;  // the other form's
   const bool odd = kTwoLane && (threadIdx.x & 1);
   const double tau = 1.0, eps1 = 1e-15, eps2 = 1e-3, eps3 = 1e-20;
   const int itmax = 50;
   PairGeo<double> g;
-  V3<double> x1 = mk3(0.0, 0.0, 0.0);  // kDelta: body 1's centre (the evaluation frame's origin)
-  V3<double> pA1 = x1, pA2 = x1;      // kDelta: the anchor's points (relative frame)
-  double aA0 = 0, aA1 = 0, cdA = 0;   // kDelta: the anchor's angles and touching operand
-  bool haveAnc = false;
-  unsigned long long ndbl = 0;
   double p0 = 0, p1v = 0, Dp0 = 0, Dp1 = 0, mu = 0, p_fL2 = 0, A00 = 0, A01 = 0, A11 = 0, dA0 = 0, dA1 = 0;
   double jTf0 = 0, jTf1 = 0, p_L2 = 0, q0 = 0, qq1 = 0;
   V3<double> p_f = mk3(0.0, 0.0, 0.0), f0 = p_f, f1 = p_f, ct = p_f, lp1 = p_f, lp2 = p_f, lc = p_f;
@@ -894,10 +840,8 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
           GeoRec r2 = load_rec(A.geo, A.S.id2[w]);
           if (A.S.sh) r2.pos = r2.pos + ld3s(A.S.sh, w, A.S.cap);  // position2 = pos + shift2 (:1886)
           const GeoRec r1 = load_rec(A.geo, A.S.id1[w]);
-          pair_geo<double>(r1, r2, kDelta, g);
-          x1 = r1.pos;
+          pair_geo<double>(r1, r2, false, g);
         }
-        haveAnc = false;
         p0 = A.S.a0[w];
         p1v = A.S.a1[w];
         ct = ld3s(A.S.ct, w, A.S.cap);
@@ -930,71 +874,9 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
       lc = mk3(0.5 * (lp1.x + lp2.x), 0.5 * (lp1.y + lp2.y), 0.5 * (lp1.z + lp2.z));
       fe = lp2 - lp1;
       if (dot(contact, fe) > 0.0) stop_flag = true;
-    } else if constexpr (!kDelta) {
+    } else {
       lm_eval<double, kFast>(g, e0, e1, lp1, lp2, lc, stop_flag, nev);
       fe = lp2 - lp1;
-    } else {
-      V3<double> q1, q2;
-      bool done_eval = false;
-      if (state == LM_INIT || state == LM_J0) {
-        if (haveAnc && e0 == aA0 && e1 == aA1) {
-          // the Jacobian's f(p) at the anchor itself (after the first evaluation)
-          q1 = pA1;
-          q2 = pA2;
-          if (cdA > 0.0) stop_flag = true;
-          nev++;
-          done_eval = true;
-        } else {
-          DeltaPair P;
-          bool t = true, ok = true;
-          double cd = 0.0;
-          const M3<double> I = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
-          const double f = delta_anchor(g, I, e0, e1, P, q1, q2, t, cd, ok, kLmEpsMax, kLmMinTrig);
-          haveAnc = ok && isfinite(f);
-          if (haveAnc) {
-            ndbl++;
-            store_anchor(A.S, w, P);
-            aA0 = e0;
-            aA1 = e1;
-            pA1 = q1;
-            pA2 = q2;
-            cdA = cd;
-            if (cd > 0.0) stop_flag = true;
-            nev++;
-            done_eval = true;
-          }
-        }
-      } else if (haveAnc) {
-        DeltaPair P;
-        lm_load_anchor(A.S, w, g, P);
-        V3<float> ds1, ds2;
-        float cd = 0.0f, err = 0.0f;
-        bool ok = true;
-        const V3<double> d =
-            delta_eval_vec(P, (float)(e0 - aA0), (float)(e1 - aA1), ds1, ds2, cd, err, ok, kLmLnMinTrig);
-        // decisions taken on this value must not depend on its rounding: the touching test
-        // (sticky) and, for a trial step, the acceptance test dF > 0; within the bound the
-        // point is evaluated in double
-        const float nd = (float)norm(d);
-        const float bnd = kLmDeltaErr * err + 2.4e-7f * nd;
-        const bool sure = fabsf(cd) > bnd && (state != LM_TRIAL || fabsf((float)(p_fL2 - (double)nd)) > bnd);
-        if (ok && sure && isfinite(d.x + d.y + d.z)) {
-          q1 = pA1 + cast3<double>(ds1);
-          q2 = q1 + d;
-          if (cd > 0.0f) stop_flag = true;
-          nev++;
-          done_eval = true;
-        }
-      }
-      if (!done_eval) {  // in double at this point
-        V3<double> c;
-        lm_eval<double, true>(g, e0, e1, q1, q2, c, stop_flag, nev);
-        ndbl++;
-      }
-      fe = q2 - q1;
-      lp1 = q1 + x1;
-      lp2 = q2 + x1;
-      lc = mk3(0.5 * (lp1.x + lp2.x), 0.5 * (lp1.y + lp2.y), 0.5 * (lp1.z + lp2.z));
     }
     bool finished = false;
     bool outer = false;  // go to the outer-iteration checks
@@ -1008,9 +890,6 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
       } else {
         outer = true;
       }
-    } else if (state == LM_J0) {
-      f0 = fe;
-      state = LM_J1;
     } else if (state == LM_J1) {
       f1 = fe;
       state = LM_J2;
@@ -1116,17 +995,12 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
           stop = 8;
           finished = true;
         } else {
-          if constexpr (!kDelta) {
-            // JacfunctionApp's f(p) (Superquadrics_Ig2.cpp:1037) re-evaluates the point
-            // evaluated last -- the start point or the accepted trial, whose result is p_f,
-            // lp1, lp2 and lc -- at the same parameters: the same values, so it is counted
-            // and taken from there
-            f0 = p_f;
-            nev++;
-            state = LM_J1;
-          } else {
-            state = LM_J0;
-          }
+          // JacfunctionApp's f(p) (Superquadrics_Ig2.cpp:1037) re-evaluates the point evaluated
+          // last -- the start point or the accepted trial, whose result is p_f, lp1, lp2 and
+          // lc -- at the same parameters: the same values, so it is counted and taken from there
+          f0 = p_f;
+          nev++;
+          state = LM_J1;
         }
       }
     }
@@ -1149,5 +1023,5 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
     }
   }
   add_evals(A.evals, tot);
-  if (A.dbl) add_evals(A.dbl, kDelta ? ndbl : tot);  // the mixed form's evaluations in double
+  if (A.dbl) add_evals(A.dbl, tot);  // the mixed form: every evaluation in double
 }

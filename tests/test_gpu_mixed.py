@@ -9,7 +9,8 @@ step from identical states, every contact): the same contacts, overlap within 1e
 for at least 99.9 % of them and within 1e-8 for 99 %, at least 99 % on the oracle's solver
 path, normals within 1e-6 and normal forces within 1e-6 relative for 99.9 % (the curvature
 and Hertz terms are double in both forms).  The Levenberg-Marquardt solve's mixed form (linear
-law, periodic cell and box) is checked at the end of this file.
+law, periodic cell and box: double with the device's functions) is checked at the end of this
+file.
 """
 import numpy as np
 import pytest
@@ -157,13 +158,12 @@ def _linear_bed(gpu):
 @pytest.mark.parametrize("case", ["periodic", "periodic_shaken", "linear_bed"])
 def test_mixed_levenberg_marquardt_against_oracle(gpu, oracle, case):
     """The mixed form of the Levenberg-Marquardt solve (Superquadrics_Ig2.cpp:1056-1354): the
-    state machine, its decisions and each outer iteration's base point stay in double (a new
-    anchor, device double functions); the two forward-difference Jacobian columns and the trial
-    points are float differences from it, certified on the touching test and on the acceptance
-    test dF > 0 and done in double inside their rounding bound.  Exponents up to 1.95 and
-    support factors down to 1e-6 take the float form.  Bars from identical states, one step,
-    every contact: no contact classified differently, the oracle's solver path for 99 %, depth
-    within 1e-5 relative for 99 % (measured 99.8 % periodic, 100 % bed)."""
+    certified float screen of the first evaluation, then the solve in double with the device's
+    own functions (not the correctly rounded ones of the FP64 form).  Bars from identical
+    states, one step, every contact: no contact classified differently, the oracle's solver
+    path for 99 %, depth within 1e-5 relative for 99 % (measured 99.8 % periodic: the rest are
+    solves that a last-bit difference of a device function sends to another end point; 100 %
+    bed)."""
     if case == "linear_bed":
         cfg, warm, vel = _linear_bed(gpu), 2, None
     else:
@@ -192,6 +192,4 @@ def test_mixed_levenberg_marquardt_against_oracle(gpu, oracle, case):
     assert r["same_path"] >= 0.99
     assert np.mean(d <= 1e-5) >= 0.99
     assert np.mean(r["normal_force"] <= 1e-5) >= 0.99
-    assert 0 < ndbl <= nev
-    if case != "linear_bed":  # long solves: the Jacobian columns and trials run in float
-        assert ndbl < 0.8 * nev
+    assert ndbl >= nev > 0  # every evaluation in double (restarts included in ndbl)
