@@ -44,7 +44,8 @@ enum {
 enum {
   SDM_FP64 = 0,  /* all arithmetic in double, reference semantics (pow, atan2, atan, sin and cos
                     correctly rounded, the values glibc gives the reference) */
-  SDM_MIXED = 1  /* narrowphase iterations in float on relative frames, final contact refinement in double */
+  SDM_MIXED = 1  /* Nelder-Mead iterations in float on relative frames, final contact refinement in double;
+                    Levenberg-Marquardt in double with the device's own functions */
 };
 
 enum { SDM_BODY_SUPERQUADRIC = 0, SDM_BODY_WALL = 1 };
