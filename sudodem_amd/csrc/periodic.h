@@ -130,6 +130,37 @@ __device__ __forceinline__ void screen_box(double m, double M, double L, float& 
   wide = wide || !(M - m < SDM_PERI_WIDE * L);
 }
 
+// The screen of a pair of narrow boxes (centre c, half widths h; .w of the centre 1 for a
+// wide box): apart on some axis by the minimum-image distance of the centres, beyond a margin
+// far above the screen's rounding, so no pair the exact test accepts is dropped.  Wide boxes
+// are never screened: only the exact test (which reports a box spanning into the next period)
+// decides their pairs.
+__device__ __forceinline__ bool screen_apart(const PeriCell& pc, float4 ca, float4 ha, float4 cb, float4 hb) {
+  if (ca.w != 0.0f || cb.w != 0.0f) return false;
+  const float d[3] = {ca.x - cb.x, ca.y - cb.y, ca.z - cb.z};
+  const float h[3] = {ha.x + hb.x, ha.y + hb.y, ha.z + hb.z};
+  bool apart = false;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float dk = d[k] - pc.Lf[k] * rintf(d[k] * pc.iLf[k]);
+    apart = apart || fabsf(dk) > h[k] + pc.mf[k];
+  }
+  return apart;
+}
+
+// The screen boxes of every body alone (a later collider run: the ring walk's crossings).
+__global__ void k_screen_boxes(BodyArrays B, PeriCell pc, float4* box) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t N = B.n;
+  if (i >= N) return;
+  float cen[3], hw[3];
+  bool wide = false;
+#pragma unroll
+  for (int k = 0; k < 3; k++) screen_box(B.bmin[k * N + i], B.bmax[k * N + i], pc.L[k], cen[k], hw[k], wide);
+  box[2 * i] = make_float4(cen[0], cen[1], cen[2], wide ? 1.0f : 0.0f);
+  box[2 * i + 1] = make_float4(hw[0], hw[1], hw[2], 0.0f);
+}
+
 // Cell range of every body: start cell s[k] and count c[k] per axis (c = n: the whole
 // axis), and the number of cells the box touches.  Also the boxes of the pair screen,
 // box[2i], box[2i+1] = centre and half widths; box[2i].w is 1 for a wide box.
@@ -315,19 +346,7 @@ __global__ __launch_bounds__(64) void k_peri_cell_pairs(BodyArrays B, PeriCell p
       if (!first) continue;
       const int b = ent_id(j);
       const float4 cb = ent_box(j, 0);
-      const bool screened = ca.w == 0.0f && cb.w == 0.0f;
-      if (screened) {
-        const float4 hb = ent_box(j, 1);
-        const float d[3] = {ca.x - cb.x, ca.y - cb.y, ca.z - cb.z};
-        const float h[3] = {ha.x + hb.x, ha.y + hb.y, ha.z + hb.z};
-        bool apart = false;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          const float dk = d[k] - pc.Lf[k] * rintf(d[k] * pc.iLf[k]);
-          apart = apart || fabsf(dk) > h[k] + pc.mf[k];
-        }
-        if (apart) continue;
-      }
+      if (ca.w == 0.0f && cb.w == 0.0f && screen_apart(pc, ca, ha, cb, ent_box(j, 1))) continue;
       peri_candidate(B, pc, po, a, b, tab, mask, C, err);
     }
   }

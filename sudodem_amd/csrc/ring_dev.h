@@ -350,17 +350,26 @@ struct RingCross {
   uint32_t* longList;  // lower bounds that moved far: one wave each (k_ring_cross_long)
   unsigned long long* longCount;
   int64_t longCap;
+  const float4* box;   // screen boxes of the bodies (k_screen_boxes): two float4 each
 };
 
+// One crossing candidate: the inserted lower bound of `id` (box m1, M1; screen box ca, ha) and
+// the element at sorted position pp.  A pair the single-precision screen puts apart is one the
+// exact test rejects without an error (both boxes narrow), so it is dropped on 32 bytes of
+// the other body instead of six scattered doubles.
 __device__ __forceinline__ void ring_cross_one(const BodyArrays& B, const PeriCell& pc, const RingCross& X,
-                                               const double m1[3], const double M1[3], int id, uint32_t q,
-                                               uint64_t base, int64_t pp, const Cands& C,
+                                               const double m1[3], const double M1[3], float4 ca, float4 ha, int id,
+                                               uint32_t q, uint64_t base, int64_t pp, const Cands& C,
                                                unsigned long long* perr) {
   if (X.vals[pp] >= q) return;
   const int32_t h2 = X.hn[pp];
   if (!(h2 & 1)) return;
   const int id2 = h2 >> 1;
   if (id2 == id) return;
+  if (ca.w == 0.0f) {
+    const float4 cb = X.box[2 * id2];
+    if (cb.w == 0.0f && screen_apart(pc, ca, ha, cb, X.box[2 * id2 + 1])) return;
+  }
   const int64_t N = B.n;
   bool ok = true;
 #pragma unroll
@@ -399,8 +408,10 @@ __global__ void k_ring_cross(BodyArrays B, PeriCell pc, RingCross X, Cands C, un
     m1[k] = B.bmin[k * N + id];
     M1[k] = B.bmax[k * N + id];
   }
+  const float4 ca = X.box[2 * id], ha = X.box[2 * id + 1];
   const uint64_t base = ((uint64_t)X.a << 62) | ((uint64_t)(X.ws + q) << 31);
-  for (int64_t pp = p + 1; pp < X.m && X.smin[pp] < q; pp++) ring_cross_one(B, pc, X, m1, M1, id, q, base, pp, C, perr);
+  for (int64_t pp = p + 1; pp < X.m && X.smin[pp] < q; pp++)
+    ring_cross_one(B, pc, X, m1, M1, ca, ha, id, q, base, pp, C, perr);
 }
 
 // The far-moved lower bounds, one wave each: lane l takes the positions p+1+l, p+65+l, ...
@@ -420,10 +431,11 @@ __global__ __launch_bounds__(256) void k_ring_cross_long(BodyArrays B, PeriCell 
       m1[k] = B.bmin[k * N + id];
       M1[k] = B.bmax[k * N + id];
     }
+    const float4 ca = X.box[2 * id], ha = X.box[2 * id + 1];
     const uint64_t base = ((uint64_t)X.a << 62) | ((uint64_t)(X.ws + q) << 31);
     for (int64_t c0 = p + 1; c0 < X.m && X.smin[c0] < q; c0 += 64) {
       const int64_t pp = c0 + lane;
-      if (pp < X.m && X.smin[pp] < q) ring_cross_one(B, pc, X, m1, M1, id, q, base, pp, C, perr);
+      if (pp < X.m && X.smin[pp] < q) ring_cross_one(B, pc, X, m1, M1, ca, ha, id, q, base, pp, C, perr);
     }
   }
 }

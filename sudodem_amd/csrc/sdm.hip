@@ -528,6 +528,9 @@ int64_t ring_run(sdm_ctx* c, const PeriCell& pc) {
     c->rtx = dalloc<double>(n2);
     c->rdev = dalloc<int64_t>(64);
   }
+  // the single-precision screen boxes of this run's bounds, for the crossing scans
+  if (!c->pbox) c->pbox = dalloc<float4>(2 * N);
+  hipLaunchKernelGGL(k_screen_boxes, dim3(nblk(N)), dim3(256), 0, c->st, c->B, pc, c->pbox);
   RingSet R;
   for (int a = 0; a < 3; a++) {
     R.h[a] = c->bb[a];
@@ -634,6 +637,7 @@ int64_t ring_run(sdm_ctx* c, const PeriCell& pc) {
         X.longList = c->u1;
         X.longCount = (unsigned long long*)(c->rdev + 30);
         X.longCap = m;
+        X.box = c->pbox;
         HIPCHK(hipMemsetAsync(X.longCount, 0, sizeof(unsigned long long), c->st));
         hipLaunchKernelGGL(k_ring_cross, dim3(nblk(m)), dim3(256), 0, c->st, c->B, pc, X, c->C, perr);
         hipLaunchKernelGGL(k_ring_cross_long, dim3(std::max(1, c->numCU) * 2), dim3(256), 0, c->st, c->B, pc, X, c->C,
