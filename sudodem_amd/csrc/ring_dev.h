@@ -383,7 +383,7 @@ __device__ __forceinline__ void ring_cross_one(const BodyArrays& B, const PeriCe
   if (ok) push_cand(C, id, id2, base | (uint64_t)(0x7FFFFFFFu - (uint32_t)pp));
 }
 
-#define SDM_RING_LONG 96  // a lower bound that moved back further is scanned by a whole wave
+#define SDM_RING_LONG 192  // a lower bound that moved back further is scanned by a whole wave
 
 __global__ void k_ring_cross(BodyArrays B, PeriCell pc, RingCross X, Cands C, unsigned long long* perr) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
