@@ -143,7 +143,8 @@ int sdm_get_timers(sdm_ctx* ctx, double out[8]);
  * bound update of a collider run (BoundDispatcher), 3 the interaction loop's dispatch pass,
  * 4 the solver queue (flag scan and fill), 5 the wall and sphere pairs, 6 the Nelder-Mead first
  * vertex, 7 the Nelder-Mead solver list (scan, fill, cost sort), 8 the main solver launch,
- * 9 the mixed form's resume launch or the Levenberg-Marquardt restarts, 10 the mixed form's
+ * 9 the mixed form's resume launch (the Levenberg-Marquardt restarts run inside the main
+ * solver launch), 10 the mixed form's
  * double refinement, 11 the finish (curvature, stiffness, force law, stores), 12 the periodic
  * collider's ring walk in its sequential phases (with the run's coordinates), 13 its parallel
  * phases (window checks, sorts, crossings); out[28] the Nelder-Mead pairs touching at their

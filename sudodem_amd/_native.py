@@ -300,9 +300,10 @@ class Context:
     def kernel_timers(self):
         """Device time [ms] and launches of each kernel (or group of launches) of the step since
         set_timing(True): the integrator, the force sums, the bound update, the interaction-loop
-        dispatch pass, the solver queue, the wall and sphere pairs, the Nelder-Mead first vertex
-        and solver list, the main solver launch, the resume launch or LM restarts, the mixed
-        form's double refinement and the finish; with the mixed Hertz form, the Nelder-Mead
+        dispatch pass, the solver queue, the wall and sphere pairs, the first evaluation (the
+        Nelder-Mead first vertex or the LM screen) and the solver list, the main solver launch
+        (the LM restarts included), the mixed Nelder-Mead resume launch, the mixed form's double
+        refinement and the finish; with the mixed Hertz form, the Nelder-Mead
         pairs listed for the solver and the pairs handed to its resume launch; with either mixed
         form, the solver's evaluations done in double (anchors included); and the launches
         that extended a sequential phase of the periodic collider's ring walk."""

@@ -831,47 +831,6 @@ __global__ __launch_bounds__(256) void k_list_fill(const uint32_t* flag, const u
   listOut[pos[w]] = (uint32_t)w;
 }
 
-// Restart checks of Ig2_..._SuperquadricsGeom::go after an LM pass (Superquadrics_Ig2.cpp:2013-2052).
-// kind 1: NaN parameters or stop 7 while touching; kind 2: a suspiciously deep contact whose
-// points are not inside the other body.  Restarted slots are listed for another pass.
-template <int KIND>
-__global__ __launch_bounds__(256) void k_lm_check(StepArgs A, SolveArgs SA, int32_t* list, unsigned long long* cnt) {
-  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool restart = false;
-  const SolveIO& S = SA.S;
-  if (w < work_n(SA) && S.touch[w]) {
-    const int64_t N = A.B.n;
-    const int id1 = S.id1[w], id2 = S.id2[w];
-    if (KIND == 1) {
-      restart = isnan(S.a0[w]) || isnan(S.a1[w]) || S.stop[w] == 7;
-    } else {
-      const V3<double> p1 = ld3s(S.p1, w, S.cap), p2 = ld3s(S.p2, w, S.cap);
-      const V3<double> x1 = ld3(A.B.pos, id1, N);
-      const V3<double> x2 = S.sh ? ld3(A.B.pos, id2, N) + ld3s(S.sh, w, S.cap) : ld3(A.B.pos, id2, N);
-      if (10.0 * norm(p2 - p1) > A.B.rmax[id1]) {
-        Shape<double> s1, s2;
-        s1.rx = A.B.rxyz[id1]; s1.ry = A.B.rxyz[N + id1]; s1.rz = A.B.rxyz[2 * N + id1];
-        s1.eps1 = A.B.eps[id1]; s1.eps2 = A.B.eps[N + id1];
-        s2.rx = A.B.rxyz[id2]; s2.ry = A.B.rxyz[N + id2]; s2.rz = A.B.rxyz[2 * N + id2];
-        s2.eps1 = A.B.eps[id2]; s2.eps2 = A.B.eps[N + id2];
-        const M3<double> G1 = ld9(A.B.rotG, id1, N), G2 = ld9(A.B.rotG, id2, N);
-        restart = !(is_inside(s1, G1, p2 - x1)) || !(is_inside(s2, G2, p1 - x2));
-      }
-    }
-    if (restart) {
-      const V3<double> sh = S.sh ? ld3s(S.sh, w, S.cap) : mk3(0.0, 0.0, 0.0);
-      const V3<double> d0 = (ld3(A.B.pos, id2, N) + sh) - ld3(A.B.pos, id1, N);
-      double a0 = xatan2(d0.y, d0.x);
-      double a1 = xatan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y));
-      S.a0[w] = a0 + 1e-5;
-      S.a1[w] = a1 + 1e-5;
-      S.tag[w] = KIND == 1 ? 100 : 200;
-    }
-  }
-  const int64_t slot = fetch_work(restart, cnt);
-  if (restart) list[slot] = (int32_t)w;
-}
-
 // Curvature directions and the Hertz terms of a Geom2 contact (Superquadrics_Ig2.cpp:2459-2503)
 // in double, with the correctly rounded functions (the FP64 form) or the device's (kFast, the
 // mixed form).  A float form of these was off by up to 0.1 rad in the angle between the

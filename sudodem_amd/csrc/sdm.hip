@@ -45,7 +45,7 @@ enum {
   KT_FIRST = 6,      // k_nm_first
   KT_LIST = 7,       // Nelder-Mead list: flag scan, fill, cost keys, cost sort
   KT_SOLVE = 8,      // main solver launch (k_solve_nm / k_solve_lm)
-  KT_RESUME = 9,     // mixed NM resume launch; LM restart checks and passes
+  KT_RESUME = 9,     // mixed NM resume launch
   KT_REFINE = 10,    // k_nm_refine
   KT_FINISH = 11,    // k_pairs_finish
   KT_RING_WALK = 12, // periodic ring walk: coordinates and the sequential phases
@@ -151,7 +151,7 @@ struct sdm_ctx {
   uint32_t* qpos = nullptr;
   int numCU = 0;
   int residentNM[3] = {0, 0, 0};  // resident blocks of k_solve_nm<float>, <double>, <float, true> (0: not queried)
-  int residentLM[2] = {0, 0};  // resident blocks of the main and restart k_solve_lm launches
+  int residentLM = 0;  // resident blocks of k_solve_lm
   bool matChecked = false;
   bool solverRan = false;
   bool geoDirty = true;
@@ -442,7 +442,7 @@ void zero_counter(sdm_ctx* c, int k) { hipLaunchKernelGGL(k_zero_u64, dim3(1), d
 // [13] the container size and [14] Scene::iter of the step, [15] the mixed form's solver
 // evaluations done in double, [16] its pairs handed to the resume launch.
 constexpr int kCounters = 24;
-enum { DC_NEXT = 3, DC_MISC = 6, DC_NQ = 10, DC_NT = 11, DC_NR = 12, DC_NP = 13, DC_NDBL = 15, DC_NHO = 16 };
+enum { DC_NEXT = 3, DC_MISC = 6, DC_NQ = 10, DC_NT = 11, DC_NP = 13, DC_NDBL = 15, DC_NHO = 16 };
 
 // Exclusive scan of in[0..n) into out[0..n) with the total left on the device in dcnt[slot].
 void scan_to_counter(sdm_ctx* c, const uint32_t* in, uint32_t* out, int64_t n, int slot) {
@@ -1152,11 +1152,10 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
   if (LAW == 0) {
     zero_counter(c, DC_NEXT);
     // The FP64 form solves in double with the correctly rounded functions, the mixed form in
-    // double with the device's own; both on two lanes per pair (each lane one body's support
-    // point), the main launch at two waves per SIMD, the restarts' fewer and longer solves at one
+    // double with the device's own; on two lanes per pair (each lane one body's support point)
+    // at two waves per SIMD, the Geom functor's two restarts inline
     const bool mixedLM = c->prm.precision == SDM_MIXED;
     const auto lmKernel = mixedLM ? k_solve_lm<true, 2, true> : k_solve_lm<false, 2, true>;
-    const auto lmRestart = mixedLM ? k_solve_lm<true, 1, true> : k_solve_lm<false, 1, true>;
     if (mixedLM) {
       SA.dbl = c->dcnt + DC_NDBL;
       zero_counter(c, DC_NDBL);
@@ -1175,30 +1174,14 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
     SolveArgs SL = SA;
     SL.list = reinterpret_cast<const int32_t*>(c->v0);
     SL.count = c->dcnt + DC_NT;
-    const unsigned grid = resident_grid(c, lmKernel, c->residentLM[0], 2 * A.np);
+    SL.lmRestarts = true;
+    const unsigned grid = resident_grid(c, lmKernel, c->residentLM, 2 * A.np);
     if (c->timing) HIPCHK(hipEventRecord(c->ev[4], c->st));
     KT_START(c, KT_SOLVE);
     hipLaunchKernelGGL(lmKernel, dim3(grid), dim3(256), 0, c->st, SL);
     KT_STOP(c, KT_SOLVE);
     if (c->timing) HIPCHK(hipEventRecord(c->ev[5], c->st));
     c->solverRan = true;
-    // the two restarts of Ig2_..._SuperquadricsGeom::go, each a further pass over its list
-    KT_START(c, KT_RESUME);
-    for (int kind = 1; kind <= 2; kind++) {
-      zero_counter(c, DC_NR);
-      if (kind == 1)
-        hipLaunchKernelGGL(k_lm_check<1>, dim3(nblk(A.np)), dim3(256), 0, c->st, A, SA, c->retryList, c->dcnt + DC_NR);
-      else
-        hipLaunchKernelGGL(k_lm_check<2>, dim3(nblk(A.np)), dim3(256), 0, c->st, A, SA, c->retryList, c->dcnt + DC_NR);
-      SolveArgs RA = SA;
-      RA.list = c->retryList;
-      RA.count = c->dcnt + DC_NR;
-      RA.evals = c->dcnt + 5;
-      zero_counter(c, DC_NEXT);
-      hipLaunchKernelGGL(lmRestart, dim3(resident_grid(c, lmRestart, c->residentLM[1], 2 * A.np)), dim3(256), 0,
-                         c->st, RA);
-    }
-    KT_STOP(c, KT_RESUME);
     KT_START(c, KT_FINISH);
     hipLaunchKernelGGL(k_pairs_finish<0>, dim3(nblk(A.np)), dim3(256), 0, c->st, A, c->S, nullptr, A.np, nq);
     KT_STOP(c, KT_FINISH);
@@ -1316,9 +1299,7 @@ void one_step(sdm_ctx* c) {
     ensure_queue(c, A.np);
     ensure_scratch(c, A.np + 1);
     resident_grid(c, c->prm.precision == SDM_MIXED ? k_solve_lm<true, 2, true> : k_solve_lm<false, 2, true>,
-                  c->residentLM[0], 1);
-    resident_grid(c, c->prm.precision == SDM_MIXED ? k_solve_lm<true, 1, true> : k_solve_lm<false, 1, true>,
-                  c->residentLM[1], 1);
+                  c->residentLM, 1);
     resident_grid(c, k_solve_nm<float>, c->residentNM[0], 1);
     resident_grid(c, k_solve_nm<double>, c->residentNM[1], 1);
     resident_grid(c, k_solve_nm<float, true>, c->residentNM[2], 1);

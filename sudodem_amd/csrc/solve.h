@@ -71,6 +71,7 @@ struct SolveArgs {
   unsigned long long* dbl = nullptr;  // mixed NM: evaluations done in double (added)
   int32_t* hoList = nullptr;          // mixed NM: slots handed to the resume launch
   unsigned long long* hoCount = nullptr;  // their number (zero before the main launch)
+  bool lmRestarts = false;  // LM (two lanes per pair): the Geom functor's restarts inline
 };
 
 // Work items of a launch: the device-side count (written by an earlier kernel of the same
@@ -802,12 +803,19 @@ constexpr double kLmMinTrig = 1e-6;
 // set by its longest chains of dependent evaluations (~250 for the solves that reach itmax),
 // which start when the queue has drained.  Measured on MI355X (1M-body periodic cell, FP64):
 // main launch 12.0 -> 9.7 ms/step against one lane per pair; restarts 7.5 -> 5.5 ms/step.
-// kWaves: resident waves per SIMD the register allocation is held to: two for the main launch
-// (12 % faster than one, at 144 B of spill per lane), one for the restarts (fewer pairs than
-// lanes: latency-bound).
+// kWaves: resident waves per SIMD the register allocation is held to.
+// A.lmRestarts (two lanes per pair): the two restarts of Ig2_..._SuperquadricsGeom::go
+// (Superquadrics_Ig2.cpp:2013-2052) run in the lane pair as soon as its solve ends -- kind 1
+// (NaN parameters or stop 7) once after the first solve, kind 2 (a deep contact, 10 |p2 - p1|
+// > r_max of body 1, whose points are not inside the other body) after the first solve or the
+// kind-1 restart's, each a fresh solve from the line of centres + 1e-5 with the stop code
+// tagged 100 / 200 -- so their long chains start during the launch instead of in launches
+// after it (1M periodic cell, FP64: main launch + restarts 15.1 -> 13.7 ms/step).
 template <bool kFast, int kWaves = 1, bool kTwoLane = false>
 __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
   const bool odd = kTwoLane && (threadIdx.x & 1);
+  int phase = 0;          // A.lmRestarts: 0 the first solve, 1 after a kind-1 restart, 2 after a kind-2
+  double myRmax = 0.0;    // A.lmRestarts: this lane's body's r_max
   const double tau = 1.0, eps1 = 1e-15, eps2 = 1e-3, eps3 = 1e-20;
   const int itmax = 50;
   PairGeo<double> g;
@@ -836,6 +844,8 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
           g.s1 = r.s;
           g.G1 = r.G;
           g.pos1 = r.pos;
+          myRmax = r.rmax;
+          phase = 0;
         } else {
           GeoRec r2 = load_rec(A.geo, A.S.id2[w]);
           if (A.S.sh) r2.pos = r2.pos + ld3s(A.S.sh, w, A.S.cap);  // position2 = pos + shift2 (:1886)
@@ -1002,6 +1012,36 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
           nev++;
           state = LM_J1;
         }
+      }
+    }
+    if (finished && kTwoLane && A.lmRestarts && touching && phase < 2) {
+      if (k >= itmax) stop = 3;
+      // kind 1: NaN parameters or stop 7 (checked once, after the first solve); kind 2: a deep
+      // contact whose points are not inside the other body (after the first solve or the
+      // kind-1 restart's)
+      // (every lane shuffle here runs in both lanes of the pair: a lane reads its partner's
+      // register, which must have been written by the same instruction)
+      const double rmax1 = __shfl(myRmax, (int)(threadIdx.x & 62));
+      int kind = 0;
+      if (phase == 0 && (isnan(p0) || isnan(p1v) || stop == 7)) {
+        kind = 1;
+      } else if (10.0 * norm(lp2 - lp1) > rmax1) {
+        const bool in = is_inside(g.s1, g.G1, (odd ? lp1 : lp2) - g.pos1);  // p2 in body 1, p1 in body 2
+        const bool inOther = __shfl_xor((int)in, 1) != 0;
+        if (!in || !inOther) kind = 2;
+      }
+      if (kind) {
+        const V3<double> o = mk3(__shfl_xor(g.pos1.x, 1), __shfl_xor(g.pos1.y, 1), __shfl_xor(g.pos1.z, 1));
+        const V3<double> d0 = (odd ? g.pos1 : o) - (odd ? o : g.pos1);  // (position2 + shift2) - position1
+        p0 = xatan2(d0.y, d0.x) + 1e-5;
+        p1v = xatan(d0.z / sqrt(d0.x * d0.x + d0.y * d0.y)) + 1e-5;
+        if (!odd) A.S.tag[w] = kind == 1 ? 100 : 200;
+        phase = kind;
+        Dp0 = 1e-7; Dp1 = 1e-7;
+        mu = 0; nu = 2; stop = 0; k = 0;
+        stop_flag = false;
+        state = LM_INIT;
+        finished = false;  // the evaluation count runs on: the pair's total is the passes' sum
       }
     }
     if (finished) {
