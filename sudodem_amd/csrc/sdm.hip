@@ -1155,7 +1155,7 @@ void interaction_loop(sdm_ctx* c, const StepArgs& A) {
     // double with the device's own; on two lanes per pair (each lane one body's support point)
     // at two waves per SIMD, the Geom functor's two restarts inline
     const bool mixedLM = c->prm.precision == SDM_MIXED;
-    const auto lmKernel = mixedLM ? k_solve_lm<true, 2, true> : k_solve_lm<false, 2, true>;
+    const auto lmKernel = mixedLM ? k_solve_lm<true, 2> : k_solve_lm<false, 2>;
     if (mixedLM) {
       SA.dbl = c->dcnt + DC_NDBL;
       zero_counter(c, DC_NDBL);
@@ -1298,7 +1298,7 @@ void one_step(sdm_ctx* c) {
     A.np = c->P.cap;
     ensure_queue(c, A.np);
     ensure_scratch(c, A.np + 1);
-    resident_grid(c, c->prm.precision == SDM_MIXED ? k_solve_lm<true, 2, true> : k_solve_lm<false, 2, true>,
+    resident_grid(c, c->prm.precision == SDM_MIXED ? k_solve_lm<true, 2> : k_solve_lm<false, 2>,
                   c->residentLM, 1);
     resident_grid(c, k_solve_nm<float>, c->residentNM[0], 1);
     resident_grid(c, k_solve_nm<double>, c->residentNM[1], 1);

@@ -795,7 +795,7 @@ enum { LM_INIT = 0, LM_J1 = 2, LM_J2 = 3, LM_TRIAL = 4 };  // f(p) of the Jacobi
 constexpr double kLmMinTrig = 1e-6;
 
 // kFast: the device's double functions (the mixed form); otherwise correctly rounded (FP64).
-// kTwoLane: two adjacent lanes take each pair, each holding one body: both run the pair's
+// Two adjacent lanes take each pair, each holding one body: both run the pair's
 // state machine (the same arithmetic on the same values, so they stay in step) and each
 // evaluates its own body's support point, which they exchange -- p2 is body 2's support along
 // -c (Disfunction's mulT(G2, c) negated is mulT(G2, -c) exactly).  An evaluation then takes
@@ -804,21 +804,21 @@ constexpr double kLmMinTrig = 1e-6;
 // which start when the queue has drained.  Measured on MI355X (1M-body periodic cell, FP64):
 // main launch 12.0 -> 9.7 ms/step against one lane per pair; restarts 7.5 -> 5.5 ms/step.
 // kWaves: resident waves per SIMD the register allocation is held to.
-// A.lmRestarts (two lanes per pair): the two restarts of Ig2_..._SuperquadricsGeom::go
+// A.lmRestarts: the two restarts of Ig2_..._SuperquadricsGeom::go
 // (Superquadrics_Ig2.cpp:2013-2052) run in the lane pair as soon as its solve ends -- kind 1
 // (NaN parameters or stop 7) once after the first solve, kind 2 (a deep contact, 10 |p2 - p1|
 // > r_max of body 1, whose points are not inside the other body) after the first solve or the
 // kind-1 restart's, each a fresh solve from the line of centres + 1e-5 with the stop code
 // tagged 100 / 200 -- so their long chains start during the launch instead of in launches
 // after it (1M periodic cell, FP64: main launch + restarts 15.1 -> 13.7 ms/step).
-template <bool kFast, int kWaves = 1, bool kTwoLane = false>
+template <bool kFast, int kWaves = 1>
 __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
-  const bool odd = kTwoLane && (threadIdx.x & 1);
+  const bool odd = (threadIdx.x & 1) != 0;  // body 2's lane of the pair
   int phase = 0;          // A.lmRestarts: 0 the first solve, 1 after a kind-1 restart, 2 after a kind-2
   double myRmax = 0.0;    // A.lmRestarts: this lane's body's r_max
   const double tau = 1.0, eps1 = 1e-15, eps2 = 1e-3, eps3 = 1e-20;
   const int itmax = 50;
-  PairGeo<double> g;
+  PairGeo<double> g;  // this lane's body in s1, G1, pos1
   double p0 = 0, p1v = 0, Dp0 = 0, Dp1 = 0, mu = 0, p_fL2 = 0, A00 = 0, A01 = 0, A11 = 0, dA0 = 0, dA1 = 0;
   double jTf0 = 0, jTf1 = 0, p_L2 = 0, q0 = 0, qq1 = 0;
   V3<double> p_f = mk3(0.0, 0.0, 0.0), f0 = p_f, f1 = p_f, ct = p_f, lp1 = p_f, lp2 = p_f, lc = p_f;
@@ -832,26 +832,20 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
   for (;;) {
     const bool idle = (w < 0) && !done;
     int64_t got = take_work(idle && !odd, batch, A.next, nwork);
-    if constexpr (kTwoLane) got = __shfl((long long)got, (int)(threadIdx.x & 62));
+    got = __shfl((long long)got, (int)(threadIdx.x & 62));
     if (idle) {
       if (got < 0) {
         done = true;
       } else {
         w = A.list ? A.list[got] : got;
-        if constexpr (kTwoLane) {  // this lane's body in g.s1, g.G1, g.pos1
-          GeoRec r = load_rec(A.geo, odd ? A.S.id2[w] : A.S.id1[w]);
-          if (odd && A.S.sh) r.pos = r.pos + ld3s(A.S.sh, w, A.S.cap);
-          g.s1 = r.s;
-          g.G1 = r.G;
-          g.pos1 = r.pos;
-          myRmax = r.rmax;
-          phase = 0;
-        } else {
-          GeoRec r2 = load_rec(A.geo, A.S.id2[w]);
-          if (A.S.sh) r2.pos = r2.pos + ld3s(A.S.sh, w, A.S.cap);  // position2 = pos + shift2 (:1886)
-          const GeoRec r1 = load_rec(A.geo, A.S.id1[w]);
-          pair_geo<double>(r1, r2, false, g);
-        }
+        // this lane's body in g.s1, g.G1, g.pos1; position2 = pos + shift2 (:1886)
+        GeoRec r = load_rec(A.geo, odd ? A.S.id2[w] : A.S.id1[w]);
+        if (odd && A.S.sh) r.pos = r.pos + ld3s(A.S.sh, w, A.S.cap);
+        g.s1 = r.s;
+        g.G1 = r.G;
+        g.pos1 = r.pos;
+        myRmax = r.rmax;
+        phase = 0;
         p0 = A.S.a0[w];
         p1v = A.S.a1[w];
         ct = ld3s(A.S.ct, w, A.S.cap);
@@ -870,7 +864,7 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
     if (state == LM_J2) e1 = p1v + Dp1;
     if (state == LM_TRIAL) { e0 = q0; e1 = qq1; }
     V3<double> fe;
-    if constexpr (kTwoLane) {  // lm_eval split over the lane pair
+    {  // Disfunction (lm_eval) split over the lane pair
       nev++;
       double sa0, ca0, sa1, ca1;
       msincos<kFast>(e0, sa0, ca0);
@@ -884,9 +878,6 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
       lc = mk3(0.5 * (lp1.x + lp2.x), 0.5 * (lp1.y + lp2.y), 0.5 * (lp1.z + lp2.z));
       fe = lp2 - lp1;
       if (dot(contact, fe) > 0.0) stop_flag = true;
-    } else {
-      lm_eval<double, kFast>(g, e0, e1, lp1, lp2, lc, stop_flag, nev);
-      fe = lp2 - lp1;
     }
     bool finished = false;
     bool outer = false;  // go to the outer-iteration checks
@@ -1014,7 +1005,7 @@ __global__ __launch_bounds__(256, kWaves) void k_solve_lm(SolveArgs A) {
         }
       }
     }
-    if (finished && kTwoLane && A.lmRestarts && touching && phase < 2) {
+    if (finished && A.lmRestarts && touching && phase < 2) {
       if (k >= itmax) stop = 3;
       // kind 1: NaN parameters or stop 7 (checked once, after the first solve); kind 2: a deep
       // contact whose points are not inside the other body (after the first solve or the
